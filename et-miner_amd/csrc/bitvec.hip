@@ -1,0 +1,214 @@
+// Input side of the counting path: item-id range and histogram, the CSR ->
+// column-bitvector builder, and the K=1 column popcount.
+//
+// Builder layout: a workgroup owns a tile of 64 consecutive words (4,096
+// transactions) times a slab of SLAB columns. It zero-fills the tile in LDS,
+// ORs the bits of its rows' entries into LDS (ds_or_b64), then writes each
+// column's 64 words as one contiguous 512-byte run: every HBM write is a full,
+// coalesced segment and no global atomic is issued.
+#include "common.h"
+
+namespace {
+
+constexpr int kBuildThreads = 256;
+constexpr int kTileWords = 64;
+constexpr int kTileRows = kTileWords * 64;  // 4,096 transactions per tile
+
+template <typename Id>
+__global__ __launch_bounds__(256) void k_ids_minmax(const Id* __restrict__ ids, int64_t n, int64_t* out) {
+    __shared__ int64_t lds_min[4], lds_max[4];
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t v = (int64_t)ids[i];
+        lo = v < lo ? v : lo;
+        hi = v > hi ? v : hi;
+    }
+    lo = etm::wave_min(lo);
+    hi = etm::wave_max(hi);
+    const int wid = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) {
+        lds_min[wid] = lo;
+        lds_max[wid] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            lo = lds_min[w] < lo ? lds_min[w] : lo;
+            hi = lds_max[w] > hi ? lds_max[w] : hi;
+        }
+        lo = lds_min[0] < lo ? lds_min[0] : lo;
+        hi = lds_max[0] > hi ? lds_max[0] : hi;
+        atomicMin((long long*)&out[0], (long long)lo);
+        atomicMax((long long*)&out[1], (long long)hi);
+    }
+}
+
+__global__ void k_minmax_init(int64_t* out) {
+    out[0] = INT64_MAX;
+    out[1] = INT64_MIN;
+}
+
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Histogram through a per-workgroup LDS copy (span <= kLdsBins), flushed with one global add per bin.
+constexpr int kLdsBins = 16384;
+
+template <typename Id>
+__global__ __launch_bounds__(256) void k_hist_lds(const Id* __restrict__ ids, int64_t n, int64_t lo, int32_t span,
+                                                   int32_t* __restrict__ hist) {
+    __shared__ int32_t bins[kLdsBins];
+    for (int b = threadIdx.x; b < span; b += blockDim.x) bins[b] = 0;
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t v = (int64_t)ids[i] - lo;
+        if (v >= 0 && v < span) atomicAdd(&bins[v], 1);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < span; b += blockDim.x)
+        if (bins[b]) atomicAdd(&hist[b], bins[b]);
+}
+
+template <typename Id>
+__global__ __launch_bounds__(256) void k_hist_global(const Id* __restrict__ ids, int64_t n, int64_t lo, int64_t span,
+                                                      int32_t* __restrict__ hist) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t v = (int64_t)ids[i] - lo;
+        if (v >= 0 && v < span) atomicAdd(&hist[v], 1);
+    }
+}
+
+// One workgroup: words [w0, w0 + 64) x columns [c0, c0 + SLAB). Wave `wid`, lane `l` handles the
+// transactions r = w0*64 + l*64 + j for j in [wid*16, wid*16 + 16): lanes of a wave write different words.
+template <typename Id, int SLAB>
+__global__ __launch_bounds__(kBuildThreads) void k_bitvec_build(const int64_t* __restrict__ indptr,
+                                                                 const Id* __restrict__ ids, int64_t n_rows,
+                                                                 int64_t lo, int64_t span,
+                                                                 const int32_t* __restrict__ lut, int32_t n_cols,
+                                                                 int64_t pitch, uint64_t* __restrict__ out) {
+    __shared__ unsigned long long tile[SLAB * kTileWords];
+    const int64_t w0 = (int64_t)blockIdx.x * kTileWords;
+    const int c0 = blockIdx.y * SLAB;
+    for (int t = threadIdx.x; t < SLAB * kTileWords; t += kBuildThreads) tile[t] = 0ull;
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const int64_t r_base = w0 * 64 + (int64_t)lane * 64;
+    for (int j = wid * 16; j < wid * 16 + 16; ++j) {
+        const int64_t r = r_base + j;
+        if (r >= n_rows) break;
+        const int64_t a = indptr[r], b = indptr[r + 1];
+        const unsigned long long bit = 1ull << j;
+        for (int64_t e = a; e < b; ++e) {
+            const int64_t v = (int64_t)ids[e] - lo;
+            if (v < 0 || v >= span) continue;
+            const int c = lut[v] - c0;
+            if (c >= 0 && c < SLAB) atomicOr(&tile[c * kTileWords + lane], bit);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < SLAB * kTileWords; t += kBuildThreads) {
+        const int c = t / kTileWords;
+        const int w = t % kTileWords;
+        if (c0 + c < n_cols && w0 + w < pitch) out[(int64_t)(c0 + c) * pitch + w0 + w] = tile[t];
+    }
+}
+
+// One workgroup per column: sum of popcounts over the column's words.
+__global__ __launch_bounds__(256) void k_colpop(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
+                                                 int64_t* __restrict__ out) {
+    __shared__ int64_t lds[8];
+    const uint64_t* col = bv + (int64_t)blockIdx.x * pitch;
+    int64_t s = 0;
+    for (int64_t w = threadIdx.x; w < n_words; w += blockDim.x) s += __popcll(col[w]);
+    s = etm::block_sum<256>(s, lds);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+int grid_for(int64_t n, int per_block = 256, int cap = 8192) {
+    int64_t g = (n + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+}  // namespace
+
+extern "C" int etm_ids_minmax(const void* d_ids, int id_bytes, int64_t n, int64_t* d_out, void* stream) {
+    ETM_ARG(d_ids != nullptr && d_out != nullptr, "etm_ids_minmax: null pointer");
+    ETM_ARG(id_bytes == 4 || id_bytes == 8, "etm_ids_minmax: id_bytes must be 4 or 8, got %d", id_bytes);
+    ETM_ARG(n >= 1, "etm_ids_minmax: n must be >= 1");
+    hipStream_t s = ETM_STREAM(stream);
+    k_minmax_init<<<1, 1, 0, s>>>(d_out);
+    ETM_LAUNCHED("k_minmax_init");
+    if (id_bytes == 4)
+        k_ids_minmax<int32_t><<<grid_for(n, 256, 2048), 256, 0, s>>>((const int32_t*)d_ids, n, d_out);
+    else
+        k_ids_minmax<int64_t><<<grid_for(n, 256, 2048), 256, 0, s>>>((const int64_t*)d_ids, n, d_out);
+    ETM_LAUNCHED("k_ids_minmax");
+    return ETM_OK;
+}
+
+extern "C" int etm_ids_hist(const void* d_ids, int id_bytes, int64_t n, int64_t lo, int64_t span, int32_t* d_hist,
+                            void* stream) {
+    ETM_ARG(d_hist != nullptr, "etm_ids_hist: null histogram");
+    ETM_ARG(id_bytes == 4 || id_bytes == 8, "etm_ids_hist: id_bytes must be 4 or 8, got %d", id_bytes);
+    ETM_ARG(span >= 1 && n >= 0, "etm_ids_hist: span must be >= 1 and n >= 0");
+    ETM_ARG(n == 0 || d_ids != nullptr, "etm_ids_hist: null ids");
+    hipStream_t s = ETM_STREAM(stream);
+    k_fill_i32<<<grid_for(span), 256, 0, s>>>(d_hist, span, 0);
+    ETM_LAUNCHED("k_fill_i32");
+    if (n == 0) return ETM_OK;
+    if (span <= kLdsBins) {
+        const int g = grid_for(n, 256 * 64, 1024);
+        if (id_bytes == 4)
+            k_hist_lds<int32_t><<<g, 256, 0, s>>>((const int32_t*)d_ids, n, lo, (int32_t)span, d_hist);
+        else
+            k_hist_lds<int64_t><<<g, 256, 0, s>>>((const int64_t*)d_ids, n, lo, (int32_t)span, d_hist);
+        ETM_LAUNCHED("k_hist_lds");
+    } else {
+        if (id_bytes == 4)
+            k_hist_global<int32_t><<<grid_for(n), 256, 0, s>>>((const int32_t*)d_ids, n, lo, span, d_hist);
+        else
+            k_hist_global<int64_t><<<grid_for(n), 256, 0, s>>>((const int64_t*)d_ids, n, lo, span, d_hist);
+        ETM_LAUNCHED("k_hist_global");
+    }
+    return ETM_OK;
+}
+
+extern "C" int etm_bitvec_build(const int64_t* d_indptr, const void* d_ids, int id_bytes, int64_t n_rows, int64_t lo,
+                                int64_t span, const int32_t* d_lut, int32_t n_cols, int64_t pitch, uint64_t* d_out,
+                                void* stream) {
+    ETM_ARG(d_indptr != nullptr && d_lut != nullptr && d_out != nullptr, "etm_bitvec_build: null pointer");
+    ETM_ARG(id_bytes == 4 || id_bytes == 8, "etm_bitvec_build: id_bytes must be 4 or 8, got %d", id_bytes);
+    ETM_ARG(n_rows >= 0 && n_cols >= 0 && span >= 1, "etm_bitvec_build: negative size");
+    ETM_ARG(pitch >= (n_rows + 63) / 64, "etm_bitvec_build: pitch %lld < ceil(n_rows/64) = %lld",
+            (long long)pitch, (long long)((n_rows + 63) / 64));
+    if (n_cols == 0 || pitch == 0) return ETM_OK;
+    constexpr int SLAB = 128;  // 64 KiB of LDS per workgroup
+    const int64_t tiles = (pitch + kTileWords - 1) / kTileWords;
+    ETM_REQUIRE(tiles <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_bitvec_build: %lld word tiles exceed the grid",
+                (long long)tiles);
+    dim3 grid((unsigned)tiles, (unsigned)((n_cols + SLAB - 1) / SLAB));
+    hipStream_t s = ETM_STREAM(stream);
+    if (id_bytes == 4)
+        k_bitvec_build<int32_t, SLAB><<<grid, kBuildThreads, 0, s>>>(d_indptr, (const int32_t*)d_ids, n_rows, lo,
+                                                                     span, d_lut, n_cols, pitch, d_out);
+    else
+        k_bitvec_build<int64_t, SLAB><<<grid, kBuildThreads, 0, s>>>(d_indptr, (const int64_t*)d_ids, n_rows, lo,
+                                                                     span, d_lut, n_cols, pitch, d_out);
+    ETM_LAUNCHED("k_bitvec_build");
+    return ETM_OK;
+}
+
+extern "C" int etm_column_popcounts(const uint64_t* d_bv, int32_t n_cols, int64_t pitch, int64_t n_words,
+                                    int64_t* d_out, void* stream) {
+    ETM_ARG(d_bv != nullptr && d_out != nullptr, "etm_column_popcounts: null pointer");
+    ETM_ARG(n_cols >= 0 && n_words >= 0 && pitch >= n_words, "etm_column_popcounts: bad sizes");
+    if (n_cols == 0) return ETM_OK;
+    k_colpop<<<n_cols, 256, 0, ETM_STREAM(stream)>>>(d_bv, pitch, n_words, d_out);
+    ETM_LAUNCHED("k_colpop");
+    return ETM_OK;
+}

@@ -1,0 +1,344 @@
+// AND + popcount counting of a level's candidates.
+//
+// Two kernel families share one work description (prefix groups of a
+// lexsorted level, see etm.h):
+//
+// * k_count_small<S>: groups of 2..S suffixes. One wave64 per (group, word
+//   chunk); lane l reads word w0 + l + 64t of every column, so each load
+//   instruction fetches one contiguous 512-byte run per column. Per word the
+//   wave ANDs the prefix columns, ANDs each suffix column with that, and
+//   accumulates popcounts of all S(S-1)/2 suffix pairs in registers. Partial
+//   counts are reduced across the wave with shuffles and added to the count
+//   array with one integer atomic per pair and chunk (exact and
+//   order-independent). HBM-bound: (prefix + suffixes) x 8 bytes per word.
+//
+// * k_count_tiled: groups with more suffixes (and the K=2 pair space, one
+//   group with an empty prefix). One workgroup per (tile-pair, word chunk) of
+//   64 x 64 suffixes. Per 32-word step it stages the prefix AND and the two
+//   tiles' suffix columns (already ANDed with the prefix) in LDS, transposed
+//   to [word][suffix] so the compute loop reads are conflict-free; each of
+//   256 threads accumulates a 4 x 4 register block of pairs. VALU-bound: 4
+//   instructions (2 AND + 2 BCNT) per candidate-word.
+//
+// A tile-pair or group none of whose candidates has status ETM_CAND_COUNT
+// returns before reading any bitvector.
+#include <algorithm>
+
+#include "common.h"
+
+namespace {
+
+constexpr int kT = 64;   // suffixes per tile
+constexpr int kKW = 32;  // words per LDS step
+
+__device__ __forceinline__ int64_t pair_slot(int64_t i, int64_t j, int64_t s, int32_t layout) {
+    return layout == ETM_LAYOUT_TRI ? j * (j - 1) / 2 + i : etm::lex_pair_index(i, j, s);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
+                                                      int64_t chunk_words, int64_t n_chunks,
+                                                      const int32_t* __restrict__ rows, int32_t m,
+                                                      const int64_t* __restrict__ group_start,
+                                                      const int64_t* __restrict__ cum_pairs,
+                                                      const int32_t* __restrict__ list, int64_t n_list,
+                                                      const uint8_t* __restrict__ status,
+                                                      int32_t* __restrict__ counts, int32_t layout) {
+    constexpr int NP = S * (S - 1) / 2;
+    const int lane = threadIdx.x & 63;
+    const int64_t item = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (item >= n_list * n_chunks) return;
+    const int64_t g = list[item / n_chunks];
+    const int64_t ch = item % n_chunks;
+    const int64_t gs = group_start[g];
+    const int s = (int)(group_start[g + 1] - gs);
+    if (s < 2 || s > S) return;
+    const int np = m - 1;
+    const int64_t cbase = cum_pairs[g];
+    const int npairs = s * (s - 1) / 2;
+
+    // Which of the group's candidates are counted (lex order within the group).
+    bool any_live;
+    if (status == nullptr) {
+        any_live = true;
+    } else {
+        bool lv = false;
+        for (int p = lane; p < npairs; p += 64) lv = lv || status[cbase + p] == ETM_CAND_COUNT;
+        any_live = __ballot(lv) != 0;
+    }
+    if (!any_live) return;
+
+    int64_t scol[S];
+#pragma unroll
+    for (int t = 0; t < S; ++t) scol[t] = t < s ? (int64_t)rows[(gs + t) * m + np] * pitch : 0;
+    const int32_t* prow = rows + gs * m;
+
+    int acc[NP];
+#pragma unroll
+    for (int a = 0; a < NP; ++a) acc[a] = 0;
+
+    const int64_t w_begin = ch * chunk_words;
+    const int64_t w_end = min(n_words, w_begin + chunk_words);
+    for (int64_t w = w_begin + lane; w < w_end; w += 64) {
+        uint64_t pre = ~0ull;
+        for (int t = 0; t < np; ++t) pre &= bv[(int64_t)prow[t] * pitch + w];
+        uint64_t sv[S];
+#pragma unroll
+        for (int t = 0; t < S; ++t) sv[t] = t < s ? (bv[scol[t] + w] & pre) : 0ull;
+        int a = 0;
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+#pragma unroll
+            for (int j = i + 1; j < S; ++j) {
+                acc[a] += __popcll(sv[i] & sv[j]);
+                ++a;
+            }
+        }
+    }
+
+    int a = 0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+#pragma unroll
+        for (int j = i + 1; j < S; ++j) {
+            const int tot = etm::wave_sum(acc[a]);
+            if (j < s && lane == (a & 63)) {
+                const int64_t slot = pair_slot(i, j, s, layout);
+                if (status == nullptr || status[cbase + slot] == ETM_CAND_COUNT) atomicAdd(&counts[cbase + slot], tot);
+            }
+            ++a;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
+                                                      int64_t chunk_words, const int32_t* __restrict__ rows,
+                                                      int32_t m, const int64_t* __restrict__ group_start,
+                                                      const int64_t* __restrict__ cum_pairs,
+                                                      const int64_t* __restrict__ cum_tiles, int64_t n_groups,
+                                                      const uint8_t* __restrict__ status,
+                                                      int32_t* __restrict__ counts, int32_t layout) {
+    __shared__ uint64_t sA[kKW][kT + 1];
+    __shared__ uint64_t sB[kKW][kT + 1];
+    __shared__ uint64_t sP[kKW];
+    __shared__ int64_t colA[kT], colB[kT];
+
+    const int tid = threadIdx.x;
+    const int64_t tp = blockIdx.x;
+    const int64_t g = etm::upper_index(cum_tiles, n_groups, tp);
+    const int64_t gs = group_start[g];
+    const int64_t s = group_start[g + 1] - gs;
+    const int64_t nt = (s + kT - 1) / kT;
+    int64_t ta, tb1;
+    etm::lex_pair_decode(tp - cum_tiles[g], nt + 1, &ta, &tb1);
+    const int64_t tb = tb1 - 1;
+    const bool diag = ta == tb;
+    const int np = m - 1;
+    const int64_t cbase = cum_pairs[g];
+    const int tx = tid & 15, ty = tid >> 4;
+
+    // Liveness of this thread's 16 pairs.
+    bool mine_live = false;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int64_t i = ta * kT + tx + 16 * r, j = tb * kT + ty + 16 * c;
+            if (i < j && j < s)
+                mine_live = mine_live || status == nullptr ||
+                            status[cbase + pair_slot(i, j, s, layout)] == ETM_CAND_COUNT;
+        }
+    }
+    if (!__syncthreads_or(mine_live)) return;
+
+    if (tid < kT) {
+        const int64_t i = ta * kT + tid;
+        colA[tid] = i < s ? (int64_t)rows[(gs + i) * m + np] * pitch : -1;
+    } else if (tid < 2 * kT) {
+        const int64_t j = tb * kT + (tid - kT);
+        colB[tid - kT] = j < s ? (int64_t)rows[(gs + j) * m + np] * pitch : -1;
+    }
+    const int32_t* prow = rows + gs * m;
+
+    int acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = 0;
+
+    const int64_t w_begin = (int64_t)blockIdx.y * chunk_words;
+    const int64_t w_end = min(n_words, w_begin + chunk_words);
+    __syncthreads();
+    for (int64_t wb = w_begin; wb < w_end; wb += kKW) {
+        if (tid < kKW) {
+            const int64_t w = wb + tid;
+            uint64_t pre = w < w_end ? ~0ull : 0ull;
+            if (w < w_end)
+                for (int t = 0; t < np; ++t) pre &= bv[(int64_t)prow[t] * pitch + w];
+            sP[tid] = pre;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < (kT * kKW) / 256; ++q) {
+            const int e = tid + 256 * q;
+            const int c = e / kKW, w = e % kKW;
+            const int64_t ca = colA[c];
+            sA[w][c] = (ca >= 0 && wb + w < w_end) ? (bv[ca + wb + w] & sP[w]) : 0ull;
+            if (!diag) {
+                const int64_t cb = colB[c];
+                sB[w][c] = (cb >= 0 && wb + w < w_end) ? (bv[cb + wb + w] & sP[w]) : 0ull;
+            }
+        }
+        __syncthreads();
+        uint64_t(*B)[kT + 1] = diag ? sA : sB;
+#pragma unroll 4
+        for (int w = 0; w < kKW; ++w) {
+            uint64_t av[4], bw[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) av[r] = sA[w][tx + 16 * r];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) bw[c] = B[w][ty + 16 * c];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[r][c] += __popcll(av[r] & bw[c]);
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int64_t i = ta * kT + tx + 16 * r, j = tb * kT + ty + 16 * c;
+            if (i < j && j < s) {
+                const int64_t slot = cbase + pair_slot(i, j, s, layout);
+                if (status == nullptr || status[slot] == ETM_CAND_COUNT) atomicAdd(&counts[slot], acc[r][c]);
+            }
+        }
+    }
+}
+
+// One wave per (itemset, word chunk): AND of the itemset's columns, popcount, wave sum, one 64-bit add.
+__global__ __launch_bounds__(256) void k_count_itemsets(const uint64_t* __restrict__ bv, int64_t pitch,
+                                                         int64_t n_words, int64_t chunk_words, int64_t n_chunks,
+                                                         const int32_t* __restrict__ items,
+                                                         const int64_t* __restrict__ offsets, int64_t n_sets,
+                                                         int64_t* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t item = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (item >= n_sets * n_chunks) return;
+    const int64_t x = item / n_chunks, ch = item % n_chunks;
+    const int64_t a = offsets[x], b = offsets[x + 1];
+    const int64_t w_begin = ch * chunk_words;
+    const int64_t w_end = min(n_words, w_begin + chunk_words);
+    int64_t sum = 0;
+    for (int64_t w = w_begin + lane; w < w_end; w += 64) {
+        uint64_t v = ~0ull;
+        for (int64_t e = a; e < b; ++e) v &= bv[(int64_t)items[e] * pitch + w];
+        sum += __popcll(v);
+    }
+    sum = etm::wave_sum(sum);
+    if (lane == 0 && sum) atomicAdd((unsigned long long*)&out[x], (unsigned long long)sum);
+}
+
+__global__ void k_zero_i64n(int64_t* p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Word chunk (a multiple of `align` words) giving about `target` work items over `units` independent units.
+int64_t pick_chunk(int64_t n_words, int64_t units, int64_t target, int64_t align, int64_t min_words) {
+    if (n_words <= 0) return align;
+    int64_t chunks = units > 0 ? ceil_div(target, units) : 1;
+    int64_t max_chunks = std::max<int64_t>(1, n_words / std::max<int64_t>(min_words, align));
+    chunks = std::max<int64_t>(1, std::min(chunks, max_chunks));
+    return ceil_div(ceil_div(n_words, chunks), align) * align;
+}
+
+template <int S>
+hipError_t launch_small(const uint64_t* bv, int64_t pitch, int64_t n_words, int64_t chunk, const int32_t* rows,
+                        int32_t m, const int64_t* gs, const int64_t* cp, const int32_t* list, int64_t n_list,
+                        const uint8_t* status, int32_t* counts, int32_t layout, hipStream_t s) {
+    const int64_t n_chunks = ceil_div(n_words, chunk);
+    const int64_t waves = n_list * n_chunks;
+    const int64_t blocks = ceil_div(waves, 4);
+    if (blocks > 0x7FFFFFFF) return hipErrorInvalidConfiguration;
+    k_count_small<S><<<(unsigned)blocks, 256, 0, s>>>(bv, pitch, n_words, chunk, n_chunks, rows, m, gs, cp, list,
+                                                      n_list, status, counts, layout);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const int32_t* d_rows, int32_t m,
+                               const int64_t* d_group_start, const int64_t* d_cum_pairs, const int64_t* d_cum_tiles,
+                               int64_t n_groups, int64_t n_tile_pairs, const int32_t* d_small_list, int64_t n_small,
+                               int32_t small_max, const uint8_t* d_status, int32_t* d_counts, int32_t layout,
+                               int32_t word_chunk, void* stream) {
+    ETM_ARG(d_bv && d_rows && d_group_start && d_cum_pairs && d_counts, "etm_count_level: null pointer");
+    ETM_ARG(m >= 1 && m <= 62, "etm_count_level: 1 <= m <= 62 required, got %d", m);
+    ETM_ARG(pitch >= n_words && n_words >= 0, "etm_count_level: pitch %lld < n_words %lld", (long long)pitch,
+            (long long)n_words);
+    ETM_ARG(small_max == 2 || small_max == 4 || small_max == 8 || small_max == 16,
+            "etm_count_level: small_max must be 2, 4, 8 or 16, got %d", small_max);
+    ETM_ARG(layout == ETM_LAYOUT_LEX || (layout == ETM_LAYOUT_TRI && n_groups == 1),
+            "etm_count_level: the TRI layout needs a single group");
+    ETM_ARG(n_tile_pairs == 0 || d_cum_tiles != nullptr, "etm_count_level: null tile offsets");
+    ETM_ARG(n_small == 0 || d_small_list != nullptr, "etm_count_level: null small-group list");
+    if (n_words == 0) return ETM_OK;
+    hipStream_t s = ETM_STREAM(stream);
+    if (n_small > 0) {
+        const int64_t chunk = word_chunk > 0 ? word_chunk : pick_chunk(n_words, n_small, 16384, 64, 256);
+        hipError_t e;
+        switch (small_max) {
+            case 2: e = launch_small<2>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start, d_cum_pairs,
+                                        d_small_list, n_small, d_status, d_counts, layout, s); break;
+            case 4: e = launch_small<4>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start, d_cum_pairs,
+                                        d_small_list, n_small, d_status, d_counts, layout, s); break;
+            case 8: e = launch_small<8>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start, d_cum_pairs,
+                                        d_small_list, n_small, d_status, d_counts, layout, s); break;
+            default: e = launch_small<16>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start, d_cum_pairs,
+                                          d_small_list, n_small, d_status, d_counts, layout, s); break;
+        }
+        ETM_REQUIRE(e == hipSuccess, ETM_EHIP, "launch of k_count_small failed: %s", hipGetErrorString(e));
+    }
+    if (n_tile_pairs > 0) {
+        ETM_REQUIRE(n_tile_pairs <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld tile-pairs exceed the grid",
+                    (long long)n_tile_pairs);
+        const int64_t chunk = word_chunk > 0 ? ceil_div(word_chunk, kKW) * kKW
+                                             : pick_chunk(n_words, n_tile_pairs, 4096, kKW, 512);
+        const int64_t n_chunks = ceil_div(n_words, chunk);
+        ETM_REQUIRE(n_chunks <= 65535, ETM_EOVERFLOW, "etm_count_level: %lld word chunks exceed the grid",
+                    (long long)n_chunks);
+        dim3 grid((unsigned)n_tile_pairs, (unsigned)n_chunks);
+        k_count_tiled<<<grid, 256, 0, s>>>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start, d_cum_pairs,
+                                           d_cum_tiles, n_groups, d_status, d_counts, layout);
+        ETM_LAUNCHED("k_count_tiled");
+    }
+    return ETM_OK;
+}
+
+extern "C" int etm_count_itemsets(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const int32_t* d_items,
+                                  const int64_t* d_offsets, int64_t n_sets, int64_t* d_out, int32_t accumulate,
+                                  void* stream) {
+    ETM_ARG(d_bv && d_items && d_offsets && d_out, "etm_count_itemsets: null pointer");
+    ETM_ARG(pitch >= n_words && n_words >= 0 && n_sets >= 0, "etm_count_itemsets: bad sizes");
+    hipStream_t s = ETM_STREAM(stream);
+    if (n_sets == 0) return ETM_OK;
+    if (!accumulate) {
+        k_zero_i64n<<<(unsigned)std::min<int64_t>(ceil_div(n_sets, 256), 4096), 256, 0, s>>>(d_out, n_sets);
+        ETM_LAUNCHED("k_zero_i64n");
+    }
+    if (n_words == 0) return ETM_OK;
+    const int64_t chunk = pick_chunk(n_words, n_sets, 16384, 64, 256);
+    const int64_t n_chunks = ceil_div(n_words, chunk);
+    const int64_t blocks = ceil_div(n_sets * n_chunks, 4);
+    ETM_REQUIRE(blocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_itemsets: too many itemsets");
+    k_count_itemsets<<<(unsigned)blocks, 256, 0, s>>>(d_bv, pitch, n_words, chunk, n_chunks, d_items, d_offsets,
+                                                      n_sets, d_out);
+    ETM_LAUNCHED("k_count_itemsets");
+    return ETM_OK;
+}

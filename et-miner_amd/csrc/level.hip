@@ -1,0 +1,429 @@
+// Level management on the device: prefix groups of a lexsorted level, the
+// open-addressing hash of a level, candidate classification (the (k-1)-subset
+// test and Pascal inference), survivor compaction in lexicographic order, and
+// free flags. Nothing here returns to the host except the scalar totals the
+// caller reads to size the next buffers.
+#include "common.h"
+#include "scan.h"
+
+namespace {
+
+using etm::item_hash;
+
+constexpr int kTileT = 64;  // suffixes per tile of the tiled counting kernel (count.hip)
+
+__device__ __forceinline__ int32_t row_item(const int32_t* __restrict__ rows, int64_t r, int32_t m, int t) {
+    return rows[r * (int64_t)m + t];
+}
+
+// Row r starts a prefix group: r == 0, or its first m-1 items differ from row r-1's.
+struct IsGroupStart {
+    const int32_t* rows;
+    int32_t m;
+    __device__ int64_t operator()(int64_t r) const {
+        if (r == 0) return 1;
+        const int32_t* a = rows + r * (int64_t)m;
+        const int32_t* b = a - m;
+        for (int t = 0; t < m - 1; ++t)
+            if (a[t] != b[t]) return 1;
+        return 0;
+    }
+};
+
+struct WriteGroupStart {
+    int64_t* group_start;
+    __device__ void operator()(int64_t r, int64_t v, int64_t prefix) const {
+        if (v) group_start[prefix] = r;
+    }
+};
+
+struct SetGroupEnd {
+    int64_t* group_start;
+    int64_t n;
+    int64_t* totals;
+    __device__ void operator()(int64_t n_groups) const {
+        group_start[n_groups] = n;
+        totals[0] = n_groups;
+    }
+};
+
+// Per group g (g < totals[0]; 0 beyond): its candidate pairs, or its tile-pairs for the tiled kernel.
+struct GroupPairs {
+    const int64_t* group_start;
+    const int64_t* totals;
+    __device__ int64_t operator()(int64_t g) const {
+        if (g >= totals[0]) return 0;
+        const int64_t s = group_start[g + 1] - group_start[g];
+        return s * (s - 1) / 2;
+    }
+};
+
+struct GroupTiles {
+    const int64_t* group_start;
+    const int64_t* totals;
+    int32_t small_max;
+    __device__ int64_t operator()(int64_t g) const {
+        if (g >= totals[0]) return 0;
+        const int64_t s = group_start[g + 1] - group_start[g];
+        if (s <= small_max) return 0;
+        const int64_t nt = (s + kTileT - 1) / kTileT;
+        return nt * (nt + 1) / 2;
+    }
+};
+
+struct WritePrefix {
+    int64_t* out;
+    __device__ void operator()(int64_t i, int64_t, int64_t prefix) const { out[i] = prefix; }
+};
+
+struct StoreTotal {
+    int64_t* slot;
+    __device__ void operator()(int64_t t) const { *slot = t; }
+};
+
+__global__ void k_zero_i64(int64_t* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_group_classes(const int64_t* __restrict__ group_start, int64_t n,
+                                                        int32_t small_max, int32_t* __restrict__ small_list,
+                                                        int64_t* __restrict__ totals) {
+    const int64_t n_groups = totals[0];
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n_groups && g < n;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = group_start[g + 1] - group_start[g];
+        if (s >= 2 && s <= small_max) {
+            const unsigned long long at = atomicAdd((unsigned long long*)&totals[3], 1ull);
+            small_list[at] = (int32_t)g;
+        }
+        atomicMax((unsigned long long*)&totals[4], (unsigned long long)s);
+    }
+}
+
+// ── hash of a level ─────────────────────────────────────────────────────
+
+__global__ void k_fill_table(int32_t* t, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        t[i] = -1;
+}
+
+__global__ __launch_bounds__(256) void k_hash_insert(const int32_t* __restrict__ rows, int64_t n, int32_t m,
+                                                      int32_t* __restrict__ table, int64_t mask) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t h = 0;
+        for (int t = 0; t < m; ++t) h += item_hash(row_item(rows, r, m, t));
+        int64_t slot = (int64_t)(h & (uint64_t)mask);
+        while (atomicCAS(&table[slot], -1, (int32_t)r) != -1) slot = (slot + 1) & mask;
+    }
+}
+
+// The itemset made of `pre` (np items, dropping position `skip`; skip >= np keeps all), then a, then b
+// (use_b false: only a). Item t of it:
+struct Query {
+    const int32_t* pre;
+    int np, skip;
+    int32_t a, b;
+    bool use_b;
+    __device__ int size() const { return np - (skip < np ? 1 : 0) + 1 + (use_b ? 1 : 0); }
+    __device__ int32_t item(int t) const {
+        const int kept = np - (skip < np ? 1 : 0);
+        if (t < kept) return pre[(skip < np && t >= skip) ? t + 1 : t];
+        return t == kept ? a : b;
+    }
+};
+
+// Row id of the level row equal to q (q.size() == m), or -1.
+__device__ int64_t hash_find(const int32_t* __restrict__ rows, int32_t m, const int32_t* __restrict__ table,
+                             int64_t mask, uint64_t h, const Query& q) {
+    int64_t slot = (int64_t)(h & (uint64_t)mask);
+    for (;;) {
+        const int32_t r = table[slot];
+        if (r < 0) return -1;
+        const int32_t* row = rows + (int64_t)r * m;
+        bool eq = true;
+        for (int t = 0; t < m && eq; ++t) eq = row[t] == q.item(t);
+        if (eq) return r;
+        slot = (slot + 1) & mask;
+    }
+}
+
+// ── classification ──────────────────────────────────────────────────────
+
+__global__ __launch_bounds__(256) void k_classify(const int32_t* __restrict__ rows, int32_t m,
+                                                   const int32_t* __restrict__ prev_counts,
+                                                   const uint8_t* __restrict__ prev_free,
+                                                   const int32_t* __restrict__ table, int64_t mask,
+                                                   const int64_t* __restrict__ group_start,
+                                                   const int64_t* __restrict__ cum_pairs, int64_t n_groups,
+                                                   int64_t n_cands, int32_t mode, uint8_t* __restrict__ status,
+                                                   int32_t* __restrict__ counts) {
+    const bool infer = (mode & ETM_SUBSET_INFER) != 0;
+    const bool prune = (mode & ETM_SUBSET_PRUNE) != 0;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n_cands;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t g = etm::upper_index(cum_pairs, n_groups, c);
+        const int64_t gs = group_start[g];
+        const int64_t s = group_start[g + 1] - gs;
+        int64_t i, j;
+        etm::lex_pair_decode(c - cum_pairs[g], s, &i, &j);
+        const int32_t* pre = rows + gs * m;  // the group's prefix: the first m-1 items of its first row
+        const int np = m - 1;
+        const int64_t ri = gs + i, rj = gs + j;  // the two prefix-parents (rows of the level)
+        const int32_t a = rows[ri * m + np], b = rows[rj * m + np];
+        uint64_t h = item_hash(a) + item_hash(b);
+        for (int t = 0; t < np; ++t) h += item_hash(pre[t]);
+        int32_t lowest = prev_counts ? min(prev_counts[ri], prev_counts[rj]) : 0;
+        bool not_free = infer && (prev_free[ri] == 0 || prev_free[rj] == 0);
+        uint8_t st = ETM_CAND_COUNT;
+        if (prune || infer) {
+            for (int d = 0; d < np; ++d) {
+                Query q{pre, np, d, a, b, true};
+                const int64_t r = hash_find(rows, m, table, mask, h - item_hash(pre[d]), q);
+                if (r < 0) {
+                    if (prune) {
+                        st = ETM_CAND_SKIP;
+                        break;
+                    }
+                    not_free = false;  // without the subset Pascal cannot be applied
+                    lowest = -1;
+                    break;
+                }
+                if (infer) {
+                    lowest = min(lowest, prev_counts[r]);
+                    not_free = not_free || prev_free[r] == 0;
+                }
+            }
+        }
+        if (st == ETM_CAND_COUNT && infer && not_free && lowest >= 0) st = ETM_CAND_INFER;
+        status[c] = st;
+        counts[c] = st == ETM_CAND_COUNT ? 0 : (st == ETM_CAND_INFER ? lowest : -1);
+    }
+}
+
+// ── compaction ──────────────────────────────────────────────────────────
+
+struct Survives {
+    const uint8_t* status;
+    const int32_t* counts;
+    int64_t min_count;
+    __device__ int64_t operator()(int64_t c) const {
+        if (status && status[c] == ETM_CAND_SKIP) return 0;
+        return (int64_t)counts[c] >= min_count ? 1 : 0;
+    }
+};
+
+struct WriteSurvivor {
+    const int32_t* rows;
+    int32_t m;
+    const int64_t* group_start;
+    const int64_t* cum_pairs;
+    int64_t n_groups;
+    const int32_t* counts;
+    int32_t* out_rows;
+    int32_t* out_counts;
+    int64_t* out_cand;
+    __device__ void operator()(int64_t c, int64_t v, int64_t p) const {
+        if (!v) return;
+        const int64_t g = etm::upper_index(cum_pairs, n_groups, c);
+        const int64_t gs = group_start[g];
+        const int64_t s = group_start[g + 1] - gs;
+        int64_t i, j;
+        etm::lex_pair_decode(c - cum_pairs[g], s, &i, &j);
+        const int k = m + 1;
+        int32_t* o = out_rows + p * k;
+        const int32_t* first = rows + gs * m;
+        for (int t = 0; t < m - 1; ++t) o[t] = first[t];
+        o[m - 1] = rows[(gs + i) * m + m - 1];
+        o[m] = rows[(gs + j) * m + m - 1];
+        out_counts[p] = counts[c];
+        if (out_cand) out_cand[p] = c;
+    }
+};
+
+struct Flagged {
+    const uint8_t* flag;
+    __device__ int64_t operator()(int64_t i) const { return flag[i] ? 1 : 0; }
+};
+
+struct WriteSelected {
+    const int32_t* rows;
+    int32_t m;
+    const int32_t* counts;
+    int32_t* out_rows;
+    int32_t* out_counts;
+    __device__ void operator()(int64_t i, int64_t v, int64_t p) const {
+        if (!v) return;
+        for (int t = 0; t < m; ++t) out_rows[p * m + t] = rows[i * (int64_t)m + t];
+        out_counts[p] = counts[i];
+    }
+};
+
+// ── free flags ──────────────────────────────────────────────────────────
+
+__global__ __launch_bounds__(256) void k_free(const int32_t* __restrict__ rows, int64_t n, int32_t k,
+                                               const int32_t* __restrict__ counts,
+                                               const int32_t* __restrict__ prev_rows,
+                                               const int32_t* __restrict__ prev_counts,
+                                               const int32_t* __restrict__ table, int64_t mask,
+                                               uint8_t* __restrict__ free_out) {
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t* row = rows + x * k;
+        uint64_t h = 0;
+        for (int t = 0; t < k; ++t) h += item_hash(row[t]);
+        uint8_t is_free = 1;
+        for (int d = 0; d < k && is_free; ++d) {
+            // The subset dropping row[d]: row[0..k) without position d (k-1 items).
+            int64_t slot = (int64_t)((h - item_hash(row[d])) & (uint64_t)mask);
+            for (;;) {
+                const int32_t r = table[slot];
+                if (r < 0) break;
+                const int32_t* pr = prev_rows + (int64_t)r * (k - 1);
+                bool eq = true;
+                for (int t = 0; t < k - 1 && eq; ++t) eq = pr[t] == row[t < d ? t : t + 1];
+                if (eq) {
+                    if (prev_counts[r] == counts[x]) is_free = 0;
+                    break;
+                }
+                slot = (slot + 1) & mask;
+            }
+        }
+        free_out[x] = is_free;
+    }
+}
+
+int grid_for(int64_t n, int cap = 16384) {
+    int64_t g = (n + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+bool is_pow2(int64_t x) { return x > 0 && (x & (x - 1)) == 0; }
+
+}  // namespace
+
+extern "C" size_t etm_level_groups_ws(int64_t n) { return etm::scan_ws_bytes(n + 1); }
+
+extern "C" int etm_level_groups(const int32_t* d_rows, int64_t n, int32_t m, int32_t small_max,
+                                int64_t* d_group_start, int64_t* d_cum_pairs, int64_t* d_cum_tiles,
+                                int32_t* d_small_list, int64_t* d_totals, void* d_ws, size_t ws_bytes,
+                                void* stream) {
+    ETM_ARG(d_rows && d_group_start && d_cum_pairs && d_cum_tiles && d_small_list && d_totals && d_ws,
+            "etm_level_groups: null pointer");
+    ETM_ARG(n >= 1 && m >= 1 && m <= 62, "etm_level_groups: need n >= 1 and 1 <= m <= 62 (n=%lld, m=%d)",
+            (long long)n, m);
+    ETM_ARG(n < 0x7FFFFFFF, "etm_level_groups: %lld rows exceed int32 row ids", (long long)n);
+    ETM_ARG(small_max >= 2, "etm_level_groups: small_max must be >= 2");
+    ETM_ARG(ws_bytes >= etm_level_groups_ws(n), "etm_level_groups: workspace %zu < %zu bytes", ws_bytes,
+            etm_level_groups_ws(n));
+    hipStream_t s = ETM_STREAM(stream);
+    int64_t* ws = (int64_t*)d_ws;
+    k_zero_i64<<<1, 64, 0, s>>>(d_totals, 8);
+    ETM_LAUNCHED("k_zero_i64");
+    IsGroupStart is_start{d_rows, m};
+    ETM_HIP(etm::scan_prepare(is_start, n, ws, nullptr, s, SetGroupEnd{d_group_start, n, d_totals}));
+    ETM_HIP(etm::scan_finish(is_start, WriteGroupStart{d_group_start}, n, ws, s));
+    GroupPairs pairs{d_group_start, d_totals};
+    ETM_HIP(etm::scan_prepare(pairs, n + 1, ws, &d_totals[1], s));
+    ETM_HIP(etm::scan_finish(pairs, WritePrefix{d_cum_pairs}, n + 1, ws, s));
+    GroupTiles tiles{d_group_start, d_totals, small_max};
+    ETM_HIP(etm::scan_prepare(tiles, n + 1, ws, &d_totals[2], s));
+    ETM_HIP(etm::scan_finish(tiles, WritePrefix{d_cum_tiles}, n + 1, ws, s));
+    k_group_classes<<<grid_for(n), 256, 0, s>>>(d_group_start, n, small_max, d_small_list, d_totals);
+    ETM_LAUNCHED("k_group_classes");
+    return ETM_OK;
+}
+
+extern "C" int etm_level_hash(const int32_t* d_rows, int64_t n, int32_t m, int32_t* d_table, int64_t table_size,
+                              void* stream) {
+    ETM_ARG(d_rows && d_table, "etm_level_hash: null pointer");
+    ETM_ARG(m >= 1 && m <= 62, "etm_level_hash: 1 <= m <= 62 required, got %d", m);
+    ETM_ARG(is_pow2(table_size) && table_size >= 2 * n, "etm_level_hash: table_size %lld must be a power of two >= 2n",
+            (long long)table_size);
+    ETM_ARG(n < 0x7FFFFFFF, "etm_level_hash: %lld rows exceed int32 row ids", (long long)n);
+    hipStream_t s = ETM_STREAM(stream);
+    k_fill_table<<<grid_for(table_size), 256, 0, s>>>(d_table, table_size);
+    ETM_LAUNCHED("k_fill_table");
+    if (n > 0) {
+        k_hash_insert<<<grid_for(n), 256, 0, s>>>(d_rows, n, m, d_table, table_size - 1);
+        ETM_LAUNCHED("k_hash_insert");
+    }
+    return ETM_OK;
+}
+
+extern "C" int etm_level_classify(const int32_t* d_rows, int64_t n, int32_t m, const int32_t* d_prev_counts,
+                                  const uint8_t* d_prev_free, const int32_t* d_table, int64_t table_size,
+                                  const int64_t* d_group_start, const int64_t* d_cum_pairs, int64_t n_groups,
+                                  int64_t n_cands, int32_t mode, uint8_t* d_status, int32_t* d_counts,
+                                  void* stream) {
+    ETM_ARG(d_rows && d_table && d_group_start && d_cum_pairs && d_status && d_counts,
+            "etm_level_classify: null pointer");
+    ETM_ARG(m >= 1 && m <= 62, "etm_level_classify: 1 <= m <= 62 required, got %d", m);
+    ETM_ARG(is_pow2(table_size), "etm_level_classify: table_size must be a power of two");
+    ETM_ARG(!(mode & ETM_SUBSET_INFER) || (d_prev_free && d_prev_counts),
+            "etm_level_classify: inference needs the previous level's counts and free flags");
+    ETM_ARG(n_groups >= 1 || n_cands == 0, "etm_level_classify: no groups");
+    (void)n;
+    if (n_cands == 0) return ETM_OK;
+    k_classify<<<grid_for(n_cands, 65536), 256, 0, ETM_STREAM(stream)>>>(
+        d_rows, m, d_prev_counts, d_prev_free, d_table, table_size - 1, d_group_start, d_cum_pairs, n_groups,
+        n_cands, mode, d_status, d_counts);
+    ETM_LAUNCHED("k_classify");
+    return ETM_OK;
+}
+
+extern "C" size_t etm_level_compact_ws(int64_t n_cands) { return etm::scan_ws_bytes(n_cands); }
+
+extern "C" int etm_level_compact(const int32_t* d_rows, int32_t m, const int64_t* d_group_start,
+                                 const int64_t* d_cum_pairs, int64_t n_groups, int64_t n_cands,
+                                 const uint8_t* d_status, const int32_t* d_counts, int64_t min_count,
+                                 int32_t* d_out_rows, int32_t* d_out_counts, int64_t* d_out_cand, int64_t* d_out_n,
+                                 void* d_ws, size_t ws_bytes, void* stream) {
+    ETM_ARG(d_rows && d_group_start && d_cum_pairs && d_counts && d_out_n && d_ws,
+            "etm_level_compact: null pointer");
+    ETM_ARG(m >= 1 && m < 62, "etm_level_compact: 1 <= m < 62 required, got %d", m);
+    ETM_ARG(ws_bytes >= etm_level_compact_ws(n_cands), "etm_level_compact: workspace %zu < %zu bytes", ws_bytes,
+            etm_level_compact_ws(n_cands));
+    hipStream_t s = ETM_STREAM(stream);
+    Survives f{d_status, d_counts, min_count};
+    if (d_out_rows == nullptr) {
+        ETM_HIP(etm::scan_prepare(f, n_cands, (int64_t*)d_ws, d_out_n, s));
+        return ETM_OK;
+    }
+    ETM_ARG(d_out_counts != nullptr, "etm_level_compact: null output counts");
+    WriteSurvivor w{d_rows, m, d_group_start, d_cum_pairs, n_groups, d_counts, d_out_rows, d_out_counts, d_out_cand};
+    ETM_HIP(etm::scan_finish(f, w, n_cands, (const int64_t*)d_ws, s));
+    return ETM_OK;
+}
+
+extern "C" int etm_level_free(const int32_t* d_rows, int64_t n, int32_t k, const int32_t* d_counts,
+                              const int32_t* d_prev_rows, const int32_t* d_prev_counts, const int32_t* d_prev_table,
+                              int64_t table_size, uint8_t* d_free, void* stream) {
+    ETM_ARG(d_rows && d_counts && d_prev_rows && d_prev_counts && d_prev_table && d_free,
+            "etm_level_free: null pointer");
+    ETM_ARG(k >= 2 && k <= 63, "etm_level_free: 2 <= k <= 63 required, got %d", k);
+    ETM_ARG(is_pow2(table_size), "etm_level_free: table_size must be a power of two");
+    if (n == 0) return ETM_OK;
+    k_free<<<grid_for(n), 256, 0, ETM_STREAM(stream)>>>(d_rows, n, k, d_counts, d_prev_rows, d_prev_counts,
+                                                        d_prev_table, table_size - 1, d_free);
+    ETM_LAUNCHED("k_free");
+    return ETM_OK;
+}
+
+extern "C" int etm_select_rows(const int32_t* d_rows, int64_t n, int32_t m, const int32_t* d_counts,
+                               const uint8_t* d_flag, int32_t* d_out_rows, int32_t* d_out_counts, int64_t* d_out_n,
+                               void* d_ws, size_t ws_bytes, void* stream) {
+    ETM_ARG(d_rows && d_counts && d_flag && d_out_n && d_ws, "etm_select_rows: null pointer");
+    ETM_ARG(m >= 1 && m <= 63, "etm_select_rows: 1 <= m <= 63 required");
+    ETM_ARG(ws_bytes >= etm_level_compact_ws(n), "etm_select_rows: workspace too small");
+    hipStream_t s = ETM_STREAM(stream);
+    Flagged f{d_flag};
+    if (d_out_rows == nullptr) {
+        ETM_HIP(etm::scan_prepare(f, n, (int64_t*)d_ws, d_out_n, s));
+        return ETM_OK;
+    }
+    ETM_ARG(d_out_counts != nullptr, "etm_select_rows: null output counts");
+    ETM_HIP(etm::scan_finish(f, WriteSelected{d_rows, m, d_counts, d_out_rows, d_out_counts}, n,
+                             (const int64_t*)d_ws, s));
+    return ETM_OK;
+}

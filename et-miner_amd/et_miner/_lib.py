@@ -1,0 +1,107 @@
+"""ctypes binding of libetm.so (the C ABI declared in include/etm.h).
+
+``lib()`` loads the in-tree library once and declares every entry point's
+signature; ``call(name, *args)`` invokes one and maps a nonzero status to the
+exception the reference raises for the same condition (ValueError for bad
+arguments and sizes, MemoryError for allocation failures, RuntimeError for
+HIP errors). There is no fallback: without the library every GPU entry point
+raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from ctypes import POINTER, c_char_p, c_int, c_int32, c_int64, c_size_t, c_void_p
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libetm.so"
+ABI_VERSION = 1
+
+ETM_OK, ETM_EINVAL, ETM_ENOMEM, ETM_EHIP, ETM_EOVERFLOW, ETM_ENODEV = 0, 1, 2, 3, 4, 5
+CAND_COUNT, CAND_SKIP, CAND_INFER = 0, 1, 2
+SUBSET_PRUNE, SUBSET_INFER = 1, 2
+LAYOUT_LEX, LAYOUT_TRI = 0, 1
+
+P = c_void_p  # every device pointer crosses the ABI as void*
+
+#: name -> (restype, argtypes), mirroring include/etm.h.
+SIGNATURES: dict = {
+    "etm_abi_version": (c_int, []),
+    "etm_last_error": (c_int, [c_char_p, c_size_t]),
+    "etm_device_count": (c_int, [POINTER(c_int)]),
+    "etm_ids_minmax": (c_int, [P, c_int, c_int64, P, P]),
+    "etm_ids_hist": (c_int, [P, c_int, c_int64, c_int64, c_int64, P, P]),
+    "etm_bitvec_build": (c_int, [P, P, c_int, c_int64, c_int64, c_int64, P, c_int32, c_int64, P, P]),
+    "etm_column_popcounts": (c_int, [P, c_int32, c_int64, c_int64, P, P]),
+    "etm_level_groups_ws": (c_size_t, [c_int64]),
+    "etm_level_groups": (c_int, [P, c_int64, c_int32, c_int32, P, P, P, P, P, P, c_size_t, P]),
+    "etm_level_hash": (c_int, [P, c_int64, c_int32, P, c_int64, P]),
+    "etm_level_classify": (c_int, [P, c_int64, c_int32, P, P, P, c_int64, P, P, c_int64, c_int64, c_int32, P, P, P]),
+    "etm_count_level": (c_int, [P, c_int64, c_int64, P, c_int32, P, P, P, c_int64, c_int64, P, c_int64, c_int32,
+                                P, P, c_int32, c_int32, P]),
+    "etm_level_compact_ws": (c_size_t, [c_int64]),
+    "etm_level_compact": (c_int, [P, c_int32, P, P, c_int64, c_int64, P, P, c_int64, P, P, P, P, P, c_size_t, P]),
+    "etm_level_free": (c_int, [P, c_int64, c_int32, P, P, P, P, c_int64, P, P]),
+    "etm_select_rows": (c_int, [P, c_int64, c_int32, P, P, P, P, P, P, c_size_t, P]),
+    "etm_count_itemsets": (c_int, [P, c_int64, c_int64, P, P, c_int64, P, c_int32, P]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    """libetm.so is not built: the GPU path cannot run (there is no fallback)."""
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library with every signature declared (loaded once per process)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise LibraryMissing(
+                    f"{LIB_PATH} is not built: run `python __graft_entry__.py build` (or et_miner._build.build()) "
+                    "first; the GPU counting path has no CPU fallback"
+                )
+            handle = ctypes.CDLL(str(LIB_PATH))
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(handle, name)
+                fn.restype = res
+                fn.argtypes = args
+            if handle.etm_abi_version() != ABI_VERSION:
+                raise LibraryMissing(f"{LIB_PATH} has ABI {handle.etm_abi_version()}, expected {ABI_VERSION}")
+            _lib = handle
+    return _lib
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(1024)
+    lib().etm_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def call(name: str, *args) -> int:
+    """Calls an entry point; raises on a nonzero status."""
+    rc = getattr(lib(), name)(*args)
+    if rc == ETM_OK:
+        return rc
+    msg = f"{name}: {last_error()}"
+    if rc in (ETM_EINVAL, ETM_EOVERFLOW):
+        raise ValueError(msg)
+    if rc == ETM_ENOMEM:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)
+
+
+def ws_bytes(name: str, n: int) -> int:
+    return int(getattr(lib(), name)(int(n)))
+
+
+def device_count() -> int:
+    n = c_int(0)
+    call("etm_device_count", ctypes.byref(n))
+    return n.value

@@ -1,0 +1,1 @@
+"""Subpackage of et_miner."""

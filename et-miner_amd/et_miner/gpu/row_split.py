@@ -1,0 +1,152 @@
+"""The GPU level loop on one device: K=1 through exhaustion, every level resident in HBM.
+
+Counterpart of the reference's ``_apriori_row_split_multi_gpu`` (``gpu/row_split.py:229-1185``) for a single
+GPU. Per level k >= 2 the loop runs, all on the device:
+
+1. the prefix groups of level k-1 (``etm_level_groups``; for k = 2 the K=1 level is one group);
+2. for k >= 3, the (k-1)-subset test of every candidate against the hash of level k-1, with Pascal inference
+   when ``infer_counts`` (``etm_level_classify``);
+3. AND + popcount of the counted candidates (``etm_count_level``);
+4. compaction of the survivors (count >= min_count) in lexicographic order (``etm_level_compact``) -- the
+   candidate numbering is lexicographic, so no level-end sort is needed;
+5. free flags when tracked (``etm_level_free``), and the free-set prune (``etm_select_rows``).
+
+The host reads two scalars per level (the candidate total and the survivor total) to size the next buffers;
+the itemsets and counts of every level come back once, at the end.
+
+Free-set mode (``prune_non_free``) keeps one population per level, as the reference does
+(``gpu/row_split.py:292-309``): the free-sets, which are emitted, generated from, and tested against.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import device as dev
+from . import kernels as K
+
+
+@dataclass
+class LevelStats:
+    k: int
+    n_candidates: int
+    n_frequent: int
+    n_counted: int = 0
+    ms: float = 0.0
+
+
+@dataclass
+class MineResult:
+    """Levels as (rows of bitvector column ids int32 (n, k), counts int64), plus per-level statistics."""
+
+    levels: list = field(default_factory=list)
+    stats: list = field(default_factory=list)
+
+
+@dataclass
+class _Level:
+    rows: object  # device int32 (n, m)
+    counts: object  # device int32 (n,)
+    free: object = None  # device uint8 (n,) or None
+    table: object = None
+    table_size: int = 0
+
+    @property
+    def n(self) -> int:
+        return int(self.rows.shape[0])
+
+    @property
+    def m(self) -> int:
+        return int(self.rows.shape[1])
+
+    def hashed(self):
+        if self.table is None:
+            self.table, self.table_size = K.level_hash(self.rows, self.m)
+        return self.table, self.table_size
+
+
+def mine_levels(
+    bv,
+    n_words: int,
+    n_rows: int,
+    col_counts: np.ndarray,
+    min_count: int,
+    max_length: int | None = None,
+    prune_non_free: bool = False,
+    infer_counts: bool = False,
+    level_callback=None,
+    pitch: int | None = None,
+    keep_device: bool = False,
+) -> MineResult:
+    """Mines every frequent itemset (or free-set) of device bitvectors ``bv`` (n_cols, pitch).
+
+    ``col_counts`` are the K=1 counts of the columns (host int64). Counts are exact integers throughout.
+    With ``keep_device`` the levels stay device tensors (rows int32, counts int32) instead of host arrays.
+    """
+    if n_rows > np.iinfo(np.int32).max:
+        raise ValueError(f"n_transactions={n_rows:,} exceeds int32 max: the count arrays are int32")
+    out = MineResult()
+    n_cols = len(col_counts)
+    t0 = time.perf_counter()
+    freq = np.flatnonzero(col_counts >= min_count).astype(np.int32)
+    free1 = col_counts[freq] < n_rows
+    gen_cols = freq[free1] if prune_non_free else freq
+    k1_rows = gen_cols.reshape(-1, 1)
+    k1_counts = col_counts[gen_cols].astype(np.int64)
+    out.levels.append((k1_rows, k1_counts))
+    out.stats.append(LevelStats(1, n_cols, len(gen_cols), n_cols, (time.perf_counter() - t0) * 1000))
+    if level_callback:
+        level_callback(1, n_cols, len(gen_cols), out.stats[-1].ms)
+    if len(gen_cols) == 0:
+        return out
+    eml = min(max_length if max_length else math.inf, n_cols)
+    track_free = prune_non_free or infer_counts
+    prev = _Level(dev.to_device(k1_rows), dev.to_device(k1_counts.astype(np.int32)))
+    if track_free:
+        prev.free = dev.to_device((k1_counts < n_rows).astype(np.uint8))
+    dev_levels = []
+    k = 2
+    while k <= eml and prev.n >= k:
+        t_k = time.perf_counter()
+        m = k - 1
+        groups = K.level_groups(prev.rows, m)
+        if groups.n_cands == 0:
+            break
+        if k >= 3:
+            table, tsize = prev.hashed()
+            infer = infer_counts and not prune_non_free and prev.free is not None
+            mode = K.SUBSET_PRUNE | (K.SUBSET_INFER if infer else 0)
+            status, counts = K.level_classify(prev.rows, m, prev.counts, prev.free if infer else None, table, tsize,
+                                              groups, mode)
+        else:
+            status, counts = None, dev.zeros(groups.n_cands, "int32")
+        K.count_level(bv, n_words, prev.rows, m, groups, status, counts, pitch=pitch)
+        rows_k, counts_k = K.level_compact(prev.rows, m, groups, status, counts, min_count)
+        cur = _Level(rows_k, counts_k)
+        if track_free and cur.n:
+            table, tsize = prev.hashed()
+            cur.free = K.level_free(cur.rows, k, cur.counts, prev.rows, prev.counts, table, tsize)
+            if prune_non_free:
+                cur.rows, cur.counts = K.select_rows(cur.rows, k, cur.counts, cur.free)
+                cur.free = dev.empty(cur.n, "uint8").fill_(1) if cur.n else None
+        n_freq = cur.n
+        dev_levels.append((cur.rows, cur.counts))
+        dt = (time.perf_counter() - t_k) * 1000
+        out.stats.append(LevelStats(k, groups.n_cands, n_freq, 0, dt))
+        if level_callback:
+            level_callback(k, groups.n_cands, n_freq, dt)
+        if n_freq == 0:
+            break
+        prev = cur
+        k += 1
+    if keep_device:
+        out.levels = [(dev.to_device(out.levels[0][0]), dev.to_device(out.levels[0][1]))] + dev_levels
+        return out
+    for rows_d, counts_d in dev_levels:
+        if rows_d.shape[0]:
+            out.levels.append((dev.to_host(rows_d), dev.to_host(counts_d).astype(np.int64)))
+    return out

@@ -1,0 +1,274 @@
+"""SON two-pass streaming on one GPU (the reference's ``streaming/son.py:131-693``).
+
+Pass 1 mines every row chunk at ``local_support_factor * min_support`` on the GPU level loop; the union of
+the chunks' local levels sums, per itemset, the local counts (``known``) and the slack (local min_count - 1)
+of the chunks that emitted it. The bound ``known + total_slack - slack`` drops candidates that cannot reach
+the global min_count and is the exact count where no slack is left (``son.py:402-421``). Pass 2 counts the
+remaining candidates over every chunk with the batched itemset kernel, summing the chunks on the device.
+
+Transfers: each chunk's CSR is staged in pinned host buffers and copied with ``hipMemcpyAsync`` on a side
+stream; the copy of chunk i+1 is issued before chunk i is counted, so transfer overlaps counting. Only the
+CSR travels; bitvectors are built on the device.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from ..core.apriori import transactions_to_csr
+from ..core.profiling import ProfilingSession
+from ..core.result import _min_count, _mining_min_count, empty_result, itemset_table, to_frame
+
+#: The default local support factor (reference ``streaming/son.py:82``).
+LOCAL_SUPPORT_FACTOR = 0.95
+
+
+def _estimate_chunk_size_from_memory(memory_budget_gb: float, n_items_estimate: int = 1000) -> int:
+    """Rows per chunk for a memory budget (reference ``son.py:85-110``)."""
+    bytes_per_transaction = (n_items_estimate / 8) * 2
+    chunk_size = int(memory_budget_gb * 1024**3 / max(bytes_per_transaction, 1))
+    return max(100_000, min(chunk_size, 100_000_000))
+
+
+class _Uploader:
+    """Double-buffered pinned staging of CSR chunks, copied on a side stream by a helper thread.
+
+    ``issue(i)`` hands chunk i to the helper, which waits until the pinned buffer i % 2 is free (the copy of
+    chunk i-2 out of it has completed), stages the chunk's rows into it and enqueues the host-to-device copy
+    on the side stream. ``take(i)`` makes the current stream wait for that copy and issues chunk i+1, so the
+    staging and the copy of the next chunk run while the current one is counted.
+    """
+
+    def __init__(self, indptr: np.ndarray, ids: np.ndarray, bounds: list):
+        from concurrent.futures import ThreadPoolExecutor
+
+        from ..gpu import device as dev
+
+        self.torch = dev.torch()
+        self.dev = dev
+        self.device = dev.device()
+        self.indptr, self.ids, self.bounds = indptr, ids, bounds
+        t = self.torch
+        max_rows = max(b - a for a, b in bounds)
+        max_nnz = max(int(indptr[b] - indptr[a]) for a, b in bounds)
+        self.stream = t.cuda.Stream(device=self.device)
+        self.host = [(t.empty(max_rows + 1, dtype=t.int64).pin_memory(),
+                      t.empty(max(1, max_nnz), dtype=t.int64).pin_memory()) for _ in range(2)]
+        self.buffer_done = [None, None]
+        self.pool = ThreadPoolExecutor(1)
+        self.pending: dict = {}
+
+    def _stage(self, i: int):
+        t = self.torch
+        a, b = self.bounds[i]
+        slot = i % 2
+        if self.buffer_done[slot] is not None:
+            self.buffer_done[slot].synchronize()
+        hp, hi = self.host[slot]
+        lo, hi_e = int(self.indptr[a]), int(self.indptr[b])
+        n, nnz = b - a, hi_e - lo
+        hp[: n + 1].numpy()[:] = self.indptr[a : b + 1] - lo
+        hi[:nnz].numpy()[:] = self.ids[lo:hi_e]
+        with t.cuda.device(self.device), t.cuda.stream(self.stream):
+            d_ptr = hp[: n + 1].to(self.device, non_blocking=True)
+            d_ids = hi[:nnz].to(self.device, non_blocking=True)
+            ev = t.cuda.Event()
+            ev.record(self.stream)
+        self.buffer_done[slot] = ev
+        return d_ptr, d_ids, ev, n
+
+    def issue(self, i: int) -> None:
+        if 0 <= i < len(self.bounds) and i not in self.pending:
+            self.pending[i] = self.pool.submit(self._stage, i)
+
+    def take(self, i: int):
+        """Chunk i's device CSR (indptr, ids, rows), ordered after its copy on the current stream."""
+        self.issue(i)
+        d_ptr, d_ids, ev, n = self.pending.pop(i).result()
+        cur = self.torch.cuda.current_stream()
+        cur.wait_event(ev)
+        d_ptr.record_stream(cur)
+        d_ids.record_stream(cur)
+        self.issue(i + 1)
+        return d_ptr, d_ids, n
+
+    def close(self) -> None:
+        self.pool.shutdown(wait=True)
+
+
+def _local_levels(d_ptr, d_ids, n_rows: int, local_support: float, max_length, infer: bool):
+    """One chunk mined at the local threshold: (its column items, levels of (column rows, counts))."""
+    from ..gpu.input_layer import build_device_input
+    from ..gpu.row_split import mine_levels
+
+    mc = _mining_min_count(local_support, n_rows)
+    built = build_device_input(d_ptr, d_ids, n_rows, mc)
+    if built is None:
+        return None
+    res = mine_levels(built.bitvecs, built.n_words, n_rows, built.col_counts, mc, max_length,
+                      infer_counts=infer)
+    return built.col_to_item, res.levels
+
+
+def _sum_rows(rows: np.ndarray, weights: list) -> tuple:
+    """Lexsorted distinct rows with every weight summed per row (reference ``cpu_miner.py:381-414``)."""
+    if len(rows) == 0:
+        return rows, [w[:0] for w in weights]
+    order = np.lexsort(rows.T[::-1])
+    r = rows[order]
+    starts = np.flatnonzero(np.r_[True, (r[1:] != r[:-1]).any(axis=1)])
+    return r[starts], [np.add.reduceat(w[order], starts) for w in weights]
+
+
+def apriori_streaming(
+    transactions,
+    min_support: float = 0.5,
+    max_length: int | None = None,
+    item_col: str = "items",
+    chunk_size: int = 40_000_000,
+    memory_budget_gb: float | None = None,
+    local_support_factor: float = LOCAL_SUPPORT_FACTOR,
+    use_gpu: bool = True,
+    gpu_resident: bool = False,
+    batch_size: int | None = 10_000,
+    profile: bool = False,
+    show_progress: bool = False,
+    sparse=None,
+    n_jobs: int = 1,
+    progress_callback=None,
+    use_generator_pruning: bool = False,
+):
+    """Frequent itemsets by SON's two passes over row chunks, each chunk counted on the GPU.
+
+    Same result as the in-core ``apriori(use_gpu=True)``; see the module docstring.
+    """
+    if gpu_resident:
+        raise ValueError("gpu_resident was removed: use_gpu=True keeps the candidates on the GPU. Drop the argument.")
+    if not 0 < local_support_factor <= 1:
+        raise ValueError(f"local_support_factor must be in (0, 1], got {local_support_factor}: above 1 pass 1 mines "
+                         "each chunk above the global threshold and can miss globally frequent itemsets")
+    if memory_budget_gb is None and not chunk_size >= 1:
+        raise ValueError(f"chunk_size must be >= 1, got {chunk_size}")
+    if sparse is not None:
+        raise ValueError("sparse= was removed: no route reads it. Drop the argument.")
+    if not use_gpu:
+        raise ValueError("this build streams on the GPU only: pass use_gpu=True")
+    session = ProfilingSession() if profile else None
+    indptr, ids, items = transactions_to_csr(transactions, item_col)
+    res = son_csr(indptr, ids, min_support, max_length,
+                  _estimate_chunk_size_from_memory(memory_budget_gb) if memory_budget_gb is not None else chunk_size,
+                  local_support_factor, session, progress_callback, use_generator_pruning)
+    if items is not None:
+        import pyarrow as pa
+
+        flat = res.column("itemset").combine_chunks()
+        res = pa.table({"itemset": pa.LargeListArray.from_arrays(flat.offsets, items.take(flat.values)),
+                        "support": res.column("support")})
+    df = to_frame(res)
+    return (df, session) if profile else df
+
+
+def son_csr(indptr, ids, min_support, max_length, chunk_size, local_support_factor=LOCAL_SUPPORT_FACTOR,
+            session=None, progress_callback=None, use_generator_pruning=False, with_counts=False):
+    """SON over a host CSR of item ids; returns a pyarrow Table (``itemset``, ``support``[, ``count``])."""
+    from ..core.apriori import mine_csr
+    from ..gpu import device as dev
+    from ..gpu import kernels as K
+
+    dev.require_gpu()
+    indptr = np.asarray(indptr, dtype=np.int64)
+    ids = np.asarray(ids, dtype=np.int64)
+    n_total = len(indptr) - 1
+    if n_total == 0:
+        return itemset_table([], np.zeros(0, np.int64), 1, with_counts)
+    if n_total <= chunk_size:
+        table = mine_csr(indptr, ids, min_support, max_length, use_generator_pruning=use_generator_pruning,
+                         with_counts=True)
+        return table if with_counts else _drop_counts(table)
+    bounds = [(a, min(a + chunk_size, n_total)) for a in range(0, n_total, chunk_size)]
+    local_s = min_support * local_support_factor
+    slacks = [_mining_min_count(local_s, b - a) - 1 for a, b in bounds]
+    min_count = _min_count(min_support, n_total)
+    if sum(slacks) >= min_count:
+        raise RuntimeError(f"SON's slack {sum(slacks)} reaches min_count {min_count}: an itemset no chunk emits "
+                           "could be frequent")
+    up = _Uploader(indptr, ids, bounds)
+
+    # Pass 1: local levels per chunk, as item-id rows with their counts and the chunk's slack.
+    if session:
+        session.start_phase("pass1_local_mining")
+    pending: dict = {}
+    for ci in range(len(bounds)):
+        d_ptr, d_ids, n = up.take(ci)
+        local = _local_levels(d_ptr, d_ids, n, local_s, max_length, use_generator_pruning)
+        if local is not None:
+            col_items, levels = local
+            for rows, counts in levels:
+                if len(rows):
+                    pending.setdefault(rows.shape[1], []).append((col_items[rows], counts.astype(np.int64),
+                                                                  slacks[ci]))
+        if progress_callback:
+            progress_callback("pass1", ci, len(bounds), {"candidates": sum(len(p[0]) for v in pending.values()
+                                                                           for p in v)})
+    up.close()
+    if not pending:
+        return itemset_table([], np.zeros(0, np.int64), n_total, with_counts)
+    all_items = np.unique(np.concatenate([p[0].ravel() for v in pending.values() for p in v]))
+    total_slack = sum(slacks)
+    bounded = {}
+    for k, parts in sorted(pending.items()):
+        rows = np.searchsorted(all_items, np.concatenate([p[0] for p in parts])).astype(np.int32)
+        known = np.concatenate([p[1] for p in parts])
+        sl = np.concatenate([np.full(len(p[0]), p[2], dtype=np.int64) for p in parts])
+        rows, (known, sl) = _sum_rows(rows, [known, sl])
+        left = total_slack - sl
+        keep = known + left >= min_count
+        need = left[keep] > 0
+        bounded[k] = (rows[keep], np.where(need, 0, known[keep]).astype(np.int64), need)
+    if session:
+        session.end_phase(n_candidates=sum(len(v[0]) for v in bounded.values()))
+
+    # Pass 2: every candidate with slack left, counted over every chunk; totals summed on the device.
+    cands = {k: r[need] for k, (r, _, need) in bounded.items() if need.any()}
+    if session:
+        session.start_phase("pass2_global_counting")
+    if cands:
+        order = sorted(cands)
+        flat = np.concatenate([cands[k].ravel() for k in order]).astype(np.int32)
+        sizes = np.concatenate([np.full(len(cands[k]), k, dtype=np.int64) for k in order])
+        offsets = np.zeros(len(sizes) + 1, dtype=np.int64)
+        np.cumsum(sizes, out=offsets[1:])
+        d_flat, d_off = dev.to_device(flat), dev.to_device(offsets)
+        totals = dev.zeros(len(sizes), "int64")
+        lo, span = int(all_items[0]), int(all_items[-1] - all_items[0] + 1)
+        lut = np.full(span, -1, dtype=np.int32)
+        lut[all_items - lo] = np.arange(len(all_items), dtype=np.int32)
+        d_lut = dev.to_device(lut)
+        up2 = _Uploader(indptr, ids, bounds)
+        for ci in range(len(bounds)):
+            d_ptr, d_ids, n = up2.take(ci)
+            bv = K.build_bitvecs(d_ptr, d_ids, n, lo, span, d_lut, len(all_items))
+            K.count_itemsets_device(bv, K.words_for(n), d_flat, d_off, len(sizes), totals, accumulate=True)
+            if progress_callback:
+                progress_callback("pass2", ci, len(bounds), {"counted": ci + 1})
+        up2.close()
+        got = dev.to_host(totals)
+        at = 0
+        for k in order:
+            _, counts, need = bounded[k]
+            counts[need] = got[at : at + len(cands[k])]
+            at += len(cands[k])
+    if session:
+        session.end_phase(n_counted=sum(len(c) for c in cands.values()))
+    levels = []
+    for k in sorted(bounded):
+        rows, counts, _ = bounded[k]
+        keep = counts >= min_count
+        levels.append((rows[keep], counts[keep]))
+    return itemset_table(levels, all_items.astype(np.int64), n_total, with_counts)
+
+
+def _drop_counts(table):
+    return table.drop(["count"]) if "count" in table.column_names else table

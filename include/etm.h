@@ -1,0 +1,176 @@
+/*
+ * etm.h -- C ABI of libetm.so, the MI355X (gfx950) counting path of exact
+ * frequent-itemset mining.
+ *
+ * Conventions
+ * -----------
+ * - Every entry point returns an int status: ETM_OK (0) or an ETM_E* code.
+ *   etm_last_error() copies the calling thread's last message. No C++
+ *   exception crosses the ABI.
+ * - Pointers named d_* are device (HBM) pointers; h_* are host pointers.
+ *   Sizes are element counts unless the name ends in _bytes.
+ * - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *   Entry points only enqueue work on it: they return before the GPU is
+ *   done. The caller synchronizes before reading results on the host.
+ * - Bitvectors: column c of a (n_cols, pitch) uint64 array holds bit r%64 of
+ *   word r/64 set when transaction r holds c. pitch >= n_words =
+ *   ceil(n_rows/64); words in [n_words, pitch) are never read.
+ * - A "level" is a lexicographically sorted int32 array of itemsets of m
+ *   column ids each (row-major, ascending inside a row), with an int32 count
+ *   per row and optionally a uint8 free flag per row (1 = no (m-1)-subset
+ *   has the same count).
+ * - Prefix groups of a level: maximal runs of rows sharing their first m-1
+ *   items. A group of s rows yields s(s-1)/2 candidates of m+1 items
+ *   (prefix, suffix_i, suffix_j), i < j, numbered in lexicographic order:
+ *   candidate (i, j) of group g has index cum_pairs[g] + i*(2s-i-1)/2 + (j-i-1).
+ *   The candidates of a level are therefore lexsorted by index.
+ *
+ * Each declaration cites the reference interface it replaces (paths relative
+ * to the reference's src/et_miner).
+ */
+#ifndef ETM_H
+#define ETM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ETM_OK = 0,
+    ETM_EINVAL = 1,    /* bad argument: maps to ValueError */
+    ETM_ENOMEM = 2,    /* device allocation failed: MemoryError */
+    ETM_EHIP = 3,      /* HIP runtime or launch error: RuntimeError */
+    ETM_EOVERFLOW = 4, /* a size exceeds what the layout holds: ValueError */
+    ETM_ENODEV = 5     /* no usable gfx950 device: RuntimeError */
+};
+
+/* Candidate status values written by etm_level_classify. */
+enum { ETM_CAND_COUNT = 0, ETM_CAND_SKIP = 1, ETM_CAND_INFER = 2 };
+/* etm_level_classify modes (bit set), as SUBSET_PRUNE / SUBSET_INFER in gpu/kernels/_src/_subset_index.cu:29-31. */
+enum { ETM_SUBSET_PRUNE = 1, ETM_SUBSET_INFER = 2 };
+/* Pair layouts of the K=2 count array: lexicographic (i-major), or the reference's b(b-1)/2 + a. */
+enum { ETM_LAYOUT_LEX = 0, ETM_LAYOUT_TRI = 1 };
+
+/* ABI version of this header (bumped on any signature change). */
+#define ETM_ABI_VERSION 1
+int etm_abi_version(void);
+
+/* Copies the calling thread's last error message (NUL-terminated, truncated to n). */
+int etm_last_error(char* h_buf, size_t n);
+
+/* Number of visible HIP devices (0 when none). */
+int etm_device_count(int* h_out);
+
+/* Fills d_out[0] = min, d_out[1] = max of n item ids (int32 or int64 per id_bytes); n >= 1.
+ * Replaces the id range scan of gpu/input_layer.py:396-405 (the bincount's span). */
+int etm_ids_minmax(const void* d_ids, int id_bytes, int64_t n, int64_t* d_out, void* stream);
+
+/* Occurrence histogram: d_hist[v - lo] = number of entries equal to v, for lo <= v < lo + span.
+ * Zero-fills d_hist first. Replaces cp.bincount in gpu/input_layer.py:396-405 (_occurrences). */
+int etm_ids_hist(const void* d_ids, int id_bytes, int64_t n, int64_t lo, int64_t span, int32_t* d_hist,
+                 void* stream);
+
+/* Column bitvectors from a CSR of item ids: entry e of row r sets column d_lut[id - lo] (skipped when the
+ * id is outside [lo, lo+span) or the lut holds -1). Writes every word of the (n_cols, pitch) output,
+ * padding included. No atomics on HBM: each workgroup owns a 64-word x column-slab tile.
+ * Replaces csr_to_bitvec / ids_to_bitvec (gpu/kernels/_src/csr_to_bitvec.cu:8-73) and
+ * build_bitvecs_gpu (gpu/csr_bitvec.py:63-155). */
+int etm_bitvec_build(const int64_t* d_indptr, const void* d_ids, int id_bytes, int64_t n_rows, int64_t lo,
+                     int64_t span, const int32_t* d_lut, int32_t n_cols, int64_t pitch, uint64_t* d_out,
+                     void* stream);
+
+/* K=1: d_out[c] = popcount of column c over words [0, n_words).
+ * Replaces column_popcounts (gpu/kernels/loader.py:218-257). */
+int etm_column_popcounts(const uint64_t* d_bv, int32_t n_cols, int64_t pitch, int64_t n_words, int64_t* d_out,
+                         void* stream);
+
+/* Workspace bytes etm_level_groups needs for a level of n rows. */
+size_t etm_level_groups_ws(int64_t n);
+
+/* Prefix groups of a lexsorted level (n rows of m items, m >= 1; m == 1 gives one group of every row, the
+ * K=2 pair space). Writes, for n_groups groups:
+ *   d_group_start[n_groups + 1]  first row of each group (last entry n),
+ *   d_cum_pairs[n_groups + 1]    exclusive prefix sum of s(s-1)/2,
+ *   d_cum_tiles[n_groups + 1]    exclusive prefix sum of the tiled kernel's tile-pairs (groups with more
+ *                                than small_max suffixes; 0 for the others),
+ *   d_small_list[n_groups]       ids of the groups with 2..small_max suffixes (unordered),
+ *   d_totals[8]                  {n_groups, n_candidates, n_tile_pairs, n_small, max_suffixes, 0, 0, 0}.
+ * Capacities: n + 1 for the first three, n for d_small_list.
+ * Replaces build_k3plus_groups_from_flat (gpu/kernels/k3plus.py:243-376, rust_ext/src/core/groups.rs:59-205)
+ * and the kernel split of group_kernels (gpu/row_split_chunks.py:96-121). */
+int etm_level_groups(const int32_t* d_rows, int64_t n, int32_t m, int32_t small_max, int64_t* d_group_start,
+                     int64_t* d_cum_pairs, int64_t* d_cum_tiles, int32_t* d_small_list, int64_t* d_totals,
+                     void* d_ws, size_t ws_bytes, void* stream);
+
+/* Open-addressing hash of a level's rows: d_table[table_size] (a power of two >= 2n) of row ids, -1 empty.
+ * Replaces the sorted subset index upload (gpu/kernels/subset_index.py:63-140). */
+int etm_level_hash(const int32_t* d_rows, int64_t n, int32_t m, int32_t* d_table, int64_t table_size,
+                   void* stream);
+
+/* Subset test and Pascal inference of every candidate of a level's prefix groups, against the hash of the
+ * same level (the candidates' (k-1)-subsets). Writes d_status[c] (ETM_CAND_*) and initialises d_counts[c]
+ * (0 when counted, the inferred count when inferred, -1 when skipped). mode: ETM_SUBSET_PRUNE, optionally |
+ * ETM_SUBSET_INFER (then d_prev_free is required).
+ * Replaces _classify_candidate (gpu/kernels/_src/_subset_index.cu:139-178) without the deduction bounds. */
+int etm_level_classify(const int32_t* d_rows, int64_t n, int32_t m, const int32_t* d_prev_counts,
+                       const uint8_t* d_prev_free, const int32_t* d_table, int64_t table_size,
+                       const int64_t* d_group_start, const int64_t* d_cum_pairs, int64_t n_groups,
+                       int64_t n_cands, int32_t mode, uint8_t* d_status, int32_t* d_counts, void* stream);
+
+/* AND + popcount of every candidate whose status is ETM_CAND_COUNT (every candidate when d_status is
+ * NULL), ADDED into d_counts (which the caller initialises: etm_level_classify does). Groups of 2..small_max
+ * suffixes run one wave per (group, word chunk) with lanes over words; larger groups run LDS-tiled
+ * tile-pairs. layout is ETM_LAYOUT_LEX, or ETM_LAYOUT_TRI for a single-group level (K=2 in the reference's
+ * b(b-1)/2 + a order). word_chunk <= 0 picks a chunk from the level size.
+ * Replaces count_k3plus_dense, count_group_pairs and count_shared_tiled_dense
+ * (gpu/kernels/_src/k3plus_dense.cu:2-106, group_pairs.cu:31-217, shared_tiled.cu:217-283) and
+ * count_pairs_k2_dense (pairs_k2_dense.cu:2-45). */
+int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const int32_t* d_rows, int32_t m,
+                    const int64_t* d_group_start, const int64_t* d_cum_pairs, const int64_t* d_cum_tiles,
+                    int64_t n_groups, int64_t n_tile_pairs, const int32_t* d_small_list, int64_t n_small,
+                    int32_t small_max, const uint8_t* d_status, int32_t* d_counts, int32_t layout,
+                    int32_t word_chunk, void* stream);
+
+/* Workspace bytes etm_level_compact needs for n_cands candidates. */
+size_t etm_level_compact_ws(int64_t n_cands);
+
+/* Survivor compaction of a counted level, in candidate (lexicographic) order. A survivor has status !=
+ * ETM_CAND_SKIP (or d_status NULL) and count >= min_count. Call twice with the same workspace: first with
+ * d_out_rows == NULL, which writes the survivor count to d_out_n[0]; then with the outputs, capacity >= that
+ * count, which writes d_out_rows (n_out x (m+1)), d_out_counts and, when not NULL, d_out_cand (candidate
+ * index of each survivor). Replaces threshold_filter (gpu/kernels/filter.py:85-160) plus
+ * decode_k3plus_flat / decode_k2_pairs_flat (gpu/kernels/decode.py:9-73) and the level-end lexsort
+ * (gpu/row_split.py:1064-1067), which the lexicographic candidate order makes unnecessary. */
+int etm_level_compact(const int32_t* d_rows, int32_t m, const int64_t* d_group_start, const int64_t* d_cum_pairs,
+                      int64_t n_groups, int64_t n_cands, const uint8_t* d_status, const int32_t* d_counts,
+                      int64_t min_count, int32_t* d_out_rows, int32_t* d_out_counts, int64_t* d_out_cand,
+                      int64_t* d_out_n, void* d_ws, size_t ws_bytes, void* stream);
+
+/* Free flags of a level of k-itemsets against the hash of its complete previous level: d_free[x] = 1 when
+ * no (k-1)-subset of x has x's count. Every subset must be present in the previous level.
+ * Replaces _prune_non_free_mask (gpu/mining.py:81-121, rust_ext/src/core/groups.rs:250-308). */
+int etm_level_free(const int32_t* d_rows, int64_t n, int32_t k, const int32_t* d_counts,
+                   const int32_t* d_prev_rows, const int32_t* d_prev_counts, const int32_t* d_prev_table,
+                   int64_t table_size, uint8_t* d_free, void* stream);
+
+/* Keeps the rows whose d_flag is nonzero, in order (two-call protocol as etm_level_compact, workspace
+ * etm_level_compact_ws(n)). Replaces the host masks of the free-set prune (gpu/row_split.py:1081-1100). */
+int etm_select_rows(const int32_t* d_rows, int64_t n, int32_t m, const int32_t* d_counts, const uint8_t* d_flag,
+                    int32_t* d_out_rows, int32_t* d_out_counts, int64_t* d_out_n, void* d_ws, size_t ws_bytes,
+                    void* stream);
+
+/* Counts of arbitrary itemsets: popcount(AND of the columns d_items[d_offsets[i] .. d_offsets[i+1])), int64,
+ * stored into d_out[i] (accumulate == 0) or added to it (accumulate != 0, e.g. summing the chunks of SON's
+ * pass 2 on the device). Replaces count_itemsets_batch
+ * (gpu/kernels/_src/itemset_count.cu:4-41) and count_itemsets_flat (gpu/kernels/batch.py:32-117). */
+int etm_count_itemsets(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const int32_t* d_items,
+                       const int64_t* d_offsets, int64_t n_sets, int64_t* d_out, int32_t accumulate, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ETM_H */
