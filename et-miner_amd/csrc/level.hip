@@ -96,6 +96,10 @@ __global__ __launch_bounds__(256) void k_group_classes(const int64_t* __restrict
             const unsigned long long at = atomicAdd((unsigned long long*)&totals[3], 1ull);
             small_list[at] = (int32_t)g;
         }
+        if (s >= 2) {
+            atomicAdd((unsigned long long*)&totals[5], 1ull);
+            atomicAdd((unsigned long long*)&totals[6], (unsigned long long)s);
+        }
         atomicMax((unsigned long long*)&totals[4], (unsigned long long)s);
     }
 }
@@ -156,9 +160,11 @@ __global__ __launch_bounds__(256) void k_classify(const int32_t* __restrict__ ro
                                                    const int64_t* __restrict__ group_start,
                                                    const int64_t* __restrict__ cum_pairs, int64_t n_groups,
                                                    int64_t n_cands, int32_t mode, uint8_t* __restrict__ status,
-                                                   int32_t* __restrict__ counts) {
+                                                   int32_t* __restrict__ counts, int64_t* __restrict__ tally) {
+    __shared__ int64_t lds[5];
     const bool infer = (mode & ETM_SUBSET_INFER) != 0;
     const bool prune = (mode & ETM_SUBSET_PRUNE) != 0;
+    int64_t n_count = 0, n_infer = 0, n_skip = 0;
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n_cands;
          c += (int64_t)gridDim.x * blockDim.x) {
         const int64_t g = etm::upper_index(cum_pairs, n_groups, c);
@@ -197,6 +203,19 @@ __global__ __launch_bounds__(256) void k_classify(const int32_t* __restrict__ ro
         if (st == ETM_CAND_COUNT && infer && not_free && lowest >= 0) st = ETM_CAND_INFER;
         status[c] = st;
         counts[c] = st == ETM_CAND_COUNT ? 0 : (st == ETM_CAND_INFER ? lowest : -1);
+        n_count += st == ETM_CAND_COUNT;
+        n_infer += st == ETM_CAND_INFER;
+        n_skip += st == ETM_CAND_SKIP;
+    }
+    if (tally) {
+        n_count = etm::block_sum<256>(n_count, lds);
+        n_infer = etm::block_sum<256>(n_infer, lds);
+        n_skip = etm::block_sum<256>(n_skip, lds);
+        if (threadIdx.x == 0) {
+            if (n_count) atomicAdd((unsigned long long*)&tally[0], (unsigned long long)n_count);
+            if (n_infer) atomicAdd((unsigned long long*)&tally[1], (unsigned long long)n_infer);
+            if (n_skip) atomicAdd((unsigned long long*)&tally[2], (unsigned long long)n_skip);
+        }
     }
 }
 
@@ -355,7 +374,7 @@ extern "C" int etm_level_classify(const int32_t* d_rows, int64_t n, int32_t m, c
                                   const uint8_t* d_prev_free, const int32_t* d_table, int64_t table_size,
                                   const int64_t* d_group_start, const int64_t* d_cum_pairs, int64_t n_groups,
                                   int64_t n_cands, int32_t mode, uint8_t* d_status, int32_t* d_counts,
-                                  void* stream) {
+                                  int64_t* d_tally, void* stream) {
     ETM_ARG(d_rows && d_table && d_group_start && d_cum_pairs && d_status && d_counts,
             "etm_level_classify: null pointer");
     ETM_ARG(m >= 1 && m <= 62, "etm_level_classify: 1 <= m <= 62 required, got %d", m);
@@ -367,7 +386,7 @@ extern "C" int etm_level_classify(const int32_t* d_rows, int64_t n, int32_t m, c
     if (n_cands == 0) return ETM_OK;
     k_classify<<<grid_for(n_cands, 65536), 256, 0, ETM_STREAM(stream)>>>(
         d_rows, m, d_prev_counts, d_prev_free, d_table, table_size - 1, d_group_start, d_cum_pairs, n_groups,
-        n_cands, mode, d_status, d_counts);
+        n_cands, mode, d_status, d_counts, d_tally);
     ETM_LAUNCHED("k_classify");
     return ETM_OK;
 }

@@ -244,8 +244,10 @@ def apriori(
 
 def mine_csr(indptr, ids, min_support, max_length=None, prune_equal_support=False, use_generator_pruning=False,
              level_callback=None, session=None, items=None, warn_complexity=False, device_csr=None,
-             with_counts=False):
-    """The use_gpu route from a host CSR (or ``device_csr=(indptr_d, ids_d)`` already in HBM)."""
+             with_counts=False, timer=None, stats_out=None):
+    """The use_gpu route from a host CSR (or ``device_csr=(indptr_d, ids_d)`` already in HBM).
+
+    ``timer`` is handed to the level loop; ``stats_out`` (a list) receives its per-level statistics."""
     from ..gpu import device as dev
     from ..gpu.input_layer import build_device_input, host_csr_to_device
     from ..gpu.row_split import mine_levels
@@ -266,7 +268,9 @@ def mine_csr(indptr, ids, min_support, max_length=None, prune_equal_support=Fals
     if warn_complexity:
         _warn_complexity(int((built.col_counts >= min_count).sum()), min_support)
     res = mine_levels(built.bitvecs, built.n_words, n_rows, built.col_counts, min_count, max_length,
-                      prune_equal_support, use_generator_pruning, _timed(level_callback, session))
+                      prune_equal_support, use_generator_pruning, _timed(level_callback, session), timer=timer)
+    if stats_out is not None:
+        stats_out.extend(res.stats)
     if with_counts:
         return itemset_table(res.levels, built.col_to_item, n_rows, with_counts=True)
     return _frame(res.levels, built.col_to_item, n_rows, items)

@@ -111,6 +111,8 @@ class Groups:
     n_tile_pairs: int
     n_small: int
     max_suffixes: int
+    n_groups2: int  # groups with >= 2 rows (the groups that yield candidates)
+    rows2: int  # rows in those groups (their suffixes)
 
 
 def level_groups(rows, m: int) -> Groups:
@@ -126,7 +128,7 @@ def level_groups(rows, m: int) -> Groups:
     call("etm_level_groups", rows.data_ptr(), n, m, SMALL_MAX, gs.data_ptr(), cp.data_ptr(), ct.data_ptr(),
          small.data_ptr(), totals.data_ptr(), ws.data_ptr(), ws_n, _s())
     t = dev.to_host(totals).tolist()
-    return Groups(gs, cp, ct, small, int(t[0]), int(t[1]), int(t[2]), int(t[3]), int(t[4]))
+    return Groups(gs, cp, ct, small, *(int(x) for x in t[:7]))
 
 
 def table_size_for(n: int) -> int:
@@ -145,13 +147,16 @@ def level_hash(rows, m: int):
     return table, size
 
 
-def level_classify(rows, m: int, prev_counts, prev_free, table, table_size: int, groups: Groups, mode: int):
-    """(status uint8, counts int32) of every candidate: subset test, optional Pascal inference."""
+def level_classify(rows, m: int, prev_counts, prev_free, table, table_size: int, groups: Groups, mode: int,
+                   tally=None):
+    """(status uint8, counts int32) of every candidate: subset test, optional Pascal inference.
+
+    ``tally`` (device int64[3], optional) accumulates the counted, inferred and skipped candidates."""
     status = dev.empty(groups.n_cands, "uint8")
     counts = dev.empty(groups.n_cands, "int32")
     call("etm_level_classify", rows.data_ptr(), int(rows.shape[0]), m, dev.ptr(prev_counts), dev.ptr(prev_free),
          table.data_ptr(), table_size, groups.group_start.data_ptr(), groups.cum_pairs.data_ptr(), groups.n_groups,
-         groups.n_cands, mode, status.data_ptr(), counts.data_ptr(), _s())
+         groups.n_cands, mode, status.data_ptr(), counts.data_ptr(), dev.ptr(tally), _s())
     return status, counts
 
 

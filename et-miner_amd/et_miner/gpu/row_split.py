@@ -32,11 +32,21 @@ from . import kernels as K
 
 @dataclass
 class LevelStats:
+    """One level: candidates generated by the prefix join, and of them counted / inferred / skipped by the
+    subset test; ``bytes_alg`` is the compulsory HBM traffic of the prefix-group decomposition."""
+
     k: int
     n_candidates: int
     n_frequent: int
     n_counted: int = 0
+    n_inferred: int = 0
+    n_skipped: int = 0
+    bytes_alg: int = 0
     ms: float = 0.0
+
+    @property
+    def n_evaluated(self) -> int:
+        return self.n_counted + self.n_inferred
 
 
 @dataclass
@@ -81,11 +91,14 @@ def mine_levels(
     level_callback=None,
     pitch: int | None = None,
     keep_device: bool = False,
+    timer=None,
 ) -> MineResult:
     """Mines every frequent itemset (or free-set) of device bitvectors ``bv`` (n_cols, pitch).
 
     ``col_counts`` are the K=1 counts of the columns (host int64). Counts are exact integers throughout.
     With ``keep_device`` the levels stay device tensors (rows int32, counts int32) instead of host arrays.
+    ``timer`` (optional) brackets each level's counting launches: ``timer.begin(k)`` / ``timer.end(k, stats)``
+    are called on the host around them, on the stream they run on.
     """
     if n_rows > np.iinfo(np.int32).max:
         raise ValueError(f"n_transactions={n_rows:,} exceeds int32 max: the count arrays are int32")
@@ -116,16 +129,26 @@ def mine_levels(
         groups = K.level_groups(prev.rows, m)
         if groups.n_cands == 0:
             break
+        tally = None
         if k >= 3:
             table, tsize = prev.hashed()
             infer = infer_counts and not prune_non_free and prev.free is not None
             mode = K.SUBSET_PRUNE | (K.SUBSET_INFER if infer else 0)
+            tally = dev.zeros(3, "int64")
             status, counts = K.level_classify(prev.rows, m, prev.counts, prev.free if infer else None, table, tsize,
-                                              groups, mode)
+                                              groups, mode, tally)
         else:
             status, counts = None, dev.zeros(groups.n_cands, "int32")
+        st = LevelStats(k, groups.n_cands, 0, groups.n_cands, 0, 0,
+                        ((m - 1) * groups.n_groups2 + groups.rows2) * n_words * 8 + 4 * groups.n_cands)
+        if timer is not None:
+            timer.begin(k)
         K.count_level(bv, n_words, prev.rows, m, groups, status, counts, pitch=pitch)
+        if timer is not None:
+            timer.end(k, st)
         rows_k, counts_k = K.level_compact(prev.rows, m, groups, status, counts, min_count)
+        if tally is not None:
+            st.n_counted, st.n_inferred, st.n_skipped = (int(x) for x in dev.to_host(tally))
         cur = _Level(rows_k, counts_k)
         if track_free and cur.n:
             table, tsize = prev.hashed()
@@ -136,7 +159,8 @@ def mine_levels(
         n_freq = cur.n
         dev_levels.append((cur.rows, cur.counts))
         dt = (time.perf_counter() - t_k) * 1000
-        out.stats.append(LevelStats(k, groups.n_cands, n_freq, 0, dt))
+        st.n_frequent, st.ms = n_freq, dt
+        out.stats.append(st)
         if level_callback:
             level_callback(k, groups.n_cands, n_freq, dt)
         if n_freq == 0:

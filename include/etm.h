@@ -97,7 +97,8 @@ size_t etm_level_groups_ws(int64_t n);
  *   d_cum_tiles[n_groups + 1]    exclusive prefix sum of the tiled kernel's tile-pairs (groups with more
  *                                than small_max suffixes; 0 for the others),
  *   d_small_list[n_groups]       ids of the groups with 2..small_max suffixes (unordered),
- *   d_totals[8]                  {n_groups, n_candidates, n_tile_pairs, n_small, max_suffixes, 0, 0, 0}.
+ *   d_totals[8]                  {n_groups, n_candidates, n_tile_pairs, n_small, max_suffixes,
+ *                                 groups with >= 2 rows, rows in those groups, 0}.
  * Capacities: n + 1 for the first three, n for d_small_list.
  * Replaces build_k3plus_groups_from_flat (gpu/kernels/k3plus.py:243-376, rust_ext/src/core/groups.rs:59-205)
  * and the kernel split of group_kernels (gpu/row_split_chunks.py:96-121). */
@@ -113,12 +114,14 @@ int etm_level_hash(const int32_t* d_rows, int64_t n, int32_t m, int32_t* d_table
 /* Subset test and Pascal inference of every candidate of a level's prefix groups, against the hash of the
  * same level (the candidates' (k-1)-subsets). Writes d_status[c] (ETM_CAND_*) and initialises d_counts[c]
  * (0 when counted, the inferred count when inferred, -1 when skipped). mode: ETM_SUBSET_PRUNE, optionally |
- * ETM_SUBSET_INFER (then d_prev_free is required).
+ * ETM_SUBSET_INFER (then d_prev_free is required). When d_tally is not NULL, adds the numbers of counted,
+ * inferred and skipped candidates to d_tally[0..2].
  * Replaces _classify_candidate (gpu/kernels/_src/_subset_index.cu:139-178) without the deduction bounds. */
 int etm_level_classify(const int32_t* d_rows, int64_t n, int32_t m, const int32_t* d_prev_counts,
                        const uint8_t* d_prev_free, const int32_t* d_table, int64_t table_size,
                        const int64_t* d_group_start, const int64_t* d_cum_pairs, int64_t n_groups,
-                       int64_t n_cands, int32_t mode, uint8_t* d_status, int32_t* d_counts, void* stream);
+                       int64_t n_cands, int32_t mode, uint8_t* d_status, int32_t* d_counts, int64_t* d_tally,
+                       void* stream);
 
 /* AND + popcount of every candidate whose status is ETM_CAND_COUNT (every candidate when d_status is
  * NULL), ADDED into d_counts (which the caller initialises: etm_level_classify does). Groups of 2..small_max
