@@ -31,6 +31,11 @@ namespace {
 constexpr int kT = 64;   // suffixes per tile
 constexpr int kKW = 32;  // words per LDS step
 
+// acc + popcount(x) as two v_bcnt_u32_b32 with accumulate (one per 32-bit half).
+__device__ __forceinline__ int popc_acc(uint64_t x, int acc) {
+    return __popc((uint32_t)(x >> 32)) + (__popc((uint32_t)x) + acc);
+}
+
 __device__ __forceinline__ int64_t pair_slot(int64_t i, int64_t j, int64_t s, int32_t layout) {
     return layout == ETM_LAYOUT_TRI ? j * (j - 1) / 2 + i : etm::lex_pair_index(i, j, s);
 }
@@ -90,7 +95,7 @@ __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict_
         for (int i = 0; i < S; ++i) {
 #pragma unroll
             for (int j = i + 1; j < S; ++j) {
-                acc[a] += __popcll(sv[i] & sv[j]);
+                acc[a] = popc_acc(sv[i] & sv[j], acc[a]);
                 ++a;
             }
         }
@@ -201,7 +206,7 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
 #pragma unroll
             for (int r = 0; r < 4; ++r)
 #pragma unroll
-                for (int c = 0; c < 4; ++c) acc[r][c] += __popcll(av[r] & bw[c]);
+                for (int c = 0; c < 4; ++c) acc[r][c] = popc_acc(av[r] & bw[c], acc[r][c]);
         }
         __syncthreads();
     }
@@ -275,41 +280,51 @@ hipError_t launch_small(const uint64_t* bv, int64_t pitch, int64_t n_words, int6
 
 extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const int32_t* d_rows, int32_t m,
                                const int64_t* d_group_start, const int64_t* d_cum_pairs, const int64_t* d_cum_tiles,
-                               int64_t n_groups, int64_t n_tile_pairs, const int32_t* d_small_list, int64_t n_small,
-                               int32_t small_max, const uint8_t* d_status, int32_t* d_counts, int32_t layout,
-                               int32_t word_chunk, void* stream) {
+                               int64_t n_groups, int64_t n_tile_pairs, const int32_t* d_small_list,
+                               const int64_t* h_class_counts, const uint8_t* d_status, int32_t* d_counts,
+                               int32_t layout, int32_t word_chunk, void* stream) {
     ETM_ARG(d_bv && d_rows && d_group_start && d_cum_pairs && d_counts, "etm_count_level: null pointer");
     ETM_ARG(m >= 1 && m <= 62, "etm_count_level: 1 <= m <= 62 required, got %d", m);
     ETM_ARG(pitch >= n_words && n_words >= 0, "etm_count_level: pitch %lld < n_words %lld", (long long)pitch,
             (long long)n_words);
-    ETM_ARG(small_max == 2 || small_max == 4 || small_max == 8 || small_max == 16,
-            "etm_count_level: small_max must be 2, 4, 8 or 16, got %d", small_max);
     ETM_ARG(layout == ETM_LAYOUT_LEX || (layout == ETM_LAYOUT_TRI && n_groups == 1),
             "etm_count_level: the TRI layout needs a single group");
     ETM_ARG(n_tile_pairs == 0 || d_cum_tiles != nullptr, "etm_count_level: null tile offsets");
+    int64_t cls[4] = {0, 0, 0, 0};
+    if (h_class_counts)
+        for (int c = 0; c < 4; ++c) {
+            ETM_ARG(h_class_counts[c] >= 0, "etm_count_level: negative class size");
+            cls[c] = h_class_counts[c];
+        }
+    const int64_t n_small = cls[0] + cls[1] + cls[2] + cls[3];
     ETM_ARG(n_small == 0 || d_small_list != nullptr, "etm_count_level: null small-group list");
     if (n_words == 0) return ETM_OK;
     hipStream_t s = ETM_STREAM(stream);
-    if (n_small > 0) {
-        const int64_t chunk = word_chunk > 0 ? word_chunk : pick_chunk(n_words, n_small, 16384, 64, 256);
+    int64_t base = 0;
+    for (int c = 0; c < 4; ++c) {
+        if (cls[c] == 0) continue;
+        // Work items scale with the class's per-word VALU cost: wider classes get more waves per word range.
+        const int64_t chunk = word_chunk > 0 ? word_chunk : pick_chunk(n_words, cls[c], 16384, 64, 128);
         hipError_t e;
-        switch (small_max) {
-            case 2: e = launch_small<2>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start, d_cum_pairs,
-                                        d_small_list, n_small, d_status, d_counts, layout, s); break;
-            case 4: e = launch_small<4>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start, d_cum_pairs,
-                                        d_small_list, n_small, d_status, d_counts, layout, s); break;
-            case 8: e = launch_small<8>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start, d_cum_pairs,
-                                        d_small_list, n_small, d_status, d_counts, layout, s); break;
-            default: e = launch_small<16>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start, d_cum_pairs,
-                                          d_small_list, n_small, d_status, d_counts, layout, s); break;
+        const int32_t* list = d_small_list + base;
+        switch (c) {
+            case 0: e = launch_small<2>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start, d_cum_pairs, list,
+                                        cls[c], d_status, d_counts, layout, s); break;
+            case 1: e = launch_small<4>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start, d_cum_pairs, list,
+                                        cls[c], d_status, d_counts, layout, s); break;
+            case 2: e = launch_small<8>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start, d_cum_pairs, list,
+                                        cls[c], d_status, d_counts, layout, s); break;
+            default: e = launch_small<16>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start, d_cum_pairs, list,
+                                          cls[c], d_status, d_counts, layout, s); break;
         }
         ETM_REQUIRE(e == hipSuccess, ETM_EHIP, "launch of k_count_small failed: %s", hipGetErrorString(e));
+        base += cls[c];
     }
     if (n_tile_pairs > 0) {
         ETM_REQUIRE(n_tile_pairs <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld tile-pairs exceed the grid",
                     (long long)n_tile_pairs);
         const int64_t chunk = word_chunk > 0 ? ceil_div(word_chunk, kKW) * kKW
-                                             : pick_chunk(n_words, n_tile_pairs, 4096, kKW, 512);
+                                             : pick_chunk(n_words, n_tile_pairs, 2048, kKW, 128);
         const int64_t n_chunks = ceil_div(n_words, chunk);
         ETM_REQUIRE(n_chunks <= 65535, ETM_EOVERFLOW, "etm_count_level: %lld word chunks exceed the grid",
                     (long long)n_chunks);

@@ -85,22 +85,57 @@ __global__ void k_zero_i64(int64_t* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
+// Size class of a group of s suffixes for the wave-per-group kernel: 0 (2), 1 (3-4), 2 (5-8), 3 (9-16), or -1.
+__device__ __forceinline__ int size_class(int64_t s, int32_t small_max) {
+    if (s < 2 || s > small_max) return -1;
+    return s <= 2 ? 0 : s <= 4 ? 1 : s <= 8 ? 2 : 3;
+}
+
+// Pass 1: class sizes and the totals; pass 2 (place): each small group at its class's offset.
 __global__ __launch_bounds__(256) void k_group_classes(const int64_t* __restrict__ group_start, int64_t n,
-                                                        int32_t small_max, int32_t* __restrict__ small_list,
-                                                        int64_t* __restrict__ totals) {
+                                                        int32_t small_max, int64_t* __restrict__ totals) {
     const int64_t n_groups = totals[0];
+    int64_t n2 = 0, rows2 = 0, smax = 0, cls[4] = {0, 0, 0, 0};
     for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n_groups && g < n;
          g += (int64_t)gridDim.x * blockDim.x) {
         const int64_t s = group_start[g + 1] - group_start[g];
-        if (s >= 2 && s <= small_max) {
-            const unsigned long long at = atomicAdd((unsigned long long*)&totals[3], 1ull);
-            small_list[at] = (int32_t)g;
-        }
+        const int c = size_class(s, small_max);
+        if (c >= 0) cls[c] += 1;
         if (s >= 2) {
-            atomicAdd((unsigned long long*)&totals[5], 1ull);
-            atomicAdd((unsigned long long*)&totals[6], (unsigned long long)s);
+            n2 += 1;
+            rows2 += s;
         }
-        atomicMax((unsigned long long*)&totals[4], (unsigned long long)s);
+        smax = s > smax ? s : smax;
+    }
+    __shared__ int64_t lds[5];
+    n2 = etm::block_sum<256>(n2, lds);
+    rows2 = etm::block_sum<256>(rows2, lds);
+    for (int c = 0; c < 4; ++c) cls[c] = etm::block_sum<256>(cls[c], lds);
+    smax = etm::wave_max(smax);
+    if ((threadIdx.x & 63) == 0 && smax) atomicMax((unsigned long long*)&totals[4], (unsigned long long)smax);
+    if (threadIdx.x == 0) {
+        if (n2) atomicAdd((unsigned long long*)&totals[5], (unsigned long long)n2);
+        if (rows2) atomicAdd((unsigned long long*)&totals[6], (unsigned long long)rows2);
+        for (int c = 0; c < 4; ++c)
+            if (cls[c]) {
+                atomicAdd((unsigned long long*)&totals[8 + c], (unsigned long long)cls[c]);
+                atomicAdd((unsigned long long*)&totals[3], (unsigned long long)cls[c]);
+            }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_group_place(const int64_t* __restrict__ group_start, int64_t n,
+                                                      int32_t small_max, int32_t* __restrict__ small_list,
+                                                      int64_t* __restrict__ totals) {
+    const int64_t n_groups = totals[0];
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n_groups && g < n;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int c = size_class(group_start[g + 1] - group_start[g], small_max);
+        if (c < 0) continue;
+        int64_t base = 0;
+        for (int q = 0; q < c; ++q) base += totals[8 + q];
+        const unsigned long long at = atomicAdd((unsigned long long*)&totals[12 + c], 1ull);
+        small_list[base + (int64_t)at] = (int32_t)g;
     }
 }
 
@@ -332,12 +367,13 @@ extern "C" int etm_level_groups(const int32_t* d_rows, int64_t n, int32_t m, int
     ETM_ARG(n >= 1 && m >= 1 && m <= 62, "etm_level_groups: need n >= 1 and 1 <= m <= 62 (n=%lld, m=%d)",
             (long long)n, m);
     ETM_ARG(n < 0x7FFFFFFF, "etm_level_groups: %lld rows exceed int32 row ids", (long long)n);
-    ETM_ARG(small_max >= 2, "etm_level_groups: small_max must be >= 2");
+    ETM_ARG(small_max == 2 || small_max == 4 || small_max == 8 || small_max == 16,
+            "etm_level_groups: small_max must be 2, 4, 8 or 16, got %d", small_max);
     ETM_ARG(ws_bytes >= etm_level_groups_ws(n), "etm_level_groups: workspace %zu < %zu bytes", ws_bytes,
             etm_level_groups_ws(n));
     hipStream_t s = ETM_STREAM(stream);
     int64_t* ws = (int64_t*)d_ws;
-    k_zero_i64<<<1, 64, 0, s>>>(d_totals, 8);
+    k_zero_i64<<<1, 64, 0, s>>>(d_totals, 16);
     ETM_LAUNCHED("k_zero_i64");
     IsGroupStart is_start{d_rows, m};
     ETM_HIP(etm::scan_prepare(is_start, n, ws, nullptr, s, SetGroupEnd{d_group_start, n, d_totals}));
@@ -348,8 +384,10 @@ extern "C" int etm_level_groups(const int32_t* d_rows, int64_t n, int32_t m, int
     GroupTiles tiles{d_group_start, d_totals, small_max};
     ETM_HIP(etm::scan_prepare(tiles, n + 1, ws, &d_totals[2], s));
     ETM_HIP(etm::scan_finish(tiles, WritePrefix{d_cum_tiles}, n + 1, ws, s));
-    k_group_classes<<<grid_for(n), 256, 0, s>>>(d_group_start, n, small_max, d_small_list, d_totals);
+    k_group_classes<<<grid_for(n, 1024), 256, 0, s>>>(d_group_start, n, small_max, d_totals);
     ETM_LAUNCHED("k_group_classes");
+    k_group_place<<<grid_for(n), 256, 0, s>>>(d_group_start, n, small_max, d_small_list, d_totals);
+    ETM_LAUNCHED("k_group_place");
     return ETM_OK;
 }
 

@@ -10,7 +10,7 @@
 namespace etm {
 
 constexpr int kScanThreads = 256;
-constexpr int kScanPer = 16;
+constexpr int kScanPer = 4;
 constexpr int64_t kScanItems = (int64_t)kScanThreads * kScanPer;
 
 inline int64_t scan_blocks(int64_t n) { return (n + kScanItems - 1) / kScanItems; }

@@ -11,6 +11,7 @@ by wrapper name.
 
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass
 
 import numpy as np
@@ -113,6 +114,7 @@ class Groups:
     max_suffixes: int
     n_groups2: int  # groups with >= 2 rows (the groups that yield candidates)
     rows2: int  # rows in those groups (their suffixes)
+    class_counts: tuple  # small groups per size class (2, 3-4, 5-8, 9-16 suffixes)
 
 
 def level_groups(rows, m: int) -> Groups:
@@ -122,13 +124,13 @@ def level_groups(rows, m: int) -> Groups:
         raise ValueError("level_groups: empty level")
     gs, cp, ct = dev.empty(n + 1, "int64"), dev.empty(n + 1, "int64"), dev.empty(n + 1, "int64")
     small = dev.empty(n, "int32")
-    totals = dev.empty(8, "int64")
+    totals = dev.empty(16, "int64")
     ws_n = _lib.ws_bytes("etm_level_groups_ws", n)
     ws = dev.empty((ws_n + 7) // 8, "int64")
     call("etm_level_groups", rows.data_ptr(), n, m, SMALL_MAX, gs.data_ptr(), cp.data_ptr(), ct.data_ptr(),
          small.data_ptr(), totals.data_ptr(), ws.data_ptr(), ws_n, _s())
     t = dev.to_host(totals).tolist()
-    return Groups(gs, cp, ct, small, *(int(x) for x in t[:7]))
+    return Groups(gs, cp, ct, small, *(int(x) for x in t[:7]), tuple(int(x) for x in t[8:12]))
 
 
 def table_size_for(n: int) -> int:
@@ -164,10 +166,10 @@ def count_level(bv, n_words: int, rows, m: int, groups: Groups, status, counts, 
                 word_chunk: int = 0, pitch: int | None = None) -> None:
     """Adds the AND + popcount counts of the counted candidates into ``counts``."""
     pitch = int(bv.shape[1]) if pitch is None else int(pitch)
+    cls = (ctypes.c_int64 * 4)(*groups.class_counts)
     call("etm_count_level", bv.data_ptr(), pitch, n_words, rows.data_ptr(), m, groups.group_start.data_ptr(),
          groups.cum_pairs.data_ptr(), groups.cum_tiles.data_ptr(), groups.n_groups, groups.n_tile_pairs,
-         groups.small_list.data_ptr(), groups.n_small, SMALL_MAX, dev.ptr(status), counts.data_ptr(), layout,
-         word_chunk, _s())
+         groups.small_list.data_ptr(), cls, dev.ptr(status), counts.data_ptr(), layout, word_chunk, _s())
 
 
 def level_compact(rows, m: int, groups: Groups, status, counts, min_count: int, with_cand: bool = False):

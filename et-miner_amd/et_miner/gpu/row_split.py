@@ -111,7 +111,7 @@ def mine_levels(
     k1_rows = gen_cols.reshape(-1, 1)
     k1_counts = col_counts[gen_cols].astype(np.int64)
     out.levels.append((k1_rows, k1_counts))
-    out.stats.append(LevelStats(1, n_cols, len(gen_cols), n_cols, (time.perf_counter() - t0) * 1000))
+    out.stats.append(LevelStats(1, n_cols, len(gen_cols), n_counted=n_cols, ms=(time.perf_counter() - t0) * 1000))
     if level_callback:
         level_callback(1, n_cols, len(gen_cols), out.stats[-1].ms)
     if len(gen_cols) == 0:

@@ -96,9 +96,11 @@ size_t etm_level_groups_ws(int64_t n);
  *   d_cum_pairs[n_groups + 1]    exclusive prefix sum of s(s-1)/2,
  *   d_cum_tiles[n_groups + 1]    exclusive prefix sum of the tiled kernel's tile-pairs (groups with more
  *                                than small_max suffixes; 0 for the others),
- *   d_small_list[n_groups]       ids of the groups with 2..small_max suffixes (unordered),
- *   d_totals[8]                  {n_groups, n_candidates, n_tile_pairs, n_small, max_suffixes,
- *                                 groups with >= 2 rows, rows in those groups, 0}.
+ *   d_small_list[n_groups]       ids of the groups with 2..small_max suffixes, by size class: first the
+ *                                groups of 2 suffixes, then 3-4, 5-8, 9-16 (unordered within a class),
+ *   d_totals[16]                 {n_groups, n_candidates, n_tile_pairs, n_small, max_suffixes,
+ *                                 groups with >= 2 rows, rows in those groups, 0,
+ *                                 class sizes of d_small_list (2, 3-4, 5-8, 9-16), 4 internal cursors}.
  * Capacities: n + 1 for the first three, n for d_small_list.
  * Replaces build_k3plus_groups_from_flat (gpu/kernels/k3plus.py:243-376, rust_ext/src/core/groups.rs:59-205)
  * and the kernel split of group_kernels (gpu/row_split_chunks.py:96-121). */
@@ -125,16 +127,17 @@ int etm_level_classify(const int32_t* d_rows, int64_t n, int32_t m, const int32_
 
 /* AND + popcount of every candidate whose status is ETM_CAND_COUNT (every candidate when d_status is
  * NULL), ADDED into d_counts (which the caller initialises: etm_level_classify does). Groups of 2..small_max
- * suffixes run one wave per (group, word chunk) with lanes over words; larger groups run LDS-tiled
- * tile-pairs. layout is ETM_LAYOUT_LEX, or ETM_LAYOUT_TRI for a single-group level (K=2 in the reference's
+ * suffixes run one wave per (group, word chunk) with lanes over words, one kernel instance per size class
+ * (h_class_counts[4]: the class sizes etm_level_groups reported in d_totals[8..11]); larger groups run
+ * LDS-tiled tile-pairs. layout is ETM_LAYOUT_LEX, or ETM_LAYOUT_TRI for a single-group level (K=2 in the reference's
  * b(b-1)/2 + a order). word_chunk <= 0 picks a chunk from the level size.
  * Replaces count_k3plus_dense, count_group_pairs and count_shared_tiled_dense
  * (gpu/kernels/_src/k3plus_dense.cu:2-106, group_pairs.cu:31-217, shared_tiled.cu:217-283) and
  * count_pairs_k2_dense (pairs_k2_dense.cu:2-45). */
 int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const int32_t* d_rows, int32_t m,
                     const int64_t* d_group_start, const int64_t* d_cum_pairs, const int64_t* d_cum_tiles,
-                    int64_t n_groups, int64_t n_tile_pairs, const int32_t* d_small_list, int64_t n_small,
-                    int32_t small_max, const uint8_t* d_status, int32_t* d_counts, int32_t layout,
+                    int64_t n_groups, int64_t n_tile_pairs, const int32_t* d_small_list,
+                    const int64_t* h_class_counts, const uint8_t* d_status, int32_t* d_counts, int32_t layout,
                     int32_t word_chunk, void* stream);
 
 /* Workspace bytes etm_level_compact needs for n_cands candidates. */
