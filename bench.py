@@ -76,6 +76,8 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", default="deep_k_1m")
     ap.add_argument("--rows", type=int, default=None, help="override the workload's row count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--engine", default="native", choices=["native", "python"],
+                    help="native: the level loop inside libetm; python: the same kernels driven level by level")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(json.dumps({"error": f"--gpus {args.gpus}: this build runs on one GPU; multi-GPU counting "
@@ -107,7 +109,8 @@ def main(argv=None) -> int:
     def step(collect=False):
         stats = []
         table = mine_csr(None, None, w.min_support, w.max_length, use_generator_pruning=w.use_generator_pruning,
-                         device_csr=d_csr, with_counts=collect, timer=timer, stats_out=stats)
+                         device_csr=d_csr, with_counts=collect, timer=timer, stats_out=stats, engine=args.engine,
+                         time_kernels=timer.enabled)
         return table, stats
 
     timer.enabled = False
@@ -128,7 +131,17 @@ def main(argv=None) -> int:
     n_gen = sum(s.n_candidates for s in per_step)
     n_itemsets = sum(s.n_frequent for s in per_step)
     value = n_eval * args.steps / elapsed
-    ksum = timer.summary(n_words)
+    if args.engine == "python":
+        ksum = timer.summary(n_words)
+    else:  # the native loop records the events itself, on the stream its kernels run on
+        ksum = {"k2": [0.0, 0, 0], "k3p": [0.0, 0, 0]}
+        for stats in stats_all:
+            for st in stats:
+                if st.k >= 2:
+                    key = "k2" if st.k == 2 else "k3p"
+                    ksum[key][0] += st.count_ms
+                    ksum[key][1] += st.bytes_alg
+                    ksum[key][2] += st.n_counted * n_words
     k3_ms, k3_bytes = ksum["k3p"][0] / args.steps, ksum["k3p"][1] / args.steps
     k2_ms, k2_cw = ksum["k2"][0] / args.steps, ksum["k2"][2] / args.steps
     k3_launches = sum(1 for s in per_step if s.k >= 3)
@@ -171,6 +184,7 @@ def main(argv=None) -> int:
         "data": "synthetic",
         "config": {
             "workload": w.name,
+            "engine": args.engine,
             "n_transactions": data.n_rows,
             "n_items": w.vocab,
             "min_support": w.min_support,

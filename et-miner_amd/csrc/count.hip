@@ -40,6 +40,13 @@ __device__ __forceinline__ int64_t pair_slot(int64_t i, int64_t j, int64_t s, in
     return layout == ETM_LAYOUT_TRI ? j * (j - 1) / 2 + i : etm::lex_pair_index(i, j, s);
 }
 
+// Words a lane handles per iteration in k_count_small<S>: more independent loads in flight for the narrow
+// classes, fewer registers for the wide ones.
+template <int S>
+struct SmallUnroll {
+    static constexpr int U = S <= 4 ? 4 : (S <= 8 ? 2 : 1);
+};
+
 template <int S>
 __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
                                                       int64_t chunk_words, int64_t n_chunks,
@@ -50,6 +57,7 @@ __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict_
                                                       const uint8_t* __restrict__ status,
                                                       int32_t* __restrict__ counts, int32_t layout) {
     constexpr int NP = S * (S - 1) / 2;
+    constexpr int U = SmallUnroll<S>::U;
     const int lane = threadIdx.x & 63;
     const int64_t item = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (item >= n_list * n_chunks) return;
@@ -73,9 +81,9 @@ __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict_
     }
     if (!any_live) return;
 
-    int64_t scol[S];
+    const uint64_t* scol[S];
 #pragma unroll
-    for (int t = 0; t < S; ++t) scol[t] = t < s ? (int64_t)rows[(gs + t) * m + np] * pitch : 0;
+    for (int t = 0; t < S; ++t) scol[t] = bv + (t < s ? (int64_t)rows[(gs + t) * m + np] * pitch : 0);
     const int32_t* prow = rows + gs * m;
 
     int acc[NP];
@@ -84,19 +92,41 @@ __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict_
 
     const int64_t w_begin = ch * chunk_words;
     const int64_t w_end = min(n_words, w_begin + chunk_words);
-    for (int64_t w = w_begin + lane; w < w_end; w += 64) {
-        uint64_t pre = ~0ull;
-        for (int t = 0; t < np; ++t) pre &= bv[(int64_t)prow[t] * pitch + w];
-        uint64_t sv[S];
+    for (int64_t w0 = w_begin + lane; w0 < w_end; w0 += 64 * U) {
+        bool in[U];
+        int64_t w[U];
+        uint64_t pre[U];
 #pragma unroll
-        for (int t = 0; t < S; ++t) sv[t] = t < s ? (bv[scol[t] + w] & pre) : 0ull;
-        int a = 0;
+        for (int u = 0; u < U; ++u) {
+            w[u] = w0 + 64 * u;
+            in[u] = w[u] < w_end;
+            if (!in[u]) w[u] = w_begin;  // a valid word; its result is masked out below
+            pre[u] = in[u] ? ~0ull : 0ull;
+        }
+        // Prefix AND, four prefix columns at a time: 4 x U independent loads in flight.
+        for (int t = 0; t < np; t += 4) {
+            const uint64_t* c0 = bv + (int64_t)prow[t] * pitch;
+            const uint64_t* c1 = bv + (int64_t)prow[t + 1 < np ? t + 1 : t] * pitch;
+            const uint64_t* c2 = bv + (int64_t)prow[t + 2 < np ? t + 2 : t] * pitch;
+            const uint64_t* c3 = bv + (int64_t)prow[t + 3 < np ? t + 3 : t] * pitch;
 #pragma unroll
-        for (int i = 0; i < S; ++i) {
+            for (int u = 0; u < U; ++u) pre[u] &= c0[w[u]] & c1[w[u]] & c2[w[u]] & c3[w[u]];
+        }
+        uint64_t sv[U][S];
 #pragma unroll
-            for (int j = i + 1; j < S; ++j) {
-                acc[a] = popc_acc(sv[i] & sv[j], acc[a]);
-                ++a;
+        for (int t = 0; t < S; ++t)
+#pragma unroll
+            for (int u = 0; u < U; ++u) sv[u][t] = t < s ? (scol[t][w[u]] & pre[u]) : 0ull;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            int a = 0;
+#pragma unroll
+            for (int i = 0; i < S; ++i) {
+#pragma unroll
+                for (int j = i + 1; j < S; ++j) {
+                    acc[a] = popc_acc(sv[u][i] & sv[u][j], acc[a]);
+                    ++a;
+                }
             }
         }
     }
@@ -175,14 +205,18 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
     const int64_t w_end = min(n_words, w_begin + chunk_words);
     __syncthreads();
     for (int64_t wb = w_begin; wb < w_end; wb += kKW) {
-        if (tid < kKW) {
-            const int64_t w = wb + tid;
-            uint64_t pre = w < w_end ? ~0ull : 0ull;
-            if (w < w_end)
-                for (int t = 0; t < np; ++t) pre &= bv[(int64_t)prow[t] * pitch + w];
-            sP[tid] = pre;
-        }
+        if (tid < kKW) sP[tid] = (wb + tid < w_end) ? ~0ull : 0ull;
         __syncthreads();
+        if (np > 0) {
+            // 256 threads: word tid % 32, prefix columns tid / 32, tid / 32 + 8, ...
+            const int w = tid % kKW;
+            if (wb + w < w_end) {
+                uint64_t part = ~0ull;
+                for (int t = tid / kKW; t < np; t += 256 / kKW) part &= bv[(int64_t)prow[t] * pitch + wb + w];
+                if (tid / kKW < np) atomicAnd((unsigned long long*)&sP[w], (unsigned long long)part);
+            }
+            __syncthreads();
+        }
 #pragma unroll
         for (int q = 0; q < (kT * kKW) / 256; ++q) {
             const int e = tid + 256 * q;

@@ -1,0 +1,32 @@
+// Launchers shared between libetm's translation units (not part of the C ABI). Each enqueues kernels on `s`
+// and returns the launch status. A non-null d_n means "the count lives on the device at d_n, at most n_cap".
+#pragma once
+
+#include "common.h"
+
+namespace etm {
+namespace detail {
+
+// Largest level (rows) / candidate space the single-workgroup fast paths take.
+constexpr int64_t kSingleMax = 32768;
+
+hipError_t launch_groups_single(const int32_t* rows, const int64_t* d_n, int64_t n_cap, int32_t m, int32_t small_max,
+                                int64_t* group_start, int64_t* cum_pairs, int64_t* cum_tiles, int32_t* small_list,
+                                int64_t* totals, hipStream_t s);
+hipError_t launch_compact_single(const int32_t* rows, int32_t m, const int64_t* group_start, const int64_t* cum_pairs,
+                                 int64_t n_groups, int64_t n_cands, const uint8_t* status, const int32_t* counts,
+                                 int64_t min_count, int32_t* out_rows, int32_t* out_counts, int64_t* out_cand,
+                                 int64_t* d_out_n, hipStream_t s);
+hipError_t launch_select_single(const int32_t* rows, const int64_t* d_n, int64_t n_cap, int32_t m,
+                                const int32_t* counts, const uint8_t* flag, int32_t* out_rows, int32_t* out_counts,
+                                int64_t* d_out_n, hipStream_t s);
+hipError_t launch_hash(const int32_t* rows, int64_t n_cap, const int64_t* d_n, int32_t m, int32_t* table,
+                       int64_t table_size, hipStream_t s);
+hipError_t launch_free(const int32_t* rows, int64_t n_cap, const int64_t* d_n, int32_t k, const int32_t* counts,
+                       const int32_t* prev_rows, const int32_t* prev_counts, const int32_t* prev_table,
+                       int64_t table_size, uint8_t* free_out, hipStream_t s);
+hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
+hipError_t launch_zero_i64(int64_t* p, int n, hipStream_t s);
+
+}  // namespace detail
+}  // namespace etm

@@ -4,6 +4,7 @@
 // free flags. Nothing here returns to the host except the scalar totals the
 // caller reads to size the next buffers.
 #include "common.h"
+#include "internal.h"
 #include "scan.h"
 
 namespace {
@@ -147,7 +148,9 @@ __global__ void k_fill_table(int32_t* t, int64_t n) {
 }
 
 __global__ __launch_bounds__(256) void k_hash_insert(const int32_t* __restrict__ rows, int64_t n, int32_t m,
-                                                      int32_t* __restrict__ table, int64_t mask) {
+                                                      int32_t* __restrict__ table, int64_t mask,
+                                                      const int64_t* d_n = nullptr) {
+    if (d_n) n = min(n, *d_n);
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
         uint64_t h = 0;
         for (int t = 0; t < m; ++t) h += item_hash(row_item(rows, r, m, t));
@@ -319,7 +322,8 @@ __global__ __launch_bounds__(256) void k_free(const int32_t* __restrict__ rows, 
                                                const int32_t* __restrict__ prev_rows,
                                                const int32_t* __restrict__ prev_counts,
                                                const int32_t* __restrict__ table, int64_t mask,
-                                               uint8_t* __restrict__ free_out) {
+                                               uint8_t* __restrict__ free_out, const int64_t* d_n = nullptr) {
+    if (d_n) n = min(n, *d_n);
     for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
         const int32_t* row = rows + x * k;
         uint64_t h = 0;
@@ -345,6 +349,131 @@ __global__ __launch_bounds__(256) void k_free(const int32_t* __restrict__ rows, 
     }
 }
 
+// ── single-workgroup fast paths (small levels: one launch, sizes read on the device) ─────────
+
+constexpr int kOne = 1024;
+
+// The whole group build of a level of *d_n rows (at most n_cap) in one workgroup: group starts, cumulative
+// pairs and tile-pairs, size-class lists (groups in ascending order within each class) and the totals.
+__global__ __launch_bounds__(kOne) void k_groups_single(const int32_t* __restrict__ rows, const int64_t* d_n,
+                                                        int64_t n_cap, int32_t m, int32_t small_max,
+                                                        int64_t* __restrict__ group_start,
+                                                        int64_t* __restrict__ cum_pairs,
+                                                        int64_t* __restrict__ cum_tiles,
+                                                        int32_t* __restrict__ small_list,
+                                                        int64_t* __restrict__ totals) {
+    __shared__ int64_t lds[kOne / 64 + 1];
+    const int tid = threadIdx.x;
+    const int64_t n = d_n ? min(*d_n, n_cap) : n_cap;
+    IsGroupStart is_start{rows, m};
+    int64_t carry = 0, tot;
+    for (int64_t base = 0; base < n; base += kOne) {
+        const int64_t r = base + tid;
+        const int64_t f = r < n ? is_start(r) : 0;
+        const int64_t ex = etm::block_exclusive_scan<kOne>(f, lds, &tot);
+        if (f) group_start[carry + ex] = r;
+        carry += tot;
+    }
+    const int64_t n_groups = carry;
+    __syncthreads();
+    if (tid == 0) group_start[n_groups] = n;
+    __syncthreads();
+    int64_t cp = 0, ct = 0, n2 = 0, rows2 = 0, smax = 0, cls[4] = {0, 0, 0, 0};
+    for (int64_t base = 0; base < n_groups; base += kOne) {
+        const int64_t g = base + tid;
+        const int64_t sz = g < n_groups ? group_start[g + 1] - group_start[g] : 0;
+        const int64_t pairs = sz * (sz - 1) / 2;
+        const int64_t nt = (sz + kTileT - 1) / kTileT;
+        const int64_t tiles = sz > small_max ? nt * (nt + 1) / 2 : 0;
+        const int64_t exp_ = etm::block_exclusive_scan<kOne>(pairs, lds, &tot);
+        if (g < n_groups) cum_pairs[g] = cp + exp_;
+        cp += tot;
+        const int64_t ext = etm::block_exclusive_scan<kOne>(tiles, lds, &tot);
+        if (g < n_groups) cum_tiles[g] = ct + ext;
+        ct += tot;
+        if (sz >= 2) {
+            n2 += 1;
+            rows2 += sz;
+        }
+        smax = sz > smax ? sz : smax;
+    }
+    if (tid == 0) {
+        cum_pairs[n_groups] = cp;
+        cum_tiles[n_groups] = ct;
+    }
+    n2 = etm::block_sum<kOne>(n2, lds);
+    rows2 = etm::block_sum<kOne>(rows2, lds);
+    smax = etm::wave_max(smax);
+    if ((tid & 63) == 0) lds[tid / 64] = smax;
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 1; w < kOne / 64; ++w) smax = lds[w] > smax ? lds[w] : smax;
+        smax = lds[0] > smax ? lds[0] : smax;
+    }
+    __syncthreads();
+    // Class lists, ascending within each class.
+    int64_t placed = 0;
+    for (int c = 0; c < 4; ++c) {
+        int64_t cc = 0;
+        for (int64_t base = 0; base < n_groups; base += kOne) {
+            const int64_t g = base + tid;
+            const int64_t f = g < n_groups && size_class(group_start[g + 1] - group_start[g], small_max) == c;
+            const int64_t ex = etm::block_exclusive_scan<kOne>(f, lds, &tot);
+            if (f) small_list[placed + cc + ex] = (int32_t)g;
+            cc += tot;
+        }
+        cls[c] = cc;
+        placed += cc;
+    }
+    if (tid == 0) {
+        totals[0] = n_groups;
+        totals[1] = cp;
+        totals[2] = ct;
+        totals[3] = placed;
+        totals[4] = smax;
+        totals[5] = n2;
+        totals[6] = rows2;
+        totals[7] = n;
+        for (int c = 0; c < 4; ++c) totals[8 + c] = cls[c];
+        for (int c = 0; c < 4; ++c) totals[12 + c] = 0;
+    }
+}
+
+// Survivor compaction of up to n_cap candidates in one workgroup, into outputs of capacity n_cap; the
+// survivor count goes to *d_out_n.
+__global__ __launch_bounds__(kOne) void k_compact_single(Survives f, WriteSurvivor w, int64_t n_cands,
+                                                         int64_t* __restrict__ d_out_n) {
+    __shared__ int64_t lds[kOne / 64 + 1];
+    int64_t carry = 0, tot;
+    for (int64_t base = 0; base < n_cands; base += kOne) {
+        const int64_t c = base + threadIdx.x;
+        const int64_t v = c < n_cands ? f(c) : 0;
+        const int64_t ex = etm::block_exclusive_scan<kOne>(v, lds, &tot);
+        if (c < n_cands) w(c, v, carry + ex);
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *d_out_n = carry;
+}
+
+// Rows whose flag is set, in order, in one workgroup (n read from *d_n, at most n_cap).
+__global__ __launch_bounds__(kOne) void k_select_single(const int32_t* __restrict__ rows, const int64_t* d_n,
+                                                        int64_t n_cap, int32_t m, const int32_t* __restrict__ counts,
+                                                        const uint8_t* __restrict__ flag, int32_t* __restrict__ out_rows,
+                                                        int32_t* __restrict__ out_counts, int64_t* __restrict__ d_out_n) {
+    __shared__ int64_t lds[kOne / 64 + 1];
+    const int64_t n = d_n ? min(*d_n, n_cap) : n_cap;
+    WriteSelected w{rows, m, counts, out_rows, out_counts};
+    int64_t carry = 0, tot;
+    for (int64_t base = 0; base < n; base += kOne) {
+        const int64_t i = base + threadIdx.x;
+        const int64_t v = i < n && flag[i] ? 1 : 0;
+        const int64_t ex = etm::block_exclusive_scan<kOne>(v, lds, &tot);
+        w(i, v, carry + ex);
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *d_out_n = carry;
+}
+
 int grid_for(int64_t n, int cap = 16384) {
     int64_t g = (n + 255) / 256;
     if (g < 1) g = 1;
@@ -354,7 +483,73 @@ int grid_for(int64_t n, int cap = 16384) {
 
 bool is_pow2(int64_t x) { return x > 0 && (x & (x - 1)) == 0; }
 
+__global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 }  // namespace
+
+namespace etm {
+namespace detail {
+
+hipError_t launch_groups_single(const int32_t* rows, const int64_t* d_n, int64_t n_cap, int32_t m, int32_t small_max,
+                                int64_t* group_start, int64_t* cum_pairs, int64_t* cum_tiles, int32_t* small_list,
+                                int64_t* totals, hipStream_t s) {
+    k_groups_single<<<1, kOne, 0, s>>>(rows, d_n, n_cap, m, small_max, group_start, cum_pairs, cum_tiles,
+                                       small_list, totals);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_single(const int32_t* rows, int32_t m, const int64_t* group_start, const int64_t* cum_pairs,
+                                 int64_t n_groups, int64_t n_cands, const uint8_t* status, const int32_t* counts,
+                                 int64_t min_count, int32_t* out_rows, int32_t* out_counts, int64_t* out_cand,
+                                 int64_t* d_out_n, hipStream_t s) {
+    Survives f{status, counts, min_count};
+    WriteSurvivor w{rows, m, group_start, cum_pairs, n_groups, counts, out_rows, out_counts, out_cand};
+    k_compact_single<<<1, kOne, 0, s>>>(f, w, n_cands, d_out_n);
+    return hipGetLastError();
+}
+
+hipError_t launch_select_single(const int32_t* rows, const int64_t* d_n, int64_t n_cap, int32_t m,
+                                const int32_t* counts, const uint8_t* flag, int32_t* out_rows, int32_t* out_counts,
+                                int64_t* d_out_n, hipStream_t s) {
+    k_select_single<<<1, kOne, 0, s>>>(rows, d_n, n_cap, m, counts, flag, out_rows, out_counts, d_out_n);
+    return hipGetLastError();
+}
+
+hipError_t launch_hash(const int32_t* rows, int64_t n_cap, const int64_t* d_n, int32_t m, int32_t* table,
+                       int64_t table_size, hipStream_t s) {
+    k_fill_table<<<grid_for(table_size), 256, 0, s>>>(table, table_size);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || n_cap == 0) return e;
+    k_hash_insert<<<grid_for(n_cap), 256, 0, s>>>(rows, n_cap, m, table, table_size - 1, d_n);
+    return hipGetLastError();
+}
+
+hipError_t launch_free(const int32_t* rows, int64_t n_cap, const int64_t* d_n, int32_t k, const int32_t* counts,
+                       const int32_t* prev_rows, const int32_t* prev_counts, const int32_t* prev_table,
+                       int64_t table_size, uint8_t* free_out, hipStream_t s) {
+    if (n_cap == 0) return hipSuccess;
+    k_free<<<grid_for(n_cap), 256, 0, s>>>(rows, n_cap, k, counts, prev_rows, prev_counts, prev_table,
+                                           table_size - 1, free_out, d_n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    k_fill_u8<<<grid_for(n), 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_i64(int64_t* p, int n, hipStream_t s) {
+    k_zero_i64<<<1, 64, 0, s>>>(p, n);
+    return hipGetLastError();
+}
+
+}  // namespace detail
+}  // namespace etm
+
 
 extern "C" size_t etm_level_groups_ws(int64_t n) { return etm::scan_ws_bytes(n + 1); }
 

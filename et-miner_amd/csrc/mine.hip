@@ -1,0 +1,502 @@
+// The native level loop: mines every frequent itemset of device bitvectors (or of a device CSR, through
+// the input layer) in one call, with every level resident in HBM.
+//
+// Per level the host enqueues the group build, the subset test, the counting kernels, the compaction, the
+// free flags and the NEXT level's group build, then synchronizes once to read one block of scalars
+// (survivors, counted/inferred/skipped tallies, the next level's group totals). Levels of up to
+// kSingleMax rows (or candidates) take single-workgroup group-build and compaction kernels that read their
+// sizes on the device, so nothing between two syncs waits on the host. Larger levels fall back to the
+// multi-block scans with one extra sync. Device memory comes from the stream-ordered pool
+// (hipMallocAsync / hipFreeAsync), released when the call returns.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "internal.h"
+
+struct etm_levels {
+    std::vector<int32_t> k;
+    std::vector<std::vector<int32_t>> rows;
+    std::vector<std::vector<int32_t>> counts;
+    std::vector<etm_level_stat> stats;
+    std::vector<int64_t> col_to_item;
+    std::vector<int64_t> col_counts;
+};
+
+namespace {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(ETM_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void check_rc(int rc, const char* what) {
+    if (rc != ETM_OK) {
+        char buf[1024];
+        etm_last_error(buf, sizeof buf);
+        throw Error(rc, std::string(what) + ": " + buf);
+    }
+}
+
+// Whether the current device serves stream-ordered allocations; when it does, its default pool keeps
+// freed memory (release threshold = max) so repeated calls do not return to the driver.
+bool pools_ready() {
+    static thread_local int cached_dev = -1;
+    static thread_local bool cached = false;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    if (dev == cached_dev) return cached;
+    int ok = 0;
+    if (hipDeviceGetAttribute(&ok, hipDeviceAttributeMemoryPoolsSupported, dev) != hipSuccess) ok = 0;
+    if (ok) {
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t thr = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        } else {
+            ok = 0;
+        }
+    }
+    (void)hipGetLastError();
+    cached_dev = dev;
+    cached = ok != 0;
+    return cached;
+}
+
+// Device allocations of one call, released together (stream-ordered when the device has memory pools).
+class Arena {
+   public:
+    explicit Arena(hipStream_t s) : s_(s), async_(pools_ready()) {}
+    ~Arena() {
+        if (!async_ && !ptrs_.empty()) (void)hipStreamSynchronize(s_);
+        for (void* p : ptrs_) (void)(async_ ? hipFreeAsync(p, s_) : hipFree(p));
+    }
+    template <class T>
+    T* get(int64_t n) {
+        void* p = nullptr;
+        const size_t bytes = std::max<size_t>((size_t)std::max<int64_t>(n, 1) * sizeof(T), 256);
+        hipError_t e = async_ ? hipMallocAsync(&p, bytes, s_) : hipMalloc(&p, bytes);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            throw Error(ETM_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+        }
+        check(e, "device allocation");
+        ptrs_.push_back(p);
+        return (T*)p;
+    }
+
+   private:
+    hipStream_t s_;
+    bool async_;
+    std::vector<void*> ptrs_;
+};
+
+// Pinned host scalars for the per-level readback (one buffer per thread, kept for the process).
+struct Pinned {
+    int64_t* p = nullptr;
+    Pinned() {
+        static thread_local int64_t* buf = nullptr;
+        if (!buf) check(hipHostMalloc((void**)&buf, 64 * sizeof(int64_t), hipHostMallocDefault), "hipHostMalloc");
+        p = buf;
+    }
+};
+
+int64_t pow2_at_least(int64_t x) {
+    int64_t v = 1;
+    while (v < x) v <<= 1;
+    return v;
+}
+
+struct Level {
+    int32_t* rows = nullptr;
+    int32_t* counts = nullptr;
+    uint8_t* free = nullptr;
+    int32_t* table = nullptr;
+    int64_t table_size = 0;
+    int64_t n = 0;
+    int32_t m = 0;
+};
+
+// Device scalars of one level: [0] survivors, [1..3] tally, [4] selected (free-set prune), [8..23] totals of the
+// next level's groups.
+constexpr int kScal = 32;
+
+struct GroupsDev {
+    int64_t *gs, *cp, *ct;
+    int32_t* list;
+};
+
+GroupsDev alloc_groups(Arena& a, int64_t n) {
+    return GroupsDev{a.get<int64_t>(n + 1), a.get<int64_t>(n + 1), a.get<int64_t>(n + 1), a.get<int32_t>(n)};
+}
+
+// Group build of a level (single workgroup when small and d_n given; else the multi-block scans, which need n).
+void build_groups(Arena& a, const Level& lv, const int64_t* d_n, int64_t n_cap, GroupsDev& g, int64_t* d_totals,
+                  hipStream_t s) {
+    if (n_cap <= etm::detail::kSingleMax) {
+        check(etm::detail::launch_groups_single(lv.rows, d_n, n_cap, lv.m, 16, g.gs, g.cp, g.ct, g.list, d_totals, s),
+              "group build");
+        return;
+    }
+    const size_t ws_n = etm_level_groups_ws(n_cap);
+    int64_t* ws = a.get<int64_t>((int64_t)(ws_n + 7) / 8);
+    check_rc(etm_level_groups(lv.rows, n_cap, lv.m, 16, g.gs, g.cp, g.ct, g.list, d_totals, ws, ws_n, s),
+             "etm_level_groups");
+}
+
+void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n_rows, const int64_t* col_counts,
+                 int32_t n_cols, const etm_mine_opts& o, etm_levels& out, hipStream_t s) {
+    if (n_rows > 0x7FFFFFFF) throw Error(ETM_EOVERFLOW, "n_rows exceeds int32: the count arrays are int32");
+    const int64_t mc = std::max<int64_t>(1, o.min_count);
+    const bool prune = o.prune_non_free != 0;
+    const bool track_free = prune || o.infer_counts != 0;
+    const int64_t max_len = o.max_length > 0 ? std::min<int64_t>(o.max_length, n_cols) : n_cols;
+
+    // K = 1.
+    std::vector<int32_t> k1_rows;
+    std::vector<int32_t> k1_counts;
+    for (int32_t c = 0; c < n_cols; ++c) {
+        if (col_counts[c] < mc) continue;
+        if (prune && col_counts[c] >= n_rows) continue;
+        k1_rows.push_back(c);
+        k1_counts.push_back((int32_t)col_counts[c]);
+    }
+    out.k.push_back(1);
+    out.rows.push_back(k1_rows);
+    out.counts.push_back(k1_counts);
+    etm_level_stat st1{};
+    st1.k = 1;
+    st1.n_candidates = n_cols;
+    st1.n_counted = n_cols;
+    st1.n_frequent = (int64_t)k1_rows.size();
+    st1.bytes_alg = (int64_t)n_cols * n_words * 8;
+    out.stats.push_back(st1);
+    if (k1_rows.empty() || max_len < 2) return;
+
+    Arena a(s);
+    Pinned host;
+    const int64_t n1 = (int64_t)k1_rows.size();
+    Level prev;
+    prev.m = 1;
+    prev.n = n1;
+    prev.rows = a.get<int32_t>(n1);
+    prev.counts = a.get<int32_t>(n1);
+    check(hipMemcpyAsync(prev.rows, k1_rows.data(), n1 * 4, hipMemcpyHostToDevice, s), "upload K=1");
+    check(hipMemcpyAsync(prev.counts, k1_counts.data(), n1 * 4, hipMemcpyHostToDevice, s), "upload K=1 counts");
+    if (track_free) {
+        std::vector<uint8_t> f(n1);
+        for (int64_t i = 0; i < n1; ++i) f[i] = k1_counts[i] < n_rows ? 1 : 0;
+        prev.free = a.get<uint8_t>(n1);
+        check(hipMemcpyAsync(prev.free, f.data(), n1, hipMemcpyHostToDevice, s), "upload K=1 free");
+    }
+    GroupsDev g = alloc_groups(a, n1);
+    int64_t* scal = a.get<int64_t>(kScal);
+    build_groups(a, prev, nullptr, n1, g, scal + 8, s);
+    check(hipMemcpyAsync(host.p, scal, kScal * 8, hipMemcpyDeviceToHost, s), "readback");
+    check(hipStreamSynchronize(s), "sync");
+    int64_t tot[16];
+    std::memcpy(tot, host.p + 8, sizeof tot);
+
+    struct Pending {
+        int32_t* rows;
+        int32_t* counts;
+        int64_t n;
+        int32_t k;
+    };
+    std::vector<Pending> levels;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+
+    for (int32_t k = 2; k <= max_len && prev.n >= k; ++k) {
+        const int32_t m = k - 1;
+        const int64_t n_groups = tot[0], n_cands = tot[1], n_tiles = tot[2];
+        if (n_cands == 0) break;
+        etm_level_stat st{};
+        st.k = k;
+        st.n_candidates = n_cands;
+        st.bytes_alg = ((int64_t)(m - 1) * tot[5] + tot[6]) * n_words * 8 + 4 * n_cands;
+        // Hash of the previous level (subset test from K=3, free flags from K=2).
+        if ((k >= 3 || track_free) && prev.table == nullptr) {
+            prev.table_size = pow2_at_least(2 * std::max<int64_t>(1, prev.n));
+            prev.table = a.get<int32_t>(prev.table_size);
+            check(etm::detail::launch_hash(prev.rows, prev.n, nullptr, prev.m, prev.table, prev.table_size, s), "hash");
+        }
+        int64_t* cur_scal = a.get<int64_t>(kScal);
+        check(etm::detail::launch_zero_i64(cur_scal, kScal, s), "zero");
+        uint8_t* status = nullptr;
+        int32_t* counts = a.get<int32_t>(n_cands);
+        if (k >= 3) {
+            status = a.get<uint8_t>(n_cands);
+            const bool infer = o.infer_counts && !prune && prev.free != nullptr;
+            const int32_t mode = ETM_SUBSET_PRUNE | (infer ? ETM_SUBSET_INFER : 0);
+            check_rc(etm_level_classify(prev.rows, prev.n, m, prev.counts, infer ? prev.free : nullptr, prev.table,
+                                        prev.table_size, g.gs, g.cp, n_groups, n_cands, mode, status, counts,
+                                        cur_scal + 1, s),
+                     "etm_level_classify");
+        } else {
+            check(hipMemsetAsync(counts, 0, n_cands * 4, s), "zero counts");
+        }
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (o.time_kernels) {
+            check(hipEventCreate(&e0), "event");
+            check(hipEventCreate(&e1), "event");
+            check(hipEventRecord(e0, s), "event record");
+        }
+        const int64_t cls[4] = {tot[8], tot[9], tot[10], tot[11]};
+        check_rc(etm_count_level(d_bv, pitch, n_words, prev.rows, m, g.gs, g.cp, g.ct, n_groups, n_tiles, g.list, cls,
+                                 status, counts, ETM_LAYOUT_LEX, 0, s),
+                 "etm_count_level");
+        if (o.time_kernels) {
+            check(hipEventRecord(e1, s), "event record");
+            events.emplace_back(e0, e1);
+        }
+        // Survivors, in lexicographic order.
+        Level cur;
+        cur.m = k;
+        int64_t cap;
+        if (n_cands <= etm::detail::kSingleMax) {
+            cap = n_cands;
+            cur.rows = a.get<int32_t>(cap * k);
+            cur.counts = a.get<int32_t>(cap);
+            check(etm::detail::launch_compact_single(prev.rows, m, g.gs, g.cp, n_groups, n_cands, status, counts, mc,
+                                                     cur.rows, cur.counts, nullptr, cur_scal + 0, s),
+                  "compaction");
+        } else {
+            const size_t ws_n = etm_level_compact_ws(n_cands);
+            int64_t* ws = a.get<int64_t>((int64_t)(ws_n + 7) / 8);
+            check_rc(etm_level_compact(prev.rows, m, g.gs, g.cp, n_groups, n_cands, status, counts, mc, nullptr,
+                                       nullptr, nullptr, cur_scal + 0, ws, ws_n, s),
+                     "etm_level_compact");
+            check(hipMemcpyAsync(host.p, cur_scal, 8, hipMemcpyDeviceToHost, s), "readback");
+            check(hipStreamSynchronize(s), "sync");
+            cap = host.p[0];
+            cur.rows = a.get<int32_t>(cap * k);
+            cur.counts = a.get<int32_t>(cap);
+            if (cap)
+                check_rc(etm_level_compact(prev.rows, m, g.gs, g.cp, n_groups, n_cands, status, counts, mc, cur.rows,
+                                           cur.counts, nullptr, cur_scal + 0, ws, ws_n, s),
+                         "etm_level_compact");
+        }
+        const int64_t* d_n = cur_scal + 0;
+        if (track_free) {
+            cur.free = a.get<uint8_t>(cap);
+            check(etm::detail::launch_free(cur.rows, cap, d_n, k, cur.counts, prev.rows, prev.counts, prev.table,
+                                           prev.table_size, cur.free, s),
+                  "free flags");
+            if (prune) {
+                Level sel;
+                sel.m = k;
+                sel.rows = a.get<int32_t>(cap * k);
+                sel.counts = a.get<int32_t>(cap);
+                if (cap <= etm::detail::kSingleMax) {
+                    check(etm::detail::launch_select_single(cur.rows, d_n, cap, k, cur.counts, cur.free, sel.rows,
+                                                            sel.counts, cur_scal + 4, s),
+                          "select");
+                } else {
+                    check(hipMemcpyAsync(host.p, cur_scal, 8, hipMemcpyDeviceToHost, s), "readback");
+                    check(hipStreamSynchronize(s), "sync");
+                    const int64_t nn = host.p[0];
+                    const size_t ws_n = etm_level_compact_ws(nn);
+                    int64_t* ws = a.get<int64_t>((int64_t)(ws_n + 7) / 8);
+                    check_rc(etm_select_rows(cur.rows, nn, k, cur.counts, cur.free, nullptr, nullptr, cur_scal + 4, ws,
+                                             ws_n, s),
+                             "etm_select_rows");
+                    check_rc(etm_select_rows(cur.rows, nn, k, cur.counts, cur.free, sel.rows, sel.counts,
+                                             cur_scal + 4, ws, ws_n, s),
+                             "etm_select_rows");
+                }
+                sel.free = a.get<uint8_t>(cap);
+                check(etm::detail::launch_fill_u8(sel.free, cap, 1, s), "fill");
+                cur.rows = sel.rows;
+                cur.counts = sel.counts;
+                cur.free = sel.free;
+                d_n = cur_scal + 4;
+            }
+        }
+        // The next level's groups, sized on the device when the level is small.
+        GroupsDev gn{};
+        const bool early = cap <= etm::detail::kSingleMax && k + 1 <= max_len;
+        if (early) {
+            gn = alloc_groups(a, cap);
+            check(etm::detail::launch_groups_single(cur.rows, d_n, cap, k, 16, gn.gs, gn.cp, gn.ct, gn.list,
+                                                    cur_scal + 8, s),
+                  "group build");
+        }
+        check(hipMemcpyAsync(host.p, cur_scal, kScal * 8, hipMemcpyDeviceToHost, s), "readback");
+        check(hipStreamSynchronize(s), "sync");
+        cur.n = host.p[prune ? 4 : 0];
+        st.n_counted = k >= 3 ? host.p[1] : n_cands;
+        st.n_inferred = k >= 3 ? host.p[2] : 0;
+        st.n_skipped = k >= 3 ? host.p[3] : 0;
+        st.n_frequent = cur.n;
+        out.stats.push_back(st);
+        levels.push_back(Pending{cur.rows, cur.counts, cur.n, k});
+        if (cur.n == 0) break;
+        if (early) {
+            std::memcpy(tot, host.p + 8, sizeof tot);
+            g = gn;
+        } else if (k + 1 <= max_len) {
+            g = alloc_groups(a, cur.n);
+            build_groups(a, cur, nullptr, cur.n, g, cur_scal + 8, s);
+            check(hipMemcpyAsync(host.p, cur_scal, kScal * 8, hipMemcpyDeviceToHost, s), "readback");
+            check(hipStreamSynchronize(s), "sync");
+            std::memcpy(tot, host.p + 8, sizeof tot);
+        }
+        prev = cur;
+    }
+    // Every level back to the host in one pass.
+    for (const Pending& p : levels) {
+        out.k.push_back(p.k);
+        out.rows.emplace_back((size_t)(p.n * p.k));
+        out.counts.emplace_back((size_t)p.n);
+        if (p.n) {
+            check(hipMemcpyAsync(out.rows.back().data(), p.rows, p.n * p.k * 4, hipMemcpyDeviceToHost, s), "copy out");
+            check(hipMemcpyAsync(out.counts.back().data(), p.counts, p.n * 4, hipMemcpyDeviceToHost, s), "copy out");
+        }
+    }
+    check(hipStreamSynchronize(s), "sync");
+    for (size_t i = 0; i < events.size(); ++i) {
+        float ms = 0;
+        check(hipEventElapsedTime(&ms, events[i].first, events[i].second), "event time");
+        out.stats[i + 1].count_ms = ms;
+        (void)hipEventDestroy(events[i].first);
+        (void)hipEventDestroy(events[i].second);
+    }
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return ETM_OK;
+    } catch (const Error& e) {
+        etm::set_error("%s", e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        etm::set_error("host allocation failed");
+        return ETM_ENOMEM;
+    } catch (const std::exception& e) {
+        etm::set_error("%s", e.what());
+        return ETM_EHIP;
+    }
+}
+
+}  // namespace
+
+extern "C" int etm_mine_bitvecs(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n_rows,
+                                const int64_t* h_col_counts, int32_t n_cols, const etm_mine_opts* opts,
+                                etm_levels** out, void* stream) {
+    ETM_ARG(d_bv && opts && out && (n_cols == 0 || h_col_counts), "etm_mine_bitvecs: null pointer");
+    ETM_ARG(pitch >= n_words && n_words == (n_rows + 63) / 64, "etm_mine_bitvecs: bad pitch / n_words");
+    *out = nullptr;
+    auto res = std::make_unique<etm_levels>();
+    res->col_counts.assign(h_col_counts, h_col_counts + n_cols);
+    const int rc = guarded([&] {
+        mine_levels(d_bv, pitch, n_words, n_rows, h_col_counts, n_cols, *opts, *res, ETM_STREAM(stream));
+    });
+    if (rc == ETM_OK) *out = res.release();
+    return rc;
+}
+
+extern "C" int etm_mine_csr(const int64_t* d_indptr, const void* d_ids, int id_bytes, int64_t n_rows, int64_t nnz,
+                            int64_t span_limit, const etm_mine_opts* opts, etm_levels** out, void* stream) {
+    ETM_ARG(d_indptr && opts && out && (nnz == 0 || d_ids), "etm_mine_csr: null pointer");
+    ETM_ARG(id_bytes == 4 || id_bytes == 8, "etm_mine_csr: id_bytes must be 4 or 8");
+    ETM_ARG(n_rows >= 0 && nnz >= 0, "etm_mine_csr: negative size");
+    *out = nullptr;
+    auto res = std::make_unique<etm_levels>();
+    hipStream_t s = ETM_STREAM(stream);
+    const int rc = guarded([&] {
+        if (nnz == 0 || n_rows == 0) return;
+        const int64_t mc = std::max<int64_t>(1, opts->min_count);
+        Arena a(s);
+        Pinned host;
+        int64_t* mm = a.get<int64_t>(2);
+        check_rc(etm_ids_minmax(d_ids, id_bytes, nnz, mm, s), "etm_ids_minmax");
+        check(hipMemcpyAsync(host.p, mm, 16, hipMemcpyDeviceToHost, s), "readback");
+        check(hipStreamSynchronize(s), "sync");
+        const int64_t lo = host.p[0], span = host.p[1] - host.p[0] + 1;
+        if (span > span_limit)
+            throw Error(ETM_EINVAL, "item ids span " + std::to_string(span) + " values: code them to dense ids first");
+        int32_t* hist = a.get<int32_t>(span);
+        check_rc(etm_ids_hist(d_ids, id_bytes, nnz, lo, span, hist, s), "etm_ids_hist");
+        std::vector<int32_t> h_hist((size_t)span);
+        check(hipMemcpyAsync(h_hist.data(), hist, span * 4, hipMemcpyDeviceToHost, s), "readback");
+        check(hipStreamSynchronize(s), "sync");
+        std::vector<int32_t> lut((size_t)span, -1);
+        for (int64_t v = 0; v < span; ++v)
+            if (h_hist[v] >= mc) {
+                lut[v] = (int32_t)res->col_to_item.size();
+                res->col_to_item.push_back(lo + v);
+            }
+        const int32_t n_cols = (int32_t)res->col_to_item.size();
+        if (n_cols == 0) return;
+        int32_t* d_lut = a.get<int32_t>(span);
+        check(hipMemcpyAsync(d_lut, lut.data(), span * 4, hipMemcpyHostToDevice, s), "upload lut");
+        const int64_t n_words = (n_rows + 63) / 64;
+        const int64_t pitch = std::max<int64_t>(64, (n_words + 63) / 64 * 64);
+        uint64_t* bv = a.get<uint64_t>((int64_t)n_cols * pitch);
+        check_rc(etm_bitvec_build(d_indptr, d_ids, id_bytes, n_rows, lo, span, d_lut, n_cols, pitch, bv, s),
+                 "etm_bitvec_build");
+        int64_t* pc = a.get<int64_t>(n_cols);
+        check_rc(etm_column_popcounts(bv, n_cols, pitch, n_words, pc, s), "etm_column_popcounts");
+        res->col_counts.resize(n_cols);
+        check(hipMemcpyAsync(res->col_counts.data(), pc, n_cols * 8, hipMemcpyDeviceToHost, s), "readback");
+        check(hipStreamSynchronize(s), "sync");
+        mine_levels(bv, pitch, n_words, n_rows, res->col_counts.data(), n_cols, *opts, *res, s);
+    });
+    if (rc == ETM_OK) *out = res.release();
+    return rc;
+}
+
+extern "C" int etm_levels_count(const etm_levels* lv, int32_t* h_n_levels) {
+    ETM_ARG(lv && h_n_levels, "etm_levels_count: null pointer");
+    *h_n_levels = (int32_t)lv->k.size();
+    return ETM_OK;
+}
+
+extern "C" int etm_levels_shape(const etm_levels* lv, int32_t level, int32_t* h_k, int64_t* h_n) {
+    ETM_ARG(lv && h_k && h_n, "etm_levels_shape: null pointer");
+    ETM_ARG(level >= 0 && level < (int32_t)lv->k.size(), "etm_levels_shape: level %d out of range", level);
+    *h_k = lv->k[level];
+    *h_n = (int64_t)lv->counts[level].size();
+    return ETM_OK;
+}
+
+extern "C" int etm_levels_copy(const etm_levels* lv, int32_t level, int32_t* h_rows, int64_t* h_counts) {
+    ETM_ARG(lv, "etm_levels_copy: null result");
+    ETM_ARG(level >= 0 && level < (int32_t)lv->k.size(), "etm_levels_copy: level %d out of range", level);
+    if (h_rows) std::memcpy(h_rows, lv->rows[level].data(), lv->rows[level].size() * 4);
+    if (h_counts)
+        for (size_t i = 0; i < lv->counts[level].size(); ++i) h_counts[i] = lv->counts[level][i];
+    return ETM_OK;
+}
+
+extern "C" int etm_levels_columns(const etm_levels* lv, int32_t* h_n_cols, int64_t* h_col_to_item,
+                                  int64_t* h_col_counts) {
+    ETM_ARG(lv && h_n_cols, "etm_levels_columns: null pointer");
+    *h_n_cols = (int32_t)lv->col_counts.size();
+    if (h_col_to_item && !lv->col_to_item.empty())
+        std::memcpy(h_col_to_item, lv->col_to_item.data(), lv->col_to_item.size() * 8);
+    if (h_col_counts) std::memcpy(h_col_counts, lv->col_counts.data(), lv->col_counts.size() * 8);
+    return ETM_OK;
+}
+
+extern "C" int etm_levels_stats(const etm_levels* lv, int32_t level, etm_level_stat* h_out) {
+    ETM_ARG(lv && h_out, "etm_levels_stats: null pointer");
+    ETM_ARG(level >= 0 && level < (int32_t)lv->stats.size(), "etm_levels_stats: level %d out of range", level);
+    *h_out = lv->stats[level];
+    return ETM_OK;
+}
+
+extern "C" int etm_levels_free(etm_levels* lv) {
+    delete lv;
+    return ETM_OK;
+}

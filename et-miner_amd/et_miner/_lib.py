@@ -46,7 +46,30 @@ SIGNATURES: dict = {
     "etm_level_free": (c_int, [P, c_int64, c_int32, P, P, P, P, c_int64, P, P]),
     "etm_select_rows": (c_int, [P, c_int64, c_int32, P, P, P, P, P, P, c_size_t, P]),
     "etm_count_itemsets": (c_int, [P, c_int64, c_int64, P, P, c_int64, P, c_int32, P]),
+    "etm_mine_bitvecs": (c_int, [P, c_int64, c_int64, c_int64, P, c_int32, P, P, P]),
+    "etm_mine_csr": (c_int, [P, P, c_int, c_int64, c_int64, c_int64, P, P, P]),
+    "etm_levels_count": (c_int, [P, P]),
+    "etm_levels_shape": (c_int, [P, c_int32, P, P]),
+    "etm_levels_copy": (c_int, [P, c_int32, P, P]),
+    "etm_levels_columns": (c_int, [P, P, P, P]),
+    "etm_levels_stats": (c_int, [P, c_int32, P]),
+    "etm_levels_free": (c_int, [P]),
 }
+
+
+class MineOpts(ctypes.Structure):
+    """etm_mine_opts."""
+
+    _fields_ = [("min_count", c_int64), ("max_length", c_int32), ("prune_non_free", c_int32),
+                ("infer_counts", c_int32), ("time_kernels", c_int32)]
+
+
+class LevelStat(ctypes.Structure):
+    """etm_level_stat."""
+
+    _fields_ = [("k", c_int32), ("pad", c_int32), ("n_candidates", c_int64), ("n_counted", c_int64),
+                ("n_inferred", c_int64), ("n_skipped", c_int64), ("n_frequent", c_int64), ("bytes_alg", c_int64),
+                ("count_ms", ctypes.c_float), ("pad2", ctypes.c_float)]
 
 _lock = threading.Lock()
 _lib = None

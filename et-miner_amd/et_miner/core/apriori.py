@@ -244,13 +244,14 @@ def apriori(
 
 def mine_csr(indptr, ids, min_support, max_length=None, prune_equal_support=False, use_generator_pruning=False,
              level_callback=None, session=None, items=None, warn_complexity=False, device_csr=None,
-             with_counts=False, timer=None, stats_out=None):
+             with_counts=False, timer=None, stats_out=None, engine: str = "native", time_kernels=False):
     """The use_gpu route from a host CSR (or ``device_csr=(indptr_d, ids_d)`` already in HBM).
 
-    ``timer`` is handed to the level loop; ``stats_out`` (a list) receives its per-level statistics."""
+    ``engine="native"`` runs the whole level loop inside libetm (one C call); ``engine="python"`` drives the
+    same kernels level by level from Python through the operator wrappers (``timer`` hooks apply there).
+    ``stats_out`` (a list) receives the per-level statistics.
+    """
     from ..gpu import device as dev
-    from ..gpu.input_layer import build_device_input, host_csr_to_device
-    from ..gpu.row_split import mine_levels
 
     dev.require_gpu()
     n_rows = (len(indptr) - 1) if device_csr is None else int(device_csr[0].numel()) - 1
@@ -259,21 +260,49 @@ def mine_csr(indptr, ids, min_support, max_length=None, prune_equal_support=Fals
     min_count = _mining_min_count(min_support, n_rows)
     if session:
         session.start_phase("input_layer")
-    indptr_d, ids_d = host_csr_to_device(indptr, ids) if device_csr is None else device_csr
-    built = build_device_input(indptr_d, ids_d, n_rows, min_count)
-    if session:
-        session.end_phase(n_columns=0 if built is None else len(built.col_to_item))
-    if built is None:
-        return itemset_table([], np.zeros(0, np.int64), n_rows, with_counts=True) if with_counts else empty_result()
-    if warn_complexity:
-        _warn_complexity(int((built.col_counts >= min_count).sum()), min_support)
-    res = mine_levels(built.bitvecs, built.n_words, n_rows, built.col_counts, min_count, max_length,
-                      prune_equal_support, use_generator_pruning, _timed(level_callback, session), timer=timer)
+    if device_csr is None:
+        from ..gpu.input_layer import host_csr_to_device
+
+        device_csr = host_csr_to_device(indptr, ids)
+    indptr_d, ids_d = device_csr
+    if engine == "native":
+        from ..gpu.native import mine_csr_device
+
+        res = mine_csr_device(indptr_d, ids_d, n_rows, min_count, max_length, prune_equal_support,
+                              use_generator_pruning, time_kernels=time_kernels)
+        if session:
+            session.end_phase(n_columns=len(res.col_to_item))
+        stats, levels, col_to_item = res.stats, res.levels, res.col_to_item
+        if warn_complexity and levels:
+            _warn_complexity(len(levels[0][0]), min_support)
+        cb = _timed(level_callback, session)
+        if cb:
+            for st in stats:
+                cb(st.k, st.n_candidates, st.n_frequent, st.count_ms)
+    else:
+        from ..gpu.input_layer import build_device_input
+        from ..gpu.row_split import mine_levels
+
+        built = build_device_input(indptr_d, ids_d, n_rows, min_count)
+        if session:
+            session.end_phase(n_columns=0 if built is None else len(built.col_to_item))
+        if built is None:
+            levels, stats, col_to_item = [], [], np.zeros(0, np.int64)
+        else:
+            if warn_complexity:
+                _warn_complexity(int((built.col_counts >= min_count).sum()), min_support)
+            res = mine_levels(built.bitvecs, built.n_words, n_rows, built.col_counts, min_count, max_length,
+                              prune_equal_support, use_generator_pruning, _timed(level_callback, session),
+                              timer=timer)
+            levels, stats, col_to_item = res.levels, res.stats, built.col_to_item
     if stats_out is not None:
-        stats_out.extend(res.stats)
+        stats_out.extend(stats)
+    levels = [(r, c) for r, c in levels if len(r)]
     if with_counts:
-        return itemset_table(res.levels, built.col_to_item, n_rows, with_counts=True)
-    return _frame(res.levels, built.col_to_item, n_rows, items)
+        return itemset_table(levels, col_to_item, n_rows, with_counts=True)
+    if not levels:
+        return empty_result()
+    return _frame(levels, col_to_item, n_rows, items)
 
 
 def _timed(level_callback, session):
@@ -294,7 +323,6 @@ def _mine_bitvecs(bitvecs, min_support, max_length, prune_equal_support, use_gen
     """The bitvecs= route: validation as the reference (``core/apriori.py:477-527``), then the level loop."""
     from ..gpu import device as dev
     from ..gpu import kernels as K
-    from ..gpu.row_split import mine_levels
 
     if not isinstance(bitvecs, tuple) or len(bitvecs) != 3:
         raise ValueError("bitvecs must be a tuple of (bitvecs_gpu, col_to_item, n_transactions)")
@@ -325,9 +353,15 @@ def _mine_bitvecs(bitvecs, min_support, max_length, prune_equal_support, use_gen
     counts = dev.to_host(K.column_popcounts(view, arr.shape[1])).astype(np.int64)
     if warn_complexity:
         _warn_complexity(int((counts >= max(1, min_count)).sum()), min_support)
-    res = mine_levels(view, arr.shape[1], n_trans, counts, max(1, min_count), max_length, prune_equal_support,
-                      use_generator_pruning, _timed(level_callback, session))
-    levels = res.levels
+    from ..gpu.native import mine_bitvecs_device
+
+    res = mine_bitvecs_device(arr.ptr, arr.shape[1], arr.shape[1], n_trans, counts, max(1, min_count), max_length,
+                              prune_equal_support, use_generator_pruning)
+    cb = _timed(level_callback, session)
+    if cb:
+        for st in res.stats:
+            cb(st.k, st.n_candidates, st.n_frequent, st.count_ms)
+    levels = [(r, c) for r, c in res.levels if len(r)]
     if not order_ok:
         # Item ids that do not ascend with the columns: map each level, then sort every itemset's items.
         levels = [(np.sort(mapping[rows], axis=1), cnt) for rows, cnt in levels]

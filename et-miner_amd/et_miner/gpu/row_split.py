@@ -43,6 +43,7 @@ class LevelStats:
     n_skipped: int = 0
     bytes_alg: int = 0
     ms: float = 0.0
+    count_ms: float = 0.0  # event-timed duration of the level's counting launches (native loop, when timed)
 
     @property
     def n_evaluated(self) -> int:

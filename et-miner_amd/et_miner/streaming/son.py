@@ -99,17 +99,14 @@ class _Uploader:
 
 
 def _local_levels(d_ptr, d_ids, n_rows: int, local_support: float, max_length, infer: bool):
-    """One chunk mined at the local threshold: (its column items, levels of (column rows, counts))."""
-    from ..gpu.input_layer import build_device_input
-    from ..gpu.row_split import mine_levels
+    """One chunk mined at the local threshold on the native level loop: (its column items, levels)."""
+    from ..gpu.native import mine_csr_device
 
-    mc = _mining_min_count(local_support, n_rows)
-    built = build_device_input(d_ptr, d_ids, n_rows, mc)
-    if built is None:
+    res = mine_csr_device(d_ptr, d_ids, n_rows, _mining_min_count(local_support, n_rows), max_length,
+                          infer_counts=infer)
+    if not res.levels or len(res.col_to_item) == 0:
         return None
-    res = mine_levels(built.bitvecs, built.n_words, n_rows, built.col_counts, mc, max_length,
-                      infer_counts=infer)
-    return built.col_to_item, res.levels
+    return res.col_to_item, res.levels
 
 
 def _sum_rows(rows: np.ndarray, weights: list) -> tuple:

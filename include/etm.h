@@ -175,6 +175,60 @@ int etm_select_rows(const int32_t* d_rows, int64_t n, int32_t m, const int32_t* 
 int etm_count_itemsets(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const int32_t* d_items,
                        const int64_t* d_offsets, int64_t n_sets, int64_t* d_out, int32_t accumulate, void* stream);
 
+/* ---- The native level loop --------------------------------------------------------------------------- */
+
+/* Options of a mining call. min_count is the absolute threshold (>= 1 is enforced); max_length 0 = none. */
+typedef struct etm_mine_opts {
+    int64_t min_count;
+    int32_t max_length;
+    int32_t prune_non_free; /* free-set mode (prune_equal_support) */
+    int32_t infer_counts;   /* Pascal inference (use_generator_pruning) */
+    int32_t time_kernels;   /* record HIP events around each level's counting launches */
+} etm_mine_opts;
+
+/* Per-level statistics: candidates generated by the prefix join and, of them, counted / inferred / skipped by
+ * the subset test; survivors; the prefix-group decomposition's compulsory bytes; the counting launches'
+ * event-timed duration (when time_kernels). */
+typedef struct etm_level_stat {
+    int32_t k;
+    int32_t pad;
+    int64_t n_candidates;
+    int64_t n_counted;
+    int64_t n_inferred;
+    int64_t n_skipped;
+    int64_t n_frequent;
+    int64_t bytes_alg;
+    float count_ms;
+    float pad2;
+} etm_level_stat;
+
+/* Host-side result of a mining call: levels of (rows of column ids int32 (n, k), counts), the columns. */
+typedef struct etm_levels etm_levels;
+
+/* Mines every frequent itemset (free-set with prune_non_free) of device bitvectors, from K=1 (the columns'
+ * counts h_col_counts) to exhaustion, every level resident in HBM; one host sync per level. The caller's
+ * bitvectors are only read. Replaces _apriori_row_split_multi_gpu's level loop (gpu/row_split.py:866-1185)
+ * on one GPU. Free the result with etm_levels_free. */
+int etm_mine_bitvecs(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n_rows, const int64_t* h_col_counts,
+                     int32_t n_cols, const etm_mine_opts* opts, etm_levels** out, void* stream);
+
+/* The same from a device CSR of item ids (nnz entries), through the device input layer: id range,
+ * occurrence histogram (K=1 candidates: ids occurring >= min_count times), bitvectors, K=1 popcounts.
+ * Ids spanning more than span_limit values raise ETM_EINVAL. Replaces build_device_input
+ * (gpu/input_layer.py:469-544) followed by the level loop. */
+int etm_mine_csr(const int64_t* d_indptr, const void* d_ids, int id_bytes, int64_t n_rows, int64_t nnz,
+                 int64_t span_limit, const etm_mine_opts* opts, etm_levels** out, void* stream);
+
+/* Accessors of a result: number of levels; level shape; copy of a level (h_rows n*k int32, h_counts n int64,
+ * either may be NULL); the columns (count, item id of each column -- empty for etm_mine_bitvecs -- and K=1
+ * counts); per-level statistics; release. */
+int etm_levels_count(const etm_levels* lv, int32_t* h_n_levels);
+int etm_levels_shape(const etm_levels* lv, int32_t level, int32_t* h_k, int64_t* h_n);
+int etm_levels_copy(const etm_levels* lv, int32_t level, int32_t* h_rows, int64_t* h_counts);
+int etm_levels_columns(const etm_levels* lv, int32_t* h_n_cols, int64_t* h_col_to_item, int64_t* h_col_counts);
+int etm_levels_stats(const etm_levels* lv, int32_t level, etm_level_stat* h_out);
+int etm_levels_free(etm_levels* lv);
+
 #ifdef __cplusplus
 }
 #endif

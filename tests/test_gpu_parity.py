@@ -23,11 +23,14 @@ def _csr(rows):
     return indptr, ids
 
 
-def _gpu_mine(indptr, ids, s, ml=None, **kw):
+ENGINES = ["native", "python"]
+
+
+def _gpu_mine(indptr, ids, s, ml=None, engine="native", **kw):
     from et_miner.core.apriori import mine_csr
     from et_miner.core.result import as_dict
 
-    return as_dict(mine_csr(indptr, ids, s, ml, with_counts=True, **kw))
+    return as_dict(mine_csr(indptr, ids, s, ml, with_counts=True, engine=engine, **kw))
 
 
 def _rand_rows(rng, n, vocab, mean, zipf=1.1, dup=False):
@@ -135,24 +138,26 @@ def test_readme_and_boundary():
         assert got[s] == 700
 
 
+@pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("seed", range(8))
-def test_random_datasets_match_oracle(seed):
+def test_random_datasets_match_oracle(seed, engine):
     rng = np.random.default_rng(seed)
     rows = _rand_rows(rng, 2_000 + 1_000 * seed, 20 + 10 * seed, 3 + seed % 4, zipf=0.8 + 0.1 * (seed % 3),
                       dup=seed % 2 == 1)
     indptr, ids = _csr(rows)
     for s, ml in [(0.01, None), (0.03, 3)]:
         want = miner.mine(indptr, ids, s, ml).as_dict()
-        assert _gpu_mine(indptr, ids, s, ml) == want
-        assert _gpu_mine(indptr, ids, s, ml, use_generator_pruning=True) == want
+        assert _gpu_mine(indptr, ids, s, ml, engine) == want
+        assert _gpu_mine(indptr, ids, s, ml, engine, use_generator_pruning=True) == want
 
 
-def test_free_set_mode_matches_oracle():
+@pytest.mark.parametrize("engine", ENGINES)
+def test_free_set_mode_matches_oracle(engine):
     from et_miner.synthetic import PRESETS, generate_csr
 
     data = generate_csr(PRESETS["smoke"])
     want = miner.mine(data.indptr, data.indices, 0.01, prune_equal_support=True).as_dict()
-    assert _gpu_mine(data.indptr, data.indices, 0.01, prune_equal_support=True) == want
+    assert _gpu_mine(data.indptr, data.indices, 0.01, engine=engine, prune_equal_support=True) == want
     rng = np.random.default_rng(5)
     rows = [sorted(set(r) | {0}) for r in _rand_rows(rng, 3000, 12, 4)]
     for r in rows[:1000]:
@@ -160,31 +165,33 @@ def test_free_set_mode_matches_oracle():
         r.sort()
     indptr, ids = _csr(rows)
     want = miner.mine(indptr, ids, 0.02, prune_equal_support=True).as_dict()
-    assert _gpu_mine(indptr, ids, 0.02, prune_equal_support=True) == want
+    assert _gpu_mine(indptr, ids, 0.02, engine=engine, prune_equal_support=True) == want
 
 
+@pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("name", ["smoke-C1-base", "deepk-C1-base", "skew-C1-base"])
-def test_reference_lattices_on_gpu(name):
+def test_reference_lattices_on_gpu(name, engine):
     from et_miner.synthetic import PRESETS, generate_csr
 
     gold, params = load_lattice(name)
     data = generate_csr(PRESETS[params["dataset"]])
-    got = _gpu_mine(data.indptr, data.indices, params["min_support"], params["max_length"])
+    got = _gpu_mine(data.indptr, data.indices, params["min_support"], params["max_length"], engine)
     assert got == gold
-    got = _gpu_mine(data.indptr, data.indices, params["min_support"], params["max_length"],
+    got = _gpu_mine(data.indptr, data.indices, params["min_support"], params["max_length"], engine,
                     use_generator_pruning=True)
     assert got == gold
 
 
+@pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("name,n_rows", [("plain_60k", None), ("deep_k_1m", 200_000), ("wide_vocab_2m", 100_000),
                                          ("deep_sparse_20m", 400_000)])
-def test_workloads_match_oracle(name, n_rows):
+def test_workloads_match_oracle(name, n_rows, engine):
     from et_miner.synthetic import workload
 
     w = workload(name, n_rows)
     data = w.generate()
     want = miner.mine(data.indptr, data.indices, w.min_support, w.max_length).as_dict()
-    got = _gpu_mine(data.indptr, data.indices, w.min_support, w.max_length,
+    got = _gpu_mine(data.indptr, data.indices, w.min_support, w.max_length, engine,
                     use_generator_pruning=w.use_generator_pruning)
     assert got == want
 
@@ -241,10 +248,16 @@ def test_count_itemsets_flat_matches_oracle():
     assert got.tolist() == want
 
 
-def test_edge_cases():
-    assert _gpu_mine(*_csr([[5]]), 0.5) == {(5,): 1}
-    assert _gpu_mine(*_csr([[]] * 10), 0.1) == {}
+@pytest.mark.parametrize("engine", ENGINES)
+def test_edge_cases(engine):
+    assert _gpu_mine(*_csr([[5]]), 0.5, None, engine) == {(5,): 1}
+    assert _gpu_mine(*_csr([[]] * 10), 0.1, None, engine) == {}
     rows = [[1, 2, 3]] * 50
-    assert _gpu_mine(*_csr(rows), 1.0) == {(1,): 50, (2,): 50, (3,): 50, (1, 2): 50, (1, 3): 50, (2, 3): 50,
-                                           (1, 2, 3): 50}
-    assert _gpu_mine(*_csr(rows), 1.0, 1) == {(1,): 50, (2,): 50, (3,): 50}
+    assert _gpu_mine(*_csr(rows), 1.0, None, engine) == {(1,): 50, (2,): 50, (3,): 50, (1, 2): 50, (1, 3): 50,
+                                                         (2, 3): 50, (1, 2, 3): 50}
+    assert _gpu_mine(*_csr(rows), 1.0, 1, engine) == {(1,): 50, (2,): 50, (3,): 50}
+    # Levels larger than the single-workgroup fast path (> 32,768 rows / candidates) take the multi-block scans.
+    rng = np.random.default_rng(3)
+    big = [sorted(set(rng.choice(400, size=12, replace=False).tolist())) for _ in range(20_000)]
+    indptr, ids = _csr(big)
+    assert _gpu_mine(indptr, ids, 0.0005, 3, engine) == miner.mine(indptr, ids, 0.0005, 3).as_dict()
