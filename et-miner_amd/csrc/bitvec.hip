@@ -1,18 +1,16 @@
 // Input side of the counting path: item-id range and histogram, the CSR ->
 // column-bitvector builder, and the K=1 column popcount.
 //
-// Builder layout: a workgroup owns a tile of 64 consecutive words (4,096
-// transactions) times a slab of SLAB columns. It zero-fills the tile in LDS,
+// Builder layout: a workgroup owns a tile of 32 consecutive words (2,048
+// transactions) times a slab of 128 columns. It zero-fills the tile in LDS,
 // ORs the bits of its rows' entries into LDS (ds_or_b64), then writes each
-// column's 64 words as one contiguous 512-byte run: every HBM write is a full,
-// coalesced segment and no global atomic is issued.
+// column's 32 words as one contiguous 256-byte run: no global atomic is
+// issued and every id is read once, coalesced.
 #include "common.h"
 
 namespace {
 
 constexpr int kBuildThreads = 256;
-constexpr int kTileWords = 64;
-constexpr int kTileRows = kTileWords * 64;  // 4,096 transactions per tile
 
 template <typename Id>
 __global__ __launch_bounds__(256) void k_ids_minmax(const Id* __restrict__ ids, int64_t n, int64_t* out) {
@@ -80,38 +78,47 @@ __global__ __launch_bounds__(256) void k_hist_global(const Id* __restrict__ ids,
     }
 }
 
-// One workgroup: words [w0, w0 + 64) x columns [c0, c0 + SLAB). Wave `wid`, lane `l` handles the
-// transactions r = w0*64 + l*64 + j for j in [wid*16, wid*16 + 16): lanes of a wave write different words.
+// One workgroup: words [w0, w0 + kBuildWords) x columns [c0, c0 + SLAB). The rows' indptr slice is staged in
+// LDS; threads stride over the rows' entries (consecutive threads read consecutive ids), find each entry's row
+// by binary search in LDS, and OR its bit into the LDS tile. Each column's kBuildWords words then leave as one
+// contiguous 256-byte store.
+constexpr int kBuildWords = 32;
+constexpr int kBuildRows = kBuildWords * 64;
+
 template <typename Id, int SLAB>
 __global__ __launch_bounds__(kBuildThreads) void k_bitvec_build(const int64_t* __restrict__ indptr,
                                                                  const Id* __restrict__ ids, int64_t n_rows,
                                                                  int64_t lo, int64_t span,
                                                                  const int32_t* __restrict__ lut, int32_t n_cols,
                                                                  int64_t pitch, uint64_t* __restrict__ out) {
-    __shared__ unsigned long long tile[SLAB * kTileWords];
-    const int64_t w0 = (int64_t)blockIdx.x * kTileWords;
+    __shared__ unsigned long long tile[SLAB * kBuildWords];
+    __shared__ int64_t rp[kBuildRows + 1];
+    const int64_t w0 = (int64_t)blockIdx.x * kBuildWords;
     const int c0 = blockIdx.y * SLAB;
-    for (int t = threadIdx.x; t < SLAB * kTileWords; t += kBuildThreads) tile[t] = 0ull;
+    const int64_t r0 = w0 * 64;
+    const int nr = (int)max<int64_t>(0, min<int64_t>(kBuildRows, n_rows - r0));
+    for (int t = threadIdx.x; t < SLAB * kBuildWords; t += kBuildThreads) tile[t] = 0ull;
+    for (int t = threadIdx.x; t <= nr; t += kBuildThreads) rp[t] = indptr[r0 + t];
     __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const int wid = threadIdx.x >> 6;
-    const int64_t r_base = w0 * 64 + (int64_t)lane * 64;
-    for (int j = wid * 16; j < wid * 16 + 16; ++j) {
-        const int64_t r = r_base + j;
-        if (r >= n_rows) break;
-        const int64_t a = indptr[r], b = indptr[r + 1];
-        const unsigned long long bit = 1ull << j;
-        for (int64_t e = a; e < b; ++e) {
-            const int64_t v = (int64_t)ids[e] - lo;
-            if (v < 0 || v >= span) continue;
-            const int c = lut[v] - c0;
-            if (c >= 0 && c < SLAB) atomicOr(&tile[c * kTileWords + lane], bit);
+    const int64_t e0 = nr ? rp[0] : 0, e1 = nr ? rp[nr] : 0;
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += kBuildThreads) {
+        const int64_t v = (int64_t)ids[e] - lo;
+        if (v < 0 || v >= span) continue;
+        const int c = lut[v] - c0;
+        if (c < 0 || c >= SLAB) continue;
+        // Row of entry e: the last r with rp[r] <= e.
+        int a = 0, b = nr - 1;
+        while (a < b) {
+            const int mid = (a + b + 1) >> 1;
+            if (rp[mid] <= e) a = mid;
+            else b = mid - 1;
         }
+        atomicOr(&tile[c * kBuildWords + (a >> 6)], 1ull << (a & 63));
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < SLAB * kTileWords; t += kBuildThreads) {
-        const int c = t / kTileWords;
-        const int w = t % kTileWords;
+    for (int t = threadIdx.x; t < SLAB * kBuildWords; t += kBuildThreads) {
+        const int c = t / kBuildWords;
+        const int w = t % kBuildWords;
         if (c0 + c < n_cols && w0 + w < pitch) out[(int64_t)(c0 + c) * pitch + w0 + w] = tile[t];
     }
 }
@@ -187,8 +194,8 @@ extern "C" int etm_bitvec_build(const int64_t* d_indptr, const void* d_ids, int 
     ETM_ARG(pitch >= (n_rows + 63) / 64, "etm_bitvec_build: pitch %lld < ceil(n_rows/64) = %lld",
             (long long)pitch, (long long)((n_rows + 63) / 64));
     if (n_cols == 0 || pitch == 0) return ETM_OK;
-    constexpr int SLAB = 128;  // 64 KiB of LDS per workgroup
-    const int64_t tiles = (pitch + kTileWords - 1) / kTileWords;
+    constexpr int SLAB = 128;  // 32 KiB tile + 16 KiB indptr slice of LDS per workgroup
+    const int64_t tiles = (pitch + kBuildWords - 1) / kBuildWords;
     ETM_REQUIRE(tiles <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_bitvec_build: %lld word tiles exceed the grid",
                 (long long)tiles);
     dim3 grid((unsigned)tiles, (unsigned)((n_cols + SLAB - 1) / SLAB));

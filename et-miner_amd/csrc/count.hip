@@ -30,6 +30,7 @@ namespace {
 
 constexpr int kT = 64;   // suffixes per tile
 constexpr int kKW = 32;  // words per LDS step
+constexpr int kXcds = 8;  // MI355X: 8 XCDs of 32 CUs, each with its own L2
 
 // acc + popcount(x) as two v_bcnt_u32_b32 with accumulate (one per 32-bit half).
 __device__ __forceinline__ int popc_acc(uint64_t x, int acc) {
@@ -59,10 +60,13 @@ __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict_
     constexpr int NP = S * (S - 1) / 2;
     constexpr int U = SmallUnroll<S>::U;
     const int lane = threadIdx.x & 63;
-    const int64_t item = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (item >= n_list * n_chunks) return;
-    const int64_t g = list[item / n_chunks];
-    const int64_t ch = item % n_chunks;
+    // Workgroups go to the 8 XCDs round-robin. XCD x takes word chunks x, x+8, ..., chunk-major, so the waves
+    // running on one XCD at a time share one chunk of every column in that XCD's L2.
+    const int64_t xcd = blockIdx.x % kXcds;
+    const int64_t item = (int64_t)(blockIdx.x / kXcds) * 4 + (threadIdx.x >> 6);
+    const int64_t ch = xcd + kXcds * (item / n_list);
+    if (ch >= n_chunks) return;
+    const int64_t g = list[item % n_list];
     const int64_t gs = group_start[g];
     const int s = (int)(group_start[g + 1] - gs);
     if (s < 2 || s > S) return;
@@ -151,6 +155,7 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
                                                       int32_t m, const int64_t* __restrict__ group_start,
                                                       const int64_t* __restrict__ cum_pairs,
                                                       const int64_t* __restrict__ cum_tiles, int64_t n_groups,
+                                                      int64_t n_tile_pairs, int64_t n_chunks,
                                                       const uint8_t* __restrict__ status,
                                                       int32_t* __restrict__ counts, int32_t layout) {
     __shared__ uint64_t sA[kKW][kT + 1];
@@ -159,7 +164,12 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
     __shared__ int64_t colA[kT], colB[kT];
 
     const int tid = threadIdx.x;
-    const int64_t tp = blockIdx.x;
+    // 1-D grid: XCD x (blockIdx % 8) takes word chunks x, x+8, ..., and all tile-pairs of a chunk in turn.
+    const int64_t xcd = blockIdx.x % kXcds;
+    const int64_t local = blockIdx.x / kXcds;
+    const int64_t chunk_id = xcd + kXcds * (local / n_tile_pairs);
+    if (chunk_id >= n_chunks) return;
+    const int64_t tp = local % n_tile_pairs;
     const int64_t g = etm::upper_index(cum_tiles, n_groups, tp);
     const int64_t gs = group_start[g];
     const int64_t s = group_start[g + 1] - gs;
@@ -201,7 +211,7 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[r][c] = 0;
 
-    const int64_t w_begin = (int64_t)blockIdx.y * chunk_words;
+    const int64_t w_begin = chunk_id * chunk_words;
     const int64_t w_end = min(n_words, w_begin + chunk_words);
     __syncthreads();
     for (int64_t wb = w_begin; wb < w_end; wb += kKW) {
@@ -302,8 +312,8 @@ hipError_t launch_small(const uint64_t* bv, int64_t pitch, int64_t n_words, int6
                         int32_t m, const int64_t* gs, const int64_t* cp, const int32_t* list, int64_t n_list,
                         const uint8_t* status, int32_t* counts, int32_t layout, hipStream_t s) {
     const int64_t n_chunks = ceil_div(n_words, chunk);
-    const int64_t waves = n_list * n_chunks;
-    const int64_t blocks = ceil_div(waves, 4);
+    const int64_t waves_per_xcd = n_list * ceil_div(n_chunks, kXcds);
+    const int64_t blocks = kXcds * ceil_div(waves_per_xcd, 4);
     if (blocks > 0x7FFFFFFF) return hipErrorInvalidConfiguration;
     k_count_small<S><<<(unsigned)blocks, 256, 0, s>>>(bv, pitch, n_words, chunk, n_chunks, rows, m, gs, cp, list,
                                                       n_list, status, counts, layout);
@@ -338,7 +348,7 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
     for (int c = 0; c < 4; ++c) {
         if (cls[c] == 0) continue;
         // Work items scale with the class's per-word VALU cost: wider classes get more waves per word range.
-        const int64_t chunk = word_chunk > 0 ? word_chunk : pick_chunk(n_words, cls[c], 16384, 64, 128);
+        const int64_t chunk = word_chunk > 0 ? word_chunk : pick_chunk(n_words, cls[c], 16384, 64, 512);
         hipError_t e;
         const int32_t* list = d_small_list + base;
         switch (c) {
@@ -360,11 +370,12 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
         const int64_t chunk = word_chunk > 0 ? ceil_div(word_chunk, kKW) * kKW
                                              : pick_chunk(n_words, n_tile_pairs, 2048, kKW, 128);
         const int64_t n_chunks = ceil_div(n_words, chunk);
-        ETM_REQUIRE(n_chunks <= 65535, ETM_EOVERFLOW, "etm_count_level: %lld word chunks exceed the grid",
-                    (long long)n_chunks);
-        dim3 grid((unsigned)n_tile_pairs, (unsigned)n_chunks);
-        k_count_tiled<<<grid, 256, 0, s>>>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start, d_cum_pairs,
-                                           d_cum_tiles, n_groups, d_status, d_counts, layout);
+        const int64_t blocks = kXcds * n_tile_pairs * ceil_div(n_chunks, kXcds);
+        ETM_REQUIRE(blocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",
+                    (long long)blocks);
+        k_count_tiled<<<(unsigned)blocks, 256, 0, s>>>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start,
+                                                      d_cum_pairs, d_cum_tiles, n_groups, n_tile_pairs, n_chunks,
+                                                      d_status, d_counts, layout);
         ETM_LAUNCHED("k_count_tiled");
     }
     return ETM_OK;

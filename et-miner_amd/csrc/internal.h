@@ -15,8 +15,8 @@ hipError_t launch_groups_single(const int32_t* rows, const int64_t* d_n, int64_t
                                 int64_t* totals, hipStream_t s);
 hipError_t launch_compact_single(const int32_t* rows, int32_t m, const int64_t* group_start, const int64_t* cum_pairs,
                                  int64_t n_groups, int64_t n_cands, const uint8_t* status, const int32_t* counts,
-                                 int64_t min_count, int32_t* out_rows, int32_t* out_counts, int64_t* out_cand,
-                                 int64_t* d_out_n, hipStream_t s);
+                                 const int32_t* gid, int64_t min_count, int32_t* out_rows, int32_t* out_counts,
+                                 int64_t* out_cand, int64_t* d_out_n, hipStream_t s);
 hipError_t launch_select_single(const int32_t* rows, const int64_t* d_n, int64_t n_cap, int32_t m,
                                 const int32_t* counts, const uint8_t* flag, int32_t* out_rows, int32_t* out_counts,
                                 int64_t* d_out_n, hipStream_t s);

@@ -86,9 +86,12 @@ __global__ void k_zero_i64(int64_t* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
-// Size class of a group of s suffixes for the wave-per-group kernel: 0 (2), 1 (3-4), 2 (5-8), 3 (9-16), or -1.
+// Size class of a group of s suffixes: 0 (2), 1 (3-4), 2 (5-8), 3 (9-16) for the wave-per-group kernels,
+// 4 (more than small_max) for the tiled kernel, -1 (fewer than 2: no candidates).
+constexpr int kClasses = 5;
 __device__ __forceinline__ int size_class(int64_t s, int32_t small_max) {
-    if (s < 2 || s > small_max) return -1;
+    if (s < 2) return -1;
+    if (s > small_max) return 4;
     return s <= 2 ? 0 : s <= 4 ? 1 : s <= 8 ? 2 : 3;
 }
 
@@ -96,7 +99,7 @@ __device__ __forceinline__ int size_class(int64_t s, int32_t small_max) {
 __global__ __launch_bounds__(256) void k_group_classes(const int64_t* __restrict__ group_start, int64_t n,
                                                         int32_t small_max, int64_t* __restrict__ totals) {
     const int64_t n_groups = totals[0];
-    int64_t n2 = 0, rows2 = 0, smax = 0, cls[4] = {0, 0, 0, 0};
+    int64_t n2 = 0, rows2 = 0, smax = 0, cls[kClasses] = {0, 0, 0, 0, 0};
     for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n_groups && g < n;
          g += (int64_t)gridDim.x * blockDim.x) {
         const int64_t s = group_start[g + 1] - group_start[g];
@@ -111,16 +114,16 @@ __global__ __launch_bounds__(256) void k_group_classes(const int64_t* __restrict
     __shared__ int64_t lds[5];
     n2 = etm::block_sum<256>(n2, lds);
     rows2 = etm::block_sum<256>(rows2, lds);
-    for (int c = 0; c < 4; ++c) cls[c] = etm::block_sum<256>(cls[c], lds);
+    for (int c = 0; c < kClasses; ++c) cls[c] = etm::block_sum<256>(cls[c], lds);
     smax = etm::wave_max(smax);
     if ((threadIdx.x & 63) == 0 && smax) atomicMax((unsigned long long*)&totals[4], (unsigned long long)smax);
     if (threadIdx.x == 0) {
         if (n2) atomicAdd((unsigned long long*)&totals[5], (unsigned long long)n2);
         if (rows2) atomicAdd((unsigned long long*)&totals[6], (unsigned long long)rows2);
-        for (int c = 0; c < 4; ++c)
+        for (int c = 0; c < kClasses; ++c)
             if (cls[c]) {
                 atomicAdd((unsigned long long*)&totals[8 + c], (unsigned long long)cls[c]);
-                atomicAdd((unsigned long long*)&totals[3], (unsigned long long)cls[c]);
+                if (c < 4) atomicAdd((unsigned long long*)&totals[3], (unsigned long long)cls[c]);
             }
     }
 }
@@ -135,7 +138,7 @@ __global__ __launch_bounds__(256) void k_group_place(const int64_t* __restrict__
         if (c < 0) continue;
         int64_t base = 0;
         for (int q = 0; q < c; ++q) base += totals[8 + q];
-        const unsigned long long at = atomicAdd((unsigned long long*)&totals[12 + c], 1ull);
+        const unsigned long long at = atomicAdd((unsigned long long*)&totals[16 + c], 1ull);
         small_list[base + (int64_t)at] = (int32_t)g;
     }
 }
@@ -257,6 +260,160 @@ __global__ __launch_bounds__(256) void k_classify(const int32_t* __restrict__ ro
     }
 }
 
+// Group-centric classification. One wave per group of at most 16 suffixes (at most 120 candidates): lanes
+// take (candidate, dropped prefix item) pairs, so the k-2 subset lookups of a candidate run in parallel;
+// per-candidate results gather in LDS. Big groups: one workgroup each, a thread per candidate. Both write the
+// group id of every candidate (compaction then needs no search).
+constexpr int kMaxSmallPairs = 120;
+
+struct Classifier {
+    const int32_t* rows;
+    int32_t m;
+    const int32_t* prev_counts;
+    const uint8_t* prev_free;
+    const int32_t* table;
+    int64_t mask;
+    int32_t mode;
+
+    // Candidate (i, j) of the group starting at row gs: the row of the subset dropping prefix item d, or -1.
+    __device__ int64_t lookup(int64_t gs, int64_t i, int64_t j, int d) const {
+        const int32_t* pre = rows + gs * m;
+        const int np = m - 1;
+        const int32_t a = rows[(gs + i) * m + np], b = rows[(gs + j) * m + np];
+        uint64_t h = item_hash(a) + item_hash(b);
+        for (int t = 0; t < np; ++t)
+            if (t != d) h += item_hash(pre[t]);
+        Query q{pre, np, d, a, b, true};
+        return hash_find(rows, m, table, mask, h, q);
+    }
+};
+
+__global__ __launch_bounds__(256) void k_classify_small(Classifier C, const int64_t* __restrict__ group_start,
+                                                         const int64_t* __restrict__ cum_pairs,
+                                                         const int32_t* __restrict__ list, int64_t n_list,
+                                                         uint8_t* __restrict__ status, int32_t* __restrict__ counts,
+                                                         int32_t* __restrict__ gid, int64_t* __restrict__ tally) {
+    __shared__ int32_t s_min[4][kMaxSmallPairs];
+    __shared__ int32_t s_flags[4][kMaxSmallPairs];  // bit 0: a subset is missing, bit 1: a subset is not free
+    __shared__ int64_t lds[5];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t item = (int64_t)blockIdx.x * 4 + wv;
+    const bool infer = (C.mode & ETM_SUBSET_INFER) != 0;
+    const bool prune = (C.mode & ETM_SUBSET_PRUNE) != 0;
+    int64_t n_count = 0, n_infer = 0, n_skip = 0;
+    if (item < n_list) {
+        const int64_t g = list[item];
+        const int64_t gs = group_start[g];
+        const int64_t s = group_start[g + 1] - gs;
+        const int npairs = (int)(s * (s - 1) / 2);
+        const int np = C.m - 1;
+        const int64_t cbase = cum_pairs[g];
+        for (int p = lane; p < npairs; p += 64) {
+            int64_t i, j;
+            etm::lex_pair_decode(p, s, &i, &j);
+            const int64_t ri = gs + i, rj = gs + j;
+            s_min[wv][p] = C.prev_counts ? min(C.prev_counts[ri], C.prev_counts[rj]) : 0;
+            s_flags[wv][p] = (infer && (C.prev_free[ri] == 0 || C.prev_free[rj] == 0)) ? 2 : 0;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        if (prune || infer) {
+            for (int t = lane; t < npairs * np; t += 64) {
+                const int p = t / np, d = t % np;
+                int64_t i, j;
+                etm::lex_pair_decode(p, s, &i, &j);
+                const int64_t r = C.lookup(gs, i, j, d);
+                if (r < 0) {
+                    atomicOr(&s_flags[wv][p], 1);
+                } else if (infer) {
+                    atomicMin(&s_min[wv][p], C.prev_counts[r]);
+                    if (C.prev_free[r] == 0) atomicOr(&s_flags[wv][p], 2);
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (int p = lane; p < npairs; p += 64) {
+            const int f = s_flags[wv][p];
+            uint8_t st = ETM_CAND_COUNT;
+            if ((f & 1) && prune) st = ETM_CAND_SKIP;
+            else if (!(f & 1) && infer && (f & 2)) st = ETM_CAND_INFER;
+            const int64_t c = cbase + p;
+            status[c] = st;
+            counts[c] = st == ETM_CAND_COUNT ? 0 : (st == ETM_CAND_INFER ? s_min[wv][p] : -1);
+            if (gid) gid[c] = (int32_t)g;
+            n_count += st == ETM_CAND_COUNT;
+            n_infer += st == ETM_CAND_INFER;
+            n_skip += st == ETM_CAND_SKIP;
+        }
+    }
+    if (tally) {
+        n_count = etm::block_sum<256>(n_count, lds);
+        n_infer = etm::block_sum<256>(n_infer, lds);
+        n_skip = etm::block_sum<256>(n_skip, lds);
+        if (threadIdx.x == 0) {
+            if (n_count) atomicAdd((unsigned long long*)&tally[0], (unsigned long long)n_count);
+            if (n_infer) atomicAdd((unsigned long long*)&tally[1], (unsigned long long)n_infer);
+            if (n_skip) atomicAdd((unsigned long long*)&tally[2], (unsigned long long)n_skip);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_classify_big(Classifier C, const int64_t* __restrict__ group_start,
+                                                       const int64_t* __restrict__ cum_pairs,
+                                                       const int32_t* __restrict__ list,
+                                                       uint8_t* __restrict__ status, int32_t* __restrict__ counts,
+                                                       int32_t* __restrict__ gid, int64_t* __restrict__ tally) {
+    __shared__ int64_t lds[5];
+    const int64_t g = list[blockIdx.x];
+    const int64_t gs = group_start[g];
+    const int64_t s = group_start[g + 1] - gs;
+    const int64_t npairs = s * (s - 1) / 2;
+    const int64_t cbase = cum_pairs[g];
+    const int np = C.m - 1;
+    const bool infer = (C.mode & ETM_SUBSET_INFER) != 0;
+    const bool prune = (C.mode & ETM_SUBSET_PRUNE) != 0;
+    int64_t n_count = 0, n_infer = 0, n_skip = 0;
+    for (int64_t p = threadIdx.x; p < npairs; p += blockDim.x) {
+        int64_t i, j;
+        etm::lex_pair_decode(p, s, &i, &j);
+        const int64_t ri = gs + i, rj = gs + j;
+        int32_t lowest = C.prev_counts ? min(C.prev_counts[ri], C.prev_counts[rj]) : 0;
+        bool nf = infer && (C.prev_free[ri] == 0 || C.prev_free[rj] == 0);
+        bool missing = false;
+        if (prune || infer)
+            for (int d = 0; d < np && !missing; ++d) {
+                const int64_t r = C.lookup(gs, i, j, d);
+                if (r < 0) {
+                    missing = true;
+                } else if (infer) {
+                    lowest = min(lowest, C.prev_counts[r]);
+                    nf = nf || C.prev_free[r] == 0;
+                }
+            }
+        uint8_t st = ETM_CAND_COUNT;
+        if (missing && prune) st = ETM_CAND_SKIP;
+        else if (!missing && infer && nf) st = ETM_CAND_INFER;
+        const int64_t c = cbase + p;
+        status[c] = st;
+        counts[c] = st == ETM_CAND_COUNT ? 0 : (st == ETM_CAND_INFER ? lowest : -1);
+        if (gid) gid[c] = (int32_t)g;
+        n_count += st == ETM_CAND_COUNT;
+        n_infer += st == ETM_CAND_INFER;
+        n_skip += st == ETM_CAND_SKIP;
+    }
+    if (tally) {
+        n_count = etm::block_sum<256>(n_count, lds);
+        n_infer = etm::block_sum<256>(n_infer, lds);
+        n_skip = etm::block_sum<256>(n_skip, lds);
+        if (threadIdx.x == 0) {
+            if (n_count) atomicAdd((unsigned long long*)&tally[0], (unsigned long long)n_count);
+            if (n_infer) atomicAdd((unsigned long long*)&tally[1], (unsigned long long)n_infer);
+            if (n_skip) atomicAdd((unsigned long long*)&tally[2], (unsigned long long)n_skip);
+        }
+    }
+}
+
 // ── compaction ──────────────────────────────────────────────────────────
 
 struct Survives {
@@ -279,9 +436,10 @@ struct WriteSurvivor {
     int32_t* out_rows;
     int32_t* out_counts;
     int64_t* out_cand;
+    const int32_t* gid;  // group of every candidate, or null (then found by binary search)
     __device__ void operator()(int64_t c, int64_t v, int64_t p) const {
         if (!v) return;
-        const int64_t g = etm::upper_index(cum_pairs, n_groups, c);
+        const int64_t g = gid ? gid[c] : (n_groups == 1 ? 0 : etm::upper_index(cum_pairs, n_groups, c));
         const int64_t gs = group_start[g];
         const int64_t s = group_start[g + 1] - gs;
         int64_t i, j;
@@ -378,7 +536,7 @@ __global__ __launch_bounds__(kOne) void k_groups_single(const int32_t* __restric
     __syncthreads();
     if (tid == 0) group_start[n_groups] = n;
     __syncthreads();
-    int64_t cp = 0, ct = 0, n2 = 0, rows2 = 0, smax = 0, cls[4] = {0, 0, 0, 0};
+    int64_t cp = 0, ct = 0, n2 = 0, rows2 = 0, smax = 0, cls[kClasses] = {0, 0, 0, 0, 0};
     for (int64_t base = 0; base < n_groups; base += kOne) {
         const int64_t g = base + tid;
         const int64_t sz = g < n_groups ? group_start[g + 1] - group_start[g] : 0;
@@ -413,7 +571,7 @@ __global__ __launch_bounds__(kOne) void k_groups_single(const int32_t* __restric
     __syncthreads();
     // Class lists, ascending within each class.
     int64_t placed = 0;
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < kClasses; ++c) {
         int64_t cc = 0;
         for (int64_t base = 0; base < n_groups; base += kOne) {
             const int64_t g = base + tid;
@@ -429,13 +587,14 @@ __global__ __launch_bounds__(kOne) void k_groups_single(const int32_t* __restric
         totals[0] = n_groups;
         totals[1] = cp;
         totals[2] = ct;
-        totals[3] = placed;
+        totals[3] = placed - cls[4];
         totals[4] = smax;
         totals[5] = n2;
         totals[6] = rows2;
         totals[7] = n;
-        for (int c = 0; c < 4; ++c) totals[8 + c] = cls[c];
-        for (int c = 0; c < 4; ++c) totals[12 + c] = 0;
+        for (int c = 0; c < kClasses; ++c) totals[8 + c] = cls[c];
+        for (int c = kClasses; c < 8; ++c) totals[8 + c] = 0;
+        for (int c = 0; c < 8; ++c) totals[16 + c] = 0;
     }
 }
 
@@ -503,10 +662,10 @@ hipError_t launch_groups_single(const int32_t* rows, const int64_t* d_n, int64_t
 
 hipError_t launch_compact_single(const int32_t* rows, int32_t m, const int64_t* group_start, const int64_t* cum_pairs,
                                  int64_t n_groups, int64_t n_cands, const uint8_t* status, const int32_t* counts,
-                                 int64_t min_count, int32_t* out_rows, int32_t* out_counts, int64_t* out_cand,
-                                 int64_t* d_out_n, hipStream_t s) {
+                                 const int32_t* gid, int64_t min_count, int32_t* out_rows, int32_t* out_counts,
+                                 int64_t* out_cand, int64_t* d_out_n, hipStream_t s) {
     Survives f{status, counts, min_count};
-    WriteSurvivor w{rows, m, group_start, cum_pairs, n_groups, counts, out_rows, out_counts, out_cand};
+    WriteSurvivor w{rows, m, group_start, cum_pairs, n_groups, counts, out_rows, out_counts, out_cand, gid};
     k_compact_single<<<1, kOne, 0, s>>>(f, w, n_cands, d_out_n);
     return hipGetLastError();
 }
@@ -568,7 +727,7 @@ extern "C" int etm_level_groups(const int32_t* d_rows, int64_t n, int32_t m, int
             etm_level_groups_ws(n));
     hipStream_t s = ETM_STREAM(stream);
     int64_t* ws = (int64_t*)d_ws;
-    k_zero_i64<<<1, 64, 0, s>>>(d_totals, 16);
+    k_zero_i64<<<1, 64, 0, s>>>(d_totals, 24);
     ETM_LAUNCHED("k_zero_i64");
     IsGroupStart is_start{d_rows, m};
     ETM_HIP(etm::scan_prepare(is_start, n, ws, nullptr, s, SetGroupEnd{d_group_start, n, d_totals}));
@@ -606,20 +765,39 @@ extern "C" int etm_level_hash(const int32_t* d_rows, int64_t n, int32_t m, int32
 extern "C" int etm_level_classify(const int32_t* d_rows, int64_t n, int32_t m, const int32_t* d_prev_counts,
                                   const uint8_t* d_prev_free, const int32_t* d_table, int64_t table_size,
                                   const int64_t* d_group_start, const int64_t* d_cum_pairs, int64_t n_groups,
-                                  int64_t n_cands, int32_t mode, uint8_t* d_status, int32_t* d_counts,
+                                  int64_t n_cands, const int32_t* d_group_list, const int64_t* h_class_counts,
+                                  int32_t mode, uint8_t* d_status, int32_t* d_counts, int32_t* d_gid,
                                   int64_t* d_tally, void* stream) {
     ETM_ARG(d_rows && d_table && d_group_start && d_cum_pairs && d_status && d_counts,
             "etm_level_classify: null pointer");
-    ETM_ARG(m >= 1 && m <= 62, "etm_level_classify: 1 <= m <= 62 required, got %d", m);
+    ETM_ARG(m >= 2 && m <= 62, "etm_level_classify: 2 <= m <= 62 required, got %d", m);
     ETM_ARG(is_pow2(table_size), "etm_level_classify: table_size must be a power of two");
     ETM_ARG(!(mode & ETM_SUBSET_INFER) || (d_prev_free && d_prev_counts),
             "etm_level_classify: inference needs the previous level's counts and free flags");
     ETM_ARG(n_groups >= 1 || n_cands == 0, "etm_level_classify: no groups");
     (void)n;
     if (n_cands == 0) return ETM_OK;
-    k_classify<<<grid_for(n_cands, 65536), 256, 0, ETM_STREAM(stream)>>>(
-        d_rows, m, d_prev_counts, d_prev_free, d_table, table_size - 1, d_group_start, d_cum_pairs, n_groups,
-        n_cands, mode, d_status, d_counts, d_tally);
+    hipStream_t s = ETM_STREAM(stream);
+    if (d_group_list && h_class_counts) {
+        int64_t small = 0;
+        for (int c = 0; c < 4; ++c) small += h_class_counts[c];
+        const int64_t big = h_class_counts[4];
+        Classifier C{d_rows, m, d_prev_counts, d_prev_free, d_table, table_size - 1, mode};
+        if (small > 0) {
+            k_classify_small<<<(unsigned)((small + 3) / 4), 256, 0, s>>>(C, d_group_start, d_cum_pairs, d_group_list,
+                                                                         small, d_status, d_counts, d_gid, d_tally);
+            ETM_LAUNCHED("k_classify_small");
+        }
+        if (big > 0) {
+            k_classify_big<<<(unsigned)big, 256, 0, s>>>(C, d_group_start, d_cum_pairs, d_group_list + small,
+                                                         d_status, d_counts, d_gid, d_tally);
+            ETM_LAUNCHED("k_classify_big");
+        }
+        return ETM_OK;
+    }
+    k_classify<<<grid_for(n_cands, 65536), 256, 0, s>>>(d_rows, m, d_prev_counts, d_prev_free, d_table,
+                                                        table_size - 1, d_group_start, d_cum_pairs, n_groups,
+                                                        n_cands, mode, d_status, d_counts, d_tally);
     ETM_LAUNCHED("k_classify");
     return ETM_OK;
 }
@@ -628,7 +806,8 @@ extern "C" size_t etm_level_compact_ws(int64_t n_cands) { return etm::scan_ws_by
 
 extern "C" int etm_level_compact(const int32_t* d_rows, int32_t m, const int64_t* d_group_start,
                                  const int64_t* d_cum_pairs, int64_t n_groups, int64_t n_cands,
-                                 const uint8_t* d_status, const int32_t* d_counts, int64_t min_count,
+                                 const uint8_t* d_status, const int32_t* d_counts, const int32_t* d_gid,
+                                 int64_t min_count,
                                  int32_t* d_out_rows, int32_t* d_out_counts, int64_t* d_out_cand, int64_t* d_out_n,
                                  void* d_ws, size_t ws_bytes, void* stream) {
     ETM_ARG(d_rows && d_group_start && d_cum_pairs && d_counts && d_out_n && d_ws,
@@ -643,7 +822,8 @@ extern "C" int etm_level_compact(const int32_t* d_rows, int32_t m, const int64_t
         return ETM_OK;
     }
     ETM_ARG(d_out_counts != nullptr, "etm_level_compact: null output counts");
-    WriteSurvivor w{d_rows, m, d_group_start, d_cum_pairs, n_groups, d_counts, d_out_rows, d_out_counts, d_out_cand};
+    WriteSurvivor w{d_rows,       m,        d_group_start, d_cum_pairs, n_groups, d_counts,
+                    d_out_rows,   d_out_counts, d_out_cand, d_gid};
     ETM_HIP(etm::scan_finish(f, w, n_cands, (const int64_t*)d_ws, s));
     return ETM_OK;
 }

@@ -46,57 +46,84 @@ void check_rc(int rc, const char* what) {
     }
 }
 
-// Whether the current device serves stream-ordered allocations; when it does, its default pool keeps
-// freed memory (release threshold = max) so repeated calls do not return to the driver.
-bool pools_ready() {
-    static thread_local int cached_dev = -1;
-    static thread_local bool cached = false;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return false;
-    if (dev == cached_dev) return cached;
-    int ok = 0;
-    if (hipDeviceGetAttribute(&ok, hipDeviceAttributeMemoryPoolsSupported, dev) != hipSuccess) ok = 0;
-    if (ok) {
-        hipMemPool_t pool;
-        if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
-            uint64_t thr = UINT64_MAX;
-            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
-        } else {
-            ok = 0;
-        }
+// Device memory of mining calls: a per-thread, per-device bump allocator over blocks kept across calls.
+// A call takes what it needs by moving a pointer (no allocator API call on the critical path between the
+// level syncs); blocks added when a call outgrows the current one are merged into one larger block once the
+// call has synchronized. Every call ends with a stream synchronization, so reuse by the next call on the
+// same thread is ordered after all of this call's kernels.
+struct DeviceHeap {
+    int device = -1;
+    std::vector<std::pair<char*, size_t>> blocks;  // (base, capacity); blocks[0] is the main block
+    size_t used = 0;      // bytes taken from the last block
+    size_t demand = 0;    // bytes taken by the current call over all blocks
+    int depth = 0;        // nesting of Arena scopes (the input layer's holds the level loop's)
+    ~DeviceHeap() {
+        for (auto& b : blocks) (void)hipFree(b.first);
     }
-    (void)hipGetLastError();
-    cached_dev = dev;
-    cached = ok != 0;
-    return cached;
+};
+
+DeviceHeap& heap() {
+    static thread_local DeviceHeap h;
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    if (h.device != dev) {
+        for (auto& b : h.blocks) (void)hipFree(b.first);
+        h.blocks.clear();
+        h.device = dev;
+    }
+    return h;
 }
 
-// Device allocations of one call, released together (stream-ordered when the device has memory pools).
 class Arena {
    public:
-    explicit Arena(hipStream_t s) : s_(s), async_(pools_ready()) {}
+    explicit Arena(hipStream_t s) : s_(s), h_(heap()) {
+        if (h_.depth++ == 0) {
+            h_.used = 0;
+            h_.demand = 0;
+        }
+    }
+    // The outermost scope ends after the call's final synchronization: fold overflow blocks into one block.
     ~Arena() {
-        if (!async_ && !ptrs_.empty()) (void)hipStreamSynchronize(s_);
-        for (void* p : ptrs_) (void)(async_ ? hipFreeAsync(p, s_) : hipFree(p));
+        if (--h_.depth > 0) return;
+        if (h_.blocks.size() > 1) {
+            (void)hipStreamSynchronize(s_);
+            size_t total = 0;
+            for (auto& b : h_.blocks) {
+                total += b.second;
+                (void)hipFree(b.first);
+            }
+            h_.blocks.clear();
+            char* p = nullptr;
+            const size_t want = std::max(total, h_.demand + (h_.demand >> 2));
+            if (hipMalloc((void**)&p, want) == hipSuccess) h_.blocks.emplace_back(p, want);
+            else (void)hipGetLastError();
+        }
+        h_.used = 0;
     }
     template <class T>
     T* get(int64_t n) {
-        void* p = nullptr;
-        const size_t bytes = std::max<size_t>((size_t)std::max<int64_t>(n, 1) * sizeof(T), 256);
-        hipError_t e = async_ ? hipMallocAsync(&p, bytes, s_) : hipMalloc(&p, bytes);
-        if (e == hipErrorOutOfMemory) {
-            (void)hipGetLastError();
-            throw Error(ETM_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+        const size_t bytes = (std::max<size_t>((size_t)std::max<int64_t>(n, 1) * sizeof(T), 256) + 255) & ~(size_t)255;
+        h_.demand += bytes;
+        if (h_.blocks.empty() || h_.used + bytes > h_.blocks.back().second) {
+            const size_t want = std::max<size_t>(bytes, std::max<size_t>(64ull << 20, h_.demand * 2));
+            char* p = nullptr;
+            hipError_t e = hipMalloc((void**)&p, want);
+            if (e == hipErrorOutOfMemory) {
+                (void)hipGetLastError();
+                throw Error(ETM_ENOMEM, "device allocation of " + std::to_string(want) + " bytes failed");
+            }
+            check(e, "device allocation");
+            h_.blocks.emplace_back(p, want);
+            h_.used = 0;
         }
-        check(e, "device allocation");
-        ptrs_.push_back(p);
+        char* p = h_.blocks.back().first + h_.used;
+        h_.used += bytes;
         return (T*)p;
     }
 
    private:
     hipStream_t s_;
-    bool async_;
-    std::vector<void*> ptrs_;
+    DeviceHeap& h_;
 };
 
 // Pinned host scalars for the per-level readback (one buffer per thread, kept for the process).
@@ -108,6 +135,20 @@ struct Pinned {
         p = buf;
     }
 };
+
+// A pinned host buffer of at least n int32 (one per thread, grown on demand, kept for the process).
+int32_t* staging(int64_t n) {
+    static thread_local int32_t* buf = nullptr;
+    static thread_local int64_t cap = 0;
+    if (n > cap) {
+        if (buf) (void)hipHostFree(buf);
+        buf = nullptr;
+        const int64_t want = std::max<int64_t>(n, 1 << 20);
+        check(hipHostMalloc((void**)&buf, want * 4, hipHostMallocDefault), "hipHostMalloc");
+        cap = want;
+    }
+    return buf;
+}
 
 int64_t pow2_at_least(int64_t x) {
     int64_t v = 1;
@@ -202,7 +243,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
     build_groups(a, prev, nullptr, n1, g, scal + 8, s);
     check(hipMemcpyAsync(host.p, scal, kScal * 8, hipMemcpyDeviceToHost, s), "readback");
     check(hipStreamSynchronize(s), "sync");
-    int64_t tot[16];
+    int64_t tot[24];
     std::memcpy(tot, host.p + 8, sizeof tot);
 
     struct Pending {
@@ -231,14 +272,17 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         int64_t* cur_scal = a.get<int64_t>(kScal);
         check(etm::detail::launch_zero_i64(cur_scal, kScal, s), "zero");
         uint8_t* status = nullptr;
+        int32_t* gid = nullptr;
         int32_t* counts = a.get<int32_t>(n_cands);
+        const int64_t classes[5] = {tot[8], tot[9], tot[10], tot[11], tot[12]};
         if (k >= 3) {
             status = a.get<uint8_t>(n_cands);
+            gid = a.get<int32_t>(n_cands);
             const bool infer = o.infer_counts && !prune && prev.free != nullptr;
             const int32_t mode = ETM_SUBSET_PRUNE | (infer ? ETM_SUBSET_INFER : 0);
             check_rc(etm_level_classify(prev.rows, prev.n, m, prev.counts, infer ? prev.free : nullptr, prev.table,
-                                        prev.table_size, g.gs, g.cp, n_groups, n_cands, mode, status, counts,
-                                        cur_scal + 1, s),
+                                        prev.table_size, g.gs, g.cp, n_groups, n_cands, g.list, classes, mode,
+                                        status, counts, gid, cur_scal + 1, s),
                      "etm_level_classify");
         } else {
             check(hipMemsetAsync(counts, 0, n_cands * 4, s), "zero counts");
@@ -265,14 +309,14 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
             cap = n_cands;
             cur.rows = a.get<int32_t>(cap * k);
             cur.counts = a.get<int32_t>(cap);
-            check(etm::detail::launch_compact_single(prev.rows, m, g.gs, g.cp, n_groups, n_cands, status, counts, mc,
-                                                     cur.rows, cur.counts, nullptr, cur_scal + 0, s),
+            check(etm::detail::launch_compact_single(prev.rows, m, g.gs, g.cp, n_groups, n_cands, status, counts, gid,
+                                                     mc, cur.rows, cur.counts, nullptr, cur_scal + 0, s),
                   "compaction");
         } else {
             const size_t ws_n = etm_level_compact_ws(n_cands);
             int64_t* ws = a.get<int64_t>((int64_t)(ws_n + 7) / 8);
-            check_rc(etm_level_compact(prev.rows, m, g.gs, g.cp, n_groups, n_cands, status, counts, mc, nullptr,
-                                       nullptr, nullptr, cur_scal + 0, ws, ws_n, s),
+            check_rc(etm_level_compact(prev.rows, m, g.gs, g.cp, n_groups, n_cands, status, counts, gid, mc,
+                                       nullptr, nullptr, nullptr, cur_scal + 0, ws, ws_n, s),
                      "etm_level_compact");
             check(hipMemcpyAsync(host.p, cur_scal, 8, hipMemcpyDeviceToHost, s), "readback");
             check(hipStreamSynchronize(s), "sync");
@@ -280,8 +324,8 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
             cur.rows = a.get<int32_t>(cap * k);
             cur.counts = a.get<int32_t>(cap);
             if (cap)
-                check_rc(etm_level_compact(prev.rows, m, g.gs, g.cp, n_groups, n_cands, status, counts, mc, cur.rows,
-                                           cur.counts, nullptr, cur_scal + 0, ws, ws_n, s),
+                check_rc(etm_level_compact(prev.rows, m, g.gs, g.cp, n_groups, n_cands, status, counts, gid, mc,
+                                           cur.rows, cur.counts, nullptr, cur_scal + 0, ws, ws_n, s),
                          "etm_level_compact");
         }
         const int64_t* d_n = cur_scal + 0;
@@ -351,17 +395,26 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         }
         prev = cur;
     }
-    // Every level back to the host in one pass.
+    // Every level back to the host in one pass, through one pinned staging buffer.
+    int64_t total_ints = 0;
+    for (const Pending& p : levels) total_ints += p.n * (p.k + 1);
+    int32_t* stage = staging(total_ints);
+    int64_t at = 0;
     for (const Pending& p : levels) {
-        out.k.push_back(p.k);
-        out.rows.emplace_back((size_t)(p.n * p.k));
-        out.counts.emplace_back((size_t)p.n);
         if (p.n) {
-            check(hipMemcpyAsync(out.rows.back().data(), p.rows, p.n * p.k * 4, hipMemcpyDeviceToHost, s), "copy out");
-            check(hipMemcpyAsync(out.counts.back().data(), p.counts, p.n * 4, hipMemcpyDeviceToHost, s), "copy out");
+            check(hipMemcpyAsync(stage + at, p.rows, p.n * p.k * 4, hipMemcpyDeviceToHost, s), "copy out");
+            check(hipMemcpyAsync(stage + at + p.n * p.k, p.counts, p.n * 4, hipMemcpyDeviceToHost, s), "copy out");
         }
+        at += p.n * (p.k + 1);
     }
     check(hipStreamSynchronize(s), "sync");
+    at = 0;
+    for (const Pending& p : levels) {
+        out.k.push_back(p.k);
+        out.rows.emplace_back(stage + at, stage + at + p.n * p.k);
+        out.counts.emplace_back(stage + at + p.n * p.k, stage + at + p.n * (p.k + 1));
+        at += p.n * (p.k + 1);
+    }
     for (size_t i = 0; i < events.size(); ++i) {
         float ms = 0;
         check(hipEventElapsedTime(&ms, events[i].first, events[i].second), "event time");
@@ -493,6 +546,34 @@ extern "C" int etm_levels_stats(const etm_levels* lv, int32_t level, etm_level_s
     ETM_ARG(lv && h_out, "etm_levels_stats: null pointer");
     ETM_ARG(level >= 0 && level < (int32_t)lv->stats.size(), "etm_levels_stats: level %d out of range", level);
     *h_out = lv->stats[level];
+    return ETM_OK;
+}
+
+extern "C" int etm_levels_flat(const etm_levels* lv, int64_t* h_totals, int64_t* h_values, int64_t* h_offsets,
+                               int64_t* h_counts) {
+    ETM_ARG(lv && h_totals, "etm_levels_flat: null pointer");
+    int64_t n_sets = 0, n_vals = 0;
+    for (size_t l = 0; l < lv->k.size(); ++l) {
+        n_sets += (int64_t)lv->counts[l].size();
+        n_vals += (int64_t)lv->rows[l].size();
+    }
+    h_totals[0] = n_sets;
+    h_totals[1] = n_vals;
+    if (!h_values || !h_offsets || !h_counts) return ETM_OK;
+    const bool map = !lv->col_to_item.empty();
+    int64_t at = 0, row = 0;
+    h_offsets[0] = 0;
+    for (size_t l = 0; l < lv->k.size(); ++l) {
+        const int64_t k = lv->k[l];
+        const std::vector<int32_t>& r = lv->rows[l];
+        for (size_t i = 0; i < r.size(); ++i) h_values[at + (int64_t)i] = map ? lv->col_to_item[r[i]] : r[i];
+        const std::vector<int32_t>& c = lv->counts[l];
+        for (size_t i = 0; i < c.size(); ++i) {
+            h_counts[row] = c[i];
+            at += k;
+            h_offsets[++row] = at;
+        }
+    }
     return ETM_OK;
 }
 

@@ -37,12 +37,12 @@ SIGNATURES: dict = {
     "etm_level_groups_ws": (c_size_t, [c_int64]),
     "etm_level_groups": (c_int, [P, c_int64, c_int32, c_int32, P, P, P, P, P, P, c_size_t, P]),
     "etm_level_hash": (c_int, [P, c_int64, c_int32, P, c_int64, P]),
-    "etm_level_classify": (c_int, [P, c_int64, c_int32, P, P, P, c_int64, P, P, c_int64, c_int64, c_int32, P, P, P,
-                                   P]),
+    "etm_level_classify": (c_int, [P, c_int64, c_int32, P, P, P, c_int64, P, P, c_int64, c_int64, P, P, c_int32, P,
+                                   P, P, P, P]),
     "etm_count_level": (c_int, [P, c_int64, c_int64, P, c_int32, P, P, P, c_int64, c_int64, P, P, P, P, c_int32,
                                 c_int32, P]),
     "etm_level_compact_ws": (c_size_t, [c_int64]),
-    "etm_level_compact": (c_int, [P, c_int32, P, P, c_int64, c_int64, P, P, c_int64, P, P, P, P, P, c_size_t, P]),
+    "etm_level_compact": (c_int, [P, c_int32, P, P, c_int64, c_int64, P, P, P, c_int64, P, P, P, P, P, c_size_t, P]),
     "etm_level_free": (c_int, [P, c_int64, c_int32, P, P, P, P, c_int64, P, P]),
     "etm_select_rows": (c_int, [P, c_int64, c_int32, P, P, P, P, P, P, c_size_t, P]),
     "etm_count_itemsets": (c_int, [P, c_int64, c_int64, P, P, c_int64, P, c_int32, P]),
@@ -54,6 +54,7 @@ SIGNATURES: dict = {
     "etm_levels_columns": (c_int, [P, P, P, P]),
     "etm_levels_stats": (c_int, [P, c_int32, P]),
     "etm_levels_free": (c_int, [P]),
+    "etm_levels_flat": (c_int, [P, P, P, P, P]),
 }
 
 

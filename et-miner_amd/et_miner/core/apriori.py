@@ -269,16 +269,23 @@ def mine_csr(indptr, ids, min_support, max_length=None, prune_equal_support=Fals
         from ..gpu.native import mine_csr_device
 
         res = mine_csr_device(indptr_d, ids_d, n_rows, min_count, max_length, prune_equal_support,
-                              use_generator_pruning, time_kernels=time_kernels)
+                              use_generator_pruning, time_kernels=time_kernels, levels=False)
         if session:
             session.end_phase(n_columns=len(res.col_to_item))
-        stats, levels, col_to_item = res.stats, res.levels, res.col_to_item
-        if warn_complexity and levels:
-            _warn_complexity(len(levels[0][0]), min_support)
+        if warn_complexity and res.stats:
+            _warn_complexity(res.stats[0].n_frequent, min_support)
         cb = _timed(level_callback, session)
         if cb:
-            for st in stats:
+            for st in res.stats:
                 cb(st.k, st.n_candidates, st.n_frequent, st.count_ms)
+        if stats_out is not None:
+            stats_out.extend(res.stats)
+        table = res.table(n_rows, with_counts)
+        if items is not None:
+            flat = table.column("itemset").combine_chunks()
+            table = pa.table({"itemset": pa.LargeListArray.from_arrays(flat.offsets, items.take(flat.values)),
+                              "support": table.column("support")})
+        return table if with_counts else to_frame(table)
     else:
         from ..gpu.input_layer import build_device_input
         from ..gpu.row_split import mine_levels

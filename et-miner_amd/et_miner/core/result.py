@@ -29,12 +29,19 @@ def _mining_min_count(min_support: float, n_transactions: int) -> int:
     return max(1, _min_count(min_support, n_transactions))
 
 
+_POLARS = None
+
+
 def _have_polars() -> bool:
-    try:
-        import polars  # noqa: F401
-    except ImportError:
-        return False
-    return True
+    global _POLARS
+    if _POLARS is None:
+        try:
+            import polars  # noqa: F401
+
+            _POLARS = True
+        except ImportError:
+            _POLARS = False
+    return _POLARS
 
 
 def itemset_table(levels: list, col_to_item: np.ndarray, n_rows: int, with_counts: bool = False) -> pa.Table:

@@ -114,7 +114,7 @@ class Groups:
     max_suffixes: int
     n_groups2: int  # groups with >= 2 rows (the groups that yield candidates)
     rows2: int  # rows in those groups (their suffixes)
-    class_counts: tuple  # small groups per size class (2, 3-4, 5-8, 9-16 suffixes)
+    class_counts: tuple  # groups per size class (2, 3-4, 5-8, 9-16, > 16 suffixes)
 
 
 def level_groups(rows, m: int) -> Groups:
@@ -124,13 +124,13 @@ def level_groups(rows, m: int) -> Groups:
         raise ValueError("level_groups: empty level")
     gs, cp, ct = dev.empty(n + 1, "int64"), dev.empty(n + 1, "int64"), dev.empty(n + 1, "int64")
     small = dev.empty(n, "int32")
-    totals = dev.empty(16, "int64")
+    totals = dev.empty(24, "int64")
     ws_n = _lib.ws_bytes("etm_level_groups_ws", n)
     ws = dev.empty((ws_n + 7) // 8, "int64")
     call("etm_level_groups", rows.data_ptr(), n, m, SMALL_MAX, gs.data_ptr(), cp.data_ptr(), ct.data_ptr(),
          small.data_ptr(), totals.data_ptr(), ws.data_ptr(), ws_n, _s())
     t = dev.to_host(totals).tolist()
-    return Groups(gs, cp, ct, small, *(int(x) for x in t[:7]), tuple(int(x) for x in t[8:12]))
+    return Groups(gs, cp, ct, small, *(int(x) for x in t[:7]), tuple(int(x) for x in t[8:13]))
 
 
 def table_size_for(n: int) -> int:
@@ -150,36 +150,39 @@ def level_hash(rows, m: int):
 
 
 def level_classify(rows, m: int, prev_counts, prev_free, table, table_size: int, groups: Groups, mode: int,
-                   tally=None):
-    """(status uint8, counts int32) of every candidate: subset test, optional Pascal inference.
-
-    ``tally`` (device int64[3], optional) accumulates the counted, inferred and skipped candidates."""
+                   tally=None, group_centric: bool = True):
+    """(status uint8, counts int32, group id int32 or None) of every candidate: subset test, optional Pascal
+    inference. ``tally`` (device int64[3], optional) accumulates the counted, inferred and skipped candidates;
+    ``group_centric`` picks the wave-per-group form (else a thread per candidate with a binary search)."""
     status = dev.empty(groups.n_cands, "uint8")
     counts = dev.empty(groups.n_cands, "int32")
+    gid = dev.empty(groups.n_cands, "int32") if group_centric else None
+    cls = (ctypes.c_int64 * 5)(*groups.class_counts) if group_centric else None
     call("etm_level_classify", rows.data_ptr(), int(rows.shape[0]), m, dev.ptr(prev_counts), dev.ptr(prev_free),
          table.data_ptr(), table_size, groups.group_start.data_ptr(), groups.cum_pairs.data_ptr(), groups.n_groups,
-         groups.n_cands, mode, status.data_ptr(), counts.data_ptr(), dev.ptr(tally), _s())
-    return status, counts
+         groups.n_cands, groups.small_list.data_ptr() if group_centric else None, cls, mode, status.data_ptr(),
+         counts.data_ptr(), dev.ptr(gid), dev.ptr(tally), _s())
+    return status, counts, gid
 
 
 def count_level(bv, n_words: int, rows, m: int, groups: Groups, status, counts, layout: int = LAYOUT_LEX,
                 word_chunk: int = 0, pitch: int | None = None) -> None:
     """Adds the AND + popcount counts of the counted candidates into ``counts``."""
     pitch = int(bv.shape[1]) if pitch is None else int(pitch)
-    cls = (ctypes.c_int64 * 4)(*groups.class_counts)
+    cls = (ctypes.c_int64 * 4)(*groups.class_counts[:4])
     call("etm_count_level", bv.data_ptr(), pitch, n_words, rows.data_ptr(), m, groups.group_start.data_ptr(),
          groups.cum_pairs.data_ptr(), groups.cum_tiles.data_ptr(), groups.n_groups, groups.n_tile_pairs,
          groups.small_list.data_ptr(), cls, dev.ptr(status), counts.data_ptr(), layout, word_chunk, _s())
 
 
-def level_compact(rows, m: int, groups: Groups, status, counts, min_count: int, with_cand: bool = False):
+def level_compact(rows, m: int, groups: Groups, status, counts, min_count: int, with_cand: bool = False, gid=None):
     """(rows (n, m+1) int32, counts int32[, candidate ids int64]) of the survivors, in lexicographic order."""
     n_c = groups.n_cands
     ws_n = _lib.ws_bytes("etm_level_compact_ws", n_c)
     ws = dev.empty((ws_n + 7) // 8, "int64")
     n_out = dev.empty(1, "int64")
     args = (rows.data_ptr(), m, groups.group_start.data_ptr(), groups.cum_pairs.data_ptr(), groups.n_groups, n_c,
-            dev.ptr(status), counts.data_ptr(), int(min_count))
+            dev.ptr(status), counts.data_ptr(), dev.ptr(gid), int(min_count))
     call("etm_level_compact", *args, None, None, None, n_out.data_ptr(), ws.data_ptr(), ws_n, _s())
     n = int(dev.to_host(n_out)[0])
     out_rows = dev.empty((n, m + 1), "int32")

@@ -25,6 +25,18 @@ class NativeResult:
     stats: list = field(default_factory=list)  # LevelStats per level
     col_to_item: np.ndarray | None = None
     col_counts: np.ndarray | None = None
+    flat: tuple | None = None  # (item ids int64, offsets int64, counts int64) over every level
+
+    def table(self, n_rows: int, with_counts: bool = False):
+        """The result as a pyarrow Table (itemset, support[, count]) built from the flat arrays."""
+        import pyarrow as pa
+
+        values, offsets, counts = self.flat
+        cols = {"itemset": pa.LargeListArray.from_arrays(pa.array(offsets), pa.array(values)),
+                "support": pa.array(counts / float(n_rows))}
+        if with_counts:
+            cols["count"] = pa.array(counts)
+        return pa.table(cols)
 
 
 def _opts(min_count, max_length, prune, infer, time_kernels) -> _lib.MineOpts:
@@ -32,21 +44,31 @@ def _opts(min_count, max_length, prune, infer, time_kernels) -> _lib.MineOpts:
                          int(bool(time_kernels)))
 
 
-def _collect(handle) -> NativeResult:
+def _collect(handle, levels: bool = True) -> NativeResult:
+    """Reads a result handle (and frees it): the flat arrays always, the per-level arrays when ``levels``."""
     L = _lib.lib()
     res = NativeResult()
     try:
+        tot = np.zeros(2, dtype=np.int64)
+        _lib.call("etm_levels_flat", handle, tot.ctypes.data, None, None, None)
+        values = np.empty(tot[1], dtype=np.int64)
+        offsets = np.empty(tot[0] + 1, dtype=np.int64)
+        counts = np.empty(tot[0], dtype=np.int64)
+        _lib.call("etm_levels_flat", handle, tot.ctypes.data, values.ctypes.data, offsets.ctypes.data,
+                  counts.ctypes.data)
+        res.flat = (values, offsets, counts)
         n_levels = ctypes.c_int32()
         _lib.call("etm_levels_count", handle, ctypes.byref(n_levels))
         for lv in range(n_levels.value):
             k, n = ctypes.c_int32(), ctypes.c_int64()
             _lib.call("etm_levels_shape", handle, lv, ctypes.byref(k), ctypes.byref(n))
-            rows = np.empty((n.value, k.value), dtype=np.int32)
-            counts = np.empty(n.value, dtype=np.int64)
-            _lib.call("etm_levels_copy", handle, lv, rows.ctypes.data, counts.ctypes.data)
+            if levels:
+                rows = np.empty((n.value, k.value), dtype=np.int32)
+                lv_counts = np.empty(n.value, dtype=np.int64)
+                _lib.call("etm_levels_copy", handle, lv, rows.ctypes.data, lv_counts.ctypes.data)
+                res.levels.append((rows, lv_counts))
             st = _lib.LevelStat()
             _lib.call("etm_levels_stats", handle, lv, ctypes.byref(st))
-            res.levels.append((rows, counts))
             res.stats.append(LevelStats(st.k, st.n_candidates, st.n_frequent, st.n_counted, st.n_inferred,
                                         st.n_skipped, st.bytes_alg, 0.0, st.count_ms))
         n_cols = ctypes.c_int32()
@@ -61,7 +83,7 @@ def _collect(handle) -> NativeResult:
 
 
 def mine_csr_device(indptr_d, ids_d, n_rows: int, min_count: int, max_length=None, prune_non_free=False,
-                    infer_counts=False, time_kernels=False) -> NativeResult:
+                    infer_counts=False, time_kernels=False, levels: bool = True) -> NativeResult:
     """Every frequent itemset of a device CSR of item ids (int32 or int64), mined by the native loop."""
     torch = dev.torch()
     id_bytes = 8 if ids_d.dtype == torch.int64 else 4
@@ -71,7 +93,7 @@ def mine_csr_device(indptr_d, ids_d, n_rows: int, min_count: int, max_length=Non
     opts = _opts(min_count, max_length, prune_non_free, infer_counts, time_kernels)
     _lib.call("etm_mine_csr", indptr_d.data_ptr(), ids_d.data_ptr() if ids_d.numel() else None, id_bytes, n_rows,
               ids_d.numel(), SPAN_LIMIT, ctypes.byref(opts), ctypes.byref(handle), dev.stream_ptr())
-    return _collect(handle)
+    return _collect(handle, levels)
 
 
 def mine_bitvecs_device(bv_ptr: int, pitch: int, n_words: int, n_rows: int, col_counts: np.ndarray, min_count: int,

@@ -130,14 +130,16 @@ def mine_levels(
         groups = K.level_groups(prev.rows, m)
         if groups.n_cands == 0:
             break
-        tally = None
+        tally = gid = None
         if k >= 3:
             table, tsize = prev.hashed()
             infer = infer_counts and not prune_non_free and prev.free is not None
             mode = K.SUBSET_PRUNE | (K.SUBSET_INFER if infer else 0)
             tally = dev.zeros(3, "int64")
-            status, counts = K.level_classify(prev.rows, m, prev.counts, prev.free if infer else None, table, tsize,
-                                              groups, mode, tally)
+            # Both classification forms are exercised by this operator-level loop: group-centric on odd levels,
+            # one thread per candidate (binary search for the group) on even ones.
+            status, counts, gid = K.level_classify(prev.rows, m, prev.counts, prev.free if infer else None, table,
+                                                   tsize, groups, mode, tally, group_centric=(k % 2 == 1))
         else:
             status, counts = None, dev.zeros(groups.n_cands, "int32")
         st = LevelStats(k, groups.n_cands, 0, groups.n_cands, 0, 0,
@@ -147,7 +149,7 @@ def mine_levels(
         K.count_level(bv, n_words, prev.rows, m, groups, status, counts, pitch=pitch)
         if timer is not None:
             timer.end(k, st)
-        rows_k, counts_k = K.level_compact(prev.rows, m, groups, status, counts, min_count)
+        rows_k, counts_k = K.level_compact(prev.rows, m, groups, status, counts, min_count, gid=gid)
         if tally is not None:
             st.n_counted, st.n_inferred, st.n_skipped = (int(x) for x in dev.to_host(tally))
         cur = _Level(rows_k, counts_k)

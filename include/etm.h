@@ -96,11 +96,13 @@ size_t etm_level_groups_ws(int64_t n);
  *   d_cum_pairs[n_groups + 1]    exclusive prefix sum of s(s-1)/2,
  *   d_cum_tiles[n_groups + 1]    exclusive prefix sum of the tiled kernel's tile-pairs (groups with more
  *                                than small_max suffixes; 0 for the others),
- *   d_small_list[n_groups]       ids of the groups with 2..small_max suffixes, by size class: first the
- *                                groups of 2 suffixes, then 3-4, 5-8, 9-16 (unordered within a class),
- *   d_totals[16]                 {n_groups, n_candidates, n_tile_pairs, n_small, max_suffixes,
- *                                 groups with >= 2 rows, rows in those groups, 0,
- *                                 class sizes of d_small_list (2, 3-4, 5-8, 9-16), 4 internal cursors}.
+ *   d_small_list[n_groups]       ids of the groups with >= 2 suffixes, by size class: first the groups of
+ *                                2 suffixes, then 3-4, 5-8, 9-16 (the wave-per-group classes up to small_max),
+ *                                then those with more than small_max (the tiled kernel's); unordered within a
+ *                                class from the multi-block path, ascending from the single-workgroup path,
+ *   d_totals[24]                 {n_groups, n_candidates, n_tile_pairs, groups in the first four classes,
+ *                                 max_suffixes, groups with >= 2 rows, rows in those groups, rows,
+ *                                 class sizes (2, 3-4, 5-8, 9-16, > small_max), 0, 0, 0, 8 internal cursors}.
  * Capacities: n + 1 for the first three, n for d_small_list.
  * Replaces build_k3plus_groups_from_flat (gpu/kernels/k3plus.py:243-376, rust_ext/src/core/groups.rs:59-205)
  * and the kernel split of group_kernels (gpu/row_split_chunks.py:96-121). */
@@ -116,14 +118,17 @@ int etm_level_hash(const int32_t* d_rows, int64_t n, int32_t m, int32_t* d_table
 /* Subset test and Pascal inference of every candidate of a level's prefix groups, against the hash of the
  * same level (the candidates' (k-1)-subsets). Writes d_status[c] (ETM_CAND_*) and initialises d_counts[c]
  * (0 when counted, the inferred count when inferred, -1 when skipped). mode: ETM_SUBSET_PRUNE, optionally |
- * ETM_SUBSET_INFER (then d_prev_free is required). When d_tally is not NULL, adds the numbers of counted,
- * inferred and skipped candidates to d_tally[0..2].
+ * ETM_SUBSET_INFER (then d_prev_free is required). With d_group_list and h_class_counts[5] (etm_level_groups'
+ * class lists and sizes) the work is group-centric (one wave per small group, lanes over candidate x subset;
+ * one workgroup per big group) and d_gid (optional) receives every candidate's group; without them, one
+ * thread per candidate finds its group by binary search. When d_tally is not NULL, adds the numbers of
+ * counted, inferred and skipped candidates to d_tally[0..2].
  * Replaces _classify_candidate (gpu/kernels/_src/_subset_index.cu:139-178) without the deduction bounds. */
 int etm_level_classify(const int32_t* d_rows, int64_t n, int32_t m, const int32_t* d_prev_counts,
                        const uint8_t* d_prev_free, const int32_t* d_table, int64_t table_size,
                        const int64_t* d_group_start, const int64_t* d_cum_pairs, int64_t n_groups,
-                       int64_t n_cands, int32_t mode, uint8_t* d_status, int32_t* d_counts, int64_t* d_tally,
-                       void* stream);
+                       int64_t n_cands, const int32_t* d_group_list, const int64_t* h_class_counts, int32_t mode,
+                       uint8_t* d_status, int32_t* d_counts, int32_t* d_gid, int64_t* d_tally, void* stream);
 
 /* AND + popcount of every candidate whose status is ETM_CAND_COUNT (every candidate when d_status is
  * NULL), ADDED into d_counts (which the caller initialises: etm_level_classify does). Groups of 2..small_max
@@ -144,7 +149,8 @@ int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const 
 size_t etm_level_compact_ws(int64_t n_cands);
 
 /* Survivor compaction of a counted level, in candidate (lexicographic) order. A survivor has status !=
- * ETM_CAND_SKIP (or d_status NULL) and count >= min_count. Call twice with the same workspace: first with
+ * ETM_CAND_SKIP (or d_status NULL) and count >= min_count. d_gid (optional) gives each candidate's group,
+ * as etm_level_classify writes it; without it the group is found by binary search. Call twice with the same workspace: first with
  * d_out_rows == NULL, which writes the survivor count to d_out_n[0]; then with the outputs, capacity >= that
  * count, which writes d_out_rows (n_out x (m+1)), d_out_counts and, when not NULL, d_out_cand (candidate
  * index of each survivor). Replaces threshold_filter (gpu/kernels/filter.py:85-160) plus
@@ -152,7 +158,7 @@ size_t etm_level_compact_ws(int64_t n_cands);
  * (gpu/row_split.py:1064-1067), which the lexicographic candidate order makes unnecessary. */
 int etm_level_compact(const int32_t* d_rows, int32_t m, const int64_t* d_group_start, const int64_t* d_cum_pairs,
                       int64_t n_groups, int64_t n_cands, const uint8_t* d_status, const int32_t* d_counts,
-                      int64_t min_count, int32_t* d_out_rows, int32_t* d_out_counts, int64_t* d_out_cand,
+                      const int32_t* d_gid, int64_t min_count, int32_t* d_out_rows, int32_t* d_out_counts, int64_t* d_out_cand,
                       int64_t* d_out_n, void* d_ws, size_t ws_bytes, void* stream);
 
 /* Free flags of a level of k-itemsets against the hash of its complete previous level: d_free[x] = 1 when
@@ -228,6 +234,12 @@ int etm_levels_copy(const etm_levels* lv, int32_t level, int32_t* h_rows, int64_
 int etm_levels_columns(const etm_levels* lv, int32_t* h_n_cols, int64_t* h_col_to_item, int64_t* h_col_counts);
 int etm_levels_stats(const etm_levels* lv, int32_t level, etm_level_stat* h_out);
 int etm_levels_free(etm_levels* lv);
+
+/* Every level at once, flattened for a list column: h_totals[0] = itemsets, h_totals[1] = items over all
+ * itemsets (call with NULL outputs to size them); then h_values (item ids through the result's column->item
+ * map, or column ids when the result has none), h_offsets (itemsets + 1) and h_counts. */
+int etm_levels_flat(const etm_levels* lv, int64_t* h_totals, int64_t* h_values, int64_t* h_offsets,
+                    int64_t* h_counts);
 
 #ifdef __cplusplus
 }
