@@ -2,10 +2,10 @@
 // column-bitvector builder, and the K=1 column popcount.
 //
 // Builder layout: a workgroup owns a tile of 32 consecutive words (2,048
-// transactions) times a slab of 128 columns. It zero-fills the tile in LDS,
-// ORs the bits of its rows' entries into LDS (ds_or_b64), then writes each
-// column's 32 words as one contiguous 256-byte run: no global atomic is
-// issued and every id is read once, coalesced.
+// transactions) times a slab of 128 or 256 columns; each lane collects one
+// transaction's columns in a register bitmask and a wave ballot per column
+// produces each 64-bit word, so no atomic is issued; each column's 32 words
+// leave as one contiguous 256-byte run.
 #include "common.h"
 
 namespace {
@@ -78,12 +78,12 @@ __global__ __launch_bounds__(256) void k_hist_global(const Id* __restrict__ ids,
     }
 }
 
-// One workgroup: words [w0, w0 + kBuildWords) x columns [c0, c0 + SLAB). The rows' indptr slice is staged in
-// LDS; threads stride over the rows' entries (consecutive threads read consecutive ids), find each entry's row
-// by binary search in LDS, and OR its bit into the LDS tile. Each column's kBuildWords words then leave as one
-// contiguous 256-byte store.
+// One workgroup: words [w0, w0 + kBuildWords) x columns [c0, c0 + SLAB); wave v builds words w0 + v, w0 + v + 4,
+// ... For one word, lane l reads the entries of transaction 64*word + l and sets bit (column - c0) of a
+// SLAB-bit register mask; then, per column, one wave ballot of that bit IS the column's 64-bit word. The word
+// goes to an LDS tile (no atomics anywhere: each word is produced by one ballot), and each column's
+// kBuildWords words leave as one contiguous 256-byte store. Rows may be unsorted or repeat items.
 constexpr int kBuildWords = 32;
-constexpr int kBuildRows = kBuildWords * 64;
 
 template <typename Id, int SLAB>
 __global__ __launch_bounds__(kBuildThreads) void k_bitvec_build(const int64_t* __restrict__ indptr,
@@ -91,35 +91,44 @@ __global__ __launch_bounds__(kBuildThreads) void k_bitvec_build(const int64_t* _
                                                                  int64_t lo, int64_t span,
                                                                  const int32_t* __restrict__ lut, int32_t n_cols,
                                                                  int64_t pitch, uint64_t* __restrict__ out) {
-    __shared__ unsigned long long tile[SLAB * kBuildWords];
-    __shared__ int64_t rp[kBuildRows + 1];
+    constexpr int NM = SLAB / 64;  // 64-bit masks per lane
+    __shared__ uint64_t tile[SLAB][kBuildWords + 1];
     const int64_t w0 = (int64_t)blockIdx.x * kBuildWords;
     const int c0 = blockIdx.y * SLAB;
-    const int64_t r0 = w0 * 64;
-    const int nr = (int)max<int64_t>(0, min<int64_t>(kBuildRows, n_rows - r0));
-    for (int t = threadIdx.x; t < SLAB * kBuildWords; t += kBuildThreads) tile[t] = 0ull;
-    for (int t = threadIdx.x; t <= nr; t += kBuildThreads) rp[t] = indptr[r0 + t];
-    __syncthreads();
-    const int64_t e0 = nr ? rp[0] : 0, e1 = nr ? rp[nr] : 0;
-    for (int64_t e = e0 + threadIdx.x; e < e1; e += kBuildThreads) {
-        const int64_t v = (int64_t)ids[e] - lo;
-        if (v < 0 || v >= span) continue;
-        const int c = lut[v] - c0;
-        if (c < 0 || c >= SLAB) continue;
-        // Row of entry e: the last r with rp[r] <= e.
-        int a = 0, b = nr - 1;
-        while (a < b) {
-            const int mid = (a + b + 1) >> 1;
-            if (rp[mid] <= e) a = mid;
-            else b = mid - 1;
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    for (int wl = wv; wl < kBuildWords; wl += kBuildThreads / 64) {
+        const int64_t r = (w0 + wl) * 64 + lane;
+        uint64_t mask[NM];
+#pragma unroll
+        for (int q = 0; q < NM; ++q) mask[q] = 0;
+        if (r < n_rows) {
+            const int64_t a = indptr[r], b = indptr[r + 1];
+            for (int64_t e = a; e < b; ++e) {
+                const int64_t v = (int64_t)ids[e] - lo;
+                if (v < 0 || v >= span) continue;
+                const int c = lut[v] - c0;
+                if (c < 0 || c >= SLAB) continue;
+#pragma unroll
+                for (int q = 0; q < NM; ++q)
+                    if ((c >> 6) == q) mask[q] |= 1ull << (c & 63);
+            }
         }
-        atomicOr(&tile[c * kBuildWords + (a >> 6)], 1ull << (a & 63));
+#pragma unroll
+        for (int q = 0; q < NM; ++q) {
+            uint64_t mine = 0;
+            for (int l = 0; l < 64; ++l) {
+                const uint64_t word = __ballot((mask[q] >> l) & 1ull);
+                if (lane == l) mine = word;
+            }
+            tile[q * 64 + lane][wl] = mine;
+        }
     }
     __syncthreads();
     for (int t = threadIdx.x; t < SLAB * kBuildWords; t += kBuildThreads) {
         const int c = t / kBuildWords;
         const int w = t % kBuildWords;
-        if (c0 + c < n_cols && w0 + w < pitch) out[(int64_t)(c0 + c) * pitch + w0 + w] = tile[t];
+        if (c0 + c < n_cols && w0 + w < pitch) out[(int64_t)(c0 + c) * pitch + w0 + w] = tile[c][w];
     }
 }
 
@@ -194,18 +203,28 @@ extern "C" int etm_bitvec_build(const int64_t* d_indptr, const void* d_ids, int 
     ETM_ARG(pitch >= (n_rows + 63) / 64, "etm_bitvec_build: pitch %lld < ceil(n_rows/64) = %lld",
             (long long)pitch, (long long)((n_rows + 63) / 64));
     if (n_cols == 0 || pitch == 0) return ETM_OK;
-    constexpr int SLAB = 128;  // 32 KiB tile + 16 KiB indptr slice of LDS per workgroup
     const int64_t tiles = (pitch + kBuildWords - 1) / kBuildWords;
     ETM_REQUIRE(tiles <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_bitvec_build: %lld word tiles exceed the grid",
                 (long long)tiles);
-    dim3 grid((unsigned)tiles, (unsigned)((n_cols + SLAB - 1) / SLAB));
     hipStream_t s = ETM_STREAM(stream);
-    if (id_bytes == 4)
-        k_bitvec_build<int32_t, SLAB><<<grid, kBuildThreads, 0, s>>>(d_indptr, (const int32_t*)d_ids, n_rows, lo,
-                                                                     span, d_lut, n_cols, pitch, d_out);
-    else
-        k_bitvec_build<int64_t, SLAB><<<grid, kBuildThreads, 0, s>>>(d_indptr, (const int64_t*)d_ids, n_rows, lo,
-                                                                     span, d_lut, n_cols, pitch, d_out);
+    // 128-column slabs (one pass when n_cols <= 128), else 256 (33 KiB / 66 KiB of LDS per workgroup).
+    if (n_cols <= 128) {
+        dim3 grid((unsigned)tiles, 1);
+        if (id_bytes == 4)
+            k_bitvec_build<int32_t, 128><<<grid, kBuildThreads, 0, s>>>(d_indptr, (const int32_t*)d_ids, n_rows, lo,
+                                                                        span, d_lut, n_cols, pitch, d_out);
+        else
+            k_bitvec_build<int64_t, 128><<<grid, kBuildThreads, 0, s>>>(d_indptr, (const int64_t*)d_ids, n_rows, lo,
+                                                                        span, d_lut, n_cols, pitch, d_out);
+    } else {
+        dim3 grid((unsigned)tiles, (unsigned)((n_cols + 255) / 256));
+        if (id_bytes == 4)
+            k_bitvec_build<int32_t, 256><<<grid, kBuildThreads, 0, s>>>(d_indptr, (const int32_t*)d_ids, n_rows, lo,
+                                                                        span, d_lut, n_cols, pitch, d_out);
+        else
+            k_bitvec_build<int64_t, 256><<<grid, kBuildThreads, 0, s>>>(d_indptr, (const int64_t*)d_ids, n_rows, lo,
+                                                                        span, d_lut, n_cols, pitch, d_out);
+    }
     ETM_LAUNCHED("k_bitvec_build");
     return ETM_OK;
 }

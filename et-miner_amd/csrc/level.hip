@@ -510,6 +510,36 @@ __global__ __launch_bounds__(256) void k_free(const int32_t* __restrict__ rows, 
 // ── single-workgroup fast paths (small levels: one launch, sizes read on the device) ─────────
 
 constexpr int kOne = 1024;
+constexpr int kPer = 1;  // items per thread per pass (the per-item work is a dependent load chain: keep threads many)
+
+// One pass of a workgroup-wide exclusive scan over items [base, base + kOne*kPer): thread t evaluates f on its
+// kPer contiguous items, the workgroup scans the per-thread sums, and w(i, value, exclusive prefix) is called
+// for every item below n. Returns carry + the pass total. Ends with a barrier.
+template <class F, class W>
+__device__ __forceinline__ int64_t pass_scan(int64_t base, int64_t n, const F& f, const W& w, int64_t carry,
+                                             int64_t* lds) {
+    int64_t v[kPer];
+    int64_t sum = 0;
+    const int64_t first = base + (int64_t)threadIdx.x * kPer;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        v[j] = first + j < n ? f(first + j) : 0;
+        sum += v[j];
+    }
+    int64_t tot;
+    int64_t run = carry + etm::block_exclusive_scan<kOne>(sum, lds, &tot);
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        if (first + j < n) w(first + j, v[j], run);
+        run += v[j];
+    }
+    return carry + tot;
+}
+
+struct GroupSize {
+    const int64_t* group_start;
+    __device__ int64_t operator()(int64_t g) const { return group_start[g + 1] - group_start[g]; }
+};
 
 // The whole group build of a level of *d_n rows (at most n_cap) in one workgroup: group starts, cumulative
 // pairs and tile-pairs, size-class lists (groups in ascending order within each class) and the totals.
@@ -523,65 +553,59 @@ __global__ __launch_bounds__(kOne) void k_groups_single(const int32_t* __restric
     __shared__ int64_t lds[kOne / 64 + 1];
     const int tid = threadIdx.x;
     const int64_t n = d_n ? min(*d_n, n_cap) : n_cap;
+    constexpr int64_t kPass = (int64_t)kOne * kPer;
     IsGroupStart is_start{rows, m};
-    int64_t carry = 0, tot;
-    for (int64_t base = 0; base < n; base += kOne) {
-        const int64_t r = base + tid;
-        const int64_t f = r < n ? is_start(r) : 0;
-        const int64_t ex = etm::block_exclusive_scan<kOne>(f, lds, &tot);
-        if (f) group_start[carry + ex] = r;
-        carry += tot;
-    }
-    const int64_t n_groups = carry;
-    __syncthreads();
+    WriteGroupStart put_start{group_start};
+    int64_t n_groups = 0;
+    for (int64_t base = 0; base < n; base += kPass) n_groups = pass_scan(base, n, is_start, put_start, n_groups, lds);
     if (tid == 0) group_start[n_groups] = n;
     __syncthreads();
-    int64_t cp = 0, ct = 0, n2 = 0, rows2 = 0, smax = 0, cls[kClasses] = {0, 0, 0, 0, 0};
-    for (int64_t base = 0; base < n_groups; base += kOne) {
-        const int64_t g = base + tid;
-        const int64_t sz = g < n_groups ? group_start[g + 1] - group_start[g] : 0;
-        const int64_t pairs = sz * (sz - 1) / 2;
-        const int64_t nt = (sz + kTileT - 1) / kTileT;
-        const int64_t tiles = sz > small_max ? nt * (nt + 1) / 2 : 0;
-        const int64_t exp_ = etm::block_exclusive_scan<kOne>(pairs, lds, &tot);
-        if (g < n_groups) cum_pairs[g] = cp + exp_;
-        cp += tot;
-        const int64_t ext = etm::block_exclusive_scan<kOne>(tiles, lds, &tot);
-        if (g < n_groups) cum_tiles[g] = ct + ext;
-        ct += tot;
-        if (sz >= 2) {
-            n2 += 1;
-            rows2 += sz;
-        }
-        smax = sz > smax ? sz : smax;
+    GroupSize size{group_start};
+    auto pairs = [&](int64_t g) { const int64_t z = size(g); return z * (z - 1) / 2; };
+    auto tiles = [&](int64_t g) {
+        const int64_t z = size(g);
+        const int64_t nt = (z + kTileT - 1) / kTileT;
+        return z > small_max ? nt * (nt + 1) / 2 : (int64_t)0;
+    };
+    WritePrefix put_cp{cum_pairs}, put_ct{cum_tiles};
+    int64_t cp = 0, ct = 0;
+    for (int64_t base = 0; base < n_groups; base += kPass) {
+        cp = pass_scan(base, n_groups, pairs, put_cp, cp, lds);
+        ct = pass_scan(base, n_groups, tiles, put_ct, ct, lds);
     }
     if (tid == 0) {
         cum_pairs[n_groups] = cp;
         cum_tiles[n_groups] = ct;
+    }
+    int64_t n2 = 0, rows2 = 0, smax = 0;
+    for (int64_t g = tid; g < n_groups; g += kOne) {
+        const int64_t z = size(g);
+        if (z >= 2) {
+            n2 += 1;
+            rows2 += z;
+        }
+        smax = z > smax ? z : smax;
     }
     n2 = etm::block_sum<kOne>(n2, lds);
     rows2 = etm::block_sum<kOne>(rows2, lds);
     smax = etm::wave_max(smax);
     if ((tid & 63) == 0) lds[tid / 64] = smax;
     __syncthreads();
-    if (tid == 0) {
-        for (int w = 1; w < kOne / 64; ++w) smax = lds[w] > smax ? lds[w] : smax;
-        smax = lds[0] > smax ? lds[0] : smax;
-    }
+    if (tid == 0)
+        for (int w = 0; w < kOne / 64; ++w) smax = lds[w] > smax ? lds[w] : smax;
     __syncthreads();
     // Class lists, ascending within each class.
+    int64_t cls[kClasses];
     int64_t placed = 0;
     for (int c = 0; c < kClasses; ++c) {
-        int64_t cc = 0;
-        for (int64_t base = 0; base < n_groups; base += kOne) {
-            const int64_t g = base + tid;
-            const int64_t f = g < n_groups && size_class(group_start[g + 1] - group_start[g], small_max) == c;
-            const int64_t ex = etm::block_exclusive_scan<kOne>(f, lds, &tot);
-            if (f) small_list[placed + cc + ex] = (int32_t)g;
-            cc += tot;
-        }
-        cls[c] = cc;
-        placed += cc;
+        auto in_class = [&](int64_t g) { return (int64_t)(size_class(size(g), small_max) == c); };
+        auto put = [&](int64_t g, int64_t v, int64_t at) {
+            if (v) small_list[at] = (int32_t)g;
+        };
+        int64_t cc = placed;
+        for (int64_t base = 0; base < n_groups; base += kPass) cc = pass_scan(base, n_groups, in_class, put, cc, lds);
+        cls[c] = cc - placed;
+        placed = cc;
     }
     if (tid == 0) {
         totals[0] = n_groups;
@@ -603,14 +627,8 @@ __global__ __launch_bounds__(kOne) void k_groups_single(const int32_t* __restric
 __global__ __launch_bounds__(kOne) void k_compact_single(Survives f, WriteSurvivor w, int64_t n_cands,
                                                          int64_t* __restrict__ d_out_n) {
     __shared__ int64_t lds[kOne / 64 + 1];
-    int64_t carry = 0, tot;
-    for (int64_t base = 0; base < n_cands; base += kOne) {
-        const int64_t c = base + threadIdx.x;
-        const int64_t v = c < n_cands ? f(c) : 0;
-        const int64_t ex = etm::block_exclusive_scan<kOne>(v, lds, &tot);
-        if (c < n_cands) w(c, v, carry + ex);
-        carry += tot;
-    }
+    int64_t carry = 0;
+    for (int64_t base = 0; base < n_cands; base += (int64_t)kOne * kPer) carry = pass_scan(base, n_cands, f, w, carry, lds);
     if (threadIdx.x == 0) *d_out_n = carry;
 }
 
@@ -622,14 +640,9 @@ __global__ __launch_bounds__(kOne) void k_select_single(const int32_t* __restric
     __shared__ int64_t lds[kOne / 64 + 1];
     const int64_t n = d_n ? min(*d_n, n_cap) : n_cap;
     WriteSelected w{rows, m, counts, out_rows, out_counts};
-    int64_t carry = 0, tot;
-    for (int64_t base = 0; base < n; base += kOne) {
-        const int64_t i = base + threadIdx.x;
-        const int64_t v = i < n && flag[i] ? 1 : 0;
-        const int64_t ex = etm::block_exclusive_scan<kOne>(v, lds, &tot);
-        w(i, v, carry + ex);
-        carry += tot;
-    }
+    Flagged f{flag};
+    int64_t carry = 0;
+    for (int64_t base = 0; base < n; base += (int64_t)kOne * kPer) carry = pass_scan(base, n, f, w, carry, lds);
     if (threadIdx.x == 0) *d_out_n = carry;
 }
 
