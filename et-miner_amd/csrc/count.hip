@@ -31,6 +31,7 @@ namespace {
 constexpr int kT = 64;   // suffixes per tile
 constexpr int kKW = 32;  // words per LDS step
 constexpr int kXcds = 8;  // MI355X: 8 XCDs of 32 CUs, each with its own L2
+constexpr int kListMax = 1024;  // live pairs from which a tile-pair is counted densely (4096 slots)
 
 // acc + popcount(x) as two v_bcnt_u32_b32 with accumulate (one per 32-bit half).
 __device__ __forceinline__ int popc_acc(uint64_t x, int acc) {
@@ -162,6 +163,8 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
     __shared__ uint64_t sB[kKW][kT + 1];
     __shared__ uint64_t sP[kKW];
     __shared__ int64_t colA[kT], colB[kT];
+    __shared__ uint8_t s_la[kListMax], s_lb[kListMax];
+    __shared__ int s_nlist;
 
     const int tid = threadIdx.x;
     // 1-D grid: XCD x (blockIdx % 8) takes word chunks x, x+8, ..., and all tile-pairs of a chunk in turn.
@@ -182,19 +185,44 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
     const int64_t cbase = cum_pairs[g];
     const int tx = tid & 15, ty = tid >> 4;
 
-    // Liveness of this thread's 16 pairs.
-    bool mine_live = false;
+    // Liveness of this thread's 16 pairs; a sparsely live tile-pair lists its live pairs and counts only those.
+    int my_live = 0;
+    uint32_t live_bits = 0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const int64_t i = ta * kT + tx + 16 * r, j = tb * kT + ty + 16 * c;
-            if (i < j && j < s)
-                mine_live = mine_live || status == nullptr ||
-                            status[cbase + pair_slot(i, j, s, layout)] == ETM_CAND_COUNT;
+            if (i < j && j < s &&
+                (status == nullptr || status[cbase + pair_slot(i, j, s, layout)] == ETM_CAND_COUNT)) {
+                ++my_live;
+                live_bits |= 1u << (4 * r + c);
+            }
         }
     }
-    if (!__syncthreads_or(mine_live)) return;
+    if (tid == 0) s_nlist = 0;
+    __syncthreads();
+    if (my_live) atomicAdd(&s_nlist, my_live);
+    __syncthreads();
+    const int n_live = s_nlist;
+    if (n_live == 0) return;
+    const bool listed = n_live <= kListMax;
+    __syncthreads();
+    if (listed) {
+        if (tid == 0) s_nlist = 0;
+        __syncthreads();
+        if (my_live) {
+            int at = atomicAdd(&s_nlist, my_live);
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                if (live_bits & (1u << q)) {
+                    s_la[at] = (uint8_t)(tx + 16 * (q >> 2));
+                    s_lb[at] = (uint8_t)(ty + 16 * (q & 3));
+                    ++at;
+                }
+        }
+        __syncthreads();
+    }
 
     if (tid < kT) {
         const int64_t i = ta * kT + tid;
@@ -240,6 +268,22 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
         }
         __syncthreads();
         uint64_t(*B)[kT + 1] = diag ? sA : sB;
+        if (listed) {
+            // Thread t counts list entries t, t + 256, t + 512, t + 768.
+#pragma unroll
+            for (int e = 0; e < kListMax / 256; ++e) {
+                const int li = tid + 256 * e;
+                if (li < n_live) {
+                    const int a = s_la[li], b = s_lb[li];
+                    int sum = acc[0][e];
+#pragma unroll 8
+                    for (int w = 0; w < kKW; ++w) sum = popc_acc(sA[w][a] & B[w][b], sum);
+                    acc[0][e] = sum;
+                }
+            }
+            __syncthreads();
+            continue;
+        }
 #pragma unroll 4
         for (int w = 0; w < kKW; ++w) {
             uint64_t av[4], bw[4];
@@ -255,6 +299,17 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
         __syncthreads();
     }
 
+    if (listed) {
+#pragma unroll
+        for (int e = 0; e < kListMax / 256; ++e) {
+            const int li = tid + 256 * e;
+            if (li < n_live) {
+                const int64_t i = ta * kT + s_la[li], j = tb * kT + s_lb[li];
+                if (acc[0][e]) atomicAdd(&counts[cbase + pair_slot(i, j, s, layout)], acc[0][e]);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
 #pragma unroll
@@ -263,6 +318,121 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
             if (i < j && j < s) {
                 const int64_t slot = cbase + pair_slot(i, j, s, layout);
                 if (status == nullptr || status[slot] == ETM_CAND_COUNT) atomicAdd(&counts[slot], acc[r][c]);
+            }
+        }
+    }
+}
+
+// ── int8 MFMA Gram of a single-group level (the K=2 pair space) ───────────────────────────────────────────
+//
+// One workgroup per (128 x 128 tile-pair of columns, word chunk). Per 4-word step (256 transactions = 8 K-steps of
+// 32 rows) each thread unpacks one column's bits to 0/1 bytes in LDS (laid out [K-step][column][row half], 16
+// bytes per MFMA operand), then each of the 4 waves runs a 2 x 2 block of v_mfma_i32_32x32x32_i8 on its 64 x 64
+// quarter of the output. Operand lane l carries column l % 32 and rows 16 (l / 32) .. +15 of the K-step for both
+// A and B, so the product is the exact Gram sum whatever the instruction's K order within a lane. Counts
+// accumulate in int32 (exact for fewer than 2^31 transactions) and leave with one integer atomic per pair.
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int kMT = 128;  // columns per MFMA tile
+constexpr int kMW = 4;    // words per LDS step
+
+// Bits 4q..4q+3 of x as four 0/1 bytes (bit t -> byte t).
+__device__ __forceinline__ int spread4(uint32_t x, int q) {
+    return (int)((__umul24((x >> (4 * q)) & 0xFu, 0x00204081u)) & 0x01010101u);
+}
+
+__global__ __launch_bounds__(256) void k_gram_mfma(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
+                                                    int64_t chunk_words, const int32_t* __restrict__ cols, int64_t s,
+                                                    int64_t n_tp, int64_t n_chunks, int32_t* __restrict__ counts,
+                                                    int32_t layout) {
+    __shared__ v4i ldsA[2 * kMW][kMT][2];
+    __shared__ v4i ldsB[2 * kMW][kMT][2];
+    __shared__ int64_t colA[kMT], colB[kMT];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t xcd = blockIdx.x % kXcds;
+    const int64_t local = blockIdx.x / kXcds;
+    const int64_t chunk_id = xcd + kXcds * (local / n_tp);
+    if (chunk_id >= n_chunks) return;
+    const int64_t tp = local % n_tp;
+    const int64_t nt = (s + kMT - 1) / kMT;
+    int64_t ta, tb1;
+    etm::lex_pair_decode(tp, nt + 1, &ta, &tb1);
+    const int64_t tb = tb1 - 1;
+    if (tid < kMT) {
+        const int64_t i = ta * kMT + tid;
+        colA[tid] = i < s ? (int64_t)cols[i] * pitch : -1;
+    } else {
+        const int64_t j = tb * kMT + (tid - kMT);
+        colB[tid - kMT] = j < s ? (int64_t)cols[j] * pitch : -1;
+    }
+    const int wm = wave >> 1, wn = wave & 1;
+    v16i acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
+    const int64_t w_begin = chunk_id * chunk_words;
+    const int64_t w_end = min(n_words, w_begin + chunk_words);
+    const int c = tid % kMT;
+    const bool side_b = tid >= kMT;
+    __syncthreads();
+    const int64_t base = side_b ? colB[c] : colA[c];
+    // Register double buffering: the next step's words are in flight while this step's MFMAs run.
+    uint64_t next[kMW];
+#pragma unroll
+    for (int w = 0; w < kMW; ++w) next[w] = (base >= 0 && w_begin + w < w_end) ? bv[base + w_begin + w] : 0ull;
+    for (int64_t wb = w_begin; wb < w_end; wb += kMW) {
+        uint64_t words[kMW];
+#pragma unroll
+        for (int w = 0; w < kMW; ++w) words[w] = next[w];
+#pragma unroll
+        for (int w = 0; w < kMW; ++w) {
+#pragma unroll
+            for (int hw = 0; hw < 2; ++hw) {
+                const uint32_t b32 = (uint32_t)(words[w] >> (32 * hw));
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const uint32_t b16 = b32 >> (16 * h);
+                    v4i v;
+                    v[0] = spread4(b16, 0);
+                    v[1] = spread4(b16, 1);
+                    v[2] = spread4(b16, 2);
+                    v[3] = spread4(b16, 3);
+                    if (side_b) ldsB[2 * w + hw][c][h] = v;
+                    else ldsA[2 * w + hw][c][h] = v;
+                }
+            }
+        }
+        __syncthreads();
+        const int64_t nb = wb + kMW;
+#pragma unroll
+        for (int w = 0; w < kMW; ++w) next[w] = (base >= 0 && nb + w < w_end) ? bv[base + nb + w] : 0ull;
+#pragma unroll
+        for (int q = 0; q < 2 * kMW; ++q) {
+            v4i a0 = ldsA[q][wm * 64 + (lane & 31)][lane >> 5];
+            v4i a1 = ldsA[q][wm * 64 + 32 + (lane & 31)][lane >> 5];
+            v4i b0 = ldsB[q][wn * 64 + (lane & 31)][lane >> 5];
+            v4i b1 = ldsB[q][wn * 64 + 32 + (lane & 31)][lane >> 5];
+            acc[0][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // acc[mi][ni][r]: row 8 (r / 4) + 4 (lane / 32) + r % 4 of the A block, column lane % 32 of the B block.
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi) {
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t i = ta * kMT + wm * 64 + mi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int64_t j = tb * kMT + wn * 64 + ni * 32 + (lane & 31);
+                if (i < j && j < s && acc[mi][ni][r]) atomicAdd(&counts[pair_slot(i, j, s, layout)], acc[mi][ni][r]);
             }
         }
     }
@@ -326,7 +496,7 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
                                const int64_t* d_group_start, const int64_t* d_cum_pairs, const int64_t* d_cum_tiles,
                                int64_t n_groups, int64_t n_tile_pairs, const int32_t* d_small_list,
                                const int64_t* h_class_counts, const uint8_t* d_status, int32_t* d_counts,
-                               int32_t layout, int32_t word_chunk, void* stream) {
+                               int32_t layout, int32_t word_chunk, int32_t kernel, void* stream) {
     ETM_ARG(d_bv && d_rows && d_group_start && d_cum_pairs && d_counts, "etm_count_level: null pointer");
     ETM_ARG(m >= 1 && m <= 62, "etm_count_level: 1 <= m <= 62 required, got %d", m);
     ETM_ARG(pitch >= n_words && n_words >= 0, "etm_count_level: pitch %lld < n_words %lld", (long long)pitch,
@@ -363,6 +533,28 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
         }
         ETM_REQUIRE(e == hipSuccess, ETM_EHIP, "launch of k_count_small failed: %s", hipGetErrorString(e));
         base += cls[c];
+    }
+    // The pair space of a single-group level of one-item rows (K=2) without a status array: the int8 MFMA Gram
+    // when it has at least kMT columns (or when asked for), else the VALU tiled kernel.
+    const bool gram_level = n_groups == 1 && m == 1 && d_status == nullptr;
+    if (n_tile_pairs > 0 && gram_level && kernel != ETM_KERNEL_VALU) {
+        int64_t s_cols = 0;
+        ETM_HIP(hipMemcpyAsync(&s_cols, d_group_start + 1, sizeof s_cols, hipMemcpyDeviceToHost, s));
+        ETM_HIP(hipStreamSynchronize(s));
+        if (kernel == ETM_KERNEL_MFMA || s_cols >= kMT) {
+            const int64_t nt = ceil_div(s_cols, kMT);
+            const int64_t n_tp = nt * (nt + 1) / 2;
+            const int64_t chunk = word_chunk > 0 ? ceil_div(word_chunk, kMW) * kMW
+                                                 : pick_chunk(n_words, n_tp, 1024, kMW, 512);
+            const int64_t n_chunks = ceil_div(n_words, chunk);
+            const int64_t blocks = kXcds * n_tp * ceil_div(n_chunks, kXcds);
+            ETM_REQUIRE(blocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",
+                        (long long)blocks);
+            k_gram_mfma<<<(unsigned)blocks, 256, 0, s>>>(d_bv, pitch, n_words, chunk, d_rows, s_cols, n_tp, n_chunks,
+                                                         d_counts, layout);
+            ETM_LAUNCHED("k_gram_mfma");
+            return ETM_OK;
+        }
     }
     if (n_tile_pairs > 0) {
         ETM_REQUIRE(n_tile_pairs <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld tile-pairs exceed the grid",

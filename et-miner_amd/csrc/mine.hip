@@ -295,7 +295,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         }
         const int64_t cls[4] = {tot[8], tot[9], tot[10], tot[11]};
         check_rc(etm_count_level(d_bv, pitch, n_words, prev.rows, m, g.gs, g.cp, g.ct, n_groups, n_tiles, g.list, cls,
-                                 status, counts, ETM_LAYOUT_LEX, 0, s),
+                                 status, counts, ETM_LAYOUT_LEX, 0, k == 2 ? o.k2_kernel : ETM_KERNEL_VALU, s),
                  "etm_count_level");
         if (o.time_kernels) {
             check(hipEventRecord(e1, s), "event record");

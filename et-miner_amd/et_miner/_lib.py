@@ -22,6 +22,8 @@ ETM_OK, ETM_EINVAL, ETM_ENOMEM, ETM_EHIP, ETM_EOVERFLOW, ETM_ENODEV = 0, 1, 2, 3
 CAND_COUNT, CAND_SKIP, CAND_INFER = 0, 1, 2
 SUBSET_PRUNE, SUBSET_INFER = 1, 2
 LAYOUT_LEX, LAYOUT_TRI = 0, 1
+KERNEL_AUTO, KERNEL_VALU, KERNEL_MFMA = 0, 1, 2
+K2_KERNELS = {"auto": KERNEL_AUTO, "valu": KERNEL_VALU, "mfma": KERNEL_MFMA}
 
 P = c_void_p  # every device pointer crosses the ABI as void*
 
@@ -40,7 +42,7 @@ SIGNATURES: dict = {
     "etm_level_classify": (c_int, [P, c_int64, c_int32, P, P, P, c_int64, P, P, c_int64, c_int64, P, P, c_int32, P,
                                    P, P, P, P]),
     "etm_count_level": (c_int, [P, c_int64, c_int64, P, c_int32, P, P, P, c_int64, c_int64, P, P, P, P, c_int32,
-                                c_int32, P]),
+                                c_int32, c_int32, P]),
     "etm_level_compact_ws": (c_size_t, [c_int64]),
     "etm_level_compact": (c_int, [P, c_int32, P, P, c_int64, c_int64, P, P, P, c_int64, P, P, P, P, P, c_size_t, P]),
     "etm_level_free": (c_int, [P, c_int64, c_int32, P, P, P, P, c_int64, P, P]),
@@ -62,7 +64,7 @@ class MineOpts(ctypes.Structure):
     """etm_mine_opts."""
 
     _fields_ = [("min_count", c_int64), ("max_length", c_int32), ("prune_non_free", c_int32),
-                ("infer_counts", c_int32), ("time_kernels", c_int32)]
+                ("infer_counts", c_int32), ("time_kernels", c_int32), ("k2_kernel", c_int32)]
 
 
 class LevelStat(ctypes.Structure):

@@ -166,13 +166,16 @@ def level_classify(rows, m: int, prev_counts, prev_free, table, table_size: int,
 
 
 def count_level(bv, n_words: int, rows, m: int, groups: Groups, status, counts, layout: int = LAYOUT_LEX,
-                word_chunk: int = 0, pitch: int | None = None) -> None:
-    """Adds the AND + popcount counts of the counted candidates into ``counts``."""
+                word_chunk: int = 0, pitch: int | None = None, kernel: str = "auto") -> None:
+    """Adds the AND + popcount counts of the counted candidates into ``counts``.
+
+    ``kernel`` ("auto", "valu", "mfma") picks the K=2 pair space's kernel (ignored for other levels)."""
     pitch = int(bv.shape[1]) if pitch is None else int(pitch)
     cls = (ctypes.c_int64 * 4)(*groups.class_counts[:4])
     call("etm_count_level", bv.data_ptr(), pitch, n_words, rows.data_ptr(), m, groups.group_start.data_ptr(),
          groups.cum_pairs.data_ptr(), groups.cum_tiles.data_ptr(), groups.n_groups, groups.n_tile_pairs,
-         groups.small_list.data_ptr(), cls, dev.ptr(status), counts.data_ptr(), layout, word_chunk, _s())
+         groups.small_list.data_ptr(), cls, dev.ptr(status), counts.data_ptr(), layout, word_chunk,
+         _lib.K2_KERNELS[kernel], _s())
 
 
 def level_compact(rows, m: int, groups: Groups, status, counts, min_count: int, with_cand: bool = False, gid=None):
@@ -220,7 +223,7 @@ def select_rows(rows, m: int, counts, flag):
     return out_rows, out_counts
 
 
-def count_pairs_k2_shared(bv, freq_cols, n_words: int, layout: int = LAYOUT_TRI):
+def count_pairs_k2_shared(bv, freq_cols, n_words: int, layout: int = LAYOUT_TRI, kernel: str = "auto"):
     """Dense int32 counts of every pair of ``freq_cols`` (reference ``shared_tiled.py:145-150``).
 
     Pair (a < b) of the column list lives at b(b-1)/2 + a (LAYOUT_TRI, the reference's layout) or at its
@@ -233,7 +236,7 @@ def count_pairs_k2_shared(bv, freq_cols, n_words: int, layout: int = LAYOUT_TRI)
     rows = dev.to_device(cols)
     g = level_groups(rows, 1)
     counts = dev.zeros(g.n_cands, "int32")
-    count_level(bv, n_words, rows, 1, g, None, counts, layout=layout)
+    count_level(bv, n_words, rows, 1, g, None, counts, layout=layout, kernel=kernel)
     return counts
 
 

@@ -39,9 +39,9 @@ class NativeResult:
         return pa.table(cols)
 
 
-def _opts(min_count, max_length, prune, infer, time_kernels) -> _lib.MineOpts:
+def _opts(min_count, max_length, prune, infer, time_kernels, k2_kernel="auto") -> _lib.MineOpts:
     return _lib.MineOpts(int(max(1, min_count)), int(max_length or 0), int(bool(prune)), int(bool(infer)),
-                         int(bool(time_kernels)))
+                         int(bool(time_kernels)), _lib.K2_KERNELS[k2_kernel])
 
 
 def _collect(handle, levels: bool = True) -> NativeResult:
@@ -83,14 +83,15 @@ def _collect(handle, levels: bool = True) -> NativeResult:
 
 
 def mine_csr_device(indptr_d, ids_d, n_rows: int, min_count: int, max_length=None, prune_non_free=False,
-                    infer_counts=False, time_kernels=False, levels: bool = True) -> NativeResult:
+                    infer_counts=False, time_kernels=False, levels: bool = True,
+                    k2_kernel: str = "auto") -> NativeResult:
     """Every frequent itemset of a device CSR of item ids (int32 or int64), mined by the native loop."""
     torch = dev.torch()
     id_bytes = 8 if ids_d.dtype == torch.int64 else 4
     if ids_d.dtype not in (torch.int64, torch.int32):
         raise ValueError(f"item ids must be int32 or int64, got {ids_d.dtype}")
     handle = ctypes.c_void_p()
-    opts = _opts(min_count, max_length, prune_non_free, infer_counts, time_kernels)
+    opts = _opts(min_count, max_length, prune_non_free, infer_counts, time_kernels, k2_kernel)
     _lib.call("etm_mine_csr", indptr_d.data_ptr(), ids_d.data_ptr() if ids_d.numel() else None, id_bytes, n_rows,
               ids_d.numel(), SPAN_LIMIT, ctypes.byref(opts), ctypes.byref(handle), dev.stream_ptr())
     return _collect(handle, levels)
@@ -98,11 +99,11 @@ def mine_csr_device(indptr_d, ids_d, n_rows: int, min_count: int, max_length=Non
 
 def mine_bitvecs_device(bv_ptr: int, pitch: int, n_words: int, n_rows: int, col_counts: np.ndarray, min_count: int,
                         max_length=None, prune_non_free=False, infer_counts=False,
-                        time_kernels=False) -> NativeResult:
+                        time_kernels=False, k2_kernel: str = "auto") -> NativeResult:
     """Every frequent itemset of device bitvectors (read only), from the columns' K=1 counts."""
     counts = np.ascontiguousarray(col_counts, dtype=np.int64)
     handle = ctypes.c_void_p()
-    opts = _opts(min_count, max_length, prune_non_free, infer_counts, time_kernels)
+    opts = _opts(min_count, max_length, prune_non_free, infer_counts, time_kernels, k2_kernel)
     _lib.call("etm_mine_bitvecs", bv_ptr, pitch, n_words, n_rows, counts.ctypes.data, len(counts),
               ctypes.byref(opts), ctypes.byref(handle), dev.stream_ptr())
     return _collect(handle)

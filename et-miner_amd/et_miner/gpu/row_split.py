@@ -93,6 +93,7 @@ def mine_levels(
     pitch: int | None = None,
     keep_device: bool = False,
     timer=None,
+    k2_kernel: str = "auto",
 ) -> MineResult:
     """Mines every frequent itemset (or free-set) of device bitvectors ``bv`` (n_cols, pitch).
 
@@ -146,7 +147,7 @@ def mine_levels(
                         ((m - 1) * groups.n_groups2 + groups.rows2) * n_words * 8 + 4 * groups.n_cands)
         if timer is not None:
             timer.begin(k)
-        K.count_level(bv, n_words, prev.rows, m, groups, status, counts, pitch=pitch)
+        K.count_level(bv, n_words, prev.rows, m, groups, status, counts, pitch=pitch, kernel=k2_kernel)
         if timer is not None:
             timer.end(k, st)
         rows_k, counts_k = K.level_compact(prev.rows, m, groups, status, counts, min_count, gid=gid)

@@ -53,6 +53,9 @@ enum { ETM_CAND_COUNT = 0, ETM_CAND_SKIP = 1, ETM_CAND_INFER = 2 };
 enum { ETM_SUBSET_PRUNE = 1, ETM_SUBSET_INFER = 2 };
 /* Pair layouts of the K=2 count array: lexicographic (i-major), or the reference's b(b-1)/2 + a. */
 enum { ETM_LAYOUT_LEX = 0, ETM_LAYOUT_TRI = 1 };
+/* Kernel choice for the pair space of a single-group level (K=2): automatic (int8 MFMA Gram from 128 columns,
+ * else VALU AND+popcount tiles), VALU tiles only, or the MFMA Gram regardless of size. */
+enum { ETM_KERNEL_AUTO = 0, ETM_KERNEL_VALU = 1, ETM_KERNEL_MFMA = 2 };
 
 /* ABI version of this header (bumped on any signature change). */
 #define ETM_ABI_VERSION 1
@@ -143,7 +146,7 @@ int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const 
                     const int64_t* d_group_start, const int64_t* d_cum_pairs, const int64_t* d_cum_tiles,
                     int64_t n_groups, int64_t n_tile_pairs, const int32_t* d_small_list,
                     const int64_t* h_class_counts, const uint8_t* d_status, int32_t* d_counts, int32_t layout,
-                    int32_t word_chunk, void* stream);
+                    int32_t word_chunk, int32_t kernel, void* stream);
 
 /* Workspace bytes etm_level_compact needs for n_cands candidates. */
 size_t etm_level_compact_ws(int64_t n_cands);
@@ -190,6 +193,7 @@ typedef struct etm_mine_opts {
     int32_t prune_non_free; /* free-set mode (prune_equal_support) */
     int32_t infer_counts;   /* Pascal inference (use_generator_pruning) */
     int32_t time_kernels;   /* record HIP events around each level's counting launches */
+    int32_t k2_kernel;      /* ETM_KERNEL_* for the K=2 pair space */
 } etm_mine_opts;
 
 /* Per-level statistics: candidates generated by the prefix join and, of them, counted / inferred / skipped by

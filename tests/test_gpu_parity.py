@@ -103,8 +103,9 @@ def _numpy_pairs(bv: np.ndarray, cols: np.ndarray) -> dict:
     return out
 
 
-@pytest.mark.parametrize("n_cols,n_words", [(2, 1), (5, 7), (16, 130), (17, 64), (70, 33), (130, 100)])
-def test_k2_counts_both_layouts(n_cols, n_words):
+@pytest.mark.parametrize("kernel", ["valu", "mfma"])
+@pytest.mark.parametrize("n_cols,n_words", [(2, 1), (5, 7), (16, 130), (17, 64), (70, 33), (130, 100), (300, 21)])
+def test_k2_counts_both_layouts(n_cols, n_words, kernel):
     from et_miner.gpu import device as dev
     from et_miner.gpu import kernels as K
 
@@ -114,8 +115,8 @@ def test_k2_counts_both_layouts(n_cols, n_words):
     cols = np.sort(rng.choice(n_cols + 3, size=n_cols, replace=False)).astype(np.int32)
     d_bv = dev.to_device(bv)
     want = _numpy_pairs(bv, cols)
-    tri = dev.to_host(K.count_pairs_k2_shared(d_bv, cols, n_words, layout=K.LAYOUT_TRI))
-    lex = dev.to_host(K.count_pairs_k2_shared(d_bv, cols, n_words, layout=K.LAYOUT_LEX))
+    tri = dev.to_host(K.count_pairs_k2_shared(d_bv, cols, n_words, layout=K.LAYOUT_TRI, kernel=kernel))
+    lex = dev.to_host(K.count_pairs_k2_shared(d_bv, cols, n_words, layout=K.LAYOUT_LEX, kernel=kernel))
     s = len(cols)
     for (a, b), c in want.items():
         assert tri[b * (b - 1) // 2 + a] == c
@@ -194,6 +195,17 @@ def test_workloads_match_oracle(name, n_rows, engine):
     got = _gpu_mine(data.indptr, data.indices, w.min_support, w.max_length, engine,
                     use_generator_pruning=w.use_generator_pruning)
     assert got == want
+
+
+@pytest.mark.parametrize("k2_kernel", ["valu", "mfma"])
+def test_k2_kernels_mine_identically(k2_kernel):
+    """The int8 MFMA Gram and the VALU tiles give the same lattice (wide vocabulary: hundreds of columns)."""
+    from et_miner.synthetic import workload
+
+    w = workload("wide_vocab_2m", 150_000)
+    data = w.generate()
+    want = miner.mine(data.indptr, data.indices, w.min_support, 2).as_dict()
+    assert _gpu_mine(data.indptr, data.indices, w.min_support, 2, k2_kernel=k2_kernel) == want
 
 
 def test_son_matches_in_core():
