@@ -6,6 +6,8 @@
 // transaction's columns in a register bitmask and a wave ballot per column
 // produces each 64-bit word, so no atomic is issued; each column's 32 words
 // leave as one contiguous 256-byte run.
+#include <algorithm>
+
 #include "common.h"
 
 namespace {
@@ -51,8 +53,9 @@ __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
         p[i] = v;
 }
 
-// Histogram through a per-workgroup LDS copy (span <= kLdsBins), flushed with one global add per bin.
-constexpr int kLdsBins = 16384;
+// Histogram through a per-workgroup LDS copy (span <= kLdsBins: 128 KiB of gfx950's 160 KiB LDS), flushed with
+// one global add per bin; wider spans take global atomics.
+constexpr int kLdsBins = 32768;
 
 template <typename Id>
 __global__ __launch_bounds__(256) void k_hist_lds(const Id* __restrict__ ids, int64_t n, int64_t lo, int32_t span,
@@ -178,7 +181,8 @@ extern "C" int etm_ids_hist(const void* d_ids, int id_bytes, int64_t n, int64_t 
     ETM_LAUNCHED("k_fill_i32");
     if (n == 0) return ETM_OK;
     if (span <= kLdsBins) {
-        const int g = grid_for(n, 256 * 64, 1024);
+        // Each workgroup flushes every nonzero bin once: give it at least 4 x span ids.
+        const int g = grid_for(n, (int)std::max<int64_t>(4 * span, 16384), 512);
         if (id_bytes == 4)
             k_hist_lds<int32_t><<<g, 256, 0, s>>>((const int32_t*)d_ids, n, lo, (int32_t)span, d_hist);
         else

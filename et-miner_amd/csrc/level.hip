@@ -3,6 +3,8 @@
 // test and Pascal inference), survivor compaction in lexicographic order, and
 // free flags. Nothing here returns to the host except the scalar totals the
 // caller reads to size the next buffers.
+#include <algorithm>
+
 #include "common.h"
 #include "internal.h"
 #include "scan.h"
@@ -359,22 +361,28 @@ __global__ __launch_bounds__(256) void k_classify_small(Classifier C, const int6
     }
 }
 
+// Big groups: blockIdx.y picks the group, blockIdx.x a range of kBigPairs of its candidates.
+constexpr int64_t kBigPairs = 4096;
+
 __global__ __launch_bounds__(256) void k_classify_big(Classifier C, const int64_t* __restrict__ group_start,
                                                        const int64_t* __restrict__ cum_pairs,
                                                        const int32_t* __restrict__ list,
                                                        uint8_t* __restrict__ status, int32_t* __restrict__ counts,
                                                        int32_t* __restrict__ gid, int64_t* __restrict__ tally) {
     __shared__ int64_t lds[5];
-    const int64_t g = list[blockIdx.x];
+    const int64_t g = list[blockIdx.y];
     const int64_t gs = group_start[g];
     const int64_t s = group_start[g + 1] - gs;
     const int64_t npairs = s * (s - 1) / 2;
+    const int64_t p0 = (int64_t)blockIdx.x * kBigPairs;
+    if (p0 >= npairs) return;
+    const int64_t p1 = min(npairs, p0 + kBigPairs);
     const int64_t cbase = cum_pairs[g];
     const int np = C.m - 1;
     const bool infer = (C.mode & ETM_SUBSET_INFER) != 0;
     const bool prune = (C.mode & ETM_SUBSET_PRUNE) != 0;
     int64_t n_count = 0, n_infer = 0, n_skip = 0;
-    for (int64_t p = threadIdx.x; p < npairs; p += blockDim.x) {
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
         int64_t i, j;
         etm::lex_pair_decode(p, s, &i, &j);
         const int64_t ri = gs + i, rj = gs + j;
@@ -802,8 +810,14 @@ extern "C" int etm_level_classify(const int32_t* d_rows, int64_t n, int32_t m, c
             ETM_LAUNCHED("k_classify_small");
         }
         if (big > 0) {
-            k_classify_big<<<(unsigned)big, 256, 0, s>>>(C, d_group_start, d_cum_pairs, d_group_list + small,
-                                                         d_status, d_counts, d_gid, d_tally);
+            // Pair chunks per group: enough for the largest group (h_class_counts[5], when given, is its size).
+            const int64_t max_s = h_class_counts[5] > 0 ? h_class_counts[5] : 0;
+            const int64_t max_pairs = max_s > 0 ? max_s * (max_s - 1) / 2 : n_cands;
+            const int64_t gx = std::max<int64_t>(1, (max_pairs + kBigPairs - 1) / kBigPairs);
+            ETM_REQUIRE(gx <= 0x7FFFFFFF && big <= 65535, ETM_EOVERFLOW, "etm_level_classify: grid too large");
+            k_classify_big<<<dim3((unsigned)gx, (unsigned)big), 256, 0, s>>>(C, d_group_start, d_cum_pairs,
+                                                                             d_group_list + small, d_status, d_counts,
+                                                                             d_gid, d_tally);
             ETM_LAUNCHED("k_classify_big");
         }
         return ETM_OK;

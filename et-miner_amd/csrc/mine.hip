@@ -274,7 +274,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         uint8_t* status = nullptr;
         int32_t* gid = nullptr;
         int32_t* counts = a.get<int32_t>(n_cands);
-        const int64_t classes[5] = {tot[8], tot[9], tot[10], tot[11], tot[12]};
+        const int64_t classes[6] = {tot[8], tot[9], tot[10], tot[11], tot[12], tot[4]};
         if (k >= 3) {
             status = a.get<uint8_t>(n_cands);
             gid = a.get<int32_t>(n_cands);

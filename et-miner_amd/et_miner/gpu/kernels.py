@@ -157,7 +157,7 @@ def level_classify(rows, m: int, prev_counts, prev_free, table, table_size: int,
     status = dev.empty(groups.n_cands, "uint8")
     counts = dev.empty(groups.n_cands, "int32")
     gid = dev.empty(groups.n_cands, "int32") if group_centric else None
-    cls = (ctypes.c_int64 * 5)(*groups.class_counts) if group_centric else None
+    cls = (ctypes.c_int64 * 6)(*groups.class_counts, groups.max_suffixes) if group_centric else None
     call("etm_level_classify", rows.data_ptr(), int(rows.shape[0]), m, dev.ptr(prev_counts), dev.ptr(prev_free),
          table.data_ptr(), table_size, groups.group_start.data_ptr(), groups.cum_pairs.data_ptr(), groups.n_groups,
          groups.n_cands, groups.small_list.data_ptr() if group_centric else None, cls, mode, status.data_ptr(),

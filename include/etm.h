@@ -121,8 +121,8 @@ int etm_level_hash(const int32_t* d_rows, int64_t n, int32_t m, int32_t* d_table
 /* Subset test and Pascal inference of every candidate of a level's prefix groups, against the hash of the
  * same level (the candidates' (k-1)-subsets). Writes d_status[c] (ETM_CAND_*) and initialises d_counts[c]
  * (0 when counted, the inferred count when inferred, -1 when skipped). mode: ETM_SUBSET_PRUNE, optionally |
- * ETM_SUBSET_INFER (then d_prev_free is required). With d_group_list and h_class_counts[5] (etm_level_groups'
- * class lists and sizes) the work is group-centric (one wave per small group, lanes over candidate x subset;
+ * ETM_SUBSET_INFER (then d_prev_free is required). With d_group_list and h_class_counts[6] (etm_level_groups'
+ * five class sizes, then the largest group's size, d_totals[4]) the work is group-centric (one wave per small group, lanes over candidate x subset;
  * one workgroup per big group) and d_gid (optional) receives every candidate's group; without them, one
  * thread per candidate finds its group by binary search. When d_tally is not NULL, adds the numbers of
  * counted, inferred and skipped candidates to d_tally[0..2].
