@@ -186,17 +186,16 @@ def _distinct_draws(rng: np.random.Generator, lens: np.ndarray, cdf: np.ndarray,
     """(row ids, item ranks): per row the first ``lens[r]`` distinct values of an i.i.d. stream drawn by ``cdf``.
 
     Rows are drawn in rounds: each round draws ``need + margin`` more values for every row still short,
-    appends them to the row's stream, and keeps each value's first occurrence in stream order.
+    appends them to the row's stream, and keeps each value's first occurrence in stream order. Rows that are
+    complete leave the working set, so later rounds sort only the rows still short.
     """
     n = len(lens)
     lens = np.minimum(lens, vocab).astype(np.int64)
-    have_rows: list = []
-    have_items: list = []
-    have_pos: list = []
     got = np.zeros(n, dtype=np.int64)
     stream_len = np.zeros(n, dtype=np.int64)
     short = np.flatnonzero(lens > 0)
-    # Distinct (row, item) seen so far, as sorted keys with their stream positions.
+    done: list = []
+    # Distinct (row, item) keys of the rows still short, sorted, with their stream positions.
     seen_keys = np.empty(0, dtype=np.int64)
     seen_pos = np.empty(0, dtype=np.int64)
     while len(short):
@@ -209,22 +208,43 @@ def _distinct_draws(rng: np.random.Generator, lens: np.ndarray, cdf: np.ndarray,
         stream_len[short] += extra
         keys = np.concatenate([seen_keys, r * vocab + it])
         allpos = np.concatenate([seen_pos, pos])
-        order = np.lexsort((allpos, keys))
-        keys, allpos = keys[order], allpos[order]
-        firsts = np.r_[True, keys[1:] != keys[:-1]]
-        seen_keys, seen_pos = keys[firsts], allpos[firsts]
-        # Keep, per row, the lens[r] earliest distinct values.
-        rr = seen_keys // vocab
-        order = np.lexsort((seen_pos, rr))
-        rr_o = rr[order]
-        starts = np.searchsorted(rr_o, np.arange(n), side="left")
-        rank = np.arange(len(rr_o)) - starts[rr_o]
-        keep = rank < lens[rr_o]
-        sel = order[keep]
-        seen_keys, seen_pos = seen_keys[np.sort(sel)], seen_pos[np.sort(sel)]
-        got = np.bincount(seen_keys // vocab, minlength=n)
-        short = np.flatnonzero(got < lens)
-    return seen_keys // vocab, seen_keys % vocab
+        span = int(stream_len.max()) + 1
+        if n * vocab * span * vocab < (1 << 62):  # one int64 key per (row, item, position): plain sorts
+            c = np.sort(keys * span + allpos)
+            keys, allpos = c // span, c % span
+            firsts = np.r_[True, keys[1:] != keys[:-1]]
+            keys, allpos = keys[firsts], allpos[firsts]
+            # Keep, per row, the lens[r] earliest distinct values.
+            rr = keys // vocab
+            c = np.sort((rr * span + allpos) * vocab + keys % vocab)
+            rr_o = c // (span * vocab)
+            head = np.r_[True, rr_o[1:] != rr_o[:-1]]
+            first_of_row = np.maximum.accumulate(np.where(head, np.arange(len(rr_o)), 0))
+            keep = (np.arange(len(rr_o)) - first_of_row) < lens[rr_o]
+            c = c[keep]
+            kp = np.sort((rr_o[keep] * vocab + c % vocab) * span + (c // vocab) % span)  # back to key order
+            keys, allpos = kp // span, kp % span
+        else:
+            order = np.lexsort((allpos, keys))
+            keys, allpos = keys[order], allpos[order]
+            firsts = np.r_[True, keys[1:] != keys[:-1]]
+            keys, allpos = keys[firsts], allpos[firsts]
+            rr = keys // vocab
+            order = np.lexsort((allpos, rr))
+            rr_o = rr[order]
+            head = np.r_[True, rr_o[1:] != rr_o[:-1]]
+            first_of_row = np.maximum.accumulate(np.where(head, np.arange(len(rr_o)), 0))
+            keep = (np.arange(len(rr_o)) - first_of_row) < lens[rr_o]
+            sel = np.sort(order[keep])
+            keys, allpos = keys[sel], allpos[sel]
+        rr = keys // vocab
+        got[short] = np.bincount(rr, minlength=n)[short]
+        fin = got[rr] >= lens[rr]
+        done.append(keys[fin])
+        seen_keys, seen_pos = keys[~fin], allpos[~fin]
+        short = short[got[short] < lens[short]]
+    keys = np.sort(np.concatenate(done)) if done else np.empty(0, dtype=np.int64)
+    return keys // vocab, keys % vocab
 
 
 def _rows_to_csr(rows: np.ndarray, items: np.ndarray, n: int, vocab: int) -> tuple:
