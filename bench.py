@@ -40,6 +40,18 @@ sys.path.insert(0, str(ROOT / "et-miner_amd"))
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = ROOT / "profiles" / "deep_k_1m_pmc_traffic.md"
+
+
+def measured_traffic(workload_name: str):
+    """HBM bytes per step of the K>=3 counting dispatches, from the committed rocprofv3 PMC summary (FETCH_SIZE +
+    WRITE_SIZE, each scaled by a plain copy's reading); None when no summary covers the workload."""
+    import re
+
+    if workload_name != "deep_k_1m" or not TRAFFIC_PROFILE.exists():
+        return None
+    m = re.search(r"^traffic_bytes_per_step = (\d+)$", TRAFFIC_PROFILE.read_text(), re.M)
+    return int(m.group(1)) if m else None
 VALU_CW_PEAK = 9.8e12  # candidate-words/s: 256 CUs x 64 lanes x 2.4 GHz / 4 VALU ops per word
 
 
@@ -227,7 +239,8 @@ def main(argv=None) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(w.name) if args.rows is None else None,
+            "traffic_unit": "HBM bytes per step (rocprofv3 FETCH_SIZE + WRITE_SIZE over the same dispatches)",
             "bytes_alg_per_step": k3_bytes,
             "kernel_ms_per_step": k3_ms,
             "launches_per_step": k3_launches,
