@@ -1,11 +1,11 @@
 // Input side of the counting path: item-id range and histogram, the CSR ->
 // column-bitvector builder, and the K=1 column popcount.
 //
-// Builder layout: a workgroup owns a tile of 32 consecutive words (2,048
+// Builder layout: a workgroup owns a tile of 8 consecutive words (512
 // transactions) times a slab of 128 or 256 columns; each lane collects one
 // transaction's columns in a register bitmask and a wave ballot per column
-// produces each 64-bit word, so no atomic is issued; each column's 32 words
-// leave as one contiguous 256-byte run.
+// produces each 64-bit word, so no atomic is issued; each column's 8 words
+// leave as one contiguous 64-byte run.
 #include <algorithm>
 
 #include "common.h"
@@ -14,15 +14,71 @@ namespace {
 
 constexpr int kBuildThreads = 256;
 
+// Ids as 16-byte vectors: 2 int64 or 4 int32 per load. The vector body covers the 16-byte aligned prefix
+// that holds whole vectors; the remaining ids go through scalar loads.
+template <typename Id>
+struct IdVec;
+template <>
+struct IdVec<int64_t> {
+    using V = longlong2;
+    static constexpr int kPer = 2;
+    __device__ static void get(const V& v, int64_t (&o)[2]) {
+        o[0] = v.x;
+        o[1] = v.y;
+    }
+};
+template <>
+struct IdVec<int32_t> {
+    using V = int4;
+    static constexpr int kPer = 4;
+    __device__ static void get(const V& v, int64_t (&o)[4]) {
+        o[0] = v.x;
+        o[1] = v.y;
+        o[2] = v.z;
+        o[3] = v.w;
+    }
+};
+
+// Calls fn(id) for every id of ids[0..n), each thread keeping 4 vector loads in flight (grid-stride).
+template <typename Id, class Fn>
+__device__ __forceinline__ void for_each_id(const Id* __restrict__ ids, int64_t n, Fn fn) {
+    using IV = IdVec<Id>;
+    constexpr int P = IV::kPer;
+    const int64_t nv = ((uintptr_t)ids % 16 == 0) ? n / P : 0;
+    const typename IV::V* vec = reinterpret_cast<const typename IV::V*>(ids);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < nv; i += 4 * stride) {
+        typename IV::V v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = vec[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            int64_t o[P];
+            IV::get(v[u], o);
+#pragma unroll
+            for (int q = 0; q < P; ++q) fn(o[q]);
+        }
+    }
+    for (; i < nv; i += stride) {
+        int64_t o[P];
+        IV::get(vec[i], o);
+#pragma unroll
+        for (int q = 0; q < P; ++q) fn(o[q]);
+    }
+    for (int64_t j = nv * P + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) fn((int64_t)ids[j]);
+}
+
+// Range of the ids: each workgroup reduces its share and updates out[0] / out[1] only where it improves them
+// (the plain read first spares almost every atomic once an extreme has been seen).
 template <typename Id>
 __global__ __launch_bounds__(256) void k_ids_minmax(const Id* __restrict__ ids, int64_t n, int64_t* out) {
     __shared__ int64_t lds_min[4], lds_max[4];
     int64_t lo = INT64_MAX, hi = INT64_MIN;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t v = (int64_t)ids[i];
+    for_each_id(ids, n, [&](int64_t v) {
         lo = v < lo ? v : lo;
         hi = v > hi ? v : hi;
-    }
+    });
     lo = etm::wave_min(lo);
     hi = etm::wave_max(hi);
     const int wid = threadIdx.x / 64;
@@ -32,14 +88,12 @@ __global__ __launch_bounds__(256) void k_ids_minmax(const Id* __restrict__ ids, 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < 4; ++w) {
+        for (int w = 0; w < 4; ++w) {
             lo = lds_min[w] < lo ? lds_min[w] : lo;
             hi = lds_max[w] > hi ? lds_max[w] : hi;
         }
-        lo = lds_min[0] < lo ? lds_min[0] : lo;
-        hi = lds_max[0] > hi ? lds_max[0] : hi;
-        atomicMin((long long*)&out[0], (long long)lo);
-        atomicMax((long long*)&out[1], (long long)hi);
+        if (lo < __atomic_load_n(&out[0], __ATOMIC_RELAXED)) atomicMin((long long*)&out[0], (long long)lo);
+        if (hi > __atomic_load_n(&out[1], __ATOMIC_RELAXED)) atomicMax((long long*)&out[1], (long long)hi);
     }
 }
 
@@ -63,10 +117,10 @@ __global__ __launch_bounds__(256) void k_hist_lds(const Id* __restrict__ ids, in
     __shared__ int32_t bins[kLdsBins];
     for (int b = threadIdx.x; b < span; b += blockDim.x) bins[b] = 0;
     __syncthreads();
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t v = (int64_t)ids[i] - lo;
+    for_each_id(ids, n, [&](int64_t id) {
+        const int64_t v = id - lo;
         if (v >= 0 && v < span) atomicAdd(&bins[v], 1);
-    }
+    });
     __syncthreads();
     for (int b = threadIdx.x; b < span; b += blockDim.x)
         if (bins[b]) atomicAdd(&hist[b], bins[b]);
@@ -85,8 +139,8 @@ __global__ __launch_bounds__(256) void k_hist_global(const Id* __restrict__ ids,
 // ... For one word, lane l reads the entries of transaction 64*word + l and sets bit (column - c0) of a
 // SLAB-bit register mask; then, per column, one wave ballot of that bit IS the column's 64-bit word. The word
 // goes to an LDS tile (no atomics anywhere: each word is produced by one ballot), and each column's
-// kBuildWords words leave as one contiguous 256-byte store. Rows may be unsorted or repeat items.
-constexpr int kBuildWords = 32;
+// kBuildWords words leave as one contiguous 64-byte store. Rows may be unsorted or repeat items.
+constexpr int kBuildWords = 8;
 
 template <typename Id, int SLAB>
 __global__ __launch_bounds__(kBuildThreads) void k_bitvec_build(const int64_t* __restrict__ indptr,
@@ -105,17 +159,28 @@ __global__ __launch_bounds__(kBuildThreads) void k_bitvec_build(const int64_t* _
         uint64_t mask[NM];
 #pragma unroll
         for (int q = 0; q < NM; ++q) mask[q] = 0;
+        auto set = [&](int64_t id) {
+            const int64_t v = id - lo;
+            if (v < 0 || v >= span) return;
+            const int c = lut[v] - c0;
+            if (c < 0 || c >= SLAB) return;
+#pragma unroll
+            for (int q = 0; q < NM; ++q)
+                if ((c >> 6) == q) mask[q] |= 1ull << (c & 63);
+        };
         if (r < n_rows) {
             const int64_t a = indptr[r], b = indptr[r + 1];
-            for (int64_t e = a; e < b; ++e) {
-                const int64_t v = (int64_t)ids[e] - lo;
-                if (v < 0 || v >= span) continue;
-                const int c = lut[v] - c0;
-                if (c < 0 || c >= SLAB) continue;
-#pragma unroll
-                for (int q = 0; q < NM; ++q)
-                    if ((c >> 6) == q) mask[q] |= 1ull << (c & 63);
+            int64_t e = a;
+            // Four entries per step: their id loads, then their LUT loads, are independent and overlap.
+            for (; e + 4 <= b; e += 4) {
+                const int64_t v0 = (int64_t)ids[e], v1 = (int64_t)ids[e + 1], v2 = (int64_t)ids[e + 2],
+                              v3 = (int64_t)ids[e + 3];
+                set(v0);
+                set(v1);
+                set(v2);
+                set(v3);
             }
+            for (; e < b; ++e) set((int64_t)ids[e]);
         }
 #pragma unroll
         for (int q = 0; q < NM; ++q) {
@@ -163,9 +228,9 @@ extern "C" int etm_ids_minmax(const void* d_ids, int id_bytes, int64_t n, int64_
     k_minmax_init<<<1, 1, 0, s>>>(d_out);
     ETM_LAUNCHED("k_minmax_init");
     if (id_bytes == 4)
-        k_ids_minmax<int32_t><<<grid_for(n, 256, 2048), 256, 0, s>>>((const int32_t*)d_ids, n, d_out);
+        k_ids_minmax<int32_t><<<grid_for(n, 256 * 16, 1024), 256, 0, s>>>((const int32_t*)d_ids, n, d_out);
     else
-        k_ids_minmax<int64_t><<<grid_for(n, 256, 2048), 256, 0, s>>>((const int64_t*)d_ids, n, d_out);
+        k_ids_minmax<int64_t><<<grid_for(n, 256 * 8, 1024), 256, 0, s>>>((const int64_t*)d_ids, n, d_out);
     ETM_LAUNCHED("k_ids_minmax");
     return ETM_OK;
 }
@@ -182,7 +247,7 @@ extern "C" int etm_ids_hist(const void* d_ids, int id_bytes, int64_t n, int64_t 
     if (n == 0) return ETM_OK;
     if (span <= kLdsBins) {
         // Each workgroup flushes every nonzero bin once: give it at least 4 x span ids.
-        const int g = grid_for(n, (int)std::max<int64_t>(4 * span, 16384), 512);
+        const int g = grid_for(n, (int)std::max<int64_t>(4 * span, 8192), span <= 2048 ? 1024 : 512);
         if (id_bytes == 4)
             k_hist_lds<int32_t><<<g, 256, 0, s>>>((const int32_t*)d_ids, n, lo, (int32_t)span, d_hist);
         else
@@ -211,7 +276,7 @@ extern "C" int etm_bitvec_build(const int64_t* d_indptr, const void* d_ids, int 
     ETM_REQUIRE(tiles <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_bitvec_build: %lld word tiles exceed the grid",
                 (long long)tiles);
     hipStream_t s = ETM_STREAM(stream);
-    // 128-column slabs (one pass when n_cols <= 128), else 256 (33 KiB / 66 KiB of LDS per workgroup).
+    // 128-column slabs (one pass when n_cols <= 128), else 256 (9 KiB / 18 KiB of LDS per workgroup).
     if (n_cols <= 128) {
         dim3 grid((unsigned)tiles, 1);
         if (id_bytes == 4)

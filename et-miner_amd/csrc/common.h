@@ -92,6 +92,19 @@ __device__ __forceinline__ int64_t wave_inclusive_scan(int64_t v) {
     return v;
 }
 
+// Inclusive wave64 scan of int32 on the DPP crossbar: row_shr 1, 2, 4, 8 scan each 16-lane row, then
+// row_bcast:15 adds row r's last lane to row r+1 (rows 1, 3) and row_bcast:31 adds lane 31 to rows 2 and 3.
+// Lanes whose DPP source is outside the row read `old` = 0. No LDS traffic.
+__device__ __forceinline__ int32_t wave_scan_i32(int32_t v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return v;
+}
+
 // Exclusive scan of one int64 per thread over a workgroup of NT threads; returns the thread's exclusive
 // prefix and writes the workgroup total to *total. `lds` holds NT/64 + 1 int64. Ends with a barrier.
 template <int NT>

@@ -25,6 +25,7 @@
 #include <algorithm>
 
 #include "common.h"
+#include "internal.h"
 
 namespace {
 
@@ -32,6 +33,30 @@ constexpr int kT = 64;   // suffixes per tile
 constexpr int kKW = 32;  // words per LDS step
 constexpr int kXcds = 8;  // MI355X: 8 XCDs of 32 CUs, each with its own L2
 constexpr int kListMax = 1024;  // live pairs from which a tile-pair is counted densely (4096 slots)
+
+// The (word chunk, item) unit of a workgroup or wave: `xcd` = workgroup index mod 8 (the XCD the hardware
+// dispatches it to, round-robin), `local` = its index among that XCD's units. With at least 8 chunks, XCD x takes
+// chunks x, x+8, ..., and every item of a chunk in turn, so the waves of one XCD share chunks in its L2; with
+// fewer chunks, XCD x takes every 8th unit of the chunk-major order, so all 8 XCDs stay busy. Returns false for
+// a unit past the end.
+__device__ __forceinline__ bool xcd_unit(int64_t xcd, int64_t local, int64_t n_items, int64_t n_chunks,
+                                         int64_t* ch, int64_t* item) {
+    if (n_chunks >= kXcds) {
+        *ch = xcd + kXcds * (local / n_items);
+        *item = local % n_items;
+    } else {
+        const int64_t u = xcd + kXcds * local;
+        *ch = u / n_items;
+        *item = u % n_items;
+    }
+    return *ch < n_chunks;
+}
+
+// Units per XCD of the mapping above (the grid is 8 x this, rounded up to whole workgroups).
+int64_t xcd_units(int64_t n_items, int64_t n_chunks) {
+    const int64_t per = (n_chunks + kXcds - 1) / kXcds;
+    return n_chunks >= kXcds ? n_items * per : (n_items * n_chunks + kXcds - 1) / kXcds;
+}
 
 // acc + popcount(x) as two v_bcnt_u32_b32 with accumulate (one per 32-bit half).
 __device__ __forceinline__ int popc_acc(uint64_t x, int acc) {
@@ -61,13 +86,12 @@ __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict_
     constexpr int NP = S * (S - 1) / 2;
     constexpr int U = SmallUnroll<S>::U;
     const int lane = threadIdx.x & 63;
-    // Workgroups go to the 8 XCDs round-robin. XCD x takes word chunks x, x+8, ..., chunk-major, so the waves
-    // running on one XCD at a time share one chunk of every column in that XCD's L2.
-    const int64_t xcd = blockIdx.x % kXcds;
-    const int64_t item = (int64_t)(blockIdx.x / kXcds) * 4 + (threadIdx.x >> 6);
-    const int64_t ch = xcd + kXcds * (item / n_list);
-    if (ch >= n_chunks) return;
-    const int64_t g = list[item % n_list];
+    // One wave per (word chunk, group) unit, spread over the 8 XCDs by xcd_unit.
+    int64_t ch, item;
+    if (!xcd_unit(blockIdx.x % kXcds, (int64_t)(blockIdx.x / kXcds) * 4 + (threadIdx.x >> 6), n_list, n_chunks, &ch,
+                  &item))
+        return;
+    const int64_t g = list[item];
     const int64_t gs = group_start[g];
     const int s = (int)(group_start[g + 1] - gs);
     if (s < 2 || s > S) return;
@@ -167,12 +191,9 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
     __shared__ int s_nlist;
 
     const int tid = threadIdx.x;
-    // 1-D grid: XCD x (blockIdx % 8) takes word chunks x, x+8, ..., and all tile-pairs of a chunk in turn.
-    const int64_t xcd = blockIdx.x % kXcds;
-    const int64_t local = blockIdx.x / kXcds;
-    const int64_t chunk_id = xcd + kXcds * (local / n_tile_pairs);
-    if (chunk_id >= n_chunks) return;
-    const int64_t tp = local % n_tile_pairs;
+    // 1-D grid: one workgroup per (word chunk, tile-pair) unit, spread over the 8 XCDs by xcd_unit.
+    int64_t chunk_id, tp;
+    if (!xcd_unit(blockIdx.x % kXcds, blockIdx.x / kXcds, n_tile_pairs, n_chunks, &chunk_id, &tp)) return;
     const int64_t g = etm::upper_index(cum_tiles, n_groups, tp);
     const int64_t gs = group_start[g];
     const int64_t s = group_start[g + 1] - gs;
@@ -350,11 +371,8 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const uint64_t* __restrict__ 
     __shared__ v4i ldsB[2 * kMW][kMT][2];
     __shared__ int64_t colA[kMT], colB[kMT];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t xcd = blockIdx.x % kXcds;
-    const int64_t local = blockIdx.x / kXcds;
-    const int64_t chunk_id = xcd + kXcds * (local / n_tp);
-    if (chunk_id >= n_chunks) return;
-    const int64_t tp = local % n_tp;
+    int64_t chunk_id, tp;
+    if (!xcd_unit(blockIdx.x % kXcds, blockIdx.x / kXcds, n_tp, n_chunks, &chunk_id, &tp)) return;
     const int64_t nt = (s + kMT - 1) / kMT;
     int64_t ta, tb1;
     etm::lex_pair_decode(tp, nt + 1, &ta, &tb1);
@@ -482,8 +500,7 @@ hipError_t launch_small(const uint64_t* bv, int64_t pitch, int64_t n_words, int6
                         int32_t m, const int64_t* gs, const int64_t* cp, const int32_t* list, int64_t n_list,
                         const uint8_t* status, int32_t* counts, int32_t layout, hipStream_t s) {
     const int64_t n_chunks = ceil_div(n_words, chunk);
-    const int64_t waves_per_xcd = n_list * ceil_div(n_chunks, kXcds);
-    const int64_t blocks = kXcds * ceil_div(waves_per_xcd, 4);
+    const int64_t blocks = kXcds * ceil_div(xcd_units(n_list, n_chunks), 4);
     if (blocks > 0x7FFFFFFF) return hipErrorInvalidConfiguration;
     k_count_small<S><<<(unsigned)blocks, 256, 0, s>>>(bv, pitch, n_words, chunk, n_chunks, rows, m, gs, cp, list,
                                                       n_list, status, counts, layout);
@@ -535,19 +552,19 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
         base += cls[c];
     }
     // The pair space of a single-group level of one-item rows (K=2) without a status array: the int8 MFMA Gram
-    // when it has at least kMT columns (or when asked for), else the VALU tiled kernel.
+    // when asked for, or by default where it measured faster (gram_preferred), else the VALU tiled kernel.
     const bool gram_level = n_groups == 1 && m == 1 && d_status == nullptr;
     if (n_tile_pairs > 0 && gram_level && kernel != ETM_KERNEL_VALU) {
         int64_t s_cols = 0;
         ETM_HIP(hipMemcpyAsync(&s_cols, d_group_start + 1, sizeof s_cols, hipMemcpyDeviceToHost, s));
         ETM_HIP(hipStreamSynchronize(s));
-        if (kernel == ETM_KERNEL_MFMA || s_cols >= kMT) {
+        if (kernel == ETM_KERNEL_MFMA || etm::detail::gram_preferred(s_cols, n_words)) {
             const int64_t nt = ceil_div(s_cols, kMT);
             const int64_t n_tp = nt * (nt + 1) / 2;
             const int64_t chunk = word_chunk > 0 ? ceil_div(word_chunk, kMW) * kMW
                                                  : pick_chunk(n_words, n_tp, 1024, kMW, 512);
             const int64_t n_chunks = ceil_div(n_words, chunk);
-            const int64_t blocks = kXcds * n_tp * ceil_div(n_chunks, kXcds);
+            const int64_t blocks = kXcds * xcd_units(n_tp, n_chunks);
             ETM_REQUIRE(blocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",
                         (long long)blocks);
             k_gram_mfma<<<(unsigned)blocks, 256, 0, s>>>(d_bv, pitch, n_words, chunk, d_rows, s_cols, n_tp, n_chunks,
@@ -562,7 +579,7 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
         const int64_t chunk = word_chunk > 0 ? ceil_div(word_chunk, kKW) * kKW
                                              : pick_chunk(n_words, n_tile_pairs, 2048, kKW, 128);
         const int64_t n_chunks = ceil_div(n_words, chunk);
-        const int64_t blocks = kXcds * n_tile_pairs * ceil_div(n_chunks, kXcds);
+        const int64_t blocks = kXcds * xcd_units(n_tile_pairs, n_chunks);
         ETM_REQUIRE(blocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",
                     (long long)blocks);
         k_count_tiled<<<(unsigned)blocks, 256, 0, s>>>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start,

@@ -134,14 +134,29 @@ __global__ __launch_bounds__(256) void k_group_place(const int64_t* __restrict__
                                                       int32_t small_max, int32_t* __restrict__ small_list,
                                                       int64_t* __restrict__ totals) {
     const int64_t n_groups = totals[0];
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n_groups && g < n;
-         g += (int64_t)gridDim.x * blockDim.x) {
-        const int c = size_class(group_start[g + 1] - group_start[g], small_max);
-        if (c < 0) continue;
-        int64_t base = 0;
-        for (int q = 0; q < c; ++q) base += totals[8 + q];
-        const unsigned long long at = atomicAdd((unsigned long long*)&totals[16 + c], 1ull);
-        small_list[base + (int64_t)at] = (int32_t)g;
+    const int64_t lim = n_groups < n ? n_groups : n;
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    // Whole waves walk the groups (the loop bound is wave-uniform), so each class takes one cursor atomic per
+    // wave: the lanes of a class get consecutive slots in lane order.
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t g0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); g0 < lim; g0 += stride) {
+        const int64_t g = g0 + lane;
+        const int c = g < lim ? size_class(group_start[g + 1] - group_start[g], small_max) : -1;
+#pragma unroll
+        for (int q = 0; q < kClasses; ++q) {
+            const uint64_t mask = __ballot(c == q);
+            if (mask == 0) continue;
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned long long at = 0;
+            if (lane == leader) at = atomicAdd((unsigned long long*)&totals[16 + q], (unsigned long long)__popcll(mask));
+            at = __shfl(at, leader, 64);
+            if (c == q) {
+                int64_t base = 0;
+                for (int t = 0; t < q; ++t) base += totals[8 + t];
+                small_list[base + (int64_t)at + __popcll(mask & below)] = (int32_t)g;
+            }
+        }
     }
 }
 
@@ -518,30 +533,77 @@ __global__ __launch_bounds__(256) void k_free(const int32_t* __restrict__ rows, 
 // ── single-workgroup fast paths (small levels: one launch, sizes read on the device) ─────────
 
 constexpr int kOne = 1024;
-constexpr int kPer = 1;  // items per thread per pass (the per-item work is a dependent load chain: keep threads many)
+constexpr int kOneWaves = kOne / 64;
 
-// One pass of a workgroup-wide exclusive scan over items [base, base + kOne*kPer): thread t evaluates f on its
-// kPer contiguous items, the workgroup scans the per-thread sums, and w(i, value, exclusive prefix) is called
-// for every item below n. Returns carry + the pass total. Ends with a barrier.
-template <class F, class W>
-__device__ __forceinline__ int64_t pass_scan(int64_t base, int64_t n, const F& f, const W& w, int64_t carry,
-                                             int64_t* lds) {
-    int64_t v[kPer];
-    int64_t sum = 0;
-    const int64_t first = base + (int64_t)threadIdx.x * kPer;
+// One pass of a workgroup-wide exclusive scan of K int32 values per item over items [base, base + kOne*S).
+// Wave v owns the contiguous run [base + 64*S*v, base + 64*S*(v+1)), lane l its items 64*j + l (coalesced).
+// Every value is evaluated once, all of a lane's loads are issued before any scan; the S rows of a wave are
+// scanned on the DPP crossbar, and one barrier publishes the wave totals, which each wave scans itself.
+// f(i, v) fills v[K] for item i < n; w(i, v, prefix) is called for every item i < n with its exclusive
+// prefixes (carry included). carry[] advances by the pass totals. Values and totals must fit int32.
+template <int K, int S, class F, class W>
+__device__ __forceinline__ void stripe_pass(int64_t base, int64_t n, const F& f, const W& w, int32_t (&carry)[K],
+                                            int32_t* lds) {
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int64_t wbase = base + (int64_t)wv * 64 * S;
+    int32_t v[S][K], ex[S][K];
 #pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-        v[j] = first + j < n ? f(first + j) : 0;
-        sum += v[j];
-    }
-    int64_t tot;
-    int64_t run = carry + etm::block_exclusive_scan<kOne>(sum, lds, &tot);
+    for (int j = 0; j < S; ++j) {
+        const int64_t i = wbase + j * 64 + lane;
+        if (i < n) {
+            f(i, v[j]);
+        } else {
 #pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-        if (first + j < n) w(first + j, v[j], run);
-        run += v[j];
+            for (int q = 0; q < K; ++q) v[j][q] = 0;
+        }
     }
-    return carry + tot;
+    int32_t run[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) run[q] = 0;
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const int32_t inc = etm::wave_scan_i32(v[j][q]);
+            ex[j][q] = run[q] + inc - v[j][q];
+            run[q] += __builtin_amdgcn_readlane(inc, 63);
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) lds[q * kOneWaves + wv] = run[q];
+    }
+    __syncthreads();
+    int32_t off[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const int32_t t = lane < kOneWaves ? lds[q * kOneWaves + lane] : 0;
+        const int32_t tinc = etm::wave_scan_i32(t);
+        off[q] = carry[q] + __shfl(tinc - t, wv, 64);
+        carry[q] += __builtin_amdgcn_readlane(tinc, kOneWaves - 1);
+    }
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        const int64_t i = wbase + j * 64 + lane;
+        if (i < n) {
+            int32_t pre[K];
+#pragma unroll
+            for (int q = 0; q < K; ++q) pre[q] = off[q] + ex[j][q];
+            w(i, v[j], pre);
+        }
+    }
+    __syncthreads();
+}
+
+// A single-valued sweep over items [0, n): passes of kOne*S items. Returns the total.
+template <int S, class F, class W>
+__device__ __forceinline__ int64_t stripe_scan(int64_t n, const F& f, const W& w, int32_t* lds) {
+    int32_t carry[1] = {0};
+    auto f1 = [&](int64_t i, int32_t(&v)[1]) { v[0] = (int32_t)f(i); };
+    auto w1 = [&](int64_t i, const int32_t(&v)[1], const int32_t(&p)[1]) { w(i, (int64_t)v[0], (int64_t)p[0]); };
+    for (int64_t base = 0; base < n; base += (int64_t)kOne * S) stripe_pass<1, S>(base, n, f1, w1, carry, lds);
+    return carry[0];
 }
 
 struct GroupSize {
@@ -549,8 +611,11 @@ struct GroupSize {
     __device__ int64_t operator()(int64_t g) const { return group_start[g + 1] - group_start[g]; }
 };
 
-// The whole group build of a level of *d_n rows (at most n_cap) in one workgroup: group starts, cumulative
-// pairs and tile-pairs, size-class lists (groups in ascending order within each class) and the totals.
+// The whole group build of a level of *d_n rows (at most n_cap <= 32768) in one workgroup: group starts,
+// cumulative pairs and tile-pairs, size-class lists (groups in ascending order within each class) and the
+// totals. Three sweeps: rows (group starts), groups (pairs, tile-pairs and class counts, classes 0-3 packed
+// two 16-bit counts per int32: a class of groups of >= 2 rows holds at most 16384 of them), groups again (class
+// lists at their class's base). Groups of one row (no candidates) are in no class.
 __global__ __launch_bounds__(kOne) void k_groups_single(const int32_t* __restrict__ rows, const int64_t* d_n,
                                                         int64_t n_cap, int32_t m, int32_t small_max,
                                                         int64_t* __restrict__ group_start,
@@ -558,68 +623,87 @@ __global__ __launch_bounds__(kOne) void k_groups_single(const int32_t* __restric
                                                         int64_t* __restrict__ cum_tiles,
                                                         int32_t* __restrict__ small_list,
                                                         int64_t* __restrict__ totals) {
-    __shared__ int64_t lds[kOne / 64 + 1];
+    __shared__ int32_t lds[5 * kOneWaves];
+    __shared__ int64_t red[kOneWaves + 1];
     const int tid = threadIdx.x;
     const int64_t n = d_n ? min(*d_n, n_cap) : n_cap;
-    constexpr int64_t kPass = (int64_t)kOne * kPer;
     IsGroupStart is_start{rows, m};
-    WriteGroupStart put_start{group_start};
-    int64_t n_groups = 0;
-    for (int64_t base = 0; base < n; base += kPass) n_groups = pass_scan(base, n, is_start, put_start, n_groups, lds);
+    auto put_start = [&](int64_t r, int64_t v, int64_t prefix) {
+        if (v && prefix < n) group_start[prefix] = r;
+    };
+    const int64_t n_groups = stripe_scan<8>(n, is_start, put_start, lds);
     if (tid == 0) group_start[n_groups] = n;
     __syncthreads();
     GroupSize size{group_start};
-    auto pairs = [&](int64_t g) { const int64_t z = size(g); return z * (z - 1) / 2; };
-    auto tiles = [&](int64_t g) {
+    // Sweep 2: pairs, tile-pairs, class counts.
+    int64_t n2 = 0, rows2 = 0, smax = 0;
+    auto vals = [&](int64_t g, int32_t(&v)[5]) {
         const int64_t z = size(g);
         const int64_t nt = (z + kTileT - 1) / kTileT;
-        return z > small_max ? nt * (nt + 1) / 2 : (int64_t)0;
+        const int c = size_class(z, small_max);
+        v[0] = (int32_t)(z * (z - 1) / 2);
+        v[1] = z > small_max ? (int32_t)(nt * (nt + 1) / 2) : 0;
+        v[2] = c == 0 ? 1 : (c == 1 ? (1 << 16) : 0);
+        v[3] = c == 2 ? 1 : (c == 3 ? (1 << 16) : 0);
+        v[4] = c == 4 ? 1 : 0;
     };
-    WritePrefix put_cp{cum_pairs}, put_ct{cum_tiles};
-    int64_t cp = 0, ct = 0;
-    for (int64_t base = 0; base < n_groups; base += kPass) {
-        cp = pass_scan(base, n_groups, pairs, put_cp, cp, lds);
-        ct = pass_scan(base, n_groups, tiles, put_ct, ct, lds);
-    }
-    if (tid == 0) {
-        cum_pairs[n_groups] = cp;
-        cum_tiles[n_groups] = ct;
-    }
-    int64_t n2 = 0, rows2 = 0, smax = 0;
-    for (int64_t g = tid; g < n_groups; g += kOne) {
+    auto put = [&](int64_t g, const int32_t(&v)[5], const int32_t(&p)[5]) {
+        cum_pairs[g] = p[0];
+        cum_tiles[g] = p[1];
         const int64_t z = size(g);
         if (z >= 2) {
             n2 += 1;
             rows2 += z;
         }
         smax = z > smax ? z : smax;
-    }
-    n2 = etm::block_sum<kOne>(n2, lds);
-    rows2 = etm::block_sum<kOne>(rows2, lds);
-    smax = etm::wave_max(smax);
-    if ((tid & 63) == 0) lds[tid / 64] = smax;
-    __syncthreads();
-    if (tid == 0)
-        for (int w = 0; w < kOne / 64; ++w) smax = lds[w] > smax ? lds[w] : smax;
-    __syncthreads();
-    // Class lists, ascending within each class.
+    };
+    int32_t c4[5] = {0, 0, 0, 0, 0};
+    for (int64_t base = 0; base < n_groups; base += (int64_t)kOne * 4) stripe_pass<5, 4>(base, n_groups, vals, put, c4, lds);
+    const int64_t cp = c4[0], ct = c4[1];
     int64_t cls[kClasses];
-    int64_t placed = 0;
-    for (int c = 0; c < kClasses; ++c) {
-        auto in_class = [&](int64_t g) { return (int64_t)(size_class(size(g), small_max) == c); };
-        auto put = [&](int64_t g, int64_t v, int64_t at) {
-            if (v) small_list[at] = (int32_t)g;
-        };
-        int64_t cc = placed;
-        for (int64_t base = 0; base < n_groups; base += kPass) cc = pass_scan(base, n_groups, in_class, put, cc, lds);
-        cls[c] = cc - placed;
-        placed = cc;
+    cls[0] = c4[2] & 0xffff;
+    cls[1] = (uint32_t)c4[2] >> 16;
+    cls[2] = c4[3] & 0xffff;
+    cls[3] = (uint32_t)c4[3] >> 16;
+    cls[4] = c4[4];
+    if (tid == 0) {
+        cum_pairs[n_groups] = cp;
+        cum_tiles[n_groups] = ct;
     }
+    n2 = etm::block_sum<kOne>(n2, red);
+    rows2 = etm::block_sum<kOne>(rows2, red);
+    smax = etm::wave_max(smax);
+    if ((tid & 63) == 0) red[tid / 64] = smax;
+    __syncthreads();
+    for (int w = 0; w < kOneWaves; ++w) smax = red[w] > smax ? red[w] : smax;
+    __syncthreads();
+    // Sweep 3: the class lists, ascending within each class, class c at base[c].
+    int64_t cbase[kClasses];
+    cbase[0] = 0;
+    for (int c = 1; c < kClasses; ++c) cbase[c] = cbase[c - 1] + cls[c - 1];
+    auto cvals = [&](int64_t g, int32_t(&v)[3]) {
+        const int c = size_class(size(g), small_max);
+        v[0] = c == 0 ? 1 : (c == 1 ? (1 << 16) : 0);
+        v[1] = c == 2 ? 1 : (c == 3 ? (1 << 16) : 0);
+        v[2] = c == 4 ? 1 : 0;
+    };
+    auto cput = [&](int64_t g, const int32_t(&v)[3], const int32_t(&p)[3]) {
+        int64_t at;
+        if (v[0] & 0xffff) at = cbase[0] + (p[0] & 0xffff);
+        else if (v[0]) at = cbase[1] + ((uint32_t)p[0] >> 16);
+        else if (v[1] & 0xffff) at = cbase[2] + (p[1] & 0xffff);
+        else if (v[1]) at = cbase[3] + ((uint32_t)p[1] >> 16);
+        else if (v[2]) at = cbase[4] + p[2];
+        else return;
+        if (at < n_groups) small_list[at] = (int32_t)g;
+    };
+    int32_t c3[3] = {0, 0, 0};
+    for (int64_t base = 0; base < n_groups; base += (int64_t)kOne * 8) stripe_pass<3, 8>(base, n_groups, cvals, cput, c3, lds);
     if (tid == 0) {
         totals[0] = n_groups;
         totals[1] = cp;
         totals[2] = ct;
-        totals[3] = placed - cls[4];
+        totals[3] = cls[0] + cls[1] + cls[2] + cls[3];
         totals[4] = smax;
         totals[5] = n2;
         totals[6] = rows2;
@@ -634,10 +718,12 @@ __global__ __launch_bounds__(kOne) void k_groups_single(const int32_t* __restric
 // survivor count goes to *d_out_n.
 __global__ __launch_bounds__(kOne) void k_compact_single(Survives f, WriteSurvivor w, int64_t n_cands,
                                                          int64_t* __restrict__ d_out_n) {
-    __shared__ int64_t lds[kOne / 64 + 1];
-    int64_t carry = 0;
-    for (int64_t base = 0; base < n_cands; base += (int64_t)kOne * kPer) carry = pass_scan(base, n_cands, f, w, carry, lds);
-    if (threadIdx.x == 0) *d_out_n = carry;
+    __shared__ int32_t lds[kOneWaves];
+    auto put = [&](int64_t c, int64_t v, int64_t p) {
+        if (p < n_cands) w(c, v, p);
+    };
+    const int64_t total = stripe_scan<8>(n_cands, f, put, lds);
+    if (threadIdx.x == 0) *d_out_n = total;
 }
 
 // Rows whose flag is set, in order, in one workgroup (n read from *d_n, at most n_cap).
@@ -645,13 +731,15 @@ __global__ __launch_bounds__(kOne) void k_select_single(const int32_t* __restric
                                                         int64_t n_cap, int32_t m, const int32_t* __restrict__ counts,
                                                         const uint8_t* __restrict__ flag, int32_t* __restrict__ out_rows,
                                                         int32_t* __restrict__ out_counts, int64_t* __restrict__ d_out_n) {
-    __shared__ int64_t lds[kOne / 64 + 1];
+    __shared__ int32_t lds[kOneWaves];
     const int64_t n = d_n ? min(*d_n, n_cap) : n_cap;
     WriteSelected w{rows, m, counts, out_rows, out_counts};
     Flagged f{flag};
-    int64_t carry = 0;
-    for (int64_t base = 0; base < n; base += (int64_t)kOne * kPer) carry = pass_scan(base, n, f, w, carry, lds);
-    if (threadIdx.x == 0) *d_out_n = carry;
+    auto put = [&](int64_t i, int64_t v, int64_t p) {
+        if (p < n) w(i, v, p);
+    };
+    const int64_t total = stripe_scan<8>(n, f, put, lds);
+    if (threadIdx.x == 0) *d_out_n = total;
 }
 
 int grid_for(int64_t n, int cap = 16384) {

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -18,13 +19,21 @@
 #include "common.h"
 #include "internal.h"
 
+// Host result of a mining call. Level 1 lives in host vectors; the levels from K=2 on stay in the pinned block
+// their device-to-host copy landed in (taken from a process-wide pool and given back when the result is
+// freed), so no further host copy is made before the caller reads them.
 struct etm_levels {
     std::vector<int32_t> k;
-    std::vector<std::vector<int32_t>> rows;
-    std::vector<std::vector<int32_t>> counts;
+    std::vector<int64_t> n;
+    std::vector<const int32_t*> rows;    // n[l] * k[l] column ids per level
+    std::vector<const int32_t*> counts;  // n[l] counts per level
+    std::vector<int32_t> k1_rows, k1_counts;
+    int32_t* block = nullptr;
+    int64_t block_cap = 0;
     std::vector<etm_level_stat> stats;
     std::vector<int64_t> col_to_item;
     std::vector<int64_t> col_counts;
+    ~etm_levels();
 };
 
 namespace {
@@ -150,6 +159,65 @@ int32_t* staging(int64_t n) {
     return buf;
 }
 
+// Timing events of this thread, created on first use and reused by every later call on the same device
+// (event creation costs far more than recording).
+hipEvent_t pooled_event(size_t i) {
+    static thread_local std::vector<hipEvent_t> pool;
+    static thread_local int pool_device = -1;
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev != pool_device) {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+        pool.clear();
+        pool_device = dev;
+    }
+    while (pool.size() <= i) {
+        hipEvent_t e = nullptr;
+        check(hipEventCreate(&e), "event");
+        pool.push_back(e);
+    }
+    return pool[i];
+}
+
+// Process-wide pool of pinned host blocks for results (int32 capacity each). A block leaves the pool with a
+// result and comes back when the result is freed; at most kPoolKeep blocks wait in the pool.
+std::mutex g_pool_mu;
+std::vector<std::pair<int32_t*, int64_t>> g_pool;
+constexpr size_t kPoolKeep = 32;
+
+int32_t* pool_take(int64_t n, int64_t* cap) {
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        size_t best = g_pool.size();
+        for (size_t i = 0; i < g_pool.size(); ++i)
+            if (g_pool[i].second >= n && (best == g_pool.size() || g_pool[i].second < g_pool[best].second)) best = i;
+        if (best < g_pool.size()) {
+            auto b = g_pool[best];
+            g_pool.erase(g_pool.begin() + (std::ptrdiff_t)best);
+            *cap = b.second;
+            return b.first;
+        }
+    }
+    const int64_t want = std::max<int64_t>(n + (n >> 2), 1 << 20);
+    int32_t* p = nullptr;
+    check(hipHostMalloc((void**)&p, want * 4, hipHostMallocDefault), "hipHostMalloc");
+    *cap = want;
+    return p;
+}
+
+void pool_give(int32_t* p, int64_t cap) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_pool.emplace_back(p, cap);
+    if (g_pool.size() > kPoolKeep) {
+        size_t small = 0;
+        for (size_t i = 1; i < g_pool.size(); ++i)
+            if (g_pool[i].second < g_pool[small].second) small = i;
+        (void)hipHostFree(g_pool[small].first);
+        g_pool.erase(g_pool.begin() + (std::ptrdiff_t)small);
+    }
+}
+
 int64_t pow2_at_least(int64_t x) {
     int64_t v = 1;
     while (v < x) v <<= 1;
@@ -210,9 +278,12 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         k1_rows.push_back(c);
         k1_counts.push_back((int32_t)col_counts[c]);
     }
+    out.k1_rows = k1_rows;
+    out.k1_counts = k1_counts;
     out.k.push_back(1);
-    out.rows.push_back(k1_rows);
-    out.counts.push_back(k1_counts);
+    out.n.push_back((int64_t)out.k1_counts.size());
+    out.rows.push_back(out.k1_rows.data());
+    out.counts.push_back(out.k1_counts.data());
     etm_level_stat st1{};
     st1.k = 1;
     st1.n_candidates = n_cols;
@@ -289,13 +360,18 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         }
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (o.time_kernels) {
-            check(hipEventCreate(&e0), "event");
-            check(hipEventCreate(&e1), "event");
+            e0 = pooled_event(2 * events.size());
+            e1 = pooled_event(2 * events.size() + 1);
             check(hipEventRecord(e0, s), "event record");
         }
         const int64_t cls[4] = {tot[8], tot[9], tot[10], tot[11]};
+        // At K=2 the single group holds every column: the automatic kernel choice needs no device read.
+        int32_t kern = ETM_KERNEL_VALU;
+        if (k == 2)
+            kern = o.k2_kernel != ETM_KERNEL_AUTO ? o.k2_kernel
+                   : etm::detail::gram_preferred(prev.n, n_words) ? ETM_KERNEL_MFMA : ETM_KERNEL_VALU;
         check_rc(etm_count_level(d_bv, pitch, n_words, prev.rows, m, g.gs, g.cp, g.ct, n_groups, n_tiles, g.list, cls,
-                                 status, counts, ETM_LAYOUT_LEX, 0, k == 2 ? o.k2_kernel : ETM_KERNEL_VALU, s),
+                                 status, counts, ETM_LAYOUT_LEX, 0, kern, s),
                  "etm_count_level");
         if (o.time_kernels) {
             check(hipEventRecord(e1, s), "event record");
@@ -398,7 +474,10 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
     // Every level back to the host in one pass, through one pinned staging buffer.
     int64_t total_ints = 0;
     for (const Pending& p : levels) total_ints += p.n * (p.k + 1);
-    int32_t* stage = staging(total_ints);
+    int64_t cap_ints = 0;
+    int32_t* stage = pool_take(std::max<int64_t>(total_ints, 1), &cap_ints);
+    out.block = stage;
+    out.block_cap = cap_ints;
     int64_t at = 0;
     for (const Pending& p : levels) {
         if (p.n) {
@@ -411,16 +490,15 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
     at = 0;
     for (const Pending& p : levels) {
         out.k.push_back(p.k);
-        out.rows.emplace_back(stage + at, stage + at + p.n * p.k);
-        out.counts.emplace_back(stage + at + p.n * p.k, stage + at + p.n * (p.k + 1));
+        out.n.push_back(p.n);
+        out.rows.push_back(stage + at);
+        out.counts.push_back(stage + at + p.n * p.k);
         at += p.n * (p.k + 1);
     }
     for (size_t i = 0; i < events.size(); ++i) {
         float ms = 0;
         check(hipEventElapsedTime(&ms, events[i].first, events[i].second), "event time");
         out.stats[i + 1].count_ms = ms;
-        (void)hipEventDestroy(events[i].first);
-        (void)hipEventDestroy(events[i].second);
     }
 }
 
@@ -480,19 +558,22 @@ extern "C" int etm_mine_csr(const int64_t* d_indptr, const void* d_ids, int id_b
             throw Error(ETM_EINVAL, "item ids span " + std::to_string(span) + " values: code them to dense ids first");
         int32_t* hist = a.get<int32_t>(span);
         check_rc(etm_ids_hist(d_ids, id_bytes, nnz, lo, span, hist, s), "etm_ids_hist");
-        std::vector<int32_t> h_hist((size_t)span);
-        check(hipMemcpyAsync(h_hist.data(), hist, span * 4, hipMemcpyDeviceToHost, s), "readback");
+        // Histogram down and LUT up through the thread's pinned staging buffer (its two halves).
+        int32_t* h_hist = staging(2 * span);
+        int32_t* lut = h_hist + span;
+        check(hipMemcpyAsync(h_hist, hist, span * 4, hipMemcpyDeviceToHost, s), "readback");
         check(hipStreamSynchronize(s), "sync");
-        std::vector<int32_t> lut((size_t)span, -1);
-        for (int64_t v = 0; v < span; ++v)
+        for (int64_t v = 0; v < span; ++v) {
+            lut[v] = -1;
             if (h_hist[v] >= mc) {
                 lut[v] = (int32_t)res->col_to_item.size();
                 res->col_to_item.push_back(lo + v);
             }
+        }
         const int32_t n_cols = (int32_t)res->col_to_item.size();
         if (n_cols == 0) return;
         int32_t* d_lut = a.get<int32_t>(span);
-        check(hipMemcpyAsync(d_lut, lut.data(), span * 4, hipMemcpyHostToDevice, s), "upload lut");
+        check(hipMemcpyAsync(d_lut, lut, span * 4, hipMemcpyHostToDevice, s), "upload lut");
         const int64_t n_words = (n_rows + 63) / 64;
         const int64_t pitch = std::max<int64_t>(64, (n_words + 63) / 64 * 64);
         uint64_t* bv = a.get<uint64_t>((int64_t)n_cols * pitch);
@@ -519,16 +600,17 @@ extern "C" int etm_levels_shape(const etm_levels* lv, int32_t level, int32_t* h_
     ETM_ARG(lv && h_k && h_n, "etm_levels_shape: null pointer");
     ETM_ARG(level >= 0 && level < (int32_t)lv->k.size(), "etm_levels_shape: level %d out of range", level);
     *h_k = lv->k[level];
-    *h_n = (int64_t)lv->counts[level].size();
+    *h_n = lv->n[level];
     return ETM_OK;
 }
 
 extern "C" int etm_levels_copy(const etm_levels* lv, int32_t level, int32_t* h_rows, int64_t* h_counts) {
     ETM_ARG(lv, "etm_levels_copy: null result");
     ETM_ARG(level >= 0 && level < (int32_t)lv->k.size(), "etm_levels_copy: level %d out of range", level);
-    if (h_rows) std::memcpy(h_rows, lv->rows[level].data(), lv->rows[level].size() * 4);
+    const int64_t n = lv->n[level];
+    if (h_rows && n) std::memcpy(h_rows, lv->rows[level], (size_t)(n * lv->k[level]) * 4);
     if (h_counts)
-        for (size_t i = 0; i < lv->counts[level].size(); ++i) h_counts[i] = lv->counts[level][i];
+        for (int64_t i = 0; i < n; ++i) h_counts[i] = lv->counts[level][i];
     return ETM_OK;
 }
 
@@ -554,8 +636,8 @@ extern "C" int etm_levels_flat(const etm_levels* lv, int64_t* h_totals, int64_t*
     ETM_ARG(lv && h_totals, "etm_levels_flat: null pointer");
     int64_t n_sets = 0, n_vals = 0;
     for (size_t l = 0; l < lv->k.size(); ++l) {
-        n_sets += (int64_t)lv->counts[l].size();
-        n_vals += (int64_t)lv->rows[l].size();
+        n_sets += lv->n[l];
+        n_vals += lv->n[l] * lv->k[l];
     }
     h_totals[0] = n_sets;
     h_totals[1] = n_vals;
@@ -563,12 +645,16 @@ extern "C" int etm_levels_flat(const etm_levels* lv, int64_t* h_totals, int64_t*
     const bool map = !lv->col_to_item.empty();
     int64_t at = 0, row = 0;
     h_offsets[0] = 0;
+    const int64_t* c2i = map ? lv->col_to_item.data() : nullptr;
     for (size_t l = 0; l < lv->k.size(); ++l) {
-        const int64_t k = lv->k[l];
-        const std::vector<int32_t>& r = lv->rows[l];
-        for (size_t i = 0; i < r.size(); ++i) h_values[at + (int64_t)i] = map ? lv->col_to_item[r[i]] : r[i];
-        const std::vector<int32_t>& c = lv->counts[l];
-        for (size_t i = 0; i < c.size(); ++i) {
+        const int64_t k = lv->k[l], n = lv->n[l];
+        const int32_t* r = lv->rows[l];
+        const int32_t* c = lv->counts[l];
+        if (c2i)
+            for (int64_t i = 0; i < n * k; ++i) h_values[at + i] = c2i[r[i]];
+        else
+            for (int64_t i = 0; i < n * k; ++i) h_values[at + i] = r[i];
+        for (int64_t i = 0; i < n; ++i) {
             h_counts[row] = c[i];
             at += k;
             h_offsets[++row] = at;
@@ -576,6 +662,8 @@ extern "C" int etm_levels_flat(const etm_levels* lv, int64_t* h_totals, int64_t*
     }
     return ETM_OK;
 }
+
+etm_levels::~etm_levels() { pool_give(block, block_cap); }
 
 extern "C" int etm_levels_free(etm_levels* lv) {
     delete lv;
