@@ -277,11 +277,9 @@ __global__ __launch_bounds__(256) void k_classify(const int32_t* __restrict__ ro
     }
 }
 
-// Group-centric classification. One wave per group of at most 16 suffixes (at most 120 candidates): lanes
-// take (candidate, dropped prefix item) pairs, so the k-2 subset lookups of a candidate run in parallel;
-// per-candidate results gather in LDS. Big groups: one workgroup each, a thread per candidate. Both write the
-// group id of every candidate (compaction then needs no search).
-constexpr int kMaxSmallPairs = 120;
+// Group-centric classification, a thread per candidate: small groups packed several to a wave by size class,
+// big groups split into ranges of kBigPairs candidates. Both write the group id of every candidate (compaction
+// then needs no search).
 
 struct Classifier {
     const int32_t* rows;
@@ -292,73 +290,92 @@ struct Classifier {
     int64_t mask;
     int32_t mode;
 
-    // Candidate (i, j) of the group starting at row gs: the row of the subset dropping prefix item d, or -1.
-    __device__ int64_t lookup(int64_t gs, int64_t i, int64_t j, int d) const {
+    // Status and count of candidate (i, j) of the group starting at row gs: the hash of all its items is
+    // formed once, and each (k-1)-subset dropping a prefix item is one subtraction and one probe away.
+    __device__ void classify(int64_t gs, int64_t i, int64_t j, uint8_t* st_out, int32_t* cnt_out) const {
+        const bool infer = (mode & ETM_SUBSET_INFER) != 0;
+        const bool prune = (mode & ETM_SUBSET_PRUNE) != 0;
         const int32_t* pre = rows + gs * m;
         const int np = m - 1;
-        const int32_t a = rows[(gs + i) * m + np], b = rows[(gs + j) * m + np];
-        uint64_t h = item_hash(a) + item_hash(b);
-        for (int t = 0; t < np; ++t)
-            if (t != d) h += item_hash(pre[t]);
-        Query q{pre, np, d, a, b, true};
-        return hash_find(rows, m, table, mask, h, q);
-    }
-};
-
-__global__ __launch_bounds__(256) void k_classify_small(Classifier C, const int64_t* __restrict__ group_start,
-                                                         const int64_t* __restrict__ cum_pairs,
-                                                         const int32_t* __restrict__ list, int64_t n_list,
-                                                         uint8_t* __restrict__ status, int32_t* __restrict__ counts,
-                                                         int32_t* __restrict__ gid, int64_t* __restrict__ tally) {
-    __shared__ int32_t s_min[4][kMaxSmallPairs];
-    __shared__ int32_t s_flags[4][kMaxSmallPairs];  // bit 0: a subset is missing, bit 1: a subset is not free
-    __shared__ int64_t lds[5];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int64_t item = (int64_t)blockIdx.x * 4 + wv;
-    const bool infer = (C.mode & ETM_SUBSET_INFER) != 0;
-    const bool prune = (C.mode & ETM_SUBSET_PRUNE) != 0;
-    int64_t n_count = 0, n_infer = 0, n_skip = 0;
-    if (item < n_list) {
-        const int64_t g = list[item];
-        const int64_t gs = group_start[g];
-        const int64_t s = group_start[g + 1] - gs;
-        const int npairs = (int)(s * (s - 1) / 2);
-        const int np = C.m - 1;
-        const int64_t cbase = cum_pairs[g];
-        for (int p = lane; p < npairs; p += 64) {
-            int64_t i, j;
-            etm::lex_pair_decode(p, s, &i, &j);
-            const int64_t ri = gs + i, rj = gs + j;
-            s_min[wv][p] = C.prev_counts ? min(C.prev_counts[ri], C.prev_counts[rj]) : 0;
-            s_flags[wv][p] = (infer && (C.prev_free[ri] == 0 || C.prev_free[rj] == 0)) ? 2 : 0;
-        }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        const int64_t ri = gs + i, rj = gs + j;
+        const int32_t a = rows[ri * m + np], b = rows[rj * m + np];
+        int32_t lowest = prev_counts ? min(prev_counts[ri], prev_counts[rj]) : 0;
+        bool nf = infer && (prev_free[ri] == 0 || prev_free[rj] == 0);
+        bool missing = false;
         if (prune || infer) {
-            for (int t = lane; t < npairs * np; t += 64) {
-                const int p = t / np, d = t % np;
-                int64_t i, j;
-                etm::lex_pair_decode(p, s, &i, &j);
-                const int64_t r = C.lookup(gs, i, j, d);
+            uint64_t h = item_hash(a) + item_hash(b);
+            for (int t = 0; t < np; ++t) h += item_hash(pre[t]);
+            for (int d = 0; d < np && !missing; ++d) {
+                Query q{pre, np, d, a, b, true};
+                const int64_t r = hash_find(rows, m, table, mask, h - item_hash(pre[d]), q);
                 if (r < 0) {
-                    atomicOr(&s_flags[wv][p], 1);
+                    missing = true;
                 } else if (infer) {
-                    atomicMin(&s_min[wv][p], C.prev_counts[r]);
-                    if (C.prev_free[r] == 0) atomicOr(&s_flags[wv][p], 2);
+                    lowest = min(lowest, prev_counts[r]);
+                    nf = nf || prev_free[r] == 0;
                 }
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        for (int p = lane; p < npairs; p += 64) {
-            const int f = s_flags[wv][p];
-            uint8_t st = ETM_CAND_COUNT;
-            if ((f & 1) && prune) st = ETM_CAND_SKIP;
-            else if (!(f & 1) && infer && (f & 2)) st = ETM_CAND_INFER;
-            const int64_t c = cbase + p;
-            status[c] = st;
-            counts[c] = st == ETM_CAND_COUNT ? 0 : (st == ETM_CAND_INFER ? s_min[wv][p] : -1);
-            if (gid) gid[c] = (int32_t)g;
+        uint8_t st = ETM_CAND_COUNT;
+        if (missing && prune) st = ETM_CAND_SKIP;
+        else if (!missing && infer && nf) st = ETM_CAND_INFER;
+        *st_out = st;
+        *cnt_out = st == ETM_CAND_COUNT ? 0 : (st == ETM_CAND_INFER ? lowest : -1);
+    }
+};
+
+// Small groups packed into waves, a lane per candidate: size class c (groups of at most 2, 4, 8, 16 rows, so
+// at most 1, 6, 28, 120 candidates) puts 64, 10, 2 or 1 groups in a wave. `cls` holds the class sizes; the
+// class lists follow one another in `list`.
+struct PackedClasses {
+    int64_t n[4];
+    __device__ static int per_group(int c) { return c == 0 ? 1 : c == 1 ? 6 : c == 2 ? 28 : 120; }
+    __device__ static int groups(int c) { return c == 0 ? 64 : c == 1 ? 10 : c == 2 ? 2 : 1; }
+};
+
+int64_t packed_waves(const int64_t* cls) {
+    const int64_t g[4] = {64, 10, 2, 1};
+    int64_t w = 0;
+    for (int c = 0; c < 4; ++c) w += (cls[c] + g[c] - 1) / g[c];
+    return w;
+}
+
+__global__ __launch_bounds__(256) void k_classify_packed(Classifier C, const int64_t* __restrict__ group_start,
+                                                          const int64_t* __restrict__ cum_pairs,
+                                                          const int32_t* __restrict__ list, PackedClasses pc,
+                                                          uint8_t* __restrict__ status, int32_t* __restrict__ counts,
+                                                          int32_t* __restrict__ gid, int64_t* __restrict__ tally) {
+    __shared__ int64_t lds[5];
+    const int lane = threadIdx.x & 63;
+    int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    int c = 0;
+    int64_t base = 0;
+    for (; c < 4; ++c) {
+        const int64_t wc = (pc.n[c] + PackedClasses::groups(c) - 1) / PackedClasses::groups(c);
+        if (wave < wc) break;
+        wave -= wc;
+        base += pc.n[c];
+    }
+    int64_t n_count = 0, n_infer = 0, n_skip = 0;
+    if (c < 4) {
+        const int P = PackedClasses::per_group(c), G = PackedClasses::groups(c);
+        for (int slot = lane; slot < G * P; slot += 64) {
+            const int64_t li = wave * G + slot / P;
+            const int p = slot % P;
+            if (li >= pc.n[c]) continue;
+            const int64_t g = list[base + li];
+            const int64_t gs = group_start[g];
+            const int64_t sz = group_start[g + 1] - gs;
+            if (p >= sz * (sz - 1) / 2) continue;
+            int64_t i, j;
+            etm::lex_pair_decode(p, sz, &i, &j);
+            uint8_t st;
+            int32_t cnt;
+            C.classify(gs, i, j, &st, &cnt);
+            const int64_t cand = cum_pairs[g] + p;
+            status[cand] = st;
+            counts[cand] = cnt;
+            if (gid) gid[cand] = (int32_t)g;
             n_count += st == ETM_CAND_COUNT;
             n_infer += st == ETM_CAND_INFER;
             n_skip += st == ETM_CAND_SKIP;
@@ -393,33 +410,16 @@ __global__ __launch_bounds__(256) void k_classify_big(Classifier C, const int64_
     if (p0 >= npairs) return;
     const int64_t p1 = min(npairs, p0 + kBigPairs);
     const int64_t cbase = cum_pairs[g];
-    const int np = C.m - 1;
-    const bool infer = (C.mode & ETM_SUBSET_INFER) != 0;
-    const bool prune = (C.mode & ETM_SUBSET_PRUNE) != 0;
     int64_t n_count = 0, n_infer = 0, n_skip = 0;
     for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
         int64_t i, j;
         etm::lex_pair_decode(p, s, &i, &j);
-        const int64_t ri = gs + i, rj = gs + j;
-        int32_t lowest = C.prev_counts ? min(C.prev_counts[ri], C.prev_counts[rj]) : 0;
-        bool nf = infer && (C.prev_free[ri] == 0 || C.prev_free[rj] == 0);
-        bool missing = false;
-        if (prune || infer)
-            for (int d = 0; d < np && !missing; ++d) {
-                const int64_t r = C.lookup(gs, i, j, d);
-                if (r < 0) {
-                    missing = true;
-                } else if (infer) {
-                    lowest = min(lowest, C.prev_counts[r]);
-                    nf = nf || C.prev_free[r] == 0;
-                }
-            }
-        uint8_t st = ETM_CAND_COUNT;
-        if (missing && prune) st = ETM_CAND_SKIP;
-        else if (!missing && infer && nf) st = ETM_CAND_INFER;
+        uint8_t st;
+        int32_t cnt;
+        C.classify(gs, i, j, &st, &cnt);
         const int64_t c = cbase + p;
         status[c] = st;
-        counts[c] = st == ETM_CAND_COUNT ? 0 : (st == ETM_CAND_INFER ? lowest : -1);
+        counts[c] = cnt;
         if (gid) gid[c] = (int32_t)g;
         n_count += st == ETM_CAND_COUNT;
         n_infer += st == ETM_CAND_INFER;
@@ -893,9 +893,12 @@ extern "C" int etm_level_classify(const int32_t* d_rows, int64_t n, int32_t m, c
         const int64_t big = h_class_counts[4];
         Classifier C{d_rows, m, d_prev_counts, d_prev_free, d_table, table_size - 1, mode};
         if (small > 0) {
-            k_classify_small<<<(unsigned)((small + 3) / 4), 256, 0, s>>>(C, d_group_start, d_cum_pairs, d_group_list,
-                                                                         small, d_status, d_counts, d_gid, d_tally);
-            ETM_LAUNCHED("k_classify_small");
+            PackedClasses pc{{h_class_counts[0], h_class_counts[1], h_class_counts[2], h_class_counts[3]}};
+            const int64_t waves = packed_waves(pc.n);
+            ETM_REQUIRE((waves + 3) / 4 <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_level_classify: grid too large");
+            k_classify_packed<<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(C, d_group_start, d_cum_pairs, d_group_list,
+                                                                          pc, d_status, d_counts, d_gid, d_tally);
+            ETM_LAUNCHED("k_classify_packed");
         }
         if (big > 0) {
             // Pair chunks per group: enough for the largest group (h_class_counts[5], when given, is its size).
