@@ -261,8 +261,30 @@ void build_groups(Arena& a, const Level& lv, const int64_t* d_n, int64_t n_cap, 
              "etm_level_groups");
 }
 
+// The id CSR and its id -> column map, when the caller mined from transactions: the row-wise K=2 / K=3
+// kernels count from these rows.
+struct RowsSrc {
+    const int64_t* indptr;
+    const void* ids;
+    int id_bytes;
+    int64_t nnz, lo, span;
+    const int32_t* lut;
+};
+
+// Crossovers of the row-wise kernels: rows count K=2 when sum C(len, 2) < kK2RowsMaxR x pairs x words, and
+// K=3 when it is < kK3RowsMaxR x K=3 candidates x words with no row over kK3RowsLongest positions. Their
+// atomics resolve device-wide, so on MI355X they win only on much sparser levels than their row pairs alone
+// suggest: wide_vocab_2m's K=2 (r = 9e-4) measured 10.5 ms row-wise against 9.0 ms on the MFMA Gram. The
+// rows are built only when a level may take them: sum C(len, 2) >= (E^2/N - E)/2 for E K=1 entries over N rows
+// (Cauchy-Schwarz) must leave the rule open.
+constexpr double kK2RowsMaxR = 2.0e-4;
+constexpr double kK3RowsMaxR = 2.5e-4;
+constexpr int64_t kK3RowsLongest = 256;
+constexpr int64_t kRowsBitsMax = 512ll << 20;
+
 void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n_rows, const int64_t* col_counts,
-                 int32_t n_cols, const etm_mine_opts& o, etm_levels& out, hipStream_t s) {
+                 int32_t n_cols, const etm_mine_opts& o, etm_levels& out, hipStream_t s,
+                 const RowsSrc* src = nullptr) {
     if (n_rows > 0x7FFFFFFF) throw Error(ETM_EOVERFLOW, "n_rows exceeds int32: the count arrays are int32");
     const int64_t mc = std::max<int64_t>(1, o.min_count);
     const bool prune = o.prune_non_free != 0;
@@ -311,11 +333,48 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
     }
     GroupsDev g = alloc_groups(a, n1);
     int64_t* scal = a.get<int64_t>(kScal);
+    // Rows as K=1 positions, built on first need (a forced row-wise level, or one the rule may give them).
+    const bool rows_forced = o.k2_kernel == ETM_KERNEL_ROWS || o.k3_kernel == ETM_KERNEL_ROWS;
+    if (rows_forced && src == nullptr)
+        throw Error(ETM_EINVAL, "the row-wise kernels need the transactions (etm_mine_csr)");
+    double k1_entries = 0;
+    for (int64_t t = 0; t < n1; ++t) k1_entries += (double)col_counts[k1_rows[(size_t)t]];
+    const double row_pairs_lb = std::max(0.0, (k1_entries * k1_entries / (double)n_rows - k1_entries) / 2);
+    int64_t* rows_ptr = nullptr;
+    int32_t* rows_pos = nullptr;
+    int32_t* d_colpos = nullptr;
+    std::vector<int32_t> colpos;  // host source of d_colpos, alive until the level loop's syncs
+    bool rows_built = false, rows_ok = false;
+    int64_t row_pairs = 0, row_longest = 0;
+    auto ensure_rows = [&]() {
+        if (rows_built) return;
+        rows_built = true;
+        colpos.assign((size_t)n_cols, -1);
+        for (int64_t t = 0; t < n1; ++t) colpos[(size_t)k1_rows[(size_t)t]] = (int32_t)t;
+        d_colpos = a.get<int32_t>(n_cols);
+        check(hipMemcpyAsync(d_colpos, colpos.data(), (size_t)n_cols * 4, hipMemcpyHostToDevice, s), "upload colpos");
+        rows_ptr = a.get<int64_t>(n_rows + 1);
+        rows_pos = a.get<int32_t>(std::max<int64_t>(src->nnz, 1));
+        int64_t* stats = a.get<int64_t>(4);
+        const size_t ws_n = etm_rows_ws(n_rows);
+        void* ws = a.get<int64_t>((int64_t)(ws_n + 7) / 8);
+        check_rc(etm_rows_build(src->indptr, src->ids, src->id_bytes, n_rows, src->lo, src->span, src->lut, d_colpos,
+                                rows_ptr, rows_pos, stats, ws, ws_n, s),
+                 "etm_rows_build");
+        check(hipMemcpyAsync(host.p + 32, stats, 32, hipMemcpyDeviceToHost, s), "readback");
+        check(hipStreamSynchronize(s), "sync");
+        row_pairs = host.p[32];
+        row_longest = host.p[33];
+        rows_ok = host.p[34] == 0;
+        if (rows_forced && !rows_ok)
+            throw Error(ETM_EINVAL, "the row-wise kernels need rows of strictly ascending, distinct item ids");
+    };
     build_groups(a, prev, nullptr, n1, g, scal + 8, s);
     check(hipMemcpyAsync(host.p, scal, kScal * 8, hipMemcpyDeviceToHost, s), "readback");
     check(hipStreamSynchronize(s), "sync");
     int64_t tot[24];
     std::memcpy(tot, host.p + 8, sizeof tot);
+
 
     struct Pending {
         int32_t* rows;
@@ -367,12 +426,46 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         const int64_t cls[4] = {tot[8], tot[9], tot[10], tot[11]};
         // At K=2 the single group holds every column: the automatic kernel choice needs no device read.
         int32_t kern = ETM_KERNEL_VALU;
-        if (k == 2)
-            kern = o.k2_kernel != ETM_KERNEL_AUTO ? o.k2_kernel
-                   : etm::detail::gram_preferred(prev.n, n_words) ? ETM_KERNEL_MFMA : ETM_KERNEL_VALU;
-        check_rc(etm_count_level(d_bv, pitch, n_words, prev.rows, m, g.gs, g.cp, g.ct, n_groups, n_tiles, g.list, cls,
-                                 status, counts, ETM_LAYOUT_LEX, 0, kern, s),
-                 "etm_count_level");
+        if (k == 2) {
+            const double limit = kK2RowsMaxR * (double)n_cands * (double)n_words;
+            if (o.k2_kernel == ETM_KERNEL_ROWS ||
+                (o.k2_kernel == ETM_KERNEL_AUTO && src != nullptr && row_pairs_lb < limit)) {
+                ensure_rows();
+                if (o.k2_kernel == ETM_KERNEL_ROWS || (rows_ok && (double)row_pairs < limit)) kern = ETM_KERNEL_ROWS;
+            }
+            if (kern != ETM_KERNEL_ROWS)
+                kern = o.k2_kernel != ETM_KERNEL_AUTO ? o.k2_kernel
+                       : etm::detail::gram_preferred(prev.n, n_words) ? ETM_KERNEL_MFMA : ETM_KERNEL_VALU;
+        } else if (k == 3) {
+            const int64_t words1 = (n1 + 63) / 64;
+            const bool fits = n1 <= 65536 && n1 * words1 * 12 <= kRowsBitsMax;
+            const double limit = kK3RowsMaxR * (double)n_cands * (double)n_words;
+            if (o.k3_kernel == ETM_KERNEL_ROWS ||
+                (o.k3_kernel == ETM_KERNEL_AUTO && src != nullptr && fits && row_pairs_lb < limit)) {
+                ensure_rows();
+                if (o.k3_kernel == ETM_KERNEL_ROWS ||
+                    (rows_ok && row_longest <= kK3RowsLongest && (double)row_pairs < limit))
+                    kern = ETM_KERNEL_ROWS;
+            }
+        }
+        st.kernel = kern;
+        if (kern == ETM_KERNEL_ROWS && k == 2) {
+            check_rc(etm_count_k2_rows(rows_ptr, rows_pos, n_rows, n1, counts, s), "etm_count_k2_rows");
+        } else if (kern == ETM_KERNEL_ROWS) {
+            const int64_t words1 = (n1 + 63) / 64;
+            uint64_t* bits = a.get<uint64_t>(n1 * words1);
+            int32_t* rank = a.get<int32_t>(n1 * words1);
+            int32_t* gof = a.get<int32_t>(n1);
+            check_rc(etm_k3_rows_prepare(prev.rows, prev.n, d_colpos, n1, g.gs, n_groups, bits, rank, gof, s),
+                     "etm_k3_rows_prepare");
+            check_rc(etm_count_k3_rows(rows_ptr, rows_pos, n_rows, bits, rank, gof, n1, g.gs, g.cp, status, counts,
+                                       s),
+                     "etm_count_k3_rows");
+        } else {
+            check_rc(etm_count_level(d_bv, pitch, n_words, prev.rows, m, g.gs, g.cp, g.ct, n_groups, n_tiles, g.list,
+                                     cls, status, counts, ETM_LAYOUT_LEX, 0, kern, s),
+                     "etm_count_level");
+        }
         if (o.time_kernels) {
             check(hipEventRecord(e1, s), "event record");
             events.emplace_back(e0, e1);
@@ -584,7 +677,8 @@ extern "C" int etm_mine_csr(const int64_t* d_indptr, const void* d_ids, int id_b
         res->col_counts.resize(n_cols);
         check(hipMemcpyAsync(res->col_counts.data(), pc, n_cols * 8, hipMemcpyDeviceToHost, s), "readback");
         check(hipStreamSynchronize(s), "sync");
-        mine_levels(bv, pitch, n_words, n_rows, res->col_counts.data(), n_cols, *opts, *res, s);
+        const RowsSrc src{d_indptr, d_ids, id_bytes, nnz, lo, span, d_lut};
+        mine_levels(bv, pitch, n_words, n_rows, res->col_counts.data(), n_cols, *opts, *res, s, &src);
     });
     if (rc == ETM_OK) *out = res.release();
     return rc;

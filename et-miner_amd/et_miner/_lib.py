@@ -16,14 +16,16 @@ from ctypes import POINTER, c_char_p, c_int, c_int32, c_int64, c_size_t, c_void_
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libetm.so"
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 ETM_OK, ETM_EINVAL, ETM_ENOMEM, ETM_EHIP, ETM_EOVERFLOW, ETM_ENODEV = 0, 1, 2, 3, 4, 5
 CAND_COUNT, CAND_SKIP, CAND_INFER = 0, 1, 2
 SUBSET_PRUNE, SUBSET_INFER = 1, 2
 LAYOUT_LEX, LAYOUT_TRI = 0, 1
-KERNEL_AUTO, KERNEL_VALU, KERNEL_MFMA = 0, 1, 2
-K2_KERNELS = {"auto": KERNEL_AUTO, "valu": KERNEL_VALU, "mfma": KERNEL_MFMA}
+KERNEL_AUTO, KERNEL_VALU, KERNEL_MFMA, KERNEL_ROWS = 0, 1, 2, 3
+K2_KERNELS = {"auto": KERNEL_AUTO, "valu": KERNEL_VALU, "mfma": KERNEL_MFMA, "rows": KERNEL_ROWS}
+K3_KERNELS = {"auto": KERNEL_AUTO, "valu": KERNEL_VALU, "rows": KERNEL_ROWS}
+KERNEL_NAMES = {KERNEL_AUTO: "", KERNEL_VALU: "valu", KERNEL_MFMA: "mfma", KERNEL_ROWS: "rows"}
 
 P = c_void_p  # every device pointer crosses the ABI as void*
 
@@ -48,6 +50,11 @@ SIGNATURES: dict = {
     "etm_level_free": (c_int, [P, c_int64, c_int32, P, P, P, P, c_int64, P, P]),
     "etm_select_rows": (c_int, [P, c_int64, c_int32, P, P, P, P, P, P, c_size_t, P]),
     "etm_count_itemsets": (c_int, [P, c_int64, c_int64, P, P, c_int64, P, c_int32, P]),
+    "etm_rows_ws": (c_size_t, [c_int64]),
+    "etm_rows_build": (c_int, [P, P, c_int, c_int64, c_int64, c_int64, P, P, P, P, P, P, c_size_t, P]),
+    "etm_count_k2_rows": (c_int, [P, P, c_int64, c_int64, P, P]),
+    "etm_k3_rows_prepare": (c_int, [P, c_int64, P, c_int64, P, c_int64, P, P, P, P]),
+    "etm_count_k3_rows": (c_int, [P, P, c_int64, P, P, P, c_int64, P, P, P, P, P]),
     "etm_mine_bitvecs": (c_int, [P, c_int64, c_int64, c_int64, P, c_int32, P, P, P]),
     "etm_mine_csr": (c_int, [P, P, c_int, c_int64, c_int64, c_int64, P, P, P]),
     "etm_levels_count": (c_int, [P, P]),
@@ -64,13 +71,14 @@ class MineOpts(ctypes.Structure):
     """etm_mine_opts."""
 
     _fields_ = [("min_count", c_int64), ("max_length", c_int32), ("prune_non_free", c_int32),
-                ("infer_counts", c_int32), ("time_kernels", c_int32), ("k2_kernel", c_int32)]
+                ("infer_counts", c_int32), ("time_kernels", c_int32), ("k2_kernel", c_int32),
+                ("k3_kernel", c_int32), ("pad", c_int32)]
 
 
 class LevelStat(ctypes.Structure):
     """etm_level_stat."""
 
-    _fields_ = [("k", c_int32), ("pad", c_int32), ("n_candidates", c_int64), ("n_counted", c_int64),
+    _fields_ = [("k", c_int32), ("kernel", c_int32), ("n_candidates", c_int64), ("n_counted", c_int64),
                 ("n_inferred", c_int64), ("n_skipped", c_int64), ("n_frequent", c_int64), ("bytes_alg", c_int64),
                 ("count_ms", ctypes.c_float), ("pad2", ctypes.c_float)]
 

@@ -25,6 +25,7 @@ __all__ = [
     "SMALL_MAX", "MAX_K", "words_for", "padded_pitch", "ids_minmax", "ids_hist", "build_bitvecs",
     "column_popcounts", "Groups", "level_groups", "level_hash", "level_classify", "count_level", "level_compact",
     "level_free", "select_rows", "count_pairs_k2_shared", "count_itemsets_flat", "threshold_filter", "UNTOUCHED",
+    "DeviceRows", "upload_rows", "count_pairs_k2_rows", "K3Graph", "build_k3_graph", "count_k3_rows",
 ]
 
 #: Groups with at most this many suffixes run the wave-per-group kernel; larger ones the tiled kernel.
@@ -100,6 +101,82 @@ def column_popcounts(bv, n_words: int | None = None, pitch: int | None = None):
 
 
 @dataclass
+class DeviceRows:
+    """Transactions as ascending K=1 positions on the device (the row-wise kernels' input)."""
+
+    row_ptr: object  # int64 (n_rows + 1)
+    pos: object  # int32, row_ptr[-1] entries used
+    n_rows: int
+    row_pairs: int  # sum over rows of C(len, 2)
+    longest: int
+    ascending: bool  # every row's ids ascend strictly (the row-wise kernels need it)
+
+
+def upload_rows(indptr, ids, n_rows: int, lo: int, span: int, lut, colpos) -> DeviceRows:
+    """The rows of a device CSR as K=1 positions: id v -> colpos[lut[v - lo]] (-1 drops it). Synchronizes
+    for the row statistics (reference DeviceRows.upload, gpu/input_layer.py:160-197)."""
+    if indptr.numel() != n_rows + 1:
+        raise ValueError(f"indptr holds {indptr.numel()} entries, expected n_rows + 1 = {n_rows + 1}")
+    row_ptr = dev.empty(n_rows + 1, "int64")
+    pos = dev.empty(max(1, ids.numel()), "int32")
+    stats = dev.empty(4, "int64")
+    ws_n = _lib.ws_bytes("etm_rows_ws", n_rows)
+    ws = dev.empty((ws_n + 7) // 8, "int64")
+    call("etm_rows_build", indptr.data_ptr(), ids.data_ptr() if ids.numel() else indptr.data_ptr(), _id_bytes(ids),
+         n_rows, lo, span, lut.data_ptr(), colpos.data_ptr(), row_ptr.data_ptr(), pos.data_ptr(), stats.data_ptr(),
+         ws.data_ptr(), ws_n, _s())
+    st = dev.to_host(stats).tolist()
+    return DeviceRows(row_ptr, pos, n_rows, int(st[0]), int(st[1]), st[2] == 0)
+
+
+def count_pairs_k2_rows(rows: DeviceRows, n1: int):
+    """Row-wise K=2 over n1 K=1 positions: device int32 counts in lexicographic pair order (reference
+    gpu/kernels/k2.py:79-135)."""
+    if not rows.ascending:
+        raise ValueError("count_pairs_k2_rows: rows must hold strictly ascending, distinct ids")
+    out = dev.zeros(max(1, n1 * (n1 - 1) // 2), "int32")
+    call("etm_count_k2_rows", rows.row_ptr.data_ptr(), rows.pos.data_ptr(), rows.n_rows, n1, out.data_ptr(), _s())
+    return out
+
+
+@dataclass
+class K3Graph:
+    """The K=2 level as a pair bitmap over K=1 positions (reference K3Graph, gpu/kernels/k3_rows.py:76-90)."""
+
+    bits: object
+    rank: object
+    gof: object
+    n1: int
+
+
+def build_k3_graph(rows2, colpos, n1: int, groups) -> K3Graph:
+    """Pair bitmap, word ranks and prefix groups of the lexsorted K=2 level ``rows2`` (n2 x 2 column ids)."""
+    if n1 > 65536:
+        raise ValueError(f"build_k3_graph: {n1} K=1 positions exceed 65536")
+    words = (n1 + 63) // 64
+    bits = dev.empty(n1 * words, "int64")
+    rank = dev.empty(n1 * words, "int32")
+    gof = dev.empty(n1, "int32")
+    call("etm_k3_rows_prepare", rows2.data_ptr(), int(rows2.shape[0]), colpos.data_ptr(), n1,
+         groups.group_start.data_ptr(), groups.n_groups, bits.data_ptr(), rank.data_ptr(), gof.data_ptr(), _s())
+    return K3Graph(bits, rank, gof, n1)
+
+
+def count_k3_rows(rows: DeviceRows, graph: K3Graph, groups, status=None, counts=None):
+    """Row-wise K=3 into the K=2 level's candidate numbering: adds 1 per containing row to each candidate whose
+    status is COUNT (every candidate when ``status`` is None); returns the int32 counts (reference
+    gpu/kernels/k3_rows.py:126-178)."""
+    if not rows.ascending:
+        raise ValueError("count_k3_rows: rows must hold strictly ascending, distinct ids")
+    if counts is None:
+        counts = dev.zeros(max(1, groups.n_cands), "int32")
+    call("etm_count_k3_rows", rows.row_ptr.data_ptr(), rows.pos.data_ptr(), rows.n_rows, graph.bits.data_ptr(),
+         graph.rank.data_ptr(), graph.gof.data_ptr(), graph.n1, groups.group_start.data_ptr(),
+         groups.cum_pairs.data_ptr(), status.data_ptr() if status is not None else None, counts.data_ptr(), _s())
+    return counts
+
+
+@dataclass
 class Groups:
     """Prefix groups of a level (device arrays) and their totals (host ints)."""
 
@@ -170,6 +247,8 @@ def count_level(bv, n_words: int, rows, m: int, groups: Groups, status, counts, 
     """Adds the AND + popcount counts of the counted candidates into ``counts``.
 
     ``kernel`` ("auto", "valu", "mfma") picks the K=2 pair space's kernel (ignored for other levels)."""
+    if kernel not in ("auto", "valu", "mfma"):
+        raise ValueError(f"count_level: kernel must be 'auto', 'valu' or 'mfma', got {kernel!r}")
     pitch = int(bv.shape[1]) if pitch is None else int(pitch)
     cls = (ctypes.c_int64 * 4)(*groups.class_counts[:4])
     call("etm_count_level", bv.data_ptr(), pitch, n_words, rows.data_ptr(), m, groups.group_start.data_ptr(),

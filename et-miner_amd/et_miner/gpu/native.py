@@ -39,9 +39,13 @@ class NativeResult:
         return pa.table(cols)
 
 
-def _opts(min_count, max_length, prune, infer, time_kernels, k2_kernel="auto") -> _lib.MineOpts:
+def _opts(min_count, max_length, prune, infer, time_kernels, k2_kernel="auto", k3_kernel="auto") -> _lib.MineOpts:
+    if k2_kernel not in _lib.K2_KERNELS:
+        raise ValueError(f"k2_kernel must be one of {sorted(_lib.K2_KERNELS)}, got {k2_kernel!r}")
+    if k3_kernel not in _lib.K3_KERNELS:
+        raise ValueError(f"k3_kernel must be one of {sorted(_lib.K3_KERNELS)}, got {k3_kernel!r}")
     return _lib.MineOpts(int(max(1, min_count)), int(max_length or 0), int(bool(prune)), int(bool(infer)),
-                         int(bool(time_kernels)), _lib.K2_KERNELS[k2_kernel])
+                         int(bool(time_kernels)), _lib.K2_KERNELS[k2_kernel], _lib.K3_KERNELS[k3_kernel], 0)
 
 
 def _collect(handle, levels: bool = True) -> NativeResult:
@@ -70,7 +74,8 @@ def _collect(handle, levels: bool = True) -> NativeResult:
             st = _lib.LevelStat()
             _lib.call("etm_levels_stats", handle, lv, ctypes.byref(st))
             res.stats.append(LevelStats(st.k, st.n_candidates, st.n_frequent, st.n_counted, st.n_inferred,
-                                        st.n_skipped, st.bytes_alg, 0.0, st.count_ms))
+                                        st.n_skipped, st.bytes_alg, 0.0, st.count_ms,
+                                        _lib.KERNEL_NAMES.get(st.kernel, "")))
         n_cols = ctypes.c_int32()
         _lib.call("etm_levels_columns", handle, ctypes.byref(n_cols), None, None)
         c2i = np.empty(n_cols.value, dtype=np.int64)
@@ -84,14 +89,17 @@ def _collect(handle, levels: bool = True) -> NativeResult:
 
 def mine_csr_device(indptr_d, ids_d, n_rows: int, min_count: int, max_length=None, prune_non_free=False,
                     infer_counts=False, time_kernels=False, levels: bool = True,
-                    k2_kernel: str = "auto") -> NativeResult:
-    """Every frequent itemset of a device CSR of item ids (int32 or int64), mined by the native loop."""
+                    k2_kernel: str = "auto", k3_kernel: str = "auto") -> NativeResult:
+    """Every frequent itemset of a device CSR of item ids (int32 or int64), mined by the native loop.
+
+    ``k2_kernel``: "auto", "valu", "mfma" or "rows"; ``k3_kernel``: "auto", "valu" (bitvector kernels) or
+    "rows" (row-wise, which needs rows of strictly ascending distinct ids)."""
     torch = dev.torch()
     id_bytes = 8 if ids_d.dtype == torch.int64 else 4
     if ids_d.dtype not in (torch.int64, torch.int32):
         raise ValueError(f"item ids must be int32 or int64, got {ids_d.dtype}")
     handle = ctypes.c_void_p()
-    opts = _opts(min_count, max_length, prune_non_free, infer_counts, time_kernels, k2_kernel)
+    opts = _opts(min_count, max_length, prune_non_free, infer_counts, time_kernels, k2_kernel, k3_kernel)
     _lib.call("etm_mine_csr", indptr_d.data_ptr(), ids_d.data_ptr() if ids_d.numel() else None, id_bytes, n_rows,
               ids_d.numel(), SPAN_LIMIT, ctypes.byref(opts), ctypes.byref(handle), dev.stream_ptr())
     return _collect(handle, levels)

@@ -44,6 +44,7 @@ class LevelStats:
     bytes_alg: int = 0
     ms: float = 0.0
     count_ms: float = 0.0  # event-timed duration of the level's counting launches (native loop, when timed)
+    kernel: str = ""  # the native loop's counting kernel family for the level: "valu", "mfma" or "rows"
 
     @property
     def n_evaluated(self) -> int:

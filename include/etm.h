@@ -55,10 +55,10 @@ enum { ETM_SUBSET_PRUNE = 1, ETM_SUBSET_INFER = 2 };
 enum { ETM_LAYOUT_LEX = 0, ETM_LAYOUT_TRI = 1 };
 /* Kernel choice for the pair space of a single-group level (K=2): automatic (int8 MFMA Gram from 128 columns,
  * else VALU AND+popcount tiles), VALU tiles only, or the MFMA Gram regardless of size. */
-enum { ETM_KERNEL_AUTO = 0, ETM_KERNEL_VALU = 1, ETM_KERNEL_MFMA = 2 };
+enum { ETM_KERNEL_AUTO = 0, ETM_KERNEL_VALU = 1, ETM_KERNEL_MFMA = 2, ETM_KERNEL_ROWS = 3 };
 
 /* ABI version of this header (bumped on any signature change). */
-#define ETM_ABI_VERSION 1
+#define ETM_ABI_VERSION 2
 int etm_abi_version(void);
 
 /* Copies the calling thread's last error message (NUL-terminated, truncated to n). */
@@ -184,6 +184,40 @@ int etm_select_rows(const int32_t* d_rows, int64_t n, int32_t m, const int32_t* 
 int etm_count_itemsets(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const int32_t* d_items,
                        const int64_t* d_offsets, int64_t n_sets, int64_t* d_out, int32_t accumulate, void* stream);
 
+/* ---- Row-wise counting of K=2 and K=3 ---------------------------------------------------------------- */
+
+/* The transactions as ascending K=1 positions: position of id v = d_colpos[d_lut[v - lo]] (-1 drops it).
+ * d_row_ptr (n_rows + 1) and d_pos (capacity nnz) receive the rows; d_stats[0..3] = sum over rows of C(len, 2),
+ * the longest row, 1 when some row's ids do not ascend strictly (the row-wise kernels then do not apply), and
+ * the number of positions. d_ws holds etm_rows_ws(n_rows) bytes. Replaces DeviceRows and _canonicalize
+ * (gpu/input_layer.py:101-242) on one GPU. */
+size_t etm_rows_ws(int64_t n_rows);
+int etm_rows_build(const int64_t* d_indptr, const void* d_ids, int id_bytes, int64_t n_rows, int64_t lo, int64_t span,
+                   const int32_t* d_lut, const int32_t* d_colpos, int64_t* d_row_ptr, int32_t* d_pos,
+                   int64_t* d_stats, void* d_ws, size_t ws_bytes, void* stream);
+
+/* Row-wise K=2: every row adds 1 to d_counts[lex_pair_index(i, j, n1)] for each pair i < j of its positions
+ * (d_counts zeroed by the caller). Rows must ascend strictly. Replaces count_pairs_k2_rows
+ * (gpu/kernels/k2.py:79-135, gpu/kernels/_src/pairs_k2_rows.cu:53-69). */
+int etm_count_k2_rows(const int64_t* d_row_ptr, const int32_t* d_pos, int64_t n_rows, int64_t n1, int32_t* d_counts,
+                      void* stream);
+
+/* The K=2 level (d_rows2: n2 lexsorted pairs of column ids, its prefix groups d_group_start / n_groups) as a
+ * pair bitmap over the n1 <= 65536 K=1 positions: d_bits and d_rank are n1 x ceil(n1/64) (uint64 bits, int32
+ * set bits before each word), d_gof (n1) the group of each prefix or -1. Replaces k3_layout, k3_pair_bits and
+ * build_k3_graph (gpu/kernels/k3_rows.py:78-125). */
+int etm_k3_rows_prepare(const int32_t* d_rows2, int64_t n2, const int32_t* d_colpos, int64_t n1,
+                        const int64_t* d_group_start, int64_t n_groups, uint64_t* d_bits, int32_t* d_rank,
+                        int32_t* d_gof, void* stream);
+
+/* Row-wise K=3: every row adds 1 to each candidate (a, b, c) of its positions with (a, b) and (a, c) in the
+ * K=2 level whose status is ETM_CAND_COUNT (every one when d_status is NULL); candidates are numbered by the
+ * K=2 level's prefix groups (d_cum_pairs). Rows must ascend strictly. Replaces count_k3_rows and
+ * decode_k3_slots (gpu/kernels/k3_rows.py:126-178, gpu/kernels/_src/k3_rows.cu:49-140). */
+int etm_count_k3_rows(const int64_t* d_row_ptr, const int32_t* d_pos, int64_t n_rows, const uint64_t* d_bits,
+                      const int32_t* d_rank, const int32_t* d_gof, int64_t n1, const int64_t* d_group_start,
+                      const int64_t* d_cum_pairs, const uint8_t* d_status, int32_t* d_counts, void* stream);
+
 /* ---- The native level loop --------------------------------------------------------------------------- */
 
 /* Options of a mining call. min_count is the absolute threshold (>= 1 is enforced); max_length 0 = none. */
@@ -193,7 +227,9 @@ typedef struct etm_mine_opts {
     int32_t prune_non_free; /* free-set mode (prune_equal_support) */
     int32_t infer_counts;   /* Pascal inference (use_generator_pruning) */
     int32_t time_kernels;   /* record HIP events around each level's counting launches */
-    int32_t k2_kernel;      /* ETM_KERNEL_* for the K=2 pair space */
+    int32_t k2_kernel;      /* ETM_KERNEL_* for the K=2 pair space (ROWS: row-wise) */
+    int32_t k3_kernel;      /* ETM_KERNEL_AUTO, ETM_KERNEL_VALU (bitvector kernels) or ETM_KERNEL_ROWS for K=3 */
+    int32_t pad;
 } etm_mine_opts;
 
 /* Per-level statistics: candidates generated by the prefix join and, of them, counted / inferred / skipped by
@@ -201,7 +237,7 @@ typedef struct etm_mine_opts {
  * event-timed duration (when time_kernels). */
 typedef struct etm_level_stat {
     int32_t k;
-    int32_t pad;
+    int32_t kernel; /* ETM_KERNEL_* that counted the level (K >= 2; VALU also covers the group kernels) */
     int64_t n_candidates;
     int64_t n_counted;
     int64_t n_inferred;
