@@ -85,7 +85,7 @@ struct StoreTotal {
 };
 
 __global__ void k_zero_i64(int64_t* p, int n) {
-    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
 // Size class of a group of s suffixes: 0 (2), 1 (3-4), 2 (5-8), 3 (9-16) for the wave-per-group kernels,
@@ -811,7 +811,7 @@ hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s) {
 }
 
 hipError_t launch_zero_i64(int64_t* p, int n, hipStream_t s) {
-    k_zero_i64<<<1, 64, 0, s>>>(p, n);
+    k_zero_i64<<<std::max(1, std::min((n + 255) / 256, 1024)), 256, 0, s>>>(p, n);
     return hipGetLastError();
 }
 

@@ -159,6 +159,35 @@ int32_t* staging(int64_t n) {
     return buf;
 }
 
+// Waits for the stream by polling it: the per-level readbacks wait tens of microseconds at most, and a
+// blocking wait's wake-up latency would add as much again to every level.
+void wait_stream(hipStream_t s) {
+    for (;;) {
+        const hipError_t e = hipStreamQuery(s);
+        if (e == hipSuccess) return;
+        if (e != hipErrorNotReady) check(e, "stream");
+        __builtin_ia32_pause();
+    }
+}
+
+// Copies of several device int32 segments into one contiguous device buffer (one kernel, then one
+// device-to-host copy instead of one per segment).
+struct Seg {
+    const int32_t* src;
+    int64_t n;
+    int64_t dst;
+};
+constexpr int kGatherSegs = 32;
+struct Segs {
+    Seg s[kGatherSegs];
+};
+
+__global__ __launch_bounds__(256) void k_gather(Segs segs, int32_t* __restrict__ out) {
+    const Seg sg = segs.s[blockIdx.y];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < sg.n; i += (int64_t)gridDim.x * blockDim.x)
+        out[sg.dst + i] = sg.src[i];
+}
+
 // Timing events of this thread, created on first use and reused by every later call on the same device
 // (event creation costs far more than recording).
 hipEvent_t pooled_event(size_t i) {
@@ -362,7 +391,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
                                 rows_ptr, rows_pos, stats, ws, ws_n, s),
                  "etm_rows_build");
         check(hipMemcpyAsync(host.p + 32, stats, 32, hipMemcpyDeviceToHost, s), "readback");
-        check(hipStreamSynchronize(s), "sync");
+        wait_stream(s);
         row_pairs = host.p[32];
         row_longest = host.p[33];
         rows_ok = host.p[34] == 0;
@@ -371,7 +400,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
     };
     build_groups(a, prev, nullptr, n1, g, scal + 8, s);
     check(hipMemcpyAsync(host.p, scal, kScal * 8, hipMemcpyDeviceToHost, s), "readback");
-    check(hipStreamSynchronize(s), "sync");
+    wait_stream(s);
     int64_t tot[24];
     std::memcpy(tot, host.p + 8, sizeof tot);
 
@@ -385,7 +414,11 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
     std::vector<Pending> levels;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
 
-    for (int32_t k = 2; k <= max_len && prev.n >= k; ++k) {
+    // Device scalars of every level (k <= 62 + 2), zeroed once.
+    constexpr int kMaxLevels = 64;
+    int64_t* level_scal = a.get<int64_t>((int64_t)kMaxLevels * kScal);
+    check(etm::detail::launch_zero_i64(level_scal, kMaxLevels * kScal, s), "zero");
+    for (int32_t k = 2; k <= max_len && prev.n >= k && k - 2 < kMaxLevels; ++k) {
         const int32_t m = k - 1;
         const int64_t n_groups = tot[0], n_cands = tot[1], n_tiles = tot[2];
         if (n_cands == 0) break;
@@ -399,8 +432,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
             prev.table = a.get<int32_t>(prev.table_size);
             check(etm::detail::launch_hash(prev.rows, prev.n, nullptr, prev.m, prev.table, prev.table_size, s), "hash");
         }
-        int64_t* cur_scal = a.get<int64_t>(kScal);
-        check(etm::detail::launch_zero_i64(cur_scal, kScal, s), "zero");
+        int64_t* cur_scal = level_scal + (int64_t)(k - 2) * kScal;
         uint8_t* status = nullptr;
         int32_t* gid = nullptr;
         int32_t* counts = a.get<int32_t>(n_cands);
@@ -488,7 +520,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
                                        nullptr, nullptr, nullptr, cur_scal + 0, ws, ws_n, s),
                      "etm_level_compact");
             check(hipMemcpyAsync(host.p, cur_scal, 8, hipMemcpyDeviceToHost, s), "readback");
-            check(hipStreamSynchronize(s), "sync");
+            wait_stream(s);
             cap = host.p[0];
             cur.rows = a.get<int32_t>(cap * k);
             cur.counts = a.get<int32_t>(cap);
@@ -514,7 +546,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
                           "select");
                 } else {
                     check(hipMemcpyAsync(host.p, cur_scal, 8, hipMemcpyDeviceToHost, s), "readback");
-                    check(hipStreamSynchronize(s), "sync");
+                    wait_stream(s);
                     const int64_t nn = host.p[0];
                     const size_t ws_n = etm_level_compact_ws(nn);
                     int64_t* ws = a.get<int64_t>((int64_t)(ws_n + 7) / 8);
@@ -543,7 +575,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
                   "group build");
         }
         check(hipMemcpyAsync(host.p, cur_scal, kScal * 8, hipMemcpyDeviceToHost, s), "readback");
-        check(hipStreamSynchronize(s), "sync");
+        wait_stream(s);
         cur.n = host.p[prune ? 4 : 0];
         st.n_counted = k >= 3 ? host.p[1] : n_cands;
         st.n_inferred = k >= 3 ? host.p[2] : 0;
@@ -559,7 +591,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
             g = alloc_groups(a, cur.n);
             build_groups(a, cur, nullptr, cur.n, g, cur_scal + 8, s);
             check(hipMemcpyAsync(host.p, cur_scal, kScal * 8, hipMemcpyDeviceToHost, s), "readback");
-            check(hipStreamSynchronize(s), "sync");
+            wait_stream(s);
             std::memcpy(tot, host.p + 8, sizeof tot);
         }
         prev = cur;
@@ -572,14 +604,33 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
     out.block = stage;
     out.block_cap = cap_ints;
     int64_t at = 0;
-    for (const Pending& p : levels) {
-        if (p.n) {
-            check(hipMemcpyAsync(stage + at, p.rows, p.n * p.k * 4, hipMemcpyDeviceToHost, s), "copy out");
-            check(hipMemcpyAsync(stage + at + p.n * p.k, p.counts, p.n * 4, hipMemcpyDeviceToHost, s), "copy out");
+    if (total_ints > 0) {
+        int32_t* dev_stage = a.get<int32_t>(total_ints);
+        Segs segs{};
+        int ns = 0;
+        int64_t longest = 0;
+        auto flush = [&]() {
+            if (ns == 0) return;
+            const int64_t bx = std::min<int64_t>(std::max<int64_t>((longest + 255) / 256, 1), 1024);
+            k_gather<<<dim3((unsigned)bx, (unsigned)ns), 256, 0, s>>>(segs, dev_stage);
+            check(hipGetLastError(), "gather");
+            segs = Segs{};
+            ns = 0;
+            longest = 0;
+        };
+        for (const Pending& p : levels) {
+            if (p.n) {
+                segs.s[ns++] = Seg{p.rows, p.n * p.k, at};
+                segs.s[ns++] = Seg{p.counts, p.n, at + p.n * p.k};
+                longest = std::max(longest, p.n * p.k);
+                if (ns + 2 > kGatherSegs) flush();
+            }
+            at += p.n * (p.k + 1);
         }
-        at += p.n * (p.k + 1);
+        flush();
+        check(hipMemcpyAsync(stage, dev_stage, (size_t)total_ints * 4, hipMemcpyDeviceToHost, s), "copy out");
     }
-    check(hipStreamSynchronize(s), "sync");
+    wait_stream(s);
     at = 0;
     for (const Pending& p : levels) {
         out.k.push_back(p.k);
@@ -645,7 +696,7 @@ extern "C" int etm_mine_csr(const int64_t* d_indptr, const void* d_ids, int id_b
         int64_t* mm = a.get<int64_t>(2);
         check_rc(etm_ids_minmax(d_ids, id_bytes, nnz, mm, s), "etm_ids_minmax");
         check(hipMemcpyAsync(host.p, mm, 16, hipMemcpyDeviceToHost, s), "readback");
-        check(hipStreamSynchronize(s), "sync");
+        wait_stream(s);
         const int64_t lo = host.p[0], span = host.p[1] - host.p[0] + 1;
         if (span > span_limit)
             throw Error(ETM_EINVAL, "item ids span " + std::to_string(span) + " values: code them to dense ids first");
@@ -655,7 +706,7 @@ extern "C" int etm_mine_csr(const int64_t* d_indptr, const void* d_ids, int id_b
         int32_t* h_hist = staging(2 * span);
         int32_t* lut = h_hist + span;
         check(hipMemcpyAsync(h_hist, hist, span * 4, hipMemcpyDeviceToHost, s), "readback");
-        check(hipStreamSynchronize(s), "sync");
+        wait_stream(s);
         for (int64_t v = 0; v < span; ++v) {
             lut[v] = -1;
             if (h_hist[v] >= mc) {
@@ -676,7 +727,7 @@ extern "C" int etm_mine_csr(const int64_t* d_indptr, const void* d_ids, int id_b
         check_rc(etm_column_popcounts(bv, n_cols, pitch, n_words, pc, s), "etm_column_popcounts");
         res->col_counts.resize(n_cols);
         check(hipMemcpyAsync(res->col_counts.data(), pc, n_cols * 8, hipMemcpyDeviceToHost, s), "readback");
-        check(hipStreamSynchronize(s), "sync");
+        wait_stream(s);
         const RowsSrc src{d_indptr, d_ids, id_bytes, nnz, lo, span, d_lut};
         mine_levels(bv, pitch, n_words, n_rows, res->col_counts.data(), n_cols, *opts, *res, s, &src);
     });
