@@ -534,8 +534,10 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
     int64_t base = 0;
     for (int c = 0; c < 4; ++c) {
         if (cls[c] == 0) continue;
-        // Work items scale with the class's per-word VALU cost: wider classes get more waves per word range.
-        const int64_t chunk = word_chunk > 0 ? word_chunk : pick_chunk(n_words, cls[c], 16384, 64, 512);
+        // About 4096 waves per class and chunks of at least 256 words: classes of many groups keep long chunks
+        // (fewer partial sums, chunk-major L2 reuse per XCD), classes of few groups get enough waves (measured
+        // on MI355X over 256-4096-word floors and 4K-64K wave targets).
+        const int64_t chunk = word_chunk > 0 ? word_chunk : pick_chunk(n_words, cls[c], 4096, 64, 256);
         hipError_t e;
         const int32_t* list = d_small_list + base;
         switch (c) {

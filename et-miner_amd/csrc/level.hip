@@ -290,6 +290,18 @@ struct Classifier {
     int64_t mask;
     int32_t mode;
 
+    // Row of the subset of candidate (i, j) of the group at row gs that drops prefix item d, or -1.
+    __device__ int64_t probe(int64_t gs, int64_t i, int64_t j, int d) const {
+        const int32_t* pre = rows + gs * m;
+        const int np = m - 1;
+        const int32_t a = rows[(gs + i) * m + np], b = rows[(gs + j) * m + np];
+        uint64_t h = item_hash(a) + item_hash(b);
+        for (int t = 0; t < np; ++t)
+            if (t != d) h += item_hash(pre[t]);
+        Query q{pre, np, d, a, b, true};
+        return hash_find(rows, m, table, mask, h, q);
+    }
+
     // Status and count of candidate (i, j) of the group starting at row gs: the hash of all its items is
     // formed once, and each (k-1)-subset dropping a prefix item is one subtraction and one probe away.
     __device__ void classify(int64_t gs, int64_t i, int64_t j, uint8_t* st_out, int32_t* cnt_out) const {
@@ -324,19 +336,27 @@ struct Classifier {
     }
 };
 
-// Small groups packed into waves, a lane per candidate: size class c (groups of at most 2, 4, 8, 16 rows, so
-// at most 1, 6, 28, 120 candidates) puts 64, 10, 2 or 1 groups in a wave. `cls` holds the class sizes; the
-// class lists follow one another in `list`.
+// Small groups packed into waves, a lane per (candidate, dropped prefix item): a candidate's k-2 subset probes
+// run in parallel lanes (their latency does not add up on deep levels) and several groups share a wave when
+// they have few (candidate, probe) pairs. Size class c (groups of at most 2, 4, 8, 16 rows: at most
+// P = 1, 6, 28, 120 candidates) puts G = max(1, 64 / (P x probes)) groups in a wave; each candidate's outcome
+// gathers in LDS (at most 120 candidates per wave). `n` holds the class sizes; the class lists follow one
+// another in `list`.
+constexpr int kPackedSlots = 120;
+
 struct PackedClasses {
     int64_t n[4];
-    __device__ static int per_group(int c) { return c == 0 ? 1 : c == 1 ? 6 : c == 2 ? 28 : 120; }
-    __device__ static int groups(int c) { return c == 0 ? 64 : c == 1 ? 10 : c == 2 ? 2 : 1; }
+    int32_t probes;  // k-2 (at least 1)
+    __device__ __host__ static int per_group(int c) { return c == 0 ? 1 : c == 1 ? 6 : c == 2 ? 28 : 120; }
+    __device__ __host__ int groups(int c) const {
+        const int g = 64 / (per_group(c) * probes);
+        return g < 1 ? 1 : g;
+    }
 };
 
-int64_t packed_waves(const int64_t* cls) {
-    const int64_t g[4] = {64, 10, 2, 1};
+int64_t packed_waves(const PackedClasses& pc) {
     int64_t w = 0;
-    for (int c = 0; c < 4; ++c) w += (cls[c] + g[c] - 1) / g[c];
+    for (int c = 0; c < 4; ++c) w += (pc.n[c] + pc.groups(c) - 1) / pc.groups(c);
     return w;
 }
 
@@ -345,36 +365,80 @@ __global__ __launch_bounds__(256) void k_classify_packed(Classifier C, const int
                                                           const int32_t* __restrict__ list, PackedClasses pc,
                                                           uint8_t* __restrict__ status, int32_t* __restrict__ counts,
                                                           int32_t* __restrict__ gid, int64_t* __restrict__ tally) {
+    __shared__ int32_t s_min[4][kPackedSlots];
+    __shared__ int32_t s_flags[4][kPackedSlots];  // bit 0: a subset is missing, bit 1: a subset is not free
     __shared__ int64_t lds[5];
-    const int lane = threadIdx.x & 63;
-    int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const bool infer = (C.mode & ETM_SUBSET_INFER) != 0;
+    const bool prune = (C.mode & ETM_SUBSET_PRUNE) != 0;
+    int64_t wave = (int64_t)blockIdx.x * 4 + wv;
     int c = 0;
     int64_t base = 0;
     for (; c < 4; ++c) {
-        const int64_t wc = (pc.n[c] + PackedClasses::groups(c) - 1) / PackedClasses::groups(c);
+        const int64_t wc = (pc.n[c] + pc.groups(c) - 1) / pc.groups(c);
         if (wave < wc) break;
         wave -= wc;
         base += pc.n[c];
     }
     int64_t n_count = 0, n_infer = 0, n_skip = 0;
     if (c < 4) {
-        const int P = PackedClasses::per_group(c), G = PackedClasses::groups(c);
-        for (int slot = lane; slot < G * P; slot += 64) {
-            const int64_t li = wave * G + slot / P;
-            const int p = slot % P;
-            if (li >= pc.n[c]) continue;
-            const int64_t g = list[base + li];
-            const int64_t gs = group_start[g];
-            const int64_t sz = group_start[g + 1] - gs;
-            if (p >= sz * (sz - 1) / 2) continue;
-            int64_t i, j;
-            etm::lex_pair_decode(p, sz, &i, &j);
-            uint8_t st;
-            int32_t cnt;
-            C.classify(gs, i, j, &st, &cnt);
-            const int64_t cand = cum_pairs[g] + p;
+        const int P = PackedClasses::per_group(c), G = pc.groups(c);
+        const int np = C.m - 1;
+        const int slots = G * P;
+        // Seed each candidate slot from its two parents.
+        for (int sl = lane; sl < slots; sl += 64) {
+            const int64_t li = wave * G + sl / P;
+            const int p = sl % P;
+            int32_t mn = 0, fl = 4;  // bit 2: no such candidate
+            if (li < pc.n[c]) {
+                const int64_t g = list[base + li];
+                const int64_t gs = group_start[g];
+                const int64_t sz = group_start[g + 1] - gs;
+                if (p < sz * (sz - 1) / 2) {
+                    int64_t i, j;
+                    etm::lex_pair_decode(p, sz, &i, &j);
+                    const int64_t ri = gs + i, rj = gs + j;
+                    mn = C.prev_counts ? min(C.prev_counts[ri], C.prev_counts[rj]) : 0;
+                    fl = (infer && (C.prev_free[ri] == 0 || C.prev_free[rj] == 0)) ? 2 : 0;
+                }
+            }
+            s_min[wv][sl] = mn;
+            s_flags[wv][sl] = fl;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (prune || infer) {
+            for (int t = lane; t < slots * np; t += 64) {
+                const int sl = t / np, d = t % np;
+                if (s_flags[wv][sl] & 4) continue;
+                const int64_t g = list[base + wave * G + sl / P];
+                const int64_t gs = group_start[g];
+                const int64_t sz = group_start[g + 1] - gs;
+                int64_t i, j;
+                etm::lex_pair_decode(sl % P, sz, &i, &j);
+                const int64_t r = C.probe(gs, i, j, d);
+                if (r < 0) {
+                    atomicOr(&s_flags[wv][sl], 1);
+                } else if (infer) {
+                    atomicMin(&s_min[wv][sl], C.prev_counts[r]);
+                    if (C.prev_free[r] == 0) atomicOr(&s_flags[wv][sl], 2);
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int sl = lane; sl < slots; sl += 64) {
+            const int f = s_flags[wv][sl];
+            if (f & 4) continue;
+            const int64_t g = list[base + wave * G + sl / P];
+            uint8_t st = ETM_CAND_COUNT;
+            if ((f & 1) && prune) st = ETM_CAND_SKIP;
+            else if (!(f & 1) && infer && (f & 2)) st = ETM_CAND_INFER;
+            const int64_t cand = cum_pairs[g] + sl % P;
             status[cand] = st;
-            counts[cand] = cnt;
+            counts[cand] = st == ETM_CAND_COUNT ? 0 : (st == ETM_CAND_INFER ? s_min[wv][sl] : -1);
             if (gid) gid[cand] = (int32_t)g;
             n_count += st == ETM_CAND_COUNT;
             n_infer += st == ETM_CAND_INFER;
@@ -714,16 +778,93 @@ __global__ __launch_bounds__(kOne) void k_groups_single(const int32_t* __restric
     }
 }
 
-// Survivor compaction of up to n_cap candidates in one workgroup, into outputs of capacity n_cap; the
-// survivor count goes to *d_out_n.
-__global__ __launch_bounds__(kOne) void k_compact_single(Survives f, WriteSurvivor w, int64_t n_cands,
+// Survivor compaction of up to n_cands (<= 32768) candidates in one workgroup, in lexicographic order; the
+// survivor count goes to *d_out_n. Passes of kOne x 4 candidates, lane-striped per wave as in stripe_pass. The
+// arguments are restrict-qualified and each pass issues all its loads (status and count, then group id, then
+// group bounds, then the two suffix items) before its first store, so the dependent loads of a lane's four
+// candidates overlap instead of queueing behind one another's stores.
+__global__ __launch_bounds__(kOne) void k_compact_single(const uint8_t* __restrict__ status,
+                                                         const int32_t* __restrict__ counts, int64_t min_count,
+                                                         const int32_t* __restrict__ gid,
+                                                         const int32_t* __restrict__ rows, int32_t m,
+                                                         const int64_t* __restrict__ group_start,
+                                                         const int64_t* __restrict__ cum_pairs, int64_t n_groups,
+                                                         int64_t n_cands, int32_t* __restrict__ out_rows,
+                                                         int32_t* __restrict__ out_counts,
+                                                         int64_t* __restrict__ out_cand,
                                                          int64_t* __restrict__ d_out_n) {
+    constexpr int S = 4;
     __shared__ int32_t lds[kOneWaves];
-    auto put = [&](int64_t c, int64_t v, int64_t p) {
-        if (p < n_cands) w(c, v, p);
-    };
-    const int64_t total = stripe_scan<8>(n_cands, f, put, lds);
-    if (threadIdx.x == 0) *d_out_n = total;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int64_t carry = 0;
+    for (int64_t base = 0; base < n_cands; base += (int64_t)kOne * S) {
+        const int64_t wbase = base + (int64_t)wv * 64 * S;
+        int64_t c[S];
+        int32_t v[S], cnt[S], ex[S];
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            c[j] = wbase + j * 64 + lane;
+            v[j] = 0;
+            cnt[j] = 0;
+            if (c[j] < n_cands) {
+                cnt[j] = counts[c[j]];
+                const bool skip = status != nullptr && status[c[j]] == ETM_CAND_SKIP;
+                v[j] = (!skip && (int64_t)cnt[j] >= min_count) ? 1 : 0;
+            }
+        }
+        int32_t run = 0;
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            const int32_t inc = etm::wave_scan_i32(v[j]);
+            ex[j] = run + inc - v[j];
+            run += __builtin_amdgcn_readlane(inc, 63);
+        }
+        if (lane == 0) lds[wv] = run;
+        __syncthreads();
+        const int32_t t = lane < kOneWaves ? lds[lane] : 0;
+        const int32_t tinc = etm::wave_scan_i32(t);
+        const int64_t off = carry + __shfl(tinc - t, wv, 64);
+        const int64_t total = __builtin_amdgcn_readlane(tinc, kOneWaves - 1);
+        // Loads of every surviving candidate of the pass, in stages.
+        int64_t g[S], gs[S], sz[S], cb[S], ii[S], jj[S];
+        int32_t a[S], b[S];
+#pragma unroll
+        for (int j = 0; j < S; ++j)
+            g[j] = v[j] ? (gid ? (int64_t)gid[c[j]] : (n_groups == 1 ? 0 : etm::upper_index(cum_pairs, n_groups, c[j])))
+                        : 0;
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            gs[j] = group_start[g[j]];
+            sz[j] = group_start[g[j] + 1] - gs[j];
+            cb[j] = cum_pairs[g[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            ii[j] = 0;
+            jj[j] = 1;
+            if (v[j]) etm::lex_pair_decode(c[j] - cb[j], sz[j], &ii[j], &jj[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            a[j] = v[j] ? rows[(gs[j] + ii[j]) * m + m - 1] : 0;
+            b[j] = v[j] ? rows[(gs[j] + jj[j]) * m + m - 1] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            const int64_t p = off + ex[j];
+            if (!v[j] || p >= n_cands) continue;
+            int32_t* o = out_rows + p * (m + 1);
+            const int32_t* first = rows + gs[j] * m;
+            for (int q = 0; q < m - 1; ++q) o[q] = first[q];
+            o[m - 1] = a[j];
+            o[m] = b[j];
+            out_counts[p] = cnt[j];
+            if (out_cand) out_cand[p] = c[j];
+        }
+        carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *d_out_n = carry;
 }
 
 // Rows whose flag is set, in order, in one workgroup (n read from *d_n, at most n_cap).
@@ -773,9 +914,8 @@ hipError_t launch_compact_single(const int32_t* rows, int32_t m, const int64_t* 
                                  int64_t n_groups, int64_t n_cands, const uint8_t* status, const int32_t* counts,
                                  const int32_t* gid, int64_t min_count, int32_t* out_rows, int32_t* out_counts,
                                  int64_t* out_cand, int64_t* d_out_n, hipStream_t s) {
-    Survives f{status, counts, min_count};
-    WriteSurvivor w{rows, m, group_start, cum_pairs, n_groups, counts, out_rows, out_counts, out_cand, gid};
-    k_compact_single<<<1, kOne, 0, s>>>(f, w, n_cands, d_out_n);
+    k_compact_single<<<1, kOne, 0, s>>>(status, counts, min_count, gid, rows, m, group_start, cum_pairs, n_groups,
+                                        n_cands, out_rows, out_counts, out_cand, d_out_n);
     return hipGetLastError();
 }
 
@@ -893,8 +1033,9 @@ extern "C" int etm_level_classify(const int32_t* d_rows, int64_t n, int32_t m, c
         const int64_t big = h_class_counts[4];
         Classifier C{d_rows, m, d_prev_counts, d_prev_free, d_table, table_size - 1, mode};
         if (small > 0) {
-            PackedClasses pc{{h_class_counts[0], h_class_counts[1], h_class_counts[2], h_class_counts[3]}};
-            const int64_t waves = packed_waves(pc.n);
+            PackedClasses pc{{h_class_counts[0], h_class_counts[1], h_class_counts[2], h_class_counts[3]},
+                             std::max(1, m - 1)};
+            const int64_t waves = packed_waves(pc);
             ETM_REQUIRE((waves + 3) / 4 <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_level_classify: grid too large");
             k_classify_packed<<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(C, d_group_start, d_cum_pairs, d_group_list,
                                                                           pc, d_status, d_counts, d_gid, d_tally);
