@@ -680,17 +680,12 @@ struct GroupSize {
 // totals. Three sweeps: rows (group starts), groups (pairs, tile-pairs and class counts, classes 0-3 packed
 // two 16-bit counts per int32: a class of groups of >= 2 rows holds at most 16384 of them), groups again (class
 // lists at their class's base). Groups of one row (no candidates) are in no class.
-__global__ __launch_bounds__(kOne) void k_groups_single(const int32_t* __restrict__ rows, const int64_t* d_n,
-                                                        int64_t n_cap, int32_t m, int32_t small_max,
-                                                        int64_t* __restrict__ group_start,
-                                                        int64_t* __restrict__ cum_pairs,
-                                                        int64_t* __restrict__ cum_tiles,
-                                                        int32_t* __restrict__ small_list,
-                                                        int64_t* __restrict__ totals) {
-    __shared__ int32_t lds[5 * kOneWaves];
-    __shared__ int64_t red[kOneWaves + 1];
+__device__ __forceinline__ void groups_single_body(const int32_t* __restrict__ rows, int64_t n, int32_t m,
+                                                   int32_t small_max, int64_t* __restrict__ group_start,
+                                                   int64_t* __restrict__ cum_pairs, int64_t* __restrict__ cum_tiles,
+                                                   int32_t* __restrict__ small_list, int64_t* __restrict__ totals,
+                                                   int32_t* lds, int64_t* red) {
     const int tid = threadIdx.x;
-    const int64_t n = d_n ? min(*d_n, n_cap) : n_cap;
     IsGroupStart is_start{rows, m};
     auto put_start = [&](int64_t r, int64_t v, int64_t prefix) {
         if (v && prefix < n) group_start[prefix] = r;
@@ -778,23 +773,34 @@ __global__ __launch_bounds__(kOne) void k_groups_single(const int32_t* __restric
     }
 }
 
+__global__ __launch_bounds__(kOne) void k_groups_single(const int32_t* __restrict__ rows, const int64_t* d_n,
+                                                        int64_t n_cap, int32_t m, int32_t small_max,
+                                                        int64_t* __restrict__ group_start,
+                                                        int64_t* __restrict__ cum_pairs,
+                                                        int64_t* __restrict__ cum_tiles,
+                                                        int32_t* __restrict__ small_list,
+                                                        int64_t* __restrict__ totals) {
+    __shared__ int32_t lds[5 * kOneWaves];
+    __shared__ int64_t red[kOneWaves + 1];
+    const int64_t n = d_n ? min(*d_n, n_cap) : n_cap;
+    groups_single_body(rows, n, m, small_max, group_start, cum_pairs, cum_tiles, small_list, totals, lds, red);
+}
+
 // Survivor compaction of up to n_cands (<= 32768) candidates in one workgroup, in lexicographic order; the
 // survivor count goes to *d_out_n. Passes of kOne x 4 candidates, lane-striped per wave as in stripe_pass. The
 // arguments are restrict-qualified and each pass issues all its loads (status and count, then group id, then
 // group bounds, then the two suffix items) before its first store, so the dependent loads of a lane's four
 // candidates overlap instead of queueing behind one another's stores.
-__global__ __launch_bounds__(kOne) void k_compact_single(const uint8_t* __restrict__ status,
-                                                         const int32_t* __restrict__ counts, int64_t min_count,
-                                                         const int32_t* __restrict__ gid,
-                                                         const int32_t* __restrict__ rows, int32_t m,
-                                                         const int64_t* __restrict__ group_start,
-                                                         const int64_t* __restrict__ cum_pairs, int64_t n_groups,
-                                                         int64_t n_cands, int32_t* __restrict__ out_rows,
-                                                         int32_t* __restrict__ out_counts,
-                                                         int64_t* __restrict__ out_cand,
-                                                         int64_t* __restrict__ d_out_n) {
+__device__ __forceinline__ int64_t compact_single_body(const uint8_t* __restrict__ status,
+                                                       const int32_t* __restrict__ counts, int64_t min_count,
+                                                       const int32_t* __restrict__ gid,
+                                                       const int32_t* __restrict__ rows, int32_t m,
+                                                       const int64_t* __restrict__ group_start,
+                                                       const int64_t* __restrict__ cum_pairs, int64_t n_groups,
+                                                       int64_t n_cands, int32_t* __restrict__ out_rows,
+                                                       int32_t* __restrict__ out_counts,
+                                                       int64_t* __restrict__ out_cand, int32_t* lds) {
     constexpr int S = 4;
-    __shared__ int32_t lds[kOneWaves];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int64_t carry = 0;
     for (int64_t base = 0; base < n_cands; base += (int64_t)kOne * S) {
@@ -864,7 +870,23 @@ __global__ __launch_bounds__(kOne) void k_compact_single(const uint8_t* __restri
         carry += total;
         __syncthreads();
     }
-    if (threadIdx.x == 0) *d_out_n = carry;
+    return carry;
+}
+
+__global__ __launch_bounds__(kOne) void k_compact_single(const uint8_t* __restrict__ status,
+                                                         const int32_t* __restrict__ counts, int64_t min_count,
+                                                         const int32_t* __restrict__ gid,
+                                                         const int32_t* __restrict__ rows, int32_t m,
+                                                         const int64_t* __restrict__ group_start,
+                                                         const int64_t* __restrict__ cum_pairs, int64_t n_groups,
+                                                         int64_t n_cands, int32_t* __restrict__ out_rows,
+                                                         int32_t* __restrict__ out_counts,
+                                                         int64_t* __restrict__ out_cand,
+                                                         int64_t* __restrict__ d_out_n) {
+    __shared__ int32_t lds[kOneWaves];
+    const int64_t n = compact_single_body(status, counts, min_count, gid, rows, m, group_start, cum_pairs, n_groups,
+                                          n_cands, out_rows, out_counts, out_cand, lds);
+    if (threadIdx.x == 0) *d_out_n = n;
 }
 
 // Rows whose flag is set, in order, in one workgroup (n read from *d_n, at most n_cap).

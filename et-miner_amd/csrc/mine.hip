@@ -506,6 +506,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         Level cur;
         cur.m = k;
         int64_t cap;
+        GroupsDev gn{};
         if (n_cands <= etm::detail::kSingleMax) {
             cap = n_cands;
             cur.rows = a.get<int32_t>(cap * k);
@@ -566,7 +567,6 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
             }
         }
         // The next level's groups, sized on the device when the level is small.
-        GroupsDev gn{};
         const bool early = cap <= etm::detail::kSingleMax && k + 1 <= max_len;
         if (early) {
             gn = alloc_groups(a, cap);

@@ -403,3 +403,78 @@ extern "C" int etm_count_k3_rows(const int64_t* d_row_ptr, const int32_t* d_pos,
     ETM_LAUNCHED("k_count_k3_rows_long");
     return ETM_OK;
 }
+
+// ── SON pass-1 union ────────────────────────────────────────────────────
+
+namespace {
+
+// Inserts a chunk's local level (n rows of k column ids, mapped to item ids through col_to_item, with their
+// local counts) into the union of one itemset length: an open-addressing table of indices into the union's
+// key rows (k int64 item ids each), summing the local counts (known) and the chunk's slack per distinct
+// itemset. A chunk's rows are distinct, so no two threads of one call insert the same key: a new key takes a
+// fresh index (an atomic counter), writes its row, publishes it, then claims an empty slot by CAS; a thread
+// that finds an occupied slot compares the published row.
+__global__ __launch_bounds__(256) void k_union_insert(const int32_t* __restrict__ rows,
+                                                       const int32_t* __restrict__ counts, int64_t n, int32_t k,
+                                                       const int64_t* __restrict__ col_to_item, int64_t slack,
+                                                       int32_t* table, int64_t mask, int64_t* keys, int64_t* known,
+                                                       int64_t* slack_sum, unsigned long long* n_unique,
+                                                       int64_t cap, int32_t* overflow) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t h = 0;
+        for (int t = 0; t < k; ++t) h += etm::item_hash((int32_t)(col_to_item[rows[r * k + t]] & 0x7fffffff)) * (t + 1);
+        int64_t slot = (int64_t)(h & (uint64_t)mask);
+        int64_t mine = -1;
+        for (;;) {
+            const int32_t cur = __hip_atomic_load(&table[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            if (cur >= 0) {
+                bool eq = true;
+                for (int t = 0; t < k && eq; ++t) eq = keys[(int64_t)cur * k + t] == col_to_item[rows[r * k + t]];
+                if (eq) {
+                    atomicAdd((unsigned long long*)&known[cur], (unsigned long long)counts[r]);
+                    atomicAdd((unsigned long long*)&slack_sum[cur], (unsigned long long)slack);
+                    break;
+                }
+                slot = (slot + 1) & mask;
+                continue;
+            }
+            if (mine < 0) {
+                mine = (int64_t)atomicAdd(n_unique, 1ull);
+                if (mine >= cap) {
+                    atomicOr(overflow, 1);
+                    break;
+                }
+                for (int t = 0; t < k; ++t) keys[mine * k + t] = col_to_item[rows[r * k + t]];
+                known[mine] = counts[r];
+                slack_sum[mine] = slack;
+                __threadfence();
+            }
+            if (atomicCAS(&table[slot], -1, (int32_t)mine) == -1) break;
+            // Another key took the slot meanwhile: look again at the same slot (now occupied).
+        }
+    }
+}
+
+int grid_u(int64_t n) {
+    int64_t g = (n + 255) / 256;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, 16384));
+}
+
+}  // namespace
+
+extern "C" int etm_union_insert(const int32_t* d_rows, const int32_t* d_counts, int64_t n, int32_t k,
+                                const int64_t* d_col_to_item, int64_t slack, int32_t* d_table, int64_t table_size,
+                                int64_t* d_keys, int64_t* d_known, int64_t* d_slack, int64_t* d_n_unique, int64_t cap,
+                                int32_t* d_overflow, void* stream) {
+    ETM_ARG(d_rows && d_counts && d_col_to_item && d_table && d_keys && d_known && d_slack && d_n_unique && d_overflow,
+            "etm_union_insert: null pointer");
+    ETM_ARG(k >= 1 && k <= 62, "etm_union_insert: 1 <= k <= 62 required, got %d", k);
+    ETM_ARG(table_size > 0 && (table_size & (table_size - 1)) == 0 && table_size >= 2 * cap,
+            "etm_union_insert: table_size must be a power of two >= 2 x capacity");
+    if (n == 0) return ETM_OK;
+    k_union_insert<<<grid_u(n), 256, 0, ETM_STREAM(stream)>>>(d_rows, d_counts, n, k, d_col_to_item, slack, d_table,
+                                                              table_size - 1, d_keys, d_known, d_slack,
+                                                              (unsigned long long*)d_n_unique, cap, d_overflow);
+    ETM_LAUNCHED("k_union_insert");
+    return ETM_OK;
+}

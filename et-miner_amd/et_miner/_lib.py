@@ -55,6 +55,7 @@ SIGNATURES: dict = {
     "etm_count_k2_rows": (c_int, [P, P, c_int64, c_int64, P, P]),
     "etm_k3_rows_prepare": (c_int, [P, c_int64, P, c_int64, P, c_int64, P, P, P, P]),
     "etm_count_k3_rows": (c_int, [P, P, c_int64, P, P, P, c_int64, P, P, P, P, P]),
+    "etm_union_insert": (c_int, [P, P, c_int64, c_int32, P, c_int64, P, c_int64, P, P, P, P, c_int64, P, P]),
     "etm_mine_bitvecs": (c_int, [P, c_int64, c_int64, c_int64, P, c_int32, P, P, P]),
     "etm_mine_csr": (c_int, [P, P, c_int, c_int64, c_int64, c_int64, P, P, P]),
     "etm_levels_count": (c_int, [P, P]),

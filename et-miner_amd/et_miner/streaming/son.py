@@ -1,8 +1,8 @@
 """SON two-pass streaming on one GPU (the reference's ``streaming/son.py:131-693``).
 
 Pass 1 mines every row chunk at ``local_support_factor * min_support`` on the GPU level loop; the union of
-the chunks' local levels sums, per itemset, the local counts (``known``) and the slack (local min_count - 1)
-of the chunks that emitted it. The bound ``known + total_slack - slack`` drops candidates that cannot reach
+the chunks' local levels, a device hash map per itemset length (``etm_union_insert``), sums per itemset the
+local counts (``known``) and the slack (local min_count - 1) of the chunks that emitted it. The bound ``known + total_slack - slack`` drops candidates that cannot reach
 the global min_count and is the exact count where no slack is left (``son.py:402-421``). Pass 2 counts the
 remaining candidates over every chunk with the batched itemset kernel, summing the chunks on the device.
 
@@ -109,16 +109,6 @@ def _local_levels(d_ptr, d_ids, n_rows: int, local_support: float, max_length, i
     return res.col_to_item, res.levels
 
 
-def _sum_rows(rows: np.ndarray, weights: list) -> tuple:
-    """Lexsorted distinct rows with every weight summed per row (reference ``cpu_miner.py:381-414``)."""
-    if len(rows) == 0:
-        return rows, [w[:0] for w in weights]
-    order = np.lexsort(rows.T[::-1])
-    r = rows[order]
-    starts = np.flatnonzero(np.r_[True, (r[1:] != r[:-1]).any(axis=1)])
-    return r[starts], [np.add.reduceat(w[order], starts) for w in weights]
-
-
 def apriori_streaming(
     transactions,
     min_support: float = 0.5,
@@ -193,37 +183,37 @@ def son_csr(indptr, ids, min_support, max_length, chunk_size, local_support_fact
                            "could be frequent")
     up = _Uploader(indptr, ids, bounds)
 
-    # Pass 1: local levels per chunk, as item-id rows with their counts and the chunk's slack.
+    # Pass 1: local levels per chunk (column-id rows, counts, the chunk's column -> item map).
     if session:
         session.start_phase("pass1_local_mining")
     pending: dict = {}
+    chunk_items: dict = {}
     for ci in range(len(bounds)):
         d_ptr, d_ids, n = up.take(ci)
         local = _local_levels(d_ptr, d_ids, n, local_s, max_length, use_generator_pruning)
         if local is not None:
             col_items, levels = local
+            chunk_items[ci] = dev.to_device(np.ascontiguousarray(col_items, dtype=np.int64))
             for rows, counts in levels:
                 if len(rows):
-                    pending.setdefault(rows.shape[1], []).append((col_items[rows], counts.astype(np.int64),
-                                                                  slacks[ci]))
+                    pending.setdefault(rows.shape[1], []).append((ci, rows, counts))
         if progress_callback:
-            progress_callback("pass1", ci, len(bounds), {"candidates": sum(len(p[0]) for v in pending.values()
+            progress_callback("pass1", ci, len(bounds), {"candidates": sum(len(p[1]) for v in pending.values()
                                                                            for p in v)})
     up.close()
     if not pending:
         return itemset_table([], np.zeros(0, np.int64), n_total, with_counts)
-    all_items = np.unique(np.concatenate([p[0].ravel() for v in pending.values() for p in v]))
+    # The union of the chunks' levels, per length, in a device hash map: the local counts (known) and the slack
+    # of the emitting chunks summed per itemset; the bound ``known + total_slack - slack`` drops what cannot
+    # reach min_count and is exact where no slack is left (reference son.py:402-421).
     total_slack = sum(slacks)
     bounded = {}
     for k, parts in sorted(pending.items()):
-        rows = np.searchsorted(all_items, np.concatenate([p[0] for p in parts])).astype(np.int32)
-        known = np.concatenate([p[1] for p in parts])
-        sl = np.concatenate([np.full(len(p[0]), p[2], dtype=np.int64) for p in parts])
-        rows, (known, sl) = _sum_rows(rows, [known, sl])
+        keys, known, sl = _device_union(parts, k, slacks, chunk_items)
         left = total_slack - sl
         keep = known + left >= min_count
         need = left[keep] > 0
-        bounded[k] = (rows[keep], np.where(need, 0, known[keep]).astype(np.int64), need)
+        bounded[k] = (keys[keep], np.where(need, 0, known[keep]).astype(np.int64), need)
     if session:
         session.end_phase(n_candidates=sum(len(v[0]) for v in bounded.values()))
 
@@ -233,7 +223,8 @@ def son_csr(indptr, ids, min_support, max_length, chunk_size, local_support_fact
         session.start_phase("pass2_global_counting")
     if cands:
         order = sorted(cands)
-        flat = np.concatenate([cands[k].ravel() for k in order]).astype(np.int32)
+        all_items = np.unique(np.concatenate([cands[k].ravel() for k in order]))
+        flat = np.searchsorted(all_items, np.concatenate([cands[k].ravel() for k in order])).astype(np.int32)
         sizes = np.concatenate([np.full(len(cands[k]), k, dtype=np.int64) for k in order])
         offsets = np.zeros(len(sizes) + 1, dtype=np.int64)
         np.cumsum(sizes, out=offsets[1:])
@@ -264,7 +255,34 @@ def son_csr(indptr, ids, min_support, max_length, chunk_size, local_support_fact
         rows, counts, _ = bounded[k]
         keep = counts >= min_count
         levels.append((rows[keep], counts[keep]))
-    return itemset_table(levels, all_items.astype(np.int64), n_total, with_counts)
+    return itemset_table(levels, None, n_total, with_counts)
+
+
+def _device_union(parts: list, k: int, slacks: list, chunk_items: dict) -> tuple:
+    """(item-id rows int64 (u, k), summed local counts, summed slack) of the distinct itemsets of length k
+    over the chunks' levels ``parts`` = [(chunk, column-id rows, counts)], through ``etm_union_insert``."""
+    from .. import _lib
+    from ..gpu import device as dev
+
+    torch = dev.torch()
+    cap = sum(len(p[1]) for p in parts)
+    table_size = 1 << max(1, int(2 * cap - 1).bit_length())
+    table = torch.full((table_size,), -1, dtype=torch.int32, device=dev.device())
+    keys = dev.empty(cap * k, "int64")
+    known = dev.zeros(cap, "int64")
+    slack = dev.zeros(cap, "int64")
+    n_unique = dev.zeros(1, "int64")
+    overflow = dev.zeros(1, "int32")
+    for ci, rows, counts in parts:
+        d_rows = dev.to_device(np.ascontiguousarray(rows, dtype=np.int32))
+        d_counts = dev.to_device(np.ascontiguousarray(counts, dtype=np.int32))
+        _lib.call("etm_union_insert", d_rows.data_ptr(), d_counts.data_ptr(), len(rows), k,
+                  chunk_items[ci].data_ptr(), int(slacks[ci]), table.data_ptr(), table_size, keys.data_ptr(),
+                  known.data_ptr(), slack.data_ptr(), n_unique.data_ptr(), cap, overflow.data_ptr(), dev.stream_ptr())
+    u = int(dev.to_host(n_unique)[0])
+    if int(dev.to_host(overflow)[0]) or u > cap:
+        raise RuntimeError(f"SON union of length {k}: {u} itemsets exceed the capacity {cap}")
+    return (dev.to_host(keys[: u * k]).reshape(u, k), dev.to_host(known[:u]), dev.to_host(slack[:u]))
 
 
 def _drop_counts(table):

@@ -218,6 +218,19 @@ int etm_count_k3_rows(const int64_t* d_row_ptr, const int32_t* d_pos, int64_t n_
                       const int32_t* d_rank, const int32_t* d_gof, int64_t n1, const int64_t* d_group_start,
                       const int64_t* d_cum_pairs, const uint8_t* d_status, int32_t* d_counts, void* stream);
 
+/* ---- SON pass-1 union --------------------------------------------------------------------------------- */
+
+/* Adds a chunk's local level (n rows of k column ids, lexsorted within rows, mapped to item ids by
+ * d_col_to_item, and their local counts) to the union of all chunks' levels of length k: d_table (table_size,
+ * a power of two >= 2 x cap, filled with -1 before the first chunk) indexes d_keys (cap x k item ids), whose
+ * d_known sums the local counts and d_slack the emitting chunks' slack (each chunk adds `slack`). *d_n_unique
+ * counts the distinct itemsets; more than cap sets *d_overflow. A chunk's rows must be distinct. Replaces the
+ * host-side union of the local lattices (streaming/son.py:366-421). */
+int etm_union_insert(const int32_t* d_rows, const int32_t* d_counts, int64_t n, int32_t k,
+                     const int64_t* d_col_to_item, int64_t slack, int32_t* d_table, int64_t table_size,
+                     int64_t* d_keys, int64_t* d_known, int64_t* d_slack, int64_t* d_n_unique, int64_t cap,
+                     int32_t* d_overflow, void* stream);
+
 /* ---- The native level loop --------------------------------------------------------------------------- */
 
 /* Options of a mining call. min_count is the absolute threshold (>= 1 is enforced); max_length 0 = none. */

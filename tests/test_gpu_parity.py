@@ -222,6 +222,23 @@ def test_son_matches_in_core():
         assert son == miner.mine(data.indptr, data.indices, w.min_support, w.max_length).as_dict()
 
 
+@pytest.mark.parametrize("infer", [False, True])
+def test_son_with_uneven_chunks(infer):
+    """Chunks drawn from two different item popularities: many itemsets are frequent in some chunks only, so
+    the union's bound leaves slack and pass 2 counts them over every chunk."""
+    from et_miner.core.result import as_dict
+    from et_miner.streaming.son import son_csr
+
+    rng = np.random.default_rng(21)
+    first = _rand_rows(rng, 6000, 40, 5)
+    perm = rng.permutation(40)
+    second = [sorted(int(perm[x]) for x in r) for r in _rand_rows(rng, 6000, 40, 5)]
+    indptr, ids = _csr(first + second)
+    want = miner.mine(indptr, ids, 0.02, None, use_generator_pruning=infer).as_dict()
+    son = as_dict(son_csr(indptr, ids, 0.02, None, 2500, use_generator_pruning=infer, with_counts=True))
+    assert son == want
+
+
 def test_bitvecs_route_is_read_only_and_exact():
     import torch
 
