@@ -24,6 +24,9 @@ stream each, footprint admission, K=2 kernel autotuned per shape bucket) after o
 jobs / makespan with every job queued at t=0; latency (completion - arrival) is measured by replaying Poisson
 arrivals at twice the plain form's measured jobs/s. Every job's itemsets and counts are checked identical in
 both forms.
+
+streaming: deep_sparse_20m (20,000,000 transactions over 5,000 ids, s = 0.002, max_length 16, generator pruning)
+mined from host arrays resident and by SON in 2,000,000-row chunks; wall seconds and identical results.
 """
 
 from __future__ import annotations
@@ -101,6 +104,8 @@ def main(argv=None) -> int:
                     help="native: the level loop inside libetm; python: the same kernels driven level by level")
     ap.add_argument("--served-jobs", type=int, default=512, help="jobs of the served stream (0: skip that leg)")
     ap.add_argument("--served-workers", type=int, default=8)
+    ap.add_argument("--no-streaming", action="store_true",
+                    help="skip the streaming leg (deep_sparse_20m resident vs SON in 2,000,000-row chunks)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(json.dumps({"error": f"--gpus {args.gpus}: this build runs on one GPU; multi-GPU counting "
@@ -132,6 +137,14 @@ def main(argv=None) -> int:
         served_jobs_data = jobs_from_specs(served_jobs(512)[:args.served_jobs], workers=min(16, os.cpu_count() or 1))
         print(f"[bench] generated {len(served_jobs_data)} served jobs, "
               f"{sum(j.n_rows for j in served_jobs_data):,} rows in {time.perf_counter() - t_gen:.1f}s",
+              file=sys.stderr, flush=True)
+
+    stream_data = None
+    if not args.no_streaming:
+        t_gen = time.perf_counter()
+        stream_w = workload("deep_sparse_20m")
+        stream_data = stream_w.generate()
+        print(f"[bench] generated {stream_w.name}: {stream_data.n_rows:,} rows in {time.perf_counter() - t_gen:.1f}s",
               file=sys.stderr, flush=True)
 
     torch.cuda.set_device(0)
@@ -221,6 +234,7 @@ def main(argv=None) -> int:
         print(f"[bench] cpu oracle {t_c:.2f}s, parity={parity}", file=sys.stderr, flush=True)
 
     served = _served_leg(served_jobs_data, args.served_workers) if served_jobs_data else None
+    streaming = _streaming_leg(stream_w, stream_data) if stream_data is not None else None
 
     line = {
         "metric": "itemset-candidate counts/s",
@@ -268,9 +282,43 @@ def main(argv=None) -> int:
         "cpu_baseline": cpu,
         "pcie_inclusive": pcie,
         "served": served,
+        "streaming": streaming,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def _streaming_leg(w, data):
+    """deep_sparse_20m (use_generator_pruning) mined resident and by SON in 2,000,000-row chunks, from host
+    arrays (the CSR crosses PCIe in both); wall seconds of one warm run each, results compared by digest."""
+    import numpy as np
+
+    from et_miner.core.apriori import mine_csr
+    from et_miner.serve.server import result_digest
+    from et_miner.streaming.son import son_csr
+
+    ids = data.indices.astype(np.int64)
+
+    def digest(tb):
+        flat = tb.column("itemset").combine_chunks()
+        return result_digest(np.asarray(flat.values), np.asarray(flat.offsets), np.asarray(tb.column("count")))
+
+    out = {"workload": f"{w.name}: {data.n_rows:,} rows, {len(ids):,} entries, s={w.min_support}, "
+                       f"max_length={w.max_length}, generator pruning", "chunk_rows": w.chunk_size}
+    for name, run in (("resident", lambda: mine_csr(data.indptr, ids, w.min_support, w.max_length,
+                                                    use_generator_pruning=True, with_counts=True)),
+                      ("son", lambda: son_csr(data.indptr, ids, w.min_support, w.max_length, w.chunk_size,
+                                              use_generator_pruning=True, with_counts=True))):
+        run()  # warm: per-thread arenas sized, code objects loaded
+        t = time.perf_counter()
+        tb = run()
+        out[name + "_s"] = time.perf_counter() - t
+        out[name + "_digest"] = digest(tb)
+        out["itemsets"] = tb.num_rows
+    out["identical_results"] = out.pop("resident_digest") == out.pop("son_digest")
+    print(f"[bench] streaming: resident {out['resident_s']:.2f}s, son {out['son_s']:.2f}s, "
+          f"identical={out['identical_results']}", file=sys.stderr, flush=True)
+    return out
 
 
 def _served_leg(jobs, workers):
