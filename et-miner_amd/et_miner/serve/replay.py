@@ -1,0 +1,50 @@
+"""Replays the served-jobs stream (``et_miner.synthetic.served_jobs``) on one GPU, plain or served.
+
+    python -m et_miner.serve.replay --mode plain --jobs 512
+    python -m et_miner.serve.replay --mode served --workers 8 --jobs 512
+
+Every job's dataset is generated on the host (before the GPU is touched) and made resident in HBM. The
+stream runs ``--passes`` times with every job queued at t=0 (the first pass warms the per-thread arenas); the
+last pass's summary (jobs/s, makespan, latency percentiles) is printed as one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--mode", choices=["plain", "served"], default="served")
+    ap.add_argument("--jobs", type=int, default=512)
+    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--passes", type=int, default=2)
+    args = ap.parse_args(argv)
+
+    from ..synthetic import served_jobs
+    from .server import Server, jobs_from_specs, run_plain, serve
+
+    jobs = jobs_from_specs(served_jobs(512)[: args.jobs], workers=min(16, os.cpu_count() or 1))
+    import torch
+
+    torch.cuda.set_device(0)
+    summary = None
+    if args.mode == "plain":
+        for _ in range(args.passes):
+            _, summary = run_plain(jobs, time_scale=0.0)
+    else:
+        srv = Server(workers=args.workers)
+        try:
+            for _ in range(args.passes):
+                _, summary = serve(jobs, server=srv, time_scale=0.0)
+        finally:
+            srv.close()
+    print(json.dumps({"mode": args.mode, "jobs": len(jobs), **summary}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
