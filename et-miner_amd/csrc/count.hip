@@ -346,12 +346,13 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
 
 // ── int8 MFMA Gram of a single-group level (the K=2 pair space) ───────────────────────────────────────────
 //
-// One workgroup per (128 x 128 tile-pair of columns, word chunk). Per 4-word step (256 transactions = 8 K-steps of
-// 32 rows) each thread unpacks one column's bits to 0/1 bytes in LDS (laid out [K-step][column][row half], 16
-// bytes per MFMA operand), then each of the 4 waves runs a 2 x 2 block of v_mfma_i32_32x32x32_i8 on its 64 x 64
-// quarter of the output. Operand lane l carries column l % 32 and rows 16 (l / 32) .. +15 of the K-step for both
-// A and B, so the product is the exact Gram sum whatever the instruction's K order within a lane. Counts
-// accumulate in int32 (exact for fewer than 2^31 transactions) and leave with one integer atomic per pair.
+// One workgroup per (128 x 128 tile-pair of columns, word chunk). Per 4-word step (256 transactions = 8 K-steps
+// of 32 rows) each thread stages one column's four raw words in LDS ([word][column], 8 KB per side), then each of
+// the 4 waves runs a 2 x 2 block of v_mfma_i32_32x32x32_i8 on its 64 x 64 quarter of the output, expanding its
+// operands from the raw bits in registers: operand lane l carries column l % 32 and rows 16 (l / 32) .. +15 of
+// the K-step (one 16-bit LDS read, four 4-bit-to-4-byte spreads) for both A and B, so the product is the exact
+// Gram sum whatever the instruction's K order within a lane. LDS carries the bits once, not the 8x larger bytes.
+// Counts accumulate in int32 (exact for fewer than 2^31 transactions) and leave with one integer atomic per pair.
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
@@ -363,12 +364,22 @@ __device__ __forceinline__ int spread4(uint32_t x, int q) {
     return (int)((__umul24((x >> (4 * q)) & 0xFu, 0x00204081u)) & 0x01010101u);
 }
 
+// The 16 bits of x as sixteen 0/1 bytes (bit t -> byte t).
+__device__ __forceinline__ v4i spread16(uint32_t x) {
+    v4i v;
+    v[0] = spread4(x, 0);
+    v[1] = spread4(x, 1);
+    v[2] = spread4(x, 2);
+    v[3] = spread4(x, 3);
+    return v;
+}
+
 __global__ __launch_bounds__(256) void k_gram_mfma(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
                                                     int64_t chunk_words, const int32_t* __restrict__ cols, int64_t s,
                                                     int64_t n_tp, int64_t n_chunks, int32_t* __restrict__ counts,
                                                     int32_t layout) {
-    __shared__ v4i ldsA[2 * kMW][kMT][2];
-    __shared__ v4i ldsB[2 * kMW][kMT][2];
+    __shared__ uint64_t rawA[2][kMW][kMT];
+    __shared__ uint64_t rawB[2][kMW][kMT];
     __shared__ int64_t colA[kMT], colB[kMT];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int64_t chunk_id, tp;
@@ -398,48 +409,43 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const uint64_t* __restrict__ 
     const bool side_b = tid >= kMT;
     __syncthreads();
     const int64_t base = side_b ? colB[c] : colA[c];
-    // Register double buffering: the next step's words are in flight while this step's MFMAs run.
+    // This lane's operand columns and 16-bit offset within a word's K-step half.
+    const int ca0 = wm * 64 + (lane & 31), ca1 = ca0 + 32;
+    const int cb0 = wn * 64 + (lane & 31), cb1 = cb0 + 32;
+    const int sub = 16 * (lane >> 5);
+    // Register double buffering: the next step's words are in flight while this step's MFMAs run; LDS is double
+    // buffered, so one barrier per step separates the staging of step t+1 from the reads of step t.
     uint64_t next[kMW];
 #pragma unroll
     for (int w = 0; w < kMW; ++w) next[w] = (base >= 0 && w_begin + w < w_end) ? bv[base + w_begin + w] : 0ull;
-    for (int64_t wb = w_begin; wb < w_end; wb += kMW) {
-        uint64_t words[kMW];
-#pragma unroll
-        for (int w = 0; w < kMW; ++w) words[w] = next[w];
+    int buf = 0;
+    for (int64_t wb = w_begin; wb < w_end; wb += kMW, buf ^= 1) {
 #pragma unroll
         for (int w = 0; w < kMW; ++w) {
-#pragma unroll
-            for (int hw = 0; hw < 2; ++hw) {
-                const uint32_t b32 = (uint32_t)(words[w] >> (32 * hw));
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const uint32_t b16 = b32 >> (16 * h);
-                    v4i v;
-                    v[0] = spread4(b16, 0);
-                    v[1] = spread4(b16, 1);
-                    v[2] = spread4(b16, 2);
-                    v[3] = spread4(b16, 3);
-                    if (side_b) ldsB[2 * w + hw][c][h] = v;
-                    else ldsA[2 * w + hw][c][h] = v;
-                }
-            }
+            if (side_b) rawB[buf][w][c] = next[w];
+            else rawA[buf][w][c] = next[w];
         }
         __syncthreads();
         const int64_t nb = wb + kMW;
 #pragma unroll
         for (int w = 0; w < kMW; ++w) next[w] = (base >= 0 && nb + w < w_end) ? bv[base + nb + w] : 0ull;
 #pragma unroll
-        for (int q = 0; q < 2 * kMW; ++q) {
-            v4i a0 = ldsA[q][wm * 64 + (lane & 31)][lane >> 5];
-            v4i a1 = ldsA[q][wm * 64 + 32 + (lane & 31)][lane >> 5];
-            v4i b0 = ldsB[q][wn * 64 + (lane & 31)][lane >> 5];
-            v4i b1 = ldsB[q][wn * 64 + 32 + (lane & 31)][lane >> 5];
-            acc[0][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, acc[1][1], 0, 0, 0);
+        for (int w = 0; w < kMW; ++w) {
+            const uint64_t xa0 = rawA[buf][w][ca0], xa1 = rawA[buf][w][ca1];
+            const uint64_t xb0 = rawB[buf][w][cb0], xb1 = rawB[buf][w][cb1];
+#pragma unroll
+            for (int hw = 0; hw < 2; ++hw) {
+                const int sh = 32 * hw + sub;
+                const v4i a0 = spread16((uint32_t)(xa0 >> sh) & 0xFFFFu);
+                const v4i a1 = spread16((uint32_t)(xa1 >> sh) & 0xFFFFu);
+                const v4i b0 = spread16((uint32_t)(xb0 >> sh) & 0xFFFFu);
+                const v4i b1 = spread16((uint32_t)(xb1 >> sh) & 0xFFFFu);
+                acc[0][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, acc[0][0], 0, 0, 0);
+                acc[0][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, acc[0][1], 0, 0, 0);
+                acc[1][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, acc[1][0], 0, 0, 0);
+                acc[1][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, acc[1][1], 0, 0, 0);
+            }
         }
-        __syncthreads();
     }
     // acc[mi][ni][r]: row 8 (r / 4) + 4 (lane / 32) + r % 4 of the A block, column lane % 32 of the B block.
 #pragma unroll
