@@ -374,88 +374,111 @@ __device__ __forceinline__ v4i spread16(uint32_t x) {
     return v;
 }
 
+// BM x BM blocks of 32 x 32 MFMA tiles per wave; a workgroup of 4 waves covers a (64 BM) x (64 BM) tile-pair
+// (BM = 2: 128-column tiles, 8 KB of LDS per step).
+template <int BM>
 __global__ __launch_bounds__(256) void k_gram_mfma(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
                                                     int64_t chunk_words, const int32_t* __restrict__ cols, int64_t s,
                                                     int64_t n_tp, int64_t n_chunks, int32_t* __restrict__ counts,
                                                     int32_t layout) {
-    __shared__ uint64_t rawA[2][kMW][kMT];
-    __shared__ uint64_t rawB[2][kMW][kMT];
-    __shared__ int64_t colA[kMT], colB[kMT];
+    constexpr int T = 64 * BM;  // columns per side of the tile-pair
+    __shared__ uint64_t rawA[2][kMW][T];
+    __shared__ uint64_t rawB[2][kMW][T];
+    __shared__ int64_t colA[T], colB[T];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int64_t chunk_id, tp;
     if (!xcd_unit(blockIdx.x % kXcds, blockIdx.x / kXcds, n_tp, n_chunks, &chunk_id, &tp)) return;
-    const int64_t nt = (s + kMT - 1) / kMT;
+    const int64_t nt = (s + T - 1) / T;
     int64_t ta, tb1;
     etm::lex_pair_decode(tp, nt + 1, &ta, &tb1);
     const int64_t tb = tb1 - 1;
-    if (tid < kMT) {
-        const int64_t i = ta * kMT + tid;
-        colA[tid] = i < s ? (int64_t)cols[i] * pitch : -1;
-    } else {
-        const int64_t j = tb * kMT + (tid - kMT);
-        colB[tid - kMT] = j < s ? (int64_t)cols[j] * pitch : -1;
+    for (int t = tid; t < 2 * T; t += 256) {
+        if (t < T) {
+            const int64_t i = ta * T + t;
+            colA[t] = i < s ? (int64_t)cols[i] * pitch : -1;
+        } else {
+            const int64_t j = tb * T + (t - T);
+            colB[t - T] = j < s ? (int64_t)cols[j] * pitch : -1;
+        }
     }
     const int wm = wave >> 1, wn = wave & 1;
-    v16i acc[2][2];
+    v16i acc[BM][BM];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < BM; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < BM; ++b)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
     const int64_t w_begin = chunk_id * chunk_words;
     const int64_t w_end = min(n_words, w_begin + chunk_words);
-    const int c = tid % kMT;
-    const bool side_b = tid >= kMT;
     __syncthreads();
-    const int64_t base = side_b ? colB[c] : colA[c];
-    // This lane's operand columns and 16-bit offset within a word's K-step half.
-    const int ca0 = wm * 64 + (lane & 31), ca1 = ca0 + 32;
-    const int cb0 = wn * 64 + (lane & 31), cb1 = cb0 + 32;
-    const int sub = 16 * (lane >> 5);
-    // Register double buffering: the next step's words are in flight while this step's MFMAs run; LDS is double
-    // buffered, so one barrier per step separates the staging of step t+1 from the reads of step t.
-    uint64_t next[kMW];
+    // Staging: thread t holds the columns t, t + 256, ... of the concatenated sides A | B.
+    constexpr int PER = 2 * T / 256;  // columns staged per thread per step
+    int64_t base[PER];
 #pragma unroll
-    for (int w = 0; w < kMW; ++w) next[w] = (base >= 0 && w_begin + w < w_end) ? bv[base + w_begin + w] : 0ull;
+    for (int q = 0; q < PER; ++q) {
+        const int t = tid + 256 * q;
+        base[q] = t < T ? colA[t] : colB[t - T];
+    }
+    const int sub = 16 * (lane >> 5);
+    uint64_t next[PER][kMW];
+#pragma unroll
+    for (int q = 0; q < PER; ++q)
+#pragma unroll
+        for (int w = 0; w < kMW; ++w)
+            next[q][w] = (base[q] >= 0 && w_begin + w < w_end) ? bv[base[q] + w_begin + w] : 0ull;
     int buf = 0;
     for (int64_t wb = w_begin; wb < w_end; wb += kMW, buf ^= 1) {
 #pragma unroll
-        for (int w = 0; w < kMW; ++w) {
-            if (side_b) rawB[buf][w][c] = next[w];
-            else rawA[buf][w][c] = next[w];
+        for (int q = 0; q < PER; ++q) {
+            const int t = tid + 256 * q;
+#pragma unroll
+            for (int w = 0; w < kMW; ++w) {
+                if (t < T) rawA[buf][w][t] = next[q][w];
+                else rawB[buf][w][t - T] = next[q][w];
+            }
         }
         __syncthreads();
         const int64_t nb = wb + kMW;
 #pragma unroll
-        for (int w = 0; w < kMW; ++w) next[w] = (base >= 0 && nb + w < w_end) ? bv[base + nb + w] : 0ull;
+        for (int q = 0; q < PER; ++q)
+#pragma unroll
+            for (int w = 0; w < kMW; ++w)
+                next[q][w] = (base[q] >= 0 && nb + w < w_end) ? bv[base[q] + nb + w] : 0ull;
 #pragma unroll
         for (int w = 0; w < kMW; ++w) {
-            const uint64_t xa0 = rawA[buf][w][ca0], xa1 = rawA[buf][w][ca1];
-            const uint64_t xb0 = rawB[buf][w][cb0], xb1 = rawB[buf][w][cb1];
+            uint64_t xa[BM], xb[BM];
+#pragma unroll
+            for (int t = 0; t < BM; ++t) {
+                xa[t] = rawA[buf][w][wm * 32 * BM + 32 * t + (lane & 31)];
+                xb[t] = rawB[buf][w][wn * 32 * BM + 32 * t + (lane & 31)];
+            }
 #pragma unroll
             for (int hw = 0; hw < 2; ++hw) {
                 const int sh = 32 * hw + sub;
-                const v4i a0 = spread16((uint32_t)(xa0 >> sh) & 0xFFFFu);
-                const v4i a1 = spread16((uint32_t)(xa1 >> sh) & 0xFFFFu);
-                const v4i b0 = spread16((uint32_t)(xb0 >> sh) & 0xFFFFu);
-                const v4i b1 = spread16((uint32_t)(xb1 >> sh) & 0xFFFFu);
-                acc[0][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, acc[0][0], 0, 0, 0);
-                acc[0][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, acc[0][1], 0, 0, 0);
-                acc[1][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, acc[1][0], 0, 0, 0);
-                acc[1][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, acc[1][1], 0, 0, 0);
+                v4i a[BM], b[BM];
+#pragma unroll
+                for (int t = 0; t < BM; ++t) {
+                    a[t] = spread16((uint32_t)(xa[t] >> sh) & 0xFFFFu);
+                    b[t] = spread16((uint32_t)(xb[t] >> sh) & 0xFFFFu);
+                }
+#pragma unroll
+                for (int mi = 0; mi < BM; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < BM; ++ni)
+                        acc[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[mi], b[ni], acc[mi][ni], 0, 0, 0);
             }
         }
     }
     // acc[mi][ni][r]: row 8 (r / 4) + 4 (lane / 32) + r % 4 of the A block, column lane % 32 of the B block.
 #pragma unroll
-    for (int mi = 0; mi < 2; ++mi) {
+    for (int mi = 0; mi < BM; ++mi) {
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni) {
+        for (int ni = 0; ni < BM; ++ni) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int64_t i = ta * kMT + wm * 64 + mi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                const int64_t j = tb * kMT + wn * 64 + ni * 32 + (lane & 31);
+                const int64_t i = ta * T + wm * 32 * BM + mi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int64_t j = tb * T + wn * 32 * BM + ni * 32 + (lane & 31);
                 if (i < j && j < s && acc[mi][ni][r]) atomicAdd(&counts[pair_slot(i, j, s, layout)], acc[mi][ni][r]);
             }
         }
@@ -567,6 +590,8 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
         ETM_HIP(hipMemcpyAsync(&s_cols, d_group_start + 1, sizeof s_cols, hipMemcpyDeviceToHost, s));
         ETM_HIP(hipStreamSynchronize(s));
         if (kernel == ETM_KERNEL_MFMA || etm::detail::gram_preferred(s_cols, n_words)) {
+            // 2 x 2 blocks per wave (128-column tiles): measured faster than 4 x 4 blocks (256-column tiles, one
+            // wave per SIMD), which waste more edge pairs and hide load latency worse.
             const int64_t nt = ceil_div(s_cols, kMT);
             const int64_t n_tp = nt * (nt + 1) / 2;
             const int64_t chunk = word_chunk > 0 ? ceil_div(word_chunk, kMW) * kMW
@@ -575,8 +600,8 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
             const int64_t blocks = kXcds * xcd_units(n_tp, n_chunks);
             ETM_REQUIRE(blocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",
                         (long long)blocks);
-            k_gram_mfma<<<(unsigned)blocks, 256, 0, s>>>(d_bv, pitch, n_words, chunk, d_rows, s_cols, n_tp, n_chunks,
-                                                         d_counts, layout);
+            k_gram_mfma<2><<<(unsigned)blocks, 256, 0, s>>>(d_bv, pitch, n_words, chunk, d_rows, s_cols, n_tp, n_chunks,
+                                                            d_counts, layout);
             ETM_LAUNCHED("k_gram_mfma");
             return ETM_OK;
         }
