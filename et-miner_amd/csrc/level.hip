@@ -194,9 +194,10 @@ struct Query {
     }
 };
 
-// Row id of the level row equal to q (q.size() == m), or -1.
+// Row id of the level row equal to q (q.item(0..m)), or -1.
+template <class Q>
 __device__ int64_t hash_find(const int32_t* __restrict__ rows, int32_t m, const int32_t* __restrict__ table,
-                             int64_t mask, uint64_t h, const Query& q) {
+                             int64_t mask, uint64_t h, const Q& q) {
     int64_t slot = (int64_t)(h & (uint64_t)mask);
     for (;;) {
         const int32_t r = table[slot];
@@ -496,6 +497,181 @@ __global__ __launch_bounds__(256) void k_classify_big(Classifier C, const int64_
         if (threadIdx.x == 0) {
             if (n_count) atomicAdd((unsigned long long*)&tally[0], (unsigned long long)n_count);
             if (n_infer) atomicAdd((unsigned long long*)&tally[1], (unsigned long long)n_infer);
+            if (n_skip) atomicAdd((unsigned long long*)&tally[2], (unsigned long long)n_skip);
+        }
+    }
+}
+
+// ── deduction bounds ────────────────────────────────────────────────────
+//
+// For a candidate X of k items whose (k-1)-subsets are all frequent and free, with s(Y) the support of Y:
+//   upper u = min( min_a s(X-a), min_{a<b<c} [s(X-a)+s(X-b)+s(X-c) - s(X-ab)-s(X-ac)-s(X-bc) + s(X-abc)] )
+//   lower l = max( 0, max_{a<b} [s(X-a) + s(X-b) - s(X-ab)] )
+// (inclusion-exclusion over the rows missing a, b, c). u < min_count proves X infrequent; l = u gives its count.
+
+constexpr int kBoundsMaxK = 24;
+
+// Candidate x[0..k) without positions d0 < d1 < d2 (an unused position is kNoPos). Item t of it:
+struct DropQuery {
+    static constexpr int kNoPos = 1 << 30;
+    const int32_t* x;
+    int d0, d1, d2;
+    __device__ int32_t item(int t) const {
+        t += t >= d0;
+        t += t >= d1;
+        t += t >= d2;
+        return x[t];
+    }
+};
+
+// A level the bounds read: rows (n, m), counts, and its hash.
+struct DeepLevel {
+    const int32_t* rows;
+    const int32_t* counts;
+    const int32_t* table;
+    int64_t mask;
+};
+
+// Positions (i < j) of pair p in colex order p = j(j-1)/2 + i.
+__device__ __forceinline__ void colex_pair(int p, int* i, int* j) {
+    int b = 1;
+    while ((b + 1) * b / 2 <= p) ++b;
+    *j = b;
+    *i = p - b * (b - 1) / 2;
+}
+
+// Positions (i < j < l) of triple q in colex order q = C(l,3) + C(j,2) + i.
+__device__ __forceinline__ void colex_triple(int q, int* i, int* j, int* l) {
+    int c = 2;
+    while ((c + 1) * c * (c - 1) / 6 <= q) ++c;
+    *l = c;
+    colex_pair(q - c * (c - 1) * (c - 2) / 6, i, j);
+}
+
+__device__ __forceinline__ int64_t sub_max(int64_t v, int L) {
+    for (int off = L >> 1; off > 0; off >>= 1) v = max(v, (int64_t)__shfl_xor((long long)v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ int64_t sub_min(int64_t v, int L) {
+    for (int off = L >> 1; off > 0; off >>= 1) v = min(v, (int64_t)__shfl_xor((long long)v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A sub-wave of L lanes (a power of two <= 64) per candidate still ETM_CAND_COUNT: lanes over the k (k-1)-
+// subsets, then the C(k,2) pair drops (probes of level k-2), then the C(k,3) triple drops (probes of level k-3;
+// the empty set's count n_total at k = 3). Every lane of a wave runs the same iterations (the reductions are
+// wave-wide shuffles); each sub-wave keeps its candidate's items, their hashes, s(X-a) and s(X-ab) in LDS.
+__global__ __launch_bounds__(256) void k_deduce(Classifier C, DeepLevel d2, DeepLevel d3, int64_t n_total,
+                                                 int64_t min_count, const int64_t* __restrict__ group_start,
+                                                 const int64_t* __restrict__ cum_pairs, int64_t n_groups,
+                                                 const int32_t* __restrict__ gid, int64_t n_cands, int L,
+                                                 int stride8, uint8_t* __restrict__ status,
+                                                 int32_t* __restrict__ counts, int64_t* __restrict__ tally) {
+    extern __shared__ uint64_t dyn_lds[];
+    __shared__ int64_t lds[5];
+    const int k = C.m + 1, m = C.m;
+    const int P2 = k * (k - 1) / 2, P3 = k * (k - 1) * (k - 2) / 6;
+    const int per_block = blockDim.x / L;
+    const int sub = threadIdx.x / L, sl = threadIdx.x % L;
+    uint64_t* hx = dyn_lds + (int64_t)sub * stride8;
+    int32_t* x = (int32_t*)(hx + k);
+    int32_t* c1 = x + k;
+    int32_t* s2 = c1 + k;
+    int64_t n_inf = 0, n_skip = 0;
+    const int64_t step = (int64_t)gridDim.x * per_block;
+    for (int64_t base = (int64_t)blockIdx.x * per_block; base < n_cands; base += step) {
+        const int64_t c = base + sub;
+        const bool act = c < n_cands && status[c] == ETM_CAND_COUNT;
+        int64_t gs = 0, i = 0, j = 0;
+        if (act) {
+            const int64_t g = gid ? (int64_t)gid[c] : etm::upper_index(cum_pairs, n_groups, c);
+            gs = group_start[g];
+            etm::lex_pair_decode(c - cum_pairs[g], group_start[g + 1] - gs, &i, &j);
+            for (int t = sl; t < k; t += L) {
+                const int32_t v = t < m - 1 ? C.rows[gs * m + t] : C.rows[(gs + (t == m - 1 ? i : j)) * m + m - 1];
+                x[t] = v;
+                hx[t] = item_hash(v);
+            }
+        }
+        wave_lds_sync();
+        uint64_t h = 0;
+        int64_t miss = 0;
+        if (act) {
+            for (int t = 0; t < k; ++t) h += hx[t];
+            // s(X - x[t]): dropping a or b is a parent row of the group; a prefix item is a probe.
+            for (int t = sl; t < k; t += L) {
+                int64_t r;
+                if (t == k - 2) r = gs + j;
+                else if (t == k - 1) r = gs + i;
+                else r = hash_find(C.rows, m, C.table, C.mask, h - hx[t], DropQuery{x, t, DropQuery::kNoPos,
+                                                                                       DropQuery::kNoPos});
+                if (r < 0) miss = 1;
+                c1[t] = r < 0 ? 0 : C.prev_counts[r];
+            }
+        }
+        wave_lds_sync();
+        int64_t lower = 0;
+        if (act) {
+            for (int p = sl; p < P2; p += L) {
+                int a, b;
+                colex_pair(p, &a, &b);
+                const int64_t r = hash_find(d2.rows, m - 1, d2.table, d2.mask, h - hx[a] - hx[b],
+                                            DropQuery{x, a, b, DropQuery::kNoPos});
+                int32_t v = 0;
+                if (r < 0) miss = 1;
+                else v = d2.counts[r];
+                s2[p] = v;
+                lower = max(lower, (int64_t)c1[a] + c1[b] - v);
+            }
+        }
+        wave_lds_sync();
+        int64_t upper = INT64_MAX;
+        if (act) {
+            for (int t = 0; t < k; ++t) upper = min(upper, (int64_t)c1[t]);
+            for (int q = sl; q < P3; q += L) {
+                int a, b, e;
+                colex_triple(q, &a, &b, &e);
+                int64_t v3 = n_total;
+                if (k > 3) {
+                    const int64_t r = hash_find(d3.rows, m - 2, d3.table, d3.mask, h - hx[a] - hx[b] - hx[e],
+                                                DropQuery{x, a, b, e});
+                    if (r < 0) miss = 1;
+                    else v3 = d3.counts[r];
+                }
+                const int64_t u = (int64_t)c1[a] + c1[b] + c1[e] - s2[b * (b - 1) / 2 + a] -
+                                  s2[e * (e - 1) / 2 + a] - s2[e * (e - 1) / 2 + b] + v3;
+                upper = min(upper, u);
+            }
+        }
+        lower = sub_max(lower, L);
+        upper = sub_min(upper, L);
+        miss = sub_max(miss, L);
+        if (act && sl == 0 && !miss) {
+            if (upper < min_count) {
+                status[c] = ETM_CAND_SKIP;
+                counts[c] = -1;
+                ++n_skip;
+            } else if (lower == upper) {
+                status[c] = ETM_CAND_INFER;
+                counts[c] = (int32_t)upper;
+                ++n_inf;
+            }
+        }
+        wave_lds_sync();  // the next candidate's items overwrite this one's
+    }
+    if (tally) {
+        n_inf = etm::block_sum<256>(n_inf, lds);
+        n_skip = etm::block_sum<256>(n_skip, lds);
+        if (threadIdx.x == 0 && (n_inf || n_skip)) {
+            atomicAdd((unsigned long long*)&tally[0], (unsigned long long)(-(n_inf + n_skip)));
+            if (n_inf) atomicAdd((unsigned long long*)&tally[1], (unsigned long long)n_inf);
             if (n_skip) atomicAdd((unsigned long long*)&tally[2], (unsigned long long)n_skip);
         }
     }
@@ -1080,6 +1256,40 @@ extern "C" int etm_level_classify(const int32_t* d_rows, int64_t n, int32_t m, c
                                                         table_size - 1, d_group_start, d_cum_pairs, n_groups,
                                                         n_cands, mode, d_status, d_counts, d_tally);
     ETM_LAUNCHED("k_classify");
+    return ETM_OK;
+}
+
+extern "C" int etm_level_deduce(const int32_t* d_rows, int32_t m, const int32_t* d_prev_counts, const int32_t* d_table,
+                                int64_t table_size, const int32_t* d_rows2, const int32_t* d_counts2,
+                                const int32_t* d_table2, int64_t table_size2, const int32_t* d_rows3,
+                                const int32_t* d_counts3, const int32_t* d_table3, int64_t table_size3,
+                                int64_t n_total, int64_t min_count, const int64_t* d_group_start,
+                                const int64_t* d_cum_pairs, int64_t n_groups, int64_t n_cands, const int32_t* d_gid,
+                                uint8_t* d_status, int32_t* d_counts, int64_t* d_tally, void* stream) {
+    const int k = m + 1;
+    ETM_ARG(k >= 3 && k <= kBoundsMaxK, "etm_level_deduce: candidates of 3 to %d items, got %d", kBoundsMaxK, k);
+    ETM_ARG(d_rows && d_prev_counts && d_table && d_rows2 && d_counts2 && d_table2 && d_group_start &&
+                d_cum_pairs && d_status && d_counts,
+            "etm_level_deduce: null pointer");
+    ETM_ARG(k == 3 || (d_rows3 && d_counts3 && d_table3), "etm_level_deduce: k > 3 needs level k-3");
+    ETM_ARG(is_pow2(table_size) && is_pow2(table_size2) && (k == 3 || is_pow2(table_size3)),
+            "etm_level_deduce: table sizes must be powers of two");
+    ETM_ARG(n_groups >= 1 || n_cands == 0, "etm_level_deduce: no groups");
+    if (n_cands == 0) return ETM_OK;
+    hipStream_t s = ETM_STREAM(stream);
+    const int need = std::max(k, std::max(k * (k - 1) / 2, k * (k - 1) * (k - 2) / 6));
+    int L = 1;
+    while (L < need && L < 64) L <<= 1;
+    const int per_block = 256 / L;
+    const int stride8 = k + (2 * k + k * (k - 1) / 2 + 1) / 2;
+    const size_t dyn = (size_t)per_block * stride8 * 8;
+    Classifier C{d_rows, m, d_prev_counts, nullptr, d_table, table_size - 1, ETM_SUBSET_INFER};
+    DeepLevel d2{d_rows2, d_counts2, d_table2, table_size2 - 1};
+    DeepLevel d3{d_rows3, d_counts3, d_table3, k == 3 ? 0 : table_size3 - 1};
+    const int64_t blocks = std::min<int64_t>((n_cands + per_block - 1) / per_block, 8192);
+    k_deduce<<<(unsigned)blocks, 256, dyn, s>>>(C, d2, d3, n_total, min_count, d_group_start, d_cum_pairs, n_groups,
+                                                d_gid, n_cands, L, stride8, d_status, d_counts, d_tally);
+    ETM_LAUNCHED("k_deduce");
     return ETM_OK;
 }
 

@@ -347,7 +347,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
     Arena a(s);
     Pinned host;
     const int64_t n1 = (int64_t)k1_rows.size();
-    Level prev;
+    Level prev, prev2, prev3;  // levels k-1, k-2, k-3 of the level being mined
     prev.m = 1;
     prev.n = n1;
     prev.rows = a.get<int32_t>(n1);
@@ -446,6 +446,14 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
                                         prev.table_size, g.gs, g.cp, n_groups, n_cands, g.list, classes, mode,
                                         status, counts, gid, cur_scal + 1, s),
                      "etm_level_classify");
+            // The deduction bounds need levels k-2 and k-3 complete (the inference mode keeps every level whole).
+            if (infer && o.deduction != ETM_DEDUCTION_OFF && k <= 24) {
+                check_rc(etm_level_deduce(prev.rows, m, prev.counts, prev.table, prev.table_size, prev2.rows,
+                                          prev2.counts, prev2.table, prev2.table_size, prev3.rows, prev3.counts,
+                                          prev3.table, prev3.table_size, n_rows, mc, g.gs, g.cp, n_groups, n_cands,
+                                          gid, status, counts, cur_scal + 1, s),
+                         "etm_level_deduce");
+            }
         } else {
             check(hipMemsetAsync(counts, 0, n_cands * 4, s), "zero counts");
         }
@@ -594,6 +602,8 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
             wait_stream(s);
             std::memcpy(tot, host.p + 8, sizeof tot);
         }
+        prev3 = prev2;
+        prev2 = prev;
         prev = cur;
     }
     // Every level back to the host in one pass, through one pinned staging buffer.

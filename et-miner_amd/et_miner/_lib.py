@@ -11,16 +11,29 @@ raises.
 from __future__ import annotations
 
 import ctypes
+import os
 import threading
 from ctypes import POINTER, c_char_p, c_int, c_int32, c_int64, c_size_t, c_void_p
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libetm.so"
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 ETM_OK, ETM_EINVAL, ETM_ENOMEM, ETM_EHIP, ETM_EOVERFLOW, ETM_ENODEV = 0, 1, 2, 3, 4, 5
 CAND_COUNT, CAND_SKIP, CAND_INFER = 0, 1, 2
 SUBSET_PRUNE, SUBSET_INFER = 1, 2
+DEDUCTION_AUTO, DEDUCTION_OFF = 0, 1
+#: Largest candidate the deduction bounds settle (etm_level_deduce).
+BOUNDS_MAX_K = 24
+
+
+def deduction_setting() -> int:
+    """DEDUCTION_AUTO unless ``ET_MINER_DEDUCTION`` is "0" (the reference's switch, ``_env.py:285-291``): the
+    bounds run in the inference mode. Any value other than "", "0" or "1" raises."""
+    v = os.environ.get("ET_MINER_DEDUCTION", "").strip()
+    if v not in ("", "0", "1"):
+        raise ValueError(f"ET_MINER_DEDUCTION must be 0 or 1, got {v!r}")
+    return DEDUCTION_OFF if v == "0" else DEDUCTION_AUTO
 LAYOUT_LEX, LAYOUT_TRI = 0, 1
 KERNEL_AUTO, KERNEL_VALU, KERNEL_MFMA, KERNEL_ROWS = 0, 1, 2, 3
 K2_KERNELS = {"auto": KERNEL_AUTO, "valu": KERNEL_VALU, "mfma": KERNEL_MFMA, "rows": KERNEL_ROWS}
@@ -43,6 +56,8 @@ SIGNATURES: dict = {
     "etm_level_hash": (c_int, [P, c_int64, c_int32, P, c_int64, P]),
     "etm_level_classify": (c_int, [P, c_int64, c_int32, P, P, P, c_int64, P, P, c_int64, c_int64, P, P, c_int32, P,
                                    P, P, P, P]),
+    "etm_level_deduce": (c_int, [P, c_int32, P, P, c_int64, P, P, P, c_int64, P, P, P, c_int64, c_int64, c_int64,
+                                 P, P, c_int64, c_int64, P, P, P, P, P]),
     "etm_count_level": (c_int, [P, c_int64, c_int64, P, c_int32, P, P, P, c_int64, c_int64, P, P, P, P, c_int32,
                                 c_int32, c_int32, P]),
     "etm_level_compact_ws": (c_size_t, [c_int64]),
@@ -73,7 +88,7 @@ class MineOpts(ctypes.Structure):
 
     _fields_ = [("min_count", c_int64), ("max_length", c_int32), ("prune_non_free", c_int32),
                 ("infer_counts", c_int32), ("time_kernels", c_int32), ("k2_kernel", c_int32),
-                ("k3_kernel", c_int32), ("pad", c_int32)]
+                ("k3_kernel", c_int32), ("deduction", c_int32)]
 
 
 class LevelStat(ctypes.Structure):

@@ -242,6 +242,23 @@ def level_classify(rows, m: int, prev_counts, prev_free, table, table_size: int,
     return status, counts, gid
 
 
+def level_deduce(prev, older2, older3, n_total: int, min_count: int, groups: Groups, status, counts, gid=None,
+                 tally=None) -> None:
+    """Deduction bounds of the candidates ``level_classify`` left counted (its inference mode), in place.
+
+    ``prev``, ``older2``, ``older3`` are levels k-1, k-2, k-3 as ``(rows, counts, table, table_size)`` (``older3``
+    is None at k = 3, where the empty set's count is ``n_total``). Replaces the SUBSET_BOUNDS mode of the
+    reference's subset index (``gpu/kernels/_src/_subset_index.cu:95-136``)."""
+    rows, pc, table, tsize = prev
+    rows2, counts2, table2, tsize2 = older2
+    rows3, counts3, table3, tsize3 = older3 if older3 is not None else (None, None, None, 0)
+    call("etm_level_deduce", rows.data_ptr(), int(rows.shape[1]), pc.data_ptr(), table.data_ptr(), tsize,
+         rows2.data_ptr(), counts2.data_ptr(), table2.data_ptr(), tsize2, dev.ptr(rows3), dev.ptr(counts3),
+         dev.ptr(table3), tsize3, int(n_total), int(min_count), groups.group_start.data_ptr(),
+         groups.cum_pairs.data_ptr(), groups.n_groups, groups.n_cands, dev.ptr(gid), status.data_ptr(),
+         counts.data_ptr(), dev.ptr(tally), _s())
+
+
 def count_level(bv, n_words: int, rows, m: int, groups: Groups, status, counts, layout: int = LAYOUT_LEX,
                 word_chunk: int = 0, pitch: int | None = None, kernel: str = "auto") -> None:
     """Adds the AND + popcount counts of the counted candidates into ``counts``.

@@ -45,7 +45,8 @@ def _opts(min_count, max_length, prune, infer, time_kernels, k2_kernel="auto", k
     if k3_kernel not in _lib.K3_KERNELS:
         raise ValueError(f"k3_kernel must be one of {sorted(_lib.K3_KERNELS)}, got {k3_kernel!r}")
     return _lib.MineOpts(int(max(1, min_count)), int(max_length or 0), int(bool(prune)), int(bool(infer)),
-                         int(bool(time_kernels)), _lib.K2_KERNELS[k2_kernel], _lib.K3_KERNELS[k3_kernel], 0)
+                         int(bool(time_kernels)), _lib.K2_KERNELS[k2_kernel], _lib.K3_KERNELS[k3_kernel],
+                         _lib.deduction_setting())
 
 
 def _collect(handle, levels: bool = True) -> NativeResult:

@@ -5,7 +5,8 @@ GPU. Per level k >= 2 the loop runs, all on the device:
 
 1. the prefix groups of level k-1 (``etm_level_groups``; for k = 2 the K=1 level is one group);
 2. for k >= 3, the (k-1)-subset test of every candidate against the hash of level k-1, with Pascal inference
-   when ``infer_counts`` (``etm_level_classify``);
+   when ``infer_counts`` (``etm_level_classify``), then the deduction bounds over levels k-2 and k-3
+   (``etm_level_deduce``) unless ``ET_MINER_DEDUCTION=0``;
 3. AND + popcount of the counted candidates (``etm_count_level``);
 4. compaction of the survivors (count >= min_count) in lexicographic order (``etm_level_compact``) -- the
    candidate numbering is lexicographic, so no level-end sort is needed;
@@ -26,6 +27,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from .. import _lib
 from . import device as dev
 from . import kernels as K
 
@@ -125,6 +127,8 @@ def mine_levels(
     if track_free:
         prev.free = dev.to_device((k1_counts < n_rows).astype(np.uint8))
     dev_levels = []
+    older = []  # levels k-2 and k-3, for the deduction bounds
+    deduction = _lib.deduction_setting() == _lib.DEDUCTION_AUTO
     k = 2
     while k <= eml and prev.n >= k:
         t_k = time.perf_counter()
@@ -142,6 +146,13 @@ def mine_levels(
             # one thread per candidate (binary search for the group) on even ones.
             status, counts, gid = K.level_classify(prev.rows, m, prev.counts, prev.free if infer else None, table,
                                                    tsize, groups, mode, tally, group_centric=(k % 2 == 1))
+            if infer and deduction and k <= _lib.BOUNDS_MAX_K:
+                older3 = None
+                if k > 3:
+                    older3 = (older[-2].rows, older[-2].counts) + older[-2].hashed()
+                K.level_deduce((prev.rows, prev.counts, table, tsize),
+                               (older[-1].rows, older[-1].counts) + older[-1].hashed(), older3, n_rows, min_count,
+                               groups, status, counts, gid=gid, tally=tally)
         else:
             status, counts = None, dev.zeros(groups.n_cands, "int32")
         st = LevelStats(k, groups.n_cands, 0, groups.n_cands, 0, 0,
@@ -170,6 +181,7 @@ def mine_levels(
             level_callback(k, groups.n_cands, n_freq, dt)
         if n_freq == 0:
             break
+        older = (older + [prev])[-2:]
         prev = cur
         k += 1
     if keep_device:

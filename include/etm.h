@@ -57,8 +57,12 @@ enum { ETM_LAYOUT_LEX = 0, ETM_LAYOUT_TRI = 1 };
  * else VALU AND+popcount tiles), VALU tiles only, or the MFMA Gram regardless of size. */
 enum { ETM_KERNEL_AUTO = 0, ETM_KERNEL_VALU = 1, ETM_KERNEL_MFMA = 2, ETM_KERNEL_ROWS = 3 };
 
-/* ABI version of this header (bumped on any signature change). */
-#define ETM_ABI_VERSION 2
+/* etm_mine_opts.deduction: the bounds are on by default in the inference mode, as the reference's
+ * ET_MINER_DEDUCTION unset or "1" (gpu/row_split.py:627); OFF is its "0". */
+enum { ETM_DEDUCTION_AUTO = 0, ETM_DEDUCTION_OFF = 1 };
+
+/* ABI version of this header (bumped on any signature or struct-meaning change). */
+#define ETM_ABI_VERSION 3
 int etm_abi_version(void);
 
 /* Copies the calling thread's last error message (NUL-terminated, truncated to n). */
@@ -126,12 +130,32 @@ int etm_level_hash(const int32_t* d_rows, int64_t n, int32_t m, int32_t* d_table
  * one workgroup per big group) and d_gid (optional) receives every candidate's group; without them, one
  * thread per candidate finds its group by binary search. When d_tally is not NULL, adds the numbers of
  * counted, inferred and skipped candidates to d_tally[0..2].
- * Replaces _classify_candidate (gpu/kernels/_src/_subset_index.cu:139-178) without the deduction bounds. */
+ * Replaces _classify_candidate (gpu/kernels/_src/_subset_index.cu:139-178); the deduction bounds are
+ * etm_level_deduce. */
 int etm_level_classify(const int32_t* d_rows, int64_t n, int32_t m, const int32_t* d_prev_counts,
                        const uint8_t* d_prev_free, const int32_t* d_table, int64_t table_size,
                        const int64_t* d_group_start, const int64_t* d_cum_pairs, int64_t n_groups,
                        int64_t n_cands, const int32_t* d_group_list, const int64_t* h_class_counts, int32_t mode,
                        uint8_t* d_status, int32_t* d_counts, int32_t* d_gid, int64_t* d_tally, void* stream);
+
+/* Deduction bounds of depth 3 for the candidates of k = m + 1 items (3 <= k <= 24) that etm_level_classify left
+ * ETM_CAND_COUNT in its inference mode (their (k-1)-subsets all free). With s() the support:
+ *   u = min(min_a s(X-a), min_{a<b<c} s(X-a)+s(X-b)+s(X-c)-s(X-ab)-s(X-ac)-s(X-bc)+s(X-abc)),
+ *   l = max(0, max_{a<b} s(X-a)+s(X-b)-s(X-ab)).
+ * u < min_count makes the candidate ETM_CAND_SKIP (count -1); l == u makes it ETM_CAND_INFER with count u;
+ * otherwise it stays counted. Level k-1 is d_rows (n x m) with its counts and hash; level k-2 is d_rows2
+ * (m-1 columns), level k-3 d_rows3 (m-2 columns; unused at k = 3, where s(empty set) = n_total), each with its
+ * counts and its etm_level_hash table. Levels k-1..k-3 must hold every frequent itemset of their length.
+ * d_gid as etm_level_classify writes it (NULL: binary search). d_tally (optional) moves the settled candidates
+ * from d_tally[0] to d_tally[1] (inferred) and d_tally[2] (skipped).
+ * Replaces _deduce and the SUBSET_BOUNDS mode (gpu/kernels/_src/_subset_index.cu:95-136,
+ * gpu/kernels/subset_index.py:63-140 with deep=). */
+int etm_level_deduce(const int32_t* d_rows, int32_t m, const int32_t* d_prev_counts, const int32_t* d_table,
+                     int64_t table_size, const int32_t* d_rows2, const int32_t* d_counts2, const int32_t* d_table2,
+                     int64_t table_size2, const int32_t* d_rows3, const int32_t* d_counts3, const int32_t* d_table3,
+                     int64_t table_size3, int64_t n_total, int64_t min_count, const int64_t* d_group_start,
+                     const int64_t* d_cum_pairs, int64_t n_groups, int64_t n_cands, const int32_t* d_gid,
+                     uint8_t* d_status, int32_t* d_counts, int64_t* d_tally, void* stream);
 
 /* AND + popcount of every candidate whose status is ETM_CAND_COUNT (every candidate when d_status is
  * NULL), ADDED into d_counts (which the caller initialises: etm_level_classify does). Groups of 2..small_max
@@ -242,7 +266,7 @@ typedef struct etm_mine_opts {
     int32_t time_kernels;   /* record HIP events around each level's counting launches */
     int32_t k2_kernel;      /* ETM_KERNEL_* for the K=2 pair space (ROWS: row-wise) */
     int32_t k3_kernel;      /* ETM_KERNEL_AUTO, ETM_KERNEL_VALU (bitvector kernels) or ETM_KERNEL_ROWS for K=3 */
-    int32_t pad;
+    int32_t deduction;      /* ETM_DEDUCTION_AUTO: the deduction bounds with infer_counts (k <= 24); ETM_DEDUCTION_OFF */
 } etm_mine_opts;
 
 /* Per-level statistics: candidates generated by the prefix join and, of them, counted / inferred / skipped by

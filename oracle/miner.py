@@ -25,12 +25,16 @@ reference's ``src/et_miner``):
 * ``count_itemsets``: ``core/cpu_miner.py:838-896`` -- counts of given itemsets.
 * ``mine_son``: ``streaming/son.py:324-421``, ``:570-693`` -- SON's two passes,
   the candidate union with known counts and slack, and the bound.
+* ``deduction_outcome``: ``gpu/kernels/_src/_subset_index.cu:95-178`` -- Pascal
+  inference, then the depth-3 deduction bounds of one candidate.
 
 Pinned by (tests/test_oracle.py): the reference's committed lattices
 (``bench/results/2026-10-17-moonshots-baseline/*.lattice.parquet``) on the
 regenerated preset datasets, the reference's ``min_count_cases.json``, the
 README's known answer, the boundary case s=0.07, planted-motif lower bounds,
-and brute-force subset enumeration on small random inputs.
+and brute-force subset enumeration on small random inputs; ``deduction_outcome``
+by brute-force counts (l <= count <= u for every candidate, an inferred value
+equal to the count, a skipped candidate infrequent) and hand-worked cases.
 """
 
 from __future__ import annotations
@@ -474,3 +478,50 @@ def brute_force(indptr: np.ndarray, ids: np.ndarray, min_support: float, max_len
             for sub in combinations(fr, r):
                 counts[sub] = counts.get(sub, 0) + 1
     return {s: c for s, c in counts.items() if c >= mc}
+
+
+def _drop(x: tuple, drop: set, support: dict, n_total: int) -> int:
+    sub = tuple(v for t, v in enumerate(x) if t not in drop)
+    return support[sub] if sub else n_total
+
+
+def deduction_bounds(x: tuple, support: dict, n_total: int) -> tuple:
+    """(l, u): the depth-3 bounds of ``gpu/kernels/_src/_subset_index.cu:95-136`` on the count of ``x``.
+
+    u = min(min_a s(x-a), min_{a<b<c} s(x-a)+s(x-b)+s(x-c)-s(x-ab)-s(x-ac)-s(x-bc)+s(x-abc)),
+    l = max(0, max_{a<b} s(x-a)+s(x-b)-s(x-ab)); ``support`` holds the subsets of lengths k-1..k-3.
+    """
+    from itertools import combinations
+
+    k = len(x)
+    c1 = [_drop(x, {t}, support, n_total) for t in range(k)]
+    upper = min(c1)
+    lower = 0
+    for a, b in combinations(range(k), 2):
+        lower = max(lower, c1[a] + c1[b] - _drop(x, {a, b}, support, n_total))
+    for a, b, c in combinations(range(k), 3):
+        s2 = (_drop(x, {a, b}, support, n_total) + _drop(x, {a, c}, support, n_total)
+              + _drop(x, {b, c}, support, n_total))
+        upper = min(upper, c1[a] + c1[b] + c1[c] - s2 + _drop(x, {a, b, c}, support, n_total))
+    return lower, upper
+
+
+def deduction_outcome(x: tuple, support: dict, n_total: int, mc: int) -> tuple:
+    """('pascal' | 'skip' | 'infer' | 'count', value) of candidate ``x`` (ascending, every (k-1)-subset frequent).
+
+    ``support`` maps the subsets of x of lengths k-1, k-2 and k-3 to their counts (the empty set is ``n_total``).
+    Follows ``gpu/kernels/_src/_subset_index.cu:139-178``: a non-free (k-1)-subset gives Pascal inference (the
+    minimum of the subsets' counts); otherwise the bounds decide -- u < mc skips, l == u infers u.
+    """
+    k = len(x)
+    c1 = [_drop(x, {t}, support, n_total) for t in range(k)]
+    for t in range(k):
+        # a (k-1)-subset is free when its count differs from each of its own (k-2)-subsets'
+        if any(c1[t] == _drop(x, {t, u}, support, n_total) for u in range(k) if u != t):
+            return "pascal", min(c1)
+    lower, upper = deduction_bounds(x, support, n_total)
+    if upper < mc:
+        return "skip", 0
+    if lower == upper:
+        return "infer", upper
+    return "count", 0

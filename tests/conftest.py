@@ -56,3 +56,32 @@ def load_lattice(name: str) -> tuple:
     t = pq.read_table(GOLDEN / f"{name}.lattice.parquet").to_pydict()
     params = json.loads((GOLDEN / f"{name}.lattice.json").read_text())
     return {tuple(s): c for s, c in zip(t["itemset"], t["count"])}, params
+
+
+def dense_rows(seed: int, n_rows: int) -> list:
+    """Rows over seven items present independently (40-80 %), where a row holding item 0 also holds item 1 or
+    item 2: a dense lattice whose depth-3 upper bounds fall below min_count, and whose cover of item 0 by items 1
+    and 2 makes the bounds meet (the data shape of the reference's tests/test_deduction.py:72-88)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    p = np.array([0.8, 0.5, 0.5, 0.6, 0.6, 0.4, 0.4])
+    out = []
+    for _ in range(n_rows):
+        r = set(np.flatnonzero(rng.random(len(p)) < p).tolist())
+        if 0 in r and not {1, 2} & r:
+            r.add(1 if rng.random() < 0.5 else 2)
+        out.append(tuple(sorted(r)))
+    return out
+
+
+def lattice_counts(rows, min_count: int, max_k: int) -> dict:
+    """{itemset: count} of every itemset of at most max_k items with count >= min_count (brute force)."""
+    from collections import Counter
+    from itertools import combinations
+
+    counts: Counter = Counter()
+    for r in rows:
+        for k in range(1, max_k + 1):
+            counts.update(combinations(r, k))
+    return {s: c for s, c in counts.items() if c >= min_count}

@@ -117,3 +117,19 @@ def test_transactions_to_csr_inputs():
 
     indptr, ids, _ = transactions_to_csr(pd.DataFrame({"items": [[4, 5], [5]]}))
     assert ids.tolist() == [4, 5, 5]
+
+
+def test_deduction_switch(monkeypatch):
+    """ET_MINER_DEDUCTION: unset or "1" keeps the bounds, "0" turns them off, anything else raises (the
+    reference's _env.py:285-291)."""
+    from et_miner import _lib
+
+    monkeypatch.delenv("ET_MINER_DEDUCTION", raising=False)
+    assert _lib.deduction_setting() == _lib.DEDUCTION_AUTO
+    monkeypatch.setenv("ET_MINER_DEDUCTION", "1")
+    assert _lib.deduction_setting() == _lib.DEDUCTION_AUTO
+    monkeypatch.setenv("ET_MINER_DEDUCTION", "0")
+    assert _lib.deduction_setting() == _lib.DEDUCTION_OFF
+    monkeypatch.setenv("ET_MINER_DEDUCTION", "2")
+    with pytest.raises(ValueError, match="ET_MINER_DEDUCTION"):
+        _lib.deduction_setting()

@@ -144,3 +144,57 @@ def test_son_slack_guard_raises():
     indptr, ids = _csr([[0, 1]] * 100)
     with pytest.raises(RuntimeError):
         miner.mine_son(indptr, ids, 0.0, chunk_size=10)
+
+
+def test_deduction_hand_case():
+    """Rows {0,1} {0,2} {0,1,2} {1} {2} {}: s(0)=s(1)=s(2)=3, s(01)=s(02)=2, s(12)=1, s(012)=1, N=6.
+    Lower bound over the pair dropping items 1 and 2: s(02)+s(01)-s(0) = 1; upper: min(s(12)=1, triple
+    2+2+1-3-3-3+6 = 2) = 1. Bounds meet: inferred 1 at min_count 1, skipped at min_count 2."""
+    rows = [(0, 1), (0, 2), (0, 1, 2), (1,), (2,), ()]
+    from conftest import lattice_counts
+
+    lat = lattice_counts(rows, 1, 3)
+    assert miner.deduction_bounds((0, 1, 2), lat, 6) == (1, 1)
+    assert miner.deduction_outcome((0, 1, 2), lat, 6, 1) == ("infer", 1)
+    assert miner.deduction_outcome((0, 1, 2), lat, 6, 2) == ("skip", 0)
+    # {0,1,3} with 3 in every row holding 0: s(03) = s(0), so {0,3} is not free and Pascal applies
+    rows = [(0, 1, 3), (0, 3), (1,), (1, 3), (0, 1, 3)]
+    lat = lattice_counts(rows, 1, 3)
+    assert miner.deduction_outcome((0, 1, 3), lat, 5, 1) == ("pascal", 2)
+
+
+@pytest.mark.parametrize("k,seed,mc", [(3, 5, 40), (3, 53, 40), (4, 10, 40), (4, 17, 40), (5, 53, 15), (6, 17, 10)])
+def test_deduction_bounds_bracket_every_count(k, seed, mc):
+    """On every candidate of level k (all (k-1)-subsets frequent): l <= count <= u, Pascal's value and an
+    inferred value equal the count, a skipped candidate is infrequent -- and the bounds settle some."""
+    from conftest import dense_rows, lattice_counts
+
+    rows = dense_rows(seed, 300)
+    n = len(rows)
+    exact = lattice_counts(rows, 0, k)
+    lat = {s: c for s, c in exact.items() if c >= mc}
+    sets = [frozenset(r) for r in rows]
+    seen = {"pascal": 0, "skip": 0, "infer": 0, "count": 0}
+    for x in combinations(range(7), k):
+        if not all(sub in lat for sub in combinations(x, k - 1)):
+            continue
+        true = exact.get(x, 0)
+        lower, upper = miner.deduction_bounds(x, lat, n)
+        assert lower <= true <= upper, (x, lower, true, upper)
+        # Inclusion-exclusion, counted from the rows: the pair term is s(x) minus the rows holding x-ab but
+        # neither a nor b; the triple term is s(x) plus the rows holding x-abc but none of a, b, c.
+        def none_of(drop):
+            rest = set(x) - set(drop)
+            return sum(1 for r in sets if rest <= r and not set(drop) & r)
+
+        want_u = min([exact.get(tuple(v for v in x if v != a), 0) for a in x]
+                     + [true + none_of(t) for t in combinations(x, 3)])
+        want_l = max([0] + [true - none_of(t) for t in combinations(x, 2)])
+        assert (lower, upper) == (want_l, want_u), (x, lower, upper, want_l, want_u)
+        kind, value = miner.deduction_outcome(x, lat, n, mc)
+        seen[kind] += 1
+        if kind in ("pascal", "infer"):
+            assert value == true, (x, kind, value, true)
+        if kind == "skip":
+            assert true < mc, (x, true)
+    assert seen["skip"] + seen["infer"] > 0, seen
