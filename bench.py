@@ -102,6 +102,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--engine", default="native", choices=["native", "python"],
                     help="native: the level loop inside libetm; python: the same kernels driven level by level")
+    ap.add_argument("--row-order", default="auto", choices=["auto", "keep", "lex"],
+                    help="bitvector row order of the native loop (counts never depend on it)")
     ap.add_argument("--served-jobs", type=int, default=512, help="jobs of the served stream (0: skip that leg)")
     ap.add_argument("--served-workers", type=int, default=8)
     ap.add_argument("--no-streaming", action="store_true",
@@ -157,7 +159,7 @@ def main(argv=None) -> int:
         stats = []
         table = mine_csr(None, None, w.min_support, w.max_length, use_generator_pruning=w.use_generator_pruning,
                          device_csr=d_csr, with_counts=collect, timer=timer, stats_out=stats, engine=args.engine,
-                         time_kernels=timer.enabled)
+                         time_kernels=timer.enabled, row_order=args.row_order)
         return table, stats
 
     timer.enabled = False

@@ -8,6 +8,8 @@
 // leave as one contiguous 64-byte run.
 #include <algorithm>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "common.h"
 
 namespace {
@@ -147,7 +149,8 @@ __global__ __launch_bounds__(kBuildThreads) void k_bitvec_build(const int64_t* _
                                                                  const Id* __restrict__ ids, int64_t n_rows,
                                                                  int64_t lo, int64_t span,
                                                                  const int32_t* __restrict__ lut, int32_t n_cols,
-                                                                 int64_t pitch, uint64_t* __restrict__ out) {
+                                                                 int64_t pitch, const int32_t* __restrict__ order,
+                                                                 uint64_t* __restrict__ out) {
     constexpr int NM = SLAB / 64;  // 64-bit masks per lane
     __shared__ uint64_t tile[SLAB][kBuildWords + 1];
     const int64_t w0 = (int64_t)blockIdx.x * kBuildWords;
@@ -155,7 +158,8 @@ __global__ __launch_bounds__(kBuildThreads) void k_bitvec_build(const int64_t* _
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     for (int wl = wv; wl < kBuildWords; wl += kBuildThreads / 64) {
-        const int64_t r = (w0 + wl) * 64 + lane;
+        const int64_t pos = (w0 + wl) * 64 + lane;  // bit position; the row there is order[pos]
+        const int64_t r = (order != nullptr && pos < n_rows) ? (int64_t)order[pos] : pos;
         uint64_t mask[NM];
 #pragma unroll
         for (int q = 0; q < NM; ++q) mask[q] = 0;
@@ -211,6 +215,49 @@ __global__ __launch_bounds__(256) void k_colpop(const uint64_t* __restrict__ bv,
     if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
+// Row order key: bit 63 - r for each column of rank r < n_ranked the row holds (repeated ids set the same bit).
+template <typename Id>
+__global__ __launch_bounds__(256) void k_row_keys(const int64_t* __restrict__ indptr, const Id* __restrict__ ids,
+                                                   int64_t n_rows, int64_t lo, int64_t span,
+                                                   const int32_t* __restrict__ lut, const int32_t* __restrict__ rank,
+                                                   int32_t n_ranked, uint64_t* __restrict__ keys,
+                                                   int32_t* __restrict__ iota) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_rows;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t key = 0;
+        const int64_t b = indptr[r + 1];
+        for (int64_t e = indptr[r]; e < b; ++e) {
+            const int64_t v = (int64_t)ids[e] - lo;
+            if (v < 0 || v >= span) continue;
+            const int32_t c = lut[v];
+            if (c < 0) continue;
+            const int32_t rk = rank[c];
+            if (rk >= 0 && rk < n_ranked) key |= 1ull << (63 - rk);
+        }
+        keys[r] = key;
+        iota[r] = (int32_t)r;
+    }
+}
+
+// Tile occupancy: bit t % 64 of occ[c * occ_pitch + t / 64] is set when column c has a nonzero word in tile t
+// (words [64t, 64t + 64)). One wave per output word: 64 tiles, each one coalesced 512-byte read and a ballot.
+// Words of a row past the last tile are written as zero.
+__global__ __launch_bounds__(256) void k_occupancy(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
+                                                    int64_t n_cols, uint64_t* __restrict__ occ, int64_t occ_pitch) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wave >= n_cols * occ_pitch) return;
+    const int64_t c = wave / occ_pitch, q = wave % occ_pitch;
+    const uint64_t* col = bv + c * pitch;
+    uint64_t bits = 0;
+    for (int t = 0; t < 64; ++t) {
+        const int64_t w = (q * 64 + t) * 64 + lane;
+        const bool nz = w < n_words && col[w] != 0;
+        if (__ballot(nz) != 0) bits |= 1ull << t;
+    }
+    if (lane == 0) occ[wave] = bits;
+}
+
 int grid_for(int64_t n, int per_block = 256, int cap = 8192) {
     int64_t g = (n + per_block - 1) / per_block;
     if (g < 1) g = 1;
@@ -263,9 +310,73 @@ extern "C" int etm_ids_hist(const void* d_ids, int id_bytes, int64_t n, int64_t 
     return ETM_OK;
 }
 
+namespace {
+size_t sort_tmp_bytes(int64_t n) {
+    size_t tmp = 0;
+    rocprim::radix_sort_pairs(nullptr, tmp, (uint64_t*)nullptr, (uint64_t*)nullptr, (int32_t*)nullptr,
+                              (int32_t*)nullptr, (size_t)n, 0, 64);
+    return tmp;
+}
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+}  // namespace
+
+extern "C" size_t etm_row_order_ws(int64_t n_rows) {
+    const int64_t n = std::max<int64_t>(n_rows, 1);
+    return 2 * align256((size_t)n * 8) + align256((size_t)n * 4) + align256(sort_tmp_bytes(n));
+}
+
+extern "C" int etm_row_order(const int64_t* d_indptr, const void* d_ids, int id_bytes, int64_t n_rows, int64_t lo,
+                             int64_t span, const int32_t* d_lut, const int32_t* d_rank, int32_t n_ranked,
+                             int32_t* d_order, void* d_ws, size_t ws_bytes, void* stream) {
+    ETM_ARG(d_indptr && d_lut && d_rank && d_order && d_ws, "etm_row_order: null pointer");
+    ETM_ARG(id_bytes == 4 || id_bytes == 8, "etm_row_order: id_bytes must be 4 or 8, got %d", id_bytes);
+    ETM_ARG(n_rows >= 0 && n_rows < 0x7FFFFFFF && span >= 1, "etm_row_order: bad sizes");
+    ETM_ARG(n_ranked >= 1 && n_ranked <= 64, "etm_row_order: 1 <= n_ranked <= 64 required, got %d", n_ranked);
+    ETM_ARG(ws_bytes >= etm_row_order_ws(n_rows), "etm_row_order: workspace too small");
+    if (n_rows == 0) return ETM_OK;
+    ETM_ARG(d_ids != nullptr, "etm_row_order: null ids");
+    hipStream_t s = ETM_STREAM(stream);
+    char* ws = (char*)d_ws;
+    const size_t kb = align256((size_t)n_rows * 8);
+    uint64_t* keys = (uint64_t*)ws;
+    uint64_t* keys_out = (uint64_t*)(ws + kb);
+    int32_t* iota = (int32_t*)(ws + 2 * kb);
+    void* tmp = ws + 2 * kb + align256((size_t)n_rows * 4);
+    size_t tmp_bytes = sort_tmp_bytes(n_rows);
+    if (id_bytes == 4)
+        k_row_keys<int32_t><<<grid_for(n_rows), 256, 0, s>>>(d_indptr, (const int32_t*)d_ids, n_rows, lo, span, d_lut,
+                                                             d_rank, n_ranked, keys, iota);
+    else
+        k_row_keys<int64_t><<<grid_for(n_rows), 256, 0, s>>>(d_indptr, (const int64_t*)d_ids, n_rows, lo, span, d_lut,
+                                                             d_rank, n_ranked, keys, iota);
+    ETM_LAUNCHED("k_row_keys");
+    // The whole key is sorted: rocPRIM's multi-block path returned unsorted output here for every begin_bit
+    // above 0 tried (4, 24, 40, 56; above 1,000 rows), though only the top n_ranked bits carry ranks.
+    const hipError_t e = rocprim::radix_sort_pairs(tmp, tmp_bytes, keys, keys_out, iota, d_order, (size_t)n_rows,
+                                                   0, 64, s);
+    ETM_REQUIRE(e == hipSuccess, ETM_EHIP, "etm_row_order: radix sort failed: %s", hipGetErrorString(e));
+    return ETM_OK;
+}
+
+extern "C" int etm_tile_occupancy(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int32_t n_cols,
+                                  uint64_t* d_occ, int64_t occ_pitch, void* stream) {
+    ETM_ARG(d_bv && d_occ, "etm_tile_occupancy: null pointer");
+    ETM_ARG(n_cols >= 0 && n_words >= 0 && pitch >= n_words, "etm_tile_occupancy: bad sizes");
+    const int64_t tiles = (n_words + 63) / 64;
+    ETM_ARG(occ_pitch >= (tiles + 63) / 64 + 1, "etm_tile_occupancy: occ_pitch %lld < %lld",
+            (long long)occ_pitch, (long long)((tiles + 63) / 64 + 1));
+    const int64_t waves = (int64_t)n_cols * occ_pitch;
+    if (waves == 0) return ETM_OK;
+    ETM_REQUIRE((waves + 3) / 4 <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_tile_occupancy: grid too large");
+    k_occupancy<<<(unsigned)((waves + 3) / 4), 256, 0, ETM_STREAM(stream)>>>(d_bv, pitch, n_words, n_cols, d_occ,
+                                                                             occ_pitch);
+    ETM_LAUNCHED("k_occupancy");
+    return ETM_OK;
+}
+
 extern "C" int etm_bitvec_build(const int64_t* d_indptr, const void* d_ids, int id_bytes, int64_t n_rows, int64_t lo,
-                                int64_t span, const int32_t* d_lut, int32_t n_cols, int64_t pitch, uint64_t* d_out,
-                                void* stream) {
+                                int64_t span, const int32_t* d_lut, int32_t n_cols, int64_t pitch,
+                                const int32_t* d_order, uint64_t* d_out, void* stream) {
     ETM_ARG(d_indptr != nullptr && d_lut != nullptr && d_out != nullptr, "etm_bitvec_build: null pointer");
     ETM_ARG(id_bytes == 4 || id_bytes == 8, "etm_bitvec_build: id_bytes must be 4 or 8, got %d", id_bytes);
     ETM_ARG(n_rows >= 0 && n_cols >= 0 && span >= 1, "etm_bitvec_build: negative size");
@@ -281,18 +392,18 @@ extern "C" int etm_bitvec_build(const int64_t* d_indptr, const void* d_ids, int 
         dim3 grid((unsigned)tiles, 1);
         if (id_bytes == 4)
             k_bitvec_build<int32_t, 128><<<grid, kBuildThreads, 0, s>>>(d_indptr, (const int32_t*)d_ids, n_rows, lo,
-                                                                        span, d_lut, n_cols, pitch, d_out);
+                                                                        span, d_lut, n_cols, pitch, d_order, d_out);
         else
             k_bitvec_build<int64_t, 128><<<grid, kBuildThreads, 0, s>>>(d_indptr, (const int64_t*)d_ids, n_rows, lo,
-                                                                        span, d_lut, n_cols, pitch, d_out);
+                                                                        span, d_lut, n_cols, pitch, d_order, d_out);
     } else {
         dim3 grid((unsigned)tiles, (unsigned)((n_cols + 255) / 256));
         if (id_bytes == 4)
             k_bitvec_build<int32_t, 256><<<grid, kBuildThreads, 0, s>>>(d_indptr, (const int32_t*)d_ids, n_rows, lo,
-                                                                        span, d_lut, n_cols, pitch, d_out);
+                                                                        span, d_lut, n_cols, pitch, d_order, d_out);
         else
             k_bitvec_build<int64_t, 256><<<grid, kBuildThreads, 0, s>>>(d_indptr, (const int64_t*)d_ids, n_rows, lo,
-                                                                        span, d_lut, n_cols, pitch, d_out);
+                                                                        span, d_lut, n_cols, pitch, d_order, d_out);
     }
     ETM_LAUNCHED("k_bitvec_build");
     return ETM_OK;

@@ -82,7 +82,8 @@ __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict_
                                                       const int64_t* __restrict__ cum_pairs,
                                                       const int32_t* __restrict__ list, int64_t n_list,
                                                       const uint8_t* __restrict__ status,
-                                                      int32_t* __restrict__ counts, int32_t layout) {
+                                                      int32_t* __restrict__ counts, int32_t layout,
+                                                      const uint64_t* __restrict__ occ, int64_t occ_pitch) {
     constexpr int NP = S * (S - 1) / 2;
     constexpr int U = SmallUnroll<S>::U;
     const int lane = threadIdx.x & 63;
@@ -121,17 +122,11 @@ __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict_
 
     const int64_t w_begin = ch * chunk_words;
     const int64_t w_end = min(n_words, w_begin + chunk_words);
-    for (int64_t w0 = w_begin + lane; w0 < w_end; w0 += 64 * U) {
-        bool in[U];
-        int64_t w[U];
+    // U words per lane: w[u] (a valid word when !in[u]; its result is masked out).
+    auto step = [&](const int64_t (&w)[U], const bool (&in)[U]) {
         uint64_t pre[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            w[u] = w0 + 64 * u;
-            in[u] = w[u] < w_end;
-            if (!in[u]) w[u] = w_begin;  // a valid word; its result is masked out below
-            pre[u] = in[u] ? ~0ull : 0ull;
-        }
+        for (int u = 0; u < U; ++u) pre[u] = in[u] ? ~0ull : 0ull;
         // Prefix AND, four prefix columns at a time: 4 x U independent loads in flight.
         for (int t = 0; t < np; t += 4) {
             const uint64_t* c0 = bv + (int64_t)prow[t] * pitch;
@@ -157,6 +152,52 @@ __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict_
                     ++a;
                 }
             }
+        }
+    };
+    if (occ == nullptr) {
+        for (int64_t w0 = w_begin + lane; w0 < w_end; w0 += 64 * U) {
+            bool in[U];
+            int64_t w[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                w[u] = w0 + 64 * u;
+                in[u] = w[u] < w_end;
+                if (!in[u]) w[u] = w_begin;
+            }
+            step(w, in);
+        }
+    } else {
+        // Tiles of 64 words the chunk spans (at most 64: the launcher caps the chunk at 4096 words), kept where
+        // every prefix column and at least two suffix columns are occupied: elsewhere every pair's AND is zero.
+        const int64_t t0 = w_begin >> 6;
+        const int nt = (int)((w_end - w_begin + 63) >> 6);
+        auto window = [&](int32_t col) {
+            const uint64_t* o = occ + (int64_t)col * occ_pitch + (t0 >> 6);
+            const int sh = (int)(t0 & 63);
+            return sh ? (o[0] >> sh) | (o[1] << (64 - sh)) : o[0];
+        };
+        uint64_t tiles = nt >= 64 ? ~0ull : ((1ull << nt) - 1);
+        for (int t = 0; t < np; ++t) tiles &= window(prow[t]);
+        uint64_t one = 0, two = 0;
+        for (int t = 0; t < s; ++t) {
+            const uint64_t x = window(rows[(gs + t) * m + np]);
+            two |= one & x;
+            one |= x;
+        }
+        tiles &= two;
+        while (tiles) {
+            bool in[U];
+            int64_t w[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                in[u] = tiles != 0;
+                const int t = in[u] ? __builtin_ctzll(tiles) : 0;
+                tiles &= tiles - 1;
+                w[u] = in[u] ? w_begin + t * 64 + lane : w_begin;
+                in[u] = in[u] && w[u] < w_end;
+                if (!in[u]) w[u] = w_begin;
+            }
+            step(w, in);
         }
     }
 
@@ -527,12 +568,13 @@ int64_t pick_chunk(int64_t n_words, int64_t units, int64_t target, int64_t align
 template <int S>
 hipError_t launch_small(const uint64_t* bv, int64_t pitch, int64_t n_words, int64_t chunk, const int32_t* rows,
                         int32_t m, const int64_t* gs, const int64_t* cp, const int32_t* list, int64_t n_list,
-                        const uint8_t* status, int32_t* counts, int32_t layout, hipStream_t s) {
+                        const uint8_t* status, int32_t* counts, int32_t layout, const uint64_t* occ,
+                        int64_t occ_pitch, hipStream_t s) {
     const int64_t n_chunks = ceil_div(n_words, chunk);
     const int64_t blocks = kXcds * ceil_div(xcd_units(n_list, n_chunks), 4);
     if (blocks > 0x7FFFFFFF) return hipErrorInvalidConfiguration;
     k_count_small<S><<<(unsigned)blocks, 256, 0, s>>>(bv, pitch, n_words, chunk, n_chunks, rows, m, gs, cp, list,
-                                                      n_list, status, counts, layout);
+                                                      n_list, status, counts, layout, occ, occ_pitch);
     return hipGetLastError();
 }
 
@@ -542,8 +584,11 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
                                const int64_t* d_group_start, const int64_t* d_cum_pairs, const int64_t* d_cum_tiles,
                                int64_t n_groups, int64_t n_tile_pairs, const int32_t* d_small_list,
                                const int64_t* h_class_counts, const uint8_t* d_status, int32_t* d_counts,
-                               int32_t layout, int32_t word_chunk, int32_t kernel, void* stream) {
+                               int32_t layout, int32_t word_chunk, int32_t kernel, const uint64_t* d_occ,
+                               int64_t occ_pitch, void* stream) {
     ETM_ARG(d_bv && d_rows && d_group_start && d_cum_pairs && d_counts, "etm_count_level: null pointer");
+    ETM_ARG(d_occ == nullptr || occ_pitch >= ((n_words + 63) / 64 + 63) / 64 + 1,
+            "etm_count_level: occ_pitch %lld too small for %lld words", (long long)occ_pitch, (long long)n_words);
     ETM_ARG(m >= 1 && m <= 62, "etm_count_level: 1 <= m <= 62 required, got %d", m);
     ETM_ARG(pitch >= n_words && n_words >= 0, "etm_count_level: pitch %lld < n_words %lld", (long long)pitch,
             (long long)n_words);
@@ -566,18 +611,20 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
         // About 4096 waves per class and chunks of at least 256 words: classes of many groups keep long chunks
         // (fewer partial sums, chunk-major L2 reuse per XCD), classes of few groups get enough waves (measured
         // on MI355X over 256-4096-word floors and 4K-64K wave targets).
-        const int64_t chunk = word_chunk > 0 ? word_chunk : pick_chunk(n_words, cls[c], 4096, 64, 256);
+        int64_t chunk = word_chunk > 0 ? word_chunk : pick_chunk(n_words, cls[c], 4096, 64, 256);
+        // The tile skip reads one 64-bit occupancy window per column: at most 64 tiles of 64 words per chunk.
+        if (d_occ) chunk = std::min<int64_t>(ceil_div(chunk, 64) * 64, 4096);
         hipError_t e;
         const int32_t* list = d_small_list + base;
         switch (c) {
             case 0: e = launch_small<2>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start, d_cum_pairs, list,
-                                        cls[c], d_status, d_counts, layout, s); break;
+                                        cls[c], d_status, d_counts, layout, d_occ, occ_pitch, s); break;
             case 1: e = launch_small<4>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start, d_cum_pairs, list,
-                                        cls[c], d_status, d_counts, layout, s); break;
+                                        cls[c], d_status, d_counts, layout, d_occ, occ_pitch, s); break;
             case 2: e = launch_small<8>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start, d_cum_pairs, list,
-                                        cls[c], d_status, d_counts, layout, s); break;
+                                        cls[c], d_status, d_counts, layout, d_occ, occ_pitch, s); break;
             default: e = launch_small<16>(d_bv, pitch, n_words, chunk, d_rows, m, d_group_start, d_cum_pairs, list,
-                                          cls[c], d_status, d_counts, layout, s); break;
+                                          cls[c], d_status, d_counts, layout, d_occ, occ_pitch, s); break;
         }
         ETM_REQUIRE(e == hipSuccess, ETM_EHIP, "launch of k_count_small failed: %s", hipGetErrorString(e));
         base += cls[c];

@@ -23,6 +23,9 @@ ETM_OK, ETM_EINVAL, ETM_ENOMEM, ETM_EHIP, ETM_EOVERFLOW, ETM_ENODEV = 0, 1, 2, 3
 CAND_COUNT, CAND_SKIP, CAND_INFER = 0, 1, 2
 SUBSET_PRUNE, SUBSET_INFER = 1, 2
 DEDUCTION_AUTO, DEDUCTION_OFF = 0, 1
+ROW_ORDERS = {"auto": 0, "keep": 1, "lex": 2}
+#: Words per tile of the occupancy bitmap (ETM_OCC_TILE_WORDS).
+OCC_TILE_WORDS = 64
 #: Largest candidate the deduction bounds settle (etm_level_deduce).
 BOUNDS_MAX_K = 24
 
@@ -49,7 +52,10 @@ SIGNATURES: dict = {
     "etm_device_count": (c_int, [POINTER(c_int)]),
     "etm_ids_minmax": (c_int, [P, c_int, c_int64, P, P]),
     "etm_ids_hist": (c_int, [P, c_int, c_int64, c_int64, c_int64, P, P]),
-    "etm_bitvec_build": (c_int, [P, P, c_int, c_int64, c_int64, c_int64, P, c_int32, c_int64, P, P]),
+    "etm_bitvec_build": (c_int, [P, P, c_int, c_int64, c_int64, c_int64, P, c_int32, c_int64, P, P, P]),
+    "etm_row_order_ws": (c_size_t, [c_int64]),
+    "etm_row_order": (c_int, [P, P, c_int, c_int64, c_int64, c_int64, P, P, c_int32, P, P, c_size_t, P]),
+    "etm_tile_occupancy": (c_int, [P, c_int64, c_int64, c_int32, P, c_int64, P]),
     "etm_column_popcounts": (c_int, [P, c_int32, c_int64, c_int64, P, P]),
     "etm_level_groups_ws": (c_size_t, [c_int64]),
     "etm_level_groups": (c_int, [P, c_int64, c_int32, c_int32, P, P, P, P, P, P, c_size_t, P]),
@@ -59,7 +65,7 @@ SIGNATURES: dict = {
     "etm_level_deduce": (c_int, [P, c_int32, P, P, c_int64, P, P, P, c_int64, P, P, P, c_int64, c_int64, c_int64,
                                  P, P, c_int64, c_int64, P, P, P, P, P]),
     "etm_count_level": (c_int, [P, c_int64, c_int64, P, c_int32, P, P, P, c_int64, c_int64, P, P, P, P, c_int32,
-                                c_int32, c_int32, P]),
+                                c_int32, c_int32, P, c_int64, P]),
     "etm_level_compact_ws": (c_size_t, [c_int64]),
     "etm_level_compact": (c_int, [P, c_int32, P, P, c_int64, c_int64, P, P, P, c_int64, P, P, P, P, P, c_size_t, P]),
     "etm_level_free": (c_int, [P, c_int64, c_int32, P, P, P, P, c_int64, P, P]),
@@ -88,7 +94,7 @@ class MineOpts(ctypes.Structure):
 
     _fields_ = [("min_count", c_int64), ("max_length", c_int32), ("prune_non_free", c_int32),
                 ("infer_counts", c_int32), ("time_kernels", c_int32), ("k2_kernel", c_int32),
-                ("k3_kernel", c_int32), ("deduction", c_int32)]
+                ("k3_kernel", c_int32), ("deduction", c_int32), ("row_order", c_int32)]
 
 
 class LevelStat(ctypes.Structure):

@@ -245,14 +245,15 @@ def apriori(
 def mine_csr(indptr, ids, min_support, max_length=None, prune_equal_support=False, use_generator_pruning=False,
              level_callback=None, session=None, items=None, warn_complexity=False, device_csr=None,
              with_counts=False, timer=None, stats_out=None, engine: str = "native", time_kernels=False,
-             k2_kernel: str = "auto", k3_kernel: str = "auto"):
+             k2_kernel: str = "auto", k3_kernel: str = "auto", row_order: str = "auto"):
     """The use_gpu route from a host CSR (or ``device_csr=(indptr_d, ids_d)`` already in HBM).
 
     ``engine="native"`` runs the whole level loop inside libetm (one C call); ``engine="python"`` drives the
     same kernels level by level from Python through the operator wrappers (``timer`` hooks apply there).
     ``stats_out`` (a list) receives the per-level statistics. ``k2_kernel`` ("auto", "valu", "mfma",
     "rows") picks the K=2 pair space's kernel: the int8 MFMA Gram, the VALU AND+popcount tiles or the row-wise
-    kernel; ``k3_kernel`` ("auto", "valu", "rows") picks K=3's (native engine only).
+    kernel; ``k3_kernel`` ("auto", "valu", "rows") picks K=3's; ``row_order`` ("auto", "keep", "lex") the
+    bitvectors' row order (native engine only; counts never depend on it).
     """
     from ..gpu import device as dev
 
@@ -273,7 +274,7 @@ def mine_csr(indptr, ids, min_support, max_length=None, prune_equal_support=Fals
 
         res = mine_csr_device(indptr_d, ids_d, n_rows, min_count, max_length, prune_equal_support,
                               use_generator_pruning, time_kernels=time_kernels, levels=False, k2_kernel=k2_kernel,
-                              k3_kernel=k3_kernel)
+                              k3_kernel=k3_kernel, row_order=row_order)
         if session:
             session.end_phase(n_columns=len(res.col_to_item))
         if warn_complexity and res.stats:

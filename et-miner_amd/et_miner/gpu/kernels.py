@@ -77,17 +77,60 @@ def ids_hist(ids, lo: int, span: int):
     return out
 
 
-def build_bitvecs(indptr, ids, n_rows: int, lo: int, span: int, lut, n_cols: int, pitch: int | None = None):
-    """(n_cols, pitch) bitvectors (int64 tensor holding the uint64 words) of a device CSR of item ids."""
+def build_bitvecs(indptr, ids, n_rows: int, lo: int, span: int, lut, n_cols: int, pitch: int | None = None,
+                  order=None):
+    """(n_cols, pitch) bitvectors (int64 tensor holding the uint64 words) of a device CSR of item ids; with
+    ``order`` (device int32, a permutation from ``row_order``) bit position p holds row ``order[p]``."""
     if indptr.numel() != n_rows + 1:
         raise ValueError(f"indptr holds {indptr.numel()} entries, expected n_rows + 1 = {n_rows + 1}")
     pitch = padded_pitch(n_rows) if pitch is None else int(pitch)
     if pitch < words_for(n_rows):
         raise ValueError(f"pitch {pitch} < ceil(n_rows/64) = {words_for(n_rows)}")
+    if order is not None and order.numel() != n_rows:
+        raise ValueError(f"order holds {order.numel()} entries, expected n_rows = {n_rows}")
     out = dev.empty((n_cols, pitch), "int64")
     call("etm_bitvec_build", indptr.data_ptr(), ids.data_ptr() if ids.numel() else indptr.data_ptr(),
-         _id_bytes(ids), n_rows, lo, span, lut.data_ptr(), n_cols, pitch, out.data_ptr(), _s())
+         _id_bytes(ids), n_rows, lo, span, lut.data_ptr(), n_cols, pitch, dev.ptr(order), out.data_ptr(), _s())
     return out
+
+
+def row_order(indptr, ids, n_rows: int, lo: int, span: int, lut, rank, n_ranked: int):
+    """Device int32 permutation: the row at each bit position, rows sorted by the key of their ranked columns
+    (bit 63 - r for a column of rank r < ``n_ranked`` <= 64). Replaces ``gpu/input_layer.py:408-431``."""
+    order = dev.empty(max(n_rows, 1), "int32")
+    ws_n = _lib.ws_bytes("etm_row_order_ws", n_rows)
+    ws = dev.empty((ws_n + 7) // 8, "int64")
+    call("etm_row_order", indptr.data_ptr(), ids.data_ptr() if ids.numel() else indptr.data_ptr(), _id_bytes(ids),
+         n_rows, lo, span, lut.data_ptr(), rank.data_ptr(), n_ranked, order.data_ptr(), ws.data_ptr(), ws_n, _s())
+    return order[:n_rows]
+
+
+def frequent_first_ranks(col_counts: np.ndarray, n_ranked: int = 64) -> np.ndarray:
+    """Host int32 rank of each column for ``row_order``: the ``n_ranked`` columns of most occurrences get ranks
+    0.. in descending count order (ties by column), the others -1 (``gpu/input_layer.py:516-517``)."""
+    order = np.argsort(-np.asarray(col_counts, dtype=np.int64), kind="stable")[:n_ranked]
+    rank = np.full(len(col_counts), -1, dtype=np.int32)
+    rank[order] = np.arange(len(order), dtype=np.int32)
+    return rank
+
+
+@dataclass
+class Occupancy:
+    """Tile occupancy: bit t % 64 of ``bits[c, t // 64]`` is set when column c holds a row in tile t."""
+
+    bits: object  # device int64 (n_cols, pitch) holding uint64 words
+    pitch: int
+
+
+def tile_occupancy(bv, n_words: int, pitch: int | None = None) -> Occupancy:
+    """The 64-word tile occupancy of (n_cols, pitch) bitvectors (reference ``gpu/kernels/occupancy.py:37-58``)."""
+    n_cols = int(bv.shape[0])
+    pitch = int(bv.shape[1]) if pitch is None else int(pitch)
+    tiles = -(-int(n_words) // _lib.OCC_TILE_WORDS)
+    occ_pitch = -(-tiles // 64) + 1
+    bits = dev.empty((n_cols, occ_pitch), "int64")
+    call("etm_tile_occupancy", bv.data_ptr(), pitch, n_words, n_cols, bits.data_ptr(), occ_pitch, _s())
+    return Occupancy(bits, occ_pitch)
 
 
 def column_popcounts(bv, n_words: int | None = None, pitch: int | None = None):
@@ -260,10 +303,12 @@ def level_deduce(prev, older2, older3, n_total: int, min_count: int, groups: Gro
 
 
 def count_level(bv, n_words: int, rows, m: int, groups: Groups, status, counts, layout: int = LAYOUT_LEX,
-                word_chunk: int = 0, pitch: int | None = None, kernel: str = "auto") -> None:
+                word_chunk: int = 0, pitch: int | None = None, kernel: str = "auto",
+                occ: Occupancy | None = None) -> None:
     """Adds the AND + popcount counts of the counted candidates into ``counts``.
 
-    ``kernel`` ("auto", "valu", "mfma") picks the K=2 pair space's kernel (ignored for other levels)."""
+    ``kernel`` ("auto", "valu", "mfma") picks the K=2 pair space's kernel (ignored for other levels). With
+    ``occ`` (``tile_occupancy`` of ``bv``) the group kernels skip the tiles where their AND is zero."""
     if kernel not in ("auto", "valu", "mfma"):
         raise ValueError(f"count_level: kernel must be 'auto', 'valu' or 'mfma', got {kernel!r}")
     pitch = int(bv.shape[1]) if pitch is None else int(pitch)
@@ -271,7 +316,7 @@ def count_level(bv, n_words: int, rows, m: int, groups: Groups, status, counts, 
     call("etm_count_level", bv.data_ptr(), pitch, n_words, rows.data_ptr(), m, groups.group_start.data_ptr(),
          groups.cum_pairs.data_ptr(), groups.cum_tiles.data_ptr(), groups.n_groups, groups.n_tile_pairs,
          groups.small_list.data_ptr(), cls, dev.ptr(status), counts.data_ptr(), layout, word_chunk,
-         _lib.K2_KERNELS[kernel], _s())
+         _lib.K2_KERNELS[kernel], dev.ptr(occ.bits) if occ is not None else None, occ.pitch if occ else 0, _s())
 
 
 def level_compact(rows, m: int, groups: Groups, status, counts, min_count: int, with_cand: bool = False, gid=None):

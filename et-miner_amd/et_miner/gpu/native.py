@@ -39,14 +39,17 @@ class NativeResult:
         return pa.table(cols)
 
 
-def _opts(min_count, max_length, prune, infer, time_kernels, k2_kernel="auto", k3_kernel="auto") -> _lib.MineOpts:
+def _opts(min_count, max_length, prune, infer, time_kernels, k2_kernel="auto", k3_kernel="auto",
+          row_order="auto") -> _lib.MineOpts:
     if k2_kernel not in _lib.K2_KERNELS:
         raise ValueError(f"k2_kernel must be one of {sorted(_lib.K2_KERNELS)}, got {k2_kernel!r}")
     if k3_kernel not in _lib.K3_KERNELS:
         raise ValueError(f"k3_kernel must be one of {sorted(_lib.K3_KERNELS)}, got {k3_kernel!r}")
+    if row_order not in _lib.ROW_ORDERS:
+        raise ValueError(f"row_order must be one of {sorted(_lib.ROW_ORDERS)}, got {row_order!r}")
     return _lib.MineOpts(int(max(1, min_count)), int(max_length or 0), int(bool(prune)), int(bool(infer)),
                          int(bool(time_kernels)), _lib.K2_KERNELS[k2_kernel], _lib.K3_KERNELS[k3_kernel],
-                         _lib.deduction_setting())
+                         _lib.deduction_setting(), _lib.ROW_ORDERS[row_order])
 
 
 def _collect(handle, levels: bool = True) -> NativeResult:
@@ -90,17 +93,19 @@ def _collect(handle, levels: bool = True) -> NativeResult:
 
 def mine_csr_device(indptr_d, ids_d, n_rows: int, min_count: int, max_length=None, prune_non_free=False,
                     infer_counts=False, time_kernels=False, levels: bool = True,
-                    k2_kernel: str = "auto", k3_kernel: str = "auto") -> NativeResult:
+                    k2_kernel: str = "auto", k3_kernel: str = "auto", row_order: str = "auto") -> NativeResult:
     """Every frequent itemset of a device CSR of item ids (int32 or int64), mined by the native loop.
 
     ``k2_kernel``: "auto", "valu", "mfma" or "rows"; ``k3_kernel``: "auto", "valu" (bitvector kernels) or
-    "rows" (row-wise, which needs rows of strictly ascending distinct ids)."""
+    "rows" (row-wise, which needs rows of strictly ascending distinct ids); ``row_order``: "lex" (rows sorted by
+    their most frequent items, with the tile skip), "keep", or "auto" (lex for inputs of at least 2M rows
+    holding under 1% of the frequent items per row, when levels past K=3 may run)."""
     torch = dev.torch()
     id_bytes = 8 if ids_d.dtype == torch.int64 else 4
     if ids_d.dtype not in (torch.int64, torch.int32):
         raise ValueError(f"item ids must be int32 or int64, got {ids_d.dtype}")
     handle = ctypes.c_void_p()
-    opts = _opts(min_count, max_length, prune_non_free, infer_counts, time_kernels, k2_kernel, k3_kernel)
+    opts = _opts(min_count, max_length, prune_non_free, infer_counts, time_kernels, k2_kernel, k3_kernel, row_order)
     _lib.call("etm_mine_csr", indptr_d.data_ptr(), ids_d.data_ptr() if ids_d.numel() else None, id_bytes, n_rows,
               ids_d.numel(), SPAN_LIMIT, ctypes.byref(opts), ctypes.byref(handle), dev.stream_ptr())
     return _collect(handle, levels)

@@ -60,6 +60,8 @@ enum { ETM_KERNEL_AUTO = 0, ETM_KERNEL_VALU = 1, ETM_KERNEL_MFMA = 2, ETM_KERNEL
 /* etm_mine_opts.deduction: the bounds are on by default in the inference mode, as the reference's
  * ET_MINER_DEDUCTION unset or "1" (gpu/row_split.py:627); OFF is its "0". */
 enum { ETM_DEDUCTION_AUTO = 0, ETM_DEDUCTION_OFF = 1 };
+/* etm_mine_opts.row_order: the reference's row_order ("keep" / "lex", gpu/input_layer.py:469-529). */
+enum { ETM_ROW_ORDER_AUTO = 0, ETM_ROW_ORDER_KEEP = 1, ETM_ROW_ORDER_LEX = 2 };
 
 /* ABI version of this header (bumped on any signature or struct-meaning change). */
 #define ETM_ABI_VERSION 3
@@ -81,13 +83,36 @@ int etm_ids_hist(const void* d_ids, int id_bytes, int64_t n, int64_t lo, int64_t
                  void* stream);
 
 /* Column bitvectors from a CSR of item ids: entry e of row r sets column d_lut[id - lo] (skipped when the
- * id is outside [lo, lo+span) or the lut holds -1). Writes every word of the (n_cols, pitch) output,
+ * id is outside [lo, lo+span) or the lut holds -1). Bit position p holds row d_order[p] (a permutation of the
+ * rows, from etm_row_order), or row p when d_order is NULL. Writes every word of the (n_cols, pitch) output,
  * padding included. No atomics on HBM: each workgroup owns a 64-word x column-slab tile.
- * Replaces csr_to_bitvec / ids_to_bitvec (gpu/kernels/_src/csr_to_bitvec.cu:8-73) and
+ * Replaces csr_to_bitvec / ids_to_bitvec (gpu/kernels/_src/csr_to_bitvec.cu:8-73, with new_pos) and
  * build_bitvecs_gpu (gpu/csr_bitvec.py:63-155). */
 int etm_bitvec_build(const int64_t* d_indptr, const void* d_ids, int id_bytes, int64_t n_rows, int64_t lo,
-                     int64_t span, const int32_t* d_lut, int32_t n_cols, int64_t pitch, uint64_t* d_out,
-                     void* stream);
+                     int64_t span, const int32_t* d_lut, int32_t n_cols, int64_t pitch, const int32_t* d_order,
+                     uint64_t* d_out, void* stream);
+
+/* Words of tile occupancy: a tile is ETM_OCC_TILE_WORDS words (4,096 rows). */
+#define ETM_OCC_TILE_WORDS 64
+
+/* Workspace bytes etm_row_order needs for n_rows rows. */
+size_t etm_row_order_ws(int64_t n_rows);
+
+/* Row order: d_order[p] = the row placed at bit position p. Each row's key has bit 63 - r set for every column
+ * of rank r < n_ranked it holds (d_rank[column], -1 for unranked columns; ids through d_lut as in
+ * etm_bitvec_build); rows are sorted by ascending key, stably. Counts do not depend on the order; it gathers the
+ * rows that share ranked columns, which the tile skip of etm_count_level exploits.
+ * Replaces row_order_keys and the argsorts of gpu/input_layer.py:408-431 (kernels/_src/csr_to_bitvec.cu:79-101). */
+int etm_row_order(const int64_t* d_indptr, const void* d_ids, int id_bytes, int64_t n_rows, int64_t lo,
+                  int64_t span, const int32_t* d_lut, const int32_t* d_rank, int32_t n_ranked, int32_t* d_order,
+                  void* d_ws, size_t ws_bytes, void* stream);
+
+/* Tile occupancy of (n_cols, pitch) bitvectors: bit t % 64 of d_occ[c * occ_pitch + t / 64] is set when column
+ * c has a nonzero word in tile t (words [64t, 64t + 64)). occ_pitch >= ceil(tiles / 64) + 1; the words past
+ * the last tile are written as zero. Replaces tile_occupancy (gpu/kernels/occupancy.py:37-58), with 64-word
+ * tiles: one wave64 pass per tile. */
+int etm_tile_occupancy(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int32_t n_cols, uint64_t* d_occ,
+                       int64_t occ_pitch, void* stream);
 
 /* K=1: d_out[c] = popcount of column c over words [0, n_words).
  * Replaces column_popcounts (gpu/kernels/loader.py:218-257). */
@@ -162,15 +187,18 @@ int etm_level_deduce(const int32_t* d_rows, int32_t m, const int32_t* d_prev_cou
  * suffixes run one wave per (group, word chunk) with lanes over words, one kernel instance per size class
  * (h_class_counts[4]: the class sizes etm_level_groups reported in d_totals[8..11]); larger groups run
  * LDS-tiled tile-pairs. layout is ETM_LAYOUT_LEX, or ETM_LAYOUT_TRI for a single-group level (K=2 in the reference's
- * b(b-1)/2 + a order). word_chunk <= 0 picks a chunk from the level size.
+ * b(b-1)/2 + a order). word_chunk <= 0 picks a chunk from the level size. With d_occ (etm_tile_occupancy of
+ * the same bitvectors) the group kernels skip the tiles where a prefix column, or all but one suffix column,
+ * is empty; chunks are then at most 4,096 words.
  * Replaces count_k3plus_dense, count_group_pairs and count_shared_tiled_dense
- * (gpu/kernels/_src/k3plus_dense.cu:2-106, group_pairs.cu:31-217, shared_tiled.cu:217-283) and
- * count_pairs_k2_dense (pairs_k2_dense.cu:2-45). */
+ * (gpu/kernels/_src/k3plus_dense.cu:2-106 with its occupancy skip :78-83, group_pairs.cu:31-217,
+ * shared_tiled.cu:217-283) and count_pairs_k2_dense (pairs_k2_dense.cu:2-45). */
 int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const int32_t* d_rows, int32_t m,
                     const int64_t* d_group_start, const int64_t* d_cum_pairs, const int64_t* d_cum_tiles,
                     int64_t n_groups, int64_t n_tile_pairs, const int32_t* d_small_list,
                     const int64_t* h_class_counts, const uint8_t* d_status, int32_t* d_counts, int32_t layout,
-                    int32_t word_chunk, int32_t kernel, void* stream);
+                    int32_t word_chunk, int32_t kernel, const uint64_t* d_occ, int64_t occ_pitch,
+                    void* stream);
 
 /* Workspace bytes etm_level_compact needs for n_cands candidates. */
 size_t etm_level_compact_ws(int64_t n_cands);
@@ -267,6 +295,9 @@ typedef struct etm_mine_opts {
     int32_t k2_kernel;      /* ETM_KERNEL_* for the K=2 pair space (ROWS: row-wise) */
     int32_t k3_kernel;      /* ETM_KERNEL_AUTO, ETM_KERNEL_VALU (bitvector kernels) or ETM_KERNEL_ROWS for K=3 */
     int32_t deduction;      /* ETM_DEDUCTION_AUTO: the deduction bounds with infer_counts (k <= 24); ETM_DEDUCTION_OFF */
+    int32_t row_order;      /* etm_mine_csr: ETM_ROW_ORDER_LEX (rows sorted by their 64 most frequent columns,
+                               and the tile skip), ETM_ROW_ORDER_KEEP, or ETM_ROW_ORDER_AUTO (LEX for inputs of
+                               >= 2M rows holding < 1% of the frequent columns per row, levels past K=3 possible) */
 } etm_mine_opts;
 
 /* Per-level statistics: candidates generated by the prefix join and, of them, counted / inferred / skipped by

@@ -133,3 +133,12 @@ def test_deduction_switch(monkeypatch):
     monkeypatch.setenv("ET_MINER_DEDUCTION", "2")
     with pytest.raises(ValueError, match="ET_MINER_DEDUCTION"):
         _lib.deduction_setting()
+
+
+def test_frequent_first_ranks():
+    """Ranks 0.. go to the columns of most occurrences, ties by column; past n_ranked, -1."""
+    from et_miner.gpu.kernels import frequent_first_ranks
+
+    r = frequent_first_ranks(np.array([5, 1, 9, 1, 3]), n_ranked=3)
+    assert r.tolist() == [1, -1, 0, -1, 2]
+    assert frequent_first_ranks(np.array([2, 2]), n_ranked=64).tolist() == [0, 1]
