@@ -74,7 +74,7 @@ struct SmallUnroll {
     static constexpr int U = S <= 4 ? 4 : (S <= 8 ? 2 : 1);
 };
 
-template <int S>
+template <int S, bool SKIP>
 __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
                                                       int64_t chunk_words, int64_t n_chunks,
                                                       const int32_t* __restrict__ rows, int32_t m,
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict_
             }
         }
     };
-    if (occ == nullptr) {
+    if constexpr (!SKIP) {
         for (int64_t w0 = w_begin + lane; w0 < w_end; w0 += 64 * U) {
             bool in[U];
             int64_t w[U];
@@ -573,8 +573,12 @@ hipError_t launch_small(const uint64_t* bv, int64_t pitch, int64_t n_words, int6
     const int64_t n_chunks = ceil_div(n_words, chunk);
     const int64_t blocks = kXcds * ceil_div(xcd_units(n_list, n_chunks), 4);
     if (blocks > 0x7FFFFFFF) return hipErrorInvalidConfiguration;
-    k_count_small<S><<<(unsigned)blocks, 256, 0, s>>>(bv, pitch, n_words, chunk, n_chunks, rows, m, gs, cp, list,
-                                                      n_list, status, counts, layout, occ, occ_pitch);
+    if (occ)
+        k_count_small<S, true><<<(unsigned)blocks, 256, 0, s>>>(bv, pitch, n_words, chunk, n_chunks, rows, m, gs, cp,
+                                                                list, n_list, status, counts, layout, occ, occ_pitch);
+    else
+        k_count_small<S, false><<<(unsigned)blocks, 256, 0, s>>>(bv, pitch, n_words, chunk, n_chunks, rows, m, gs,
+                                                                 cp, list, n_list, status, counts, layout, nullptr, 0);
     return hipGetLastError();
 }
 
