@@ -10,6 +10,7 @@
 // (hipMallocAsync / hipFreeAsync), released when the call returns.
 #include <algorithm>
 #include <cstring>
+#include <exception>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -91,8 +92,11 @@ class Arena {
             h_.demand = 0;
         }
     }
-    // The outermost scope ends after the call's final synchronization: fold overflow blocks into one block.
+    // The outermost scope ends after the call's final synchronization: fold overflow blocks into one block. A
+    // call that fails part-way synchronizes here first, so that no queued copy or kernel outlives the host
+    // buffers and arena memory it uses.
     ~Arena() {
+        if (std::uncaught_exceptions() > 0) (void)hipStreamSynchronize(s_);
         if (--h_.depth > 0) return;
         if (h_.blocks.size() > 1) {
             (void)hipStreamSynchronize(s_);
@@ -358,11 +362,14 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
     prev.counts = a.get<int32_t>(n1);
     check(hipMemcpyAsync(prev.rows, k1_rows.data(), n1 * 4, hipMemcpyHostToDevice, s), "upload K=1");
     check(hipMemcpyAsync(prev.counts, k1_counts.data(), n1 * 4, hipMemcpyHostToDevice, s), "upload K=1 counts");
+    // Host sources of asynchronous uploads live until the call's end: a copy from pageable memory may still
+    // read its source after hipMemcpyAsync returns.
+    std::vector<uint8_t> k1_free;
     if (track_free) {
-        std::vector<uint8_t> f(n1);
-        for (int64_t i = 0; i < n1; ++i) f[i] = k1_counts[i] < n_rows ? 1 : 0;
+        k1_free.resize((size_t)n1);
+        for (int64_t i = 0; i < n1; ++i) k1_free[(size_t)i] = k1_counts[(size_t)i] < n_rows ? 1 : 0;
         prev.free = a.get<uint8_t>(n1);
-        check(hipMemcpyAsync(prev.free, f.data(), n1, hipMemcpyHostToDevice, s), "upload K=1 free");
+        check(hipMemcpyAsync(prev.free, k1_free.data(), n1, hipMemcpyHostToDevice, s), "upload K=1 free");
     }
     GroupsDev g = alloc_groups(a, n1);
     int64_t* scal = a.get<int64_t>(kScal);
