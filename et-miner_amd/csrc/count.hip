@@ -100,16 +100,38 @@ __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict_
     const int64_t cbase = cum_pairs[g];
     const int npairs = s * (s - 1) / 2;
 
-    // Which of the group's candidates are counted (lex order within the group).
-    bool any_live;
-    if (status == nullptr) {
-        any_live = true;
-    } else {
+    // The group's counted pairs in the unrolled (i, j) order (bit a of live[a / 64]; wave-uniform). Pairs that
+    // are not counted (past the group's size, or skipped / inferred by the subset test) cost no VALU work and
+    // no reduction below, and a suffix column in no counted pair is not read.
+    constexpr int NL = (NP + 63) / 64;
+    uint64_t live[NL];
+    uint32_t used = 0;
+    bool any_live = false;
+#pragma unroll
+    for (int h = 0; h < NL; ++h) {
+        const int a = h * 64 + lane;
         bool lv = false;
-        for (int p = lane; p < npairs; p += 64) lv = lv || status[cbase + p] == ETM_CAND_COUNT;
-        any_live = __ballot(lv) != 0;
+        if (a < NP) {
+            int i = 0, rem = a;
+            while (rem >= S - 1 - i) {
+                rem -= S - 1 - i;
+                ++i;
+            }
+            const int j = i + 1 + rem;
+            lv = j < s && (status == nullptr || status[cbase + pair_slot(i, j, s, layout)] == ETM_CAND_COUNT);
+        }
+        live[h] = __ballot(lv);
+        any_live = any_live || live[h] != 0;
     }
     if (!any_live) return;
+    {
+        int a = 0;
+#pragma unroll
+        for (int i = 0; i < S; ++i)
+#pragma unroll
+            for (int j = i + 1; j < S; ++j, ++a)
+                if ((live[a >> 6] >> (a & 63)) & 1ull) used |= (1u << i) | (1u << j);
+    }
 
     const uint64_t* scol[S];
 #pragma unroll
@@ -140,7 +162,7 @@ __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict_
 #pragma unroll
         for (int t = 0; t < S; ++t)
 #pragma unroll
-            for (int u = 0; u < U; ++u) sv[u][t] = t < s ? (scol[t][w[u]] & pre[u]) : 0ull;
+            for (int u = 0; u < U; ++u) sv[u][t] = ((used >> t) & 1u) ? (scol[t][w[u]] & pre[u]) : 0ull;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             int a = 0;
@@ -148,7 +170,7 @@ __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict_
             for (int i = 0; i < S; ++i) {
 #pragma unroll
                 for (int j = i + 1; j < S; ++j) {
-                    acc[a] = popc_acc(sv[u][i] & sv[u][j], acc[a]);
+                    if ((live[a >> 6] >> (a & 63)) & 1ull) acc[a] = popc_acc(sv[u][i] & sv[u][j], acc[a]);
                     ++a;
                 }
             }
@@ -206,10 +228,9 @@ __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict_
     for (int i = 0; i < S; ++i) {
 #pragma unroll
         for (int j = i + 1; j < S; ++j) {
-            const int tot = etm::wave_sum(acc[a]);
-            if (j < s && lane == (a & 63)) {
-                const int64_t slot = pair_slot(i, j, s, layout);
-                if (status == nullptr || status[cbase + slot] == ETM_CAND_COUNT) atomicAdd(&counts[cbase + slot], tot);
+            if ((live[a >> 6] >> (a & 63)) & 1ull) {
+                const int tot = etm::wave_sum(acc[a]);
+                if (lane == (a & 63)) atomicAdd(&counts[cbase + pair_slot(i, j, s, layout)], tot);
             }
             ++a;
         }
@@ -557,11 +578,15 @@ __global__ void k_zero_i64n(int64_t* p, int64_t n) {
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Word chunk (a multiple of `align` words) giving about `target` work items over `units` independent units.
+// The chunk count rounds up to a multiple of the 8 XCDs where the columns allow it: xcd_unit then keeps each
+// XCD on its own chunks, whose words its L2 holds; with fewer chunks every XCD reads every word.
 int64_t pick_chunk(int64_t n_words, int64_t units, int64_t target, int64_t align, int64_t min_words) {
     if (n_words <= 0) return align;
     int64_t chunks = units > 0 ? ceil_div(target, units) : 1;
     int64_t max_chunks = std::max<int64_t>(1, n_words / std::max<int64_t>(min_words, align));
     chunks = std::max<int64_t>(1, std::min(chunks, max_chunks));
+    const int64_t aligned = ceil_div(chunks, kXcds) * kXcds;
+    if (aligned <= max_chunks) chunks = aligned;
     return ceil_div(ceil_div(n_words, chunks), align) * align;
 }
 
