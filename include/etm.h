@@ -85,7 +85,7 @@ int etm_ids_hist(const void* d_ids, int id_bytes, int64_t n, int64_t lo, int64_t
 /* Column bitvectors from a CSR of item ids: entry e of row r sets column d_lut[id - lo] (skipped when the
  * id is outside [lo, lo+span) or the lut holds -1). Bit position p holds row d_order[p] (a permutation of the
  * rows, from etm_row_order), or row p when d_order is NULL. Writes every word of the (n_cols, pitch) output,
- * padding included. No atomics on HBM: each workgroup owns a 64-word x column-slab tile.
+ * padding included. No atomics on HBM: each workgroup owns an 8-word x 1,024-column tile in LDS.
  * Replaces csr_to_bitvec / ids_to_bitvec (gpu/kernels/_src/csr_to_bitvec.cu:8-73, with new_pos) and
  * build_bitvecs_gpu (gpu/csr_bitvec.py:63-155). */
 int etm_bitvec_build(const int64_t* d_indptr, const void* d_ids, int id_bytes, int64_t n_rows, int64_t lo,

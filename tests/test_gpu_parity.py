@@ -64,13 +64,17 @@ def test_bitvec_known_answer():
     assert not bv[:, 1:].any()
 
 
-@pytest.mark.parametrize("n_rows", [1, 63, 64, 65, 4095, 4096, 4097, 20_000])
-def test_bitvec_build_and_popcount_match_oracle(n_rows):
+@pytest.mark.parametrize("n_rows,vocab", [(1, 300), (63, 300), (64, 300), (65, 300), (4095, 300), (4096, 300),
+                                          (4097, 300), (20_000, 300), (1, 700), (513, 700), (20_000, 700),
+                                          (65, 5000), (4097, 5000), (20_000, 5000)])
+def test_bitvec_build_and_popcount_match_oracle(n_rows, vocab):
+    """Every other item of the vocabulary gets a column: about 150 or 350 columns (one LDS tile of 1,024
+    columns), or about 2,000 (two LDS tiles, the last partial)."""
     from et_miner.gpu import device as dev
     from et_miner.gpu import kernels as K
 
-    rng = np.random.default_rng(n_rows)
-    rows = _rand_rows(rng, n_rows, 300, 6, dup=True)
+    rng = np.random.default_rng(n_rows + vocab)
+    rows = _rand_rows(rng, n_rows, vocab, 6, zipf=0.8, dup=True)
     indptr, ids = _csr(rows)
     ids = ids + 1000  # a nonzero id base
     lo, span = int(ids.min()), int(ids.max() - ids.min() + 1)

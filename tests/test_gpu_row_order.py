@@ -95,6 +95,25 @@ def test_order_is_a_stable_key_sort_and_bitvectors_follow_it(n_rows, n_ranked):
     assert not bv[:, want.shape[1]:].any()
 
 
+def test_ordered_build_of_a_wide_column_set():
+    """About 1,500 columns (the LDS-tile builder, two tiles) through an order: the oracle's bitvectors of the
+    permuted rows."""
+    from et_miner.gpu import device as dev
+    from et_miner.gpu import kernels as K
+
+    rows = _planted_rows(4, 9_001, vocab=3000, mean=8, dup_every=3)
+    indptr, ids, n, d_ptr, d_ids, lo, span, lut, cols, rank = _device_input(rows)
+    assert len(cols) > 1024
+    d_lut = dev.to_device(lut)
+    order = dev.to_host(K.row_order(d_ptr, d_ids, n, lo, span, d_lut, dev.to_device(rank), 64))
+    bv = dev.to_host(K.build_bitvecs(d_ptr, d_ids, n, lo, span, d_lut, len(cols),
+                                     order=dev.to_device(order.astype(np.int32))))
+    p_indptr, p_ids = _csr([rows[i] for i in order])
+    want = miner.bitvectors(p_indptr, lut[p_ids - lo], len(cols))
+    assert np.array_equal(bv[:, : want.shape[1]].view(np.uint64), want)
+    assert not bv[:, want.shape[1]:].any()
+
+
 def test_occupancy_and_skip_keep_every_count():
     from et_miner.gpu import device as dev
     from et_miner.gpu import kernels as K
