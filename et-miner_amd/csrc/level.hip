@@ -19,7 +19,9 @@ __device__ __forceinline__ int32_t row_item(const int32_t* __restrict__ rows, in
     return rows[r * (int64_t)m + t];
 }
 
-// Row r starts a prefix group: r == 0, or its first m-1 items differ from row r-1's.
+// Row r starts a prefix group: r == 0, or its first m-1 items differ from row r-1's. Branch-free over the
+// items, four pairs of loads at a time: an early exit per item would make each load wait on the previous
+// comparison (m-1 dependent round trips on deep levels).
 struct IsGroupStart {
     const int32_t* rows;
     int32_t m;
@@ -27,9 +29,15 @@ struct IsGroupStart {
         if (r == 0) return 1;
         const int32_t* a = rows + r * (int64_t)m;
         const int32_t* b = a - m;
-        for (int t = 0; t < m - 1; ++t)
-            if (a[t] != b[t]) return 1;
-        return 0;
+        int32_t d = 0;
+        int t = 0;
+        for (; t + 4 <= m - 1; t += 4) {
+            const int32_t a0 = a[t], a1 = a[t + 1], a2 = a[t + 2], a3 = a[t + 3];
+            const int32_t b0 = b[t], b1 = b[t + 1], b2 = b[t + 2], b3 = b[t + 3];
+            d |= (a0 ^ b0) | (a1 ^ b1) | (a2 ^ b2) | (a3 ^ b3);
+        }
+        for (; t < m - 1; ++t) d |= a[t] ^ b[t];
+        return d != 0;
     }
 };
 
