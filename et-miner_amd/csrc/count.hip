@@ -207,16 +207,18 @@ __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict_
             one |= x;
         }
         tiles &= two;
-        while (tiles) {
+        // The plain loop's shape (U tiles per step, lane-strided words), skipping a step whose U tiles are all
+        // empty and masking the empty ones of the rest.
+        for (int64_t w0 = w_begin + lane; w0 < w_end; w0 += 64 * U) {
+            const int rel = (int)((w0 - lane - w_begin) >> 6);
+            const uint64_t sel = (tiles >> rel) & ((1ull << U) - 1);
+            if (sel == 0) continue;
             bool in[U];
             int64_t w[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                in[u] = tiles != 0;
-                const int t = in[u] ? __builtin_ctzll(tiles) : 0;
-                tiles &= tiles - 1;
-                w[u] = in[u] ? w_begin + t * 64 + lane : w_begin;
-                in[u] = in[u] && w[u] < w_end;
+                w[u] = w0 + 64 * u;
+                in[u] = w[u] < w_end && ((sel >> u) & 1ull);
                 if (!in[u]) w[u] = w_begin;
             }
             step(w, in);

@@ -310,10 +310,6 @@ struct RowsSrc {
 // suggest: wide_vocab_2m's K=2 (r = 9e-4) measured 10.5 ms row-wise against 9.0 ms on the MFMA Gram. The
 // rows are built only when a level may take them: sum C(len, 2) >= (E^2/N - E)/2 for E K=1 entries over N rows
 // (Cauchy-Schwarz) must leave the rule open.
-// The automatic row order: inputs of at least kOrderMinRows rows whose rows hold on average less than
-// kOrderMaxDensity of the frequent columns.
-constexpr int64_t kOrderMinRows = 2000000;
-constexpr double kOrderMaxDensity = 0.01;
 constexpr double kK2RowsMaxR = 2.0e-4;
 constexpr double kK3RowsMaxR = 2.5e-4;
 constexpr int64_t kK3RowsLongest = 256;
@@ -756,17 +752,10 @@ extern "C" int etm_mine_csr(const int64_t* d_indptr, const void* d_ids, int id_b
         const int64_t pitch = std::max<int64_t>(64, (n_words + 63) / 64 * 64);
         // Row order (ETM_ROW_ORDER_LEX): rows sorted by the 64 most frequent columns they hold, most frequent
         // first, so that rows sharing frequent columns gather into few 64-word tiles and the group kernels skip
-        // the tiles where a prefix is empty. The automatic choice orders large sparse inputs that may run levels
-        // past K=3: deep_sparse_20m (20M rows, 0.5% of the frequent columns per row) measured 80.0 -> 70.5 ms
-        // per step, deep_k_1m (1M rows, 4.5%) 3.16 -> 3.58 ms (its tiles stay occupied; the sort and the
-        // permuted build are pure cost).
-        double freq_entries = 0;
-        for (int32_t c = 0; c < n_cols; ++c) freq_entries += (double)h_hist[res->col_to_item[(size_t)c] - lo];
-        const double density = freq_entries / ((double)n_rows * (double)n_cols);
-        const bool deep = opts->max_length <= 0 || opts->max_length >= 4;
-        const bool ordered = opts->row_order == ETM_ROW_ORDER_LEX ||
-                             (opts->row_order == ETM_ROW_ORDER_AUTO && deep && n_rows >= kOrderMinRows &&
-                              density < kOrderMaxDensity);
+        // the tiles where a prefix is empty. The automatic choice keeps the rows: on deep_sparse_20m the skip
+        // saves 34.2 -> 30.4 ms of K>=3 counting but the sort and the permuted build cost as much (62.5 ms
+        // ordered, 63.2 ms kept), and on deep_k_1m the tiles stay occupied (3.16 -> 3.58 ms ordered).
+        const bool ordered = opts->row_order == ETM_ROW_ORDER_LEX;
         int32_t* order = nullptr;
         if (ordered) {
             std::vector<int32_t> by_count((size_t)n_cols);

@@ -98,8 +98,7 @@ def mine_csr_device(indptr_d, ids_d, n_rows: int, min_count: int, max_length=Non
 
     ``k2_kernel``: "auto", "valu", "mfma" or "rows"; ``k3_kernel``: "auto", "valu" (bitvector kernels) or
     "rows" (row-wise, which needs rows of strictly ascending distinct ids); ``row_order``: "lex" (rows sorted by
-    their most frequent items, with the tile skip), "keep", or "auto" (lex for inputs of at least 2M rows
-    holding under 1% of the frequent items per row, when levels past K=3 may run)."""
+    their most frequent items, with the tile skip), "keep", or "auto" (keeps the rows)."""
     torch = dev.torch()
     id_bytes = 8 if ids_d.dtype == torch.int64 else 4
     if ids_d.dtype not in (torch.int64, torch.int32):

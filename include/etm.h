@@ -296,8 +296,7 @@ typedef struct etm_mine_opts {
     int32_t k3_kernel;      /* ETM_KERNEL_AUTO, ETM_KERNEL_VALU (bitvector kernels) or ETM_KERNEL_ROWS for K=3 */
     int32_t deduction;      /* ETM_DEDUCTION_AUTO: the deduction bounds with infer_counts (k <= 24); ETM_DEDUCTION_OFF */
     int32_t row_order;      /* etm_mine_csr: ETM_ROW_ORDER_LEX (rows sorted by their 64 most frequent columns,
-                               and the tile skip), ETM_ROW_ORDER_KEEP, or ETM_ROW_ORDER_AUTO (LEX for inputs of
-                               >= 2M rows holding < 1% of the frequent columns per row, levels past K=3 possible) */
+                               and the tile skip), ETM_ROW_ORDER_KEEP, or ETM_ROW_ORDER_AUTO (keeps the rows) */
 } etm_mine_opts;
 
 /* Per-level statistics: candidates generated by the prefix join and, of them, counted / inferred / skipped by
