@@ -79,8 +79,12 @@ def _validate_route_support(*, streaming, use_gpu, has_bitvecs, n_gpus, profile,
     if anchor_items is not None:
         raise ValueError("anchor_items (two-phase anchored mining) is not part of this build's GPU counting path")
     if output_dir is not None or resume_from_k is not None:
-        raise ValueError("output_dir / resume_from_k (the per-K parquet flush and resume) are not part of this "
-                         "build's GPU counting path")
+        which = " and ".join(n for n, v in (("output_dir", output_dir), ("resume_from_k", resume_from_k))
+                             if v is not None)
+        if streaming and not has_bitvecs:
+            raise ValueError(f"{which} requires the level-loop miner, reached with bitvecs= or with use_gpu=True "
+                             "and streaming=False. On this route the per-K parquet flush does not run: output_dir "
+                             "would stay empty and resume_from_k would silently re-mine from K=1.")
     if memory_budget_gb is not None and not streaming:
         raise ValueError("memory_budget_gb requires streaming=True: it overrides chunk_size for the SON paths "
                          "and is read nowhere else.")
@@ -223,7 +227,7 @@ def apriori(
             raise ValueError("Cannot provide both 'transactions' and 'bitvecs'. Use 'bitvecs' for pre-built GPU "
                              "bitvectors or 'transactions' for DataFrame input.")
         return done(_mine_bitvecs(bitvecs, min_support, max_length, prune_equal_support, use_generator_pruning,
-                                  level_callback, session, warn_complexity))
+                                  level_callback, session, warn_complexity, output_dir, resume_from_k))
     if transactions is None:
         raise ValueError("Either 'transactions' or 'bitvecs' must be provided")
     if streaming:
@@ -239,13 +243,15 @@ def apriori(
     if session:
         session.end_phase(n_transactions=len(indptr) - 1)
     return done(mine_csr(indptr, ids, min_support, max_length, prune_equal_support, use_generator_pruning,
-                         level_callback, session, items=items, warn_complexity=warn_complexity))
+                         level_callback, session, items=items, warn_complexity=warn_complexity,
+                         output_dir=output_dir, resume_from_k=resume_from_k))
 
 
 def mine_csr(indptr, ids, min_support, max_length=None, prune_equal_support=False, use_generator_pruning=False,
              level_callback=None, session=None, items=None, warn_complexity=False, device_csr=None,
              with_counts=False, timer=None, stats_out=None, engine: str = "native", time_kernels=False,
-             k2_kernel: str = "auto", k3_kernel: str = "auto", row_order: str = "auto"):
+             k2_kernel: str = "auto", k3_kernel: str = "auto", row_order: str = "auto", output_dir=None,
+             resume_from_k=None):
     """The use_gpu route from a host CSR (or ``device_csr=(indptr_d, ids_d)`` already in HBM).
 
     ``engine="native"`` runs the whole level loop inside libetm (one C call); ``engine="python"`` drives the
@@ -254,6 +260,11 @@ def mine_csr(indptr, ids, min_support, max_length=None, prune_equal_support=Fals
     "rows") picks the K=2 pair space's kernel: the int8 MFMA Gram, the VALU AND+popcount tiles or the row-wise
     kernel; ``k3_kernel`` ("auto", "valu", "rows") picks K=3's; ``row_order`` ("auto", "keep", "lex") the
     bitvectors' row order (native engine only; counts never depend on it).
+
+    ``output_dir`` writes each level to ``frequent_k{k}.parquet`` there and returns an empty result, as the
+    reference's row-split miner does (``gpu/row_split.py:1172-1178``); with ``resume_from_k`` >= 2 as well, level
+    ``resume_from_k`` is read back from that directory and mining continues above it (the Python level loop;
+    ``gpu/row_split.py:539-606``).
     """
     from ..gpu import device as dev
 
@@ -262,6 +273,10 @@ def mine_csr(indptr, ids, min_support, max_length=None, prune_equal_support=Fals
     if n_rows <= 0:
         return itemset_table([], np.zeros(0, np.int64), 1, with_counts=True) if with_counts else empty_result()
     min_count = _mining_min_count(min_support, n_rows)
+    if output_dir is not None:
+        return _mine_to_dir(indptr, ids, device_csr, n_rows, min_count, min_support, max_length,
+                            prune_equal_support, use_generator_pruning, level_callback, session, items,
+                            output_dir, resume_from_k, stats_out, k2_kernel, k3_kernel, row_order)
     if session:
         session.start_phase("input_layer")
     if device_csr is None:
@@ -317,6 +332,57 @@ def mine_csr(indptr, ids, min_support, max_length=None, prune_equal_support=Fals
     return _frame(levels, col_to_item, n_rows, items)
 
 
+def _mine_to_dir(indptr, ids, device_csr, n_rows, min_count, min_support, max_length, prune_equal_support,
+                 use_generator_pruning, level_callback, session, items, output_dir, resume_from_k, stats_out,
+                 k2_kernel, k3_kernel, row_order):
+    """The output_dir route: every level to parquet (or the levels above a resumed one), an empty result back."""
+    from ..io import flush
+
+    if resume_from_k is None or resume_from_k < 2:
+        table = mine_csr(indptr, ids, min_support, max_length, prune_equal_support, use_generator_pruning,
+                         level_callback, session, items=items, device_csr=device_csr, with_counts=True,
+                         stats_out=stats_out, k2_kernel=k2_kernel, k3_kernel=k3_kernel, row_order=row_order)
+        if items is not None:
+            table = pa.table({"itemset": table.column("itemset"), "support": table.column("support")})
+        flush.write_levels(table, output_dir, n_rows)
+        return empty_result()
+    if items is not None:
+        raise ValueError("resume_from_k needs integer item ids (the flushed levels hold ids)")
+    from ..gpu import device as dev
+    from ..gpu.input_layer import build_device_input, host_csr_to_device
+    from ..gpu.row_split import mine_levels
+
+    if device_csr is None:
+        device_csr = host_csr_to_device(indptr, ids)
+    indptr_d, ids_d = device_csr
+    built = build_device_input(indptr_d, ids_d, n_rows, min_count)
+    if built is None:
+        return empty_result()
+    res = _resume_levels(mine_levels, built.bitvecs, built.n_words, n_rows, built.col_counts, built.col_to_item,
+                         min_count, max_length, prune_equal_support, use_generator_pruning, level_callback, session,
+                         output_dir, resume_from_k, None)
+    if stats_out is not None:
+        stats_out.extend(res.stats)
+    dev.synchronize()
+    return empty_result()
+
+
+def _resume_levels(mine_levels, bv, n_words, n_rows, col_counts, col_to_item, min_count, max_length, prune, infer,
+                   level_callback, session, output_dir, resume_from_k, pitch):
+    """Level ``resume_from_k`` back from ``output_dir``, the levels above it mined and written there."""
+    from ..io import flush
+
+    items_rows, counts = flush.read_level(output_dir, resume_from_k, n_rows)
+    cols, counts = flush.lexsorted(flush.columns_of(items_rows, np.asarray(col_to_item, dtype=np.int64)), counts)
+    res = mine_levels(bv, n_words, n_rows, col_counts, min_count, max_length, prune, infer,
+                      _timed(level_callback, session), pitch=pitch, start=(cols, counts))
+    c2i = np.asarray(col_to_item, dtype=np.int64)
+    levels = [(np.sort(c2i[r], axis=1), c) for r, c in res.levels if len(r)]
+    flush.write_levels(itemset_table(levels, None, n_rows, with_counts=True), output_dir, n_rows,
+                       min_k=resume_from_k + 1)
+    return res
+
+
 def _timed(level_callback, session):
     if level_callback is None and session is None:
         return None
@@ -331,7 +397,7 @@ def _timed(level_callback, session):
 
 
 def _mine_bitvecs(bitvecs, min_support, max_length, prune_equal_support, use_generator_pruning, level_callback,
-                  session, warn_complexity):
+                  session, warn_complexity, output_dir=None, resume_from_k=None):
     """The bitvecs= route: validation as the reference (``core/apriori.py:477-527``), then the level loop."""
     from ..gpu import device as dev
     from ..gpu import kernels as K
@@ -365,6 +431,14 @@ def _mine_bitvecs(bitvecs, min_support, max_length, prune_equal_support, use_gen
     counts = dev.to_host(K.column_popcounts(view, arr.shape[1])).astype(np.int64)
     if warn_complexity:
         _warn_complexity(int((counts >= max(1, min_count)).sum()), min_support)
+    if output_dir is not None and resume_from_k is not None and resume_from_k >= 2:
+        from ..gpu.row_split import mine_levels
+
+        _resume_levels(mine_levels, view, arr.shape[1], n_trans, counts, mapping, max(1, min_count), max_length,
+                       prune_equal_support, use_generator_pruning, level_callback, session, output_dir,
+                       resume_from_k, arr.shape[1])
+        dev.synchronize()
+        return empty_result()
     from ..gpu.native import mine_bitvecs_device
 
     res = mine_bitvecs_device(arr.ptr, arr.shape[1], arr.shape[1], n_trans, counts, max(1, min_count), max_length,
@@ -377,6 +451,13 @@ def _mine_bitvecs(bitvecs, min_support, max_length, prune_equal_support, use_gen
     if not order_ok:
         # Item ids that do not ascend with the columns: map each level, then sort every itemset's items.
         levels = [(np.sort(mapping[rows], axis=1), cnt) for rows, cnt in levels]
+        mapping = None
+    if output_dir is not None:
+        from ..io import flush
+
+        flush.write_levels(itemset_table(levels, mapping, n_trans, with_counts=True), output_dir, n_trans)
+        return empty_result()
+    if mapping is None:
         return to_frame(itemset_table(levels, None, n_trans))
     return _frame(levels, mapping, n_trans, None)
 

@@ -97,18 +97,25 @@ def mine_levels(
     keep_device: bool = False,
     timer=None,
     k2_kernel: str = "auto",
+    start: tuple | None = None,
 ) -> MineResult:
     """Mines every frequent itemset (or free-set) of device bitvectors ``bv`` (n_cols, pitch).
 
     ``col_counts`` are the K=1 counts of the columns (host int64). Counts are exact integers throughout.
     With ``keep_device`` the levels stay device tensors (rows int32, counts int32) instead of host arrays.
     ``timer`` (optional) brackets each level's counting launches: ``timer.begin(k)`` / ``timer.end(k, stats)``
-    are called on the host around them, on the stream they run on.
+    are called on the host around them, on the stream they run on. ``start`` = (rows int32 (n, k0) of column
+    ids, lexsorted; counts int64) resumes from a level a previous run wrote: mining continues at k0 + 1 and the
+    result holds the levels above k0 only. The loaded level's free flags are unknown, so its first successor
+    runs without inference (``gpu/row_split.py:594-601``).
     """
     if n_rows > np.iinfo(np.int32).max:
         raise ValueError(f"n_transactions={n_rows:,} exceeds int32 max: the count arrays are int32")
     out = MineResult()
     n_cols = len(col_counts)
+    if start is not None:
+        return _mine_from(bv, n_words, n_rows, n_cols, min_count, max_length, prune_non_free, infer_counts,
+                          level_callback, pitch, keep_device, timer, k2_kernel, start, out)
     t0 = time.perf_counter()
     freq = np.flatnonzero(col_counts >= min_count).astype(np.int32)
     free1 = col_counts[freq] < n_rows
@@ -126,10 +133,31 @@ def mine_levels(
     prev = _Level(dev.to_device(k1_rows), dev.to_device(k1_counts.astype(np.int32)))
     if track_free:
         prev.free = dev.to_device((k1_counts < n_rows).astype(np.uint8))
+    return _level_loop(bv, n_words, n_rows, min_count, eml, prune_non_free, infer_counts, level_callback,
+                       pitch, keep_device, timer, k2_kernel, prev, 2, [], out, first_host=out.levels[0])
+
+
+def _mine_from(bv, n_words, n_rows, n_cols, min_count, max_length, prune_non_free, infer_counts, level_callback,
+               pitch, keep_device, timer, k2_kernel, start, out) -> MineResult:
+    rows0, counts0 = start
+    rows0 = np.ascontiguousarray(rows0, dtype=np.int32)
+    if rows0.ndim != 2 or rows0.shape[1] < 1 or len(rows0) != len(counts0):
+        raise ValueError("start must be (rows (n, k0) of column ids, counts (n,))")
+    k0 = rows0.shape[1]
+    eml = min(max_length if max_length else math.inf, n_cols)
+    if len(rows0) == 0 or k0 + 1 > eml:
+        return out
+    prev = _Level(dev.to_device(rows0), dev.to_device(np.asarray(counts0, dtype=np.int32)))
+    return _level_loop(bv, n_words, n_rows, min_count, eml, prune_non_free, infer_counts, level_callback, pitch,
+                       keep_device, timer, k2_kernel, prev, k0 + 1, [], out, first_host=None)
+
+
+def _level_loop(bv, n_words, n_rows, min_count, eml, prune_non_free, infer_counts, level_callback, pitch,
+                keep_device, timer, k2_kernel, prev, k, older, out, first_host) -> MineResult:
+    """Levels k, k+1, ... from ``prev`` (level k-1 on the device); ``older`` holds levels k-3, k-2 when known."""
+    track_free = prune_non_free or infer_counts
     dev_levels = []
-    older = []  # levels k-2 and k-3, for the deduction bounds
     deduction = _lib.deduction_setting() == _lib.DEDUCTION_AUTO
-    k = 2
     while k <= eml and prev.n >= k:
         t_k = time.perf_counter()
         m = k - 1
@@ -146,7 +174,7 @@ def mine_levels(
             # one thread per candidate (binary search for the group) on even ones.
             status, counts, gid = K.level_classify(prev.rows, m, prev.counts, prev.free if infer else None, table,
                                                    tsize, groups, mode, tally, group_centric=(k % 2 == 1))
-            if infer and deduction and k <= _lib.BOUNDS_MAX_K:
+            if infer and deduction and k <= _lib.BOUNDS_MAX_K and len(older) >= (1 if k == 3 else 2):
                 older3 = None
                 if k > 3:
                     older3 = (older[-2].rows, older[-2].counts) + older[-2].hashed()
@@ -185,7 +213,8 @@ def mine_levels(
         prev = cur
         k += 1
     if keep_device:
-        out.levels = [(dev.to_device(out.levels[0][0]), dev.to_device(out.levels[0][1]))] + dev_levels
+        head = [(dev.to_device(first_host[0]), dev.to_device(first_host[1]))] if first_host is not None else []
+        out.levels = head + dev_levels
         return out
     for rows_d, counts_d in dev_levels:
         if rows_d.shape[0]:

@@ -74,7 +74,9 @@ def test_min_count_contract():
         (dict(min_support=0.1, streaming=True, prune_equal_support=True, use_gpu=True), ValueError),
         (dict(min_support=0.1, memory_budget_gb=1.0, use_gpu=True), ValueError),
         (dict(min_support=0.1, n_gpus=2, use_gpu=True), ValueError),
-        (dict(min_support=0.1, output_dir="/tmp/x", use_gpu=True), ValueError),
+        (dict(min_support=0.1, output_dir="/tmp/x", use_gpu=True, streaming=True), ValueError),
+        (dict(min_support=0.1, resume_from_k=2, use_gpu=True, streaming=True), ValueError),
+        (dict(min_support=0.1, anchor_items={1}, use_gpu=True), ValueError),
         (dict(min_support=0.1), ValueError),  # the CPU route is not part of this build
     ],
 )
