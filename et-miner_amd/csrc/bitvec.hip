@@ -300,10 +300,12 @@ extern "C" int etm_ids_hist(const void* d_ids, int id_bytes, int64_t n, int64_t 
 }
 
 namespace {
+// Temporary bytes of the key sort (0 when the query fails; the sort itself then reports the error).
 size_t sort_tmp_bytes(int64_t n) {
     size_t tmp = 0;
-    rocprim::radix_sort_pairs(nullptr, tmp, (uint64_t*)nullptr, (uint64_t*)nullptr, (int32_t*)nullptr,
-                              (int32_t*)nullptr, (size_t)n, 0, 64);
+    if (rocprim::radix_sort_pairs(nullptr, tmp, (uint64_t*)nullptr, (uint64_t*)nullptr, (int32_t*)nullptr,
+                                  (int32_t*)nullptr, (size_t)n, 0, 64) != hipSuccess)
+        return 0;
     return tmp;
 }
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
