@@ -56,6 +56,9 @@ def measured_traffic(workload_name: str):
     m = re.search(r"^traffic_bytes_per_step = (\d+)$", TRAFFIC_PROFILE.read_text(), re.M)
     return int(m.group(1)) if m else None
 VALU_CW_PEAK = 9.8e12  # candidate-words/s: 256 CUs x 64 lanes x 2.4 GHz / 4 VALU ops per word
+# candidate-words/s of the int8 MFMA Gram: 5.0 POP/s dense int8 / 128 ops per word (64 bit products, each a
+# multiply and an add)
+MFMA_I8_CW_PEAK = 5.0e15 / 128
 
 
 class EventTimer:
@@ -194,6 +197,11 @@ def main(argv=None) -> int:
     k3_ms, k3_bytes = ksum["k3p"][0] / args.steps, ksum["k3p"][1] / args.steps
     k2_ms, k2_cw = ksum["k2"][0] / args.steps, ksum["k2"][2] / args.steps
     k3_launches = sum(1 for s in per_step if s.k >= 3)
+    # the K=2 kernel family the level ran on ("valu" tiles, int8 "mfma" Gram or the CSR "rows" kernel); the
+    # python engine drives the VALU tiles
+    k2_kernel = next((s.kernel for s in per_step if s.k == 2 and s.kernel), "valu")
+    k2_bound, k2_peak = {"valu": ("valu", VALU_CW_PEAK), "mfma": ("mfma", MFMA_I8_CW_PEAK)}.get(k2_kernel,
+                                                                                               ("hbm", None))
     achieved = (k3_bytes / (k3_ms / 1000) / 1e9) if k3_ms > 0 else 0.0
     print(f"[bench] {args.steps} steps, {ms_per_step:.2f} ms/step, {n_eval:,} candidates/step, "
           f"K>=3 counting {k3_ms:.3f} ms/step over {k3_launches} launches", file=sys.stderr, flush=True)
@@ -278,9 +286,10 @@ def main(argv=None) -> int:
             "kernel_ms_per_step": k3_ms,
             "launches_per_step": k3_launches,
         },
-        "k2_tiled": {"bound": "valu", "kernel_ms_per_step": k2_ms, "candidate_words_per_step": k2_cw,
+        "k2_count": {"kernel": k2_kernel, "bound": k2_bound, "kernel_ms_per_step": k2_ms,
+                     "candidate_words_per_step": k2_cw,
                      "achieved_cw_per_s": (k2_cw / (k2_ms / 1000)) if k2_ms > 0 else 0.0,
-                     "peak_cw_per_s": VALU_CW_PEAK},
+                     "peak_cw_per_s": k2_peak},
         "cpu_baseline": cpu,
         "pcie_inclusive": pcie,
         "served": served,
