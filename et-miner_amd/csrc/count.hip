@@ -285,9 +285,26 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
             }
         }
     }
-    if (tid == 0) s_nlist = 0;
+    // Which of the tile's A and B columns take part in a live pair: in listed mode only those are staged.
+    __shared__ unsigned long long s_use[2];
+    if (tid == 0) {
+        s_nlist = 0;
+        s_use[0] = 0;
+        s_use[1] = 0;
+    }
     __syncthreads();
-    if (my_live) atomicAdd(&s_nlist, my_live);
+    if (my_live) {
+        atomicAdd(&s_nlist, my_live);
+        unsigned long long ua = 0, ub = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if ((live_bits >> (4 * r)) & 0xFu) ua |= 1ull << (tx + 16 * r);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (live_bits & (0x1111u << c)) ub |= 1ull << (ty + 16 * c);
+        atomicOr(&s_use[0], ua);
+        atomicOr(&s_use[1], ub);
+    }
     __syncthreads();
     const int n_live = s_nlist;
     if (n_live == 0) return;
@@ -329,12 +346,16 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
         __syncthreads();
     }
 
+    // A column outside every live pair stages as zeros without a load: a sparsely live tile-pair reads only the
+    // columns it counts (on a diagonal tile-pair both sides of a pair index the A tile).
+    const uint64_t useA = listed ? (diag ? s_use[0] | s_use[1] : s_use[0]) : ~0ull;
+    const uint64_t useB = listed ? s_use[1] : ~0ull;
     if (tid < kT) {
         const int64_t i = ta * kT + tid;
-        colA[tid] = i < s ? (int64_t)rows[(gs + i) * m + np] * pitch : -1;
+        colA[tid] = (i < s && ((useA >> tid) & 1ull)) ? (int64_t)rows[(gs + i) * m + np] * pitch : -1;
     } else if (tid < 2 * kT) {
         const int64_t j = tb * kT + (tid - kT);
-        colB[tid - kT] = j < s ? (int64_t)rows[(gs + j) * m + np] * pitch : -1;
+        colB[tid - kT] = (j < s && ((useB >> (tid - kT)) & 1ull)) ? (int64_t)rows[(gs + j) * m + np] * pitch : -1;
     }
     const int32_t* prow = rows + gs * m;
 
