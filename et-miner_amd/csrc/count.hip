@@ -33,6 +33,7 @@ constexpr int kT = 64;   // suffixes per tile
 constexpr int kKW = 32;  // words per LDS step
 constexpr int kXcds = 8;  // MI355X: 8 XCDs of 32 CUs, each with its own L2
 constexpr int kListMax = 1024;  // live pairs from which a tile-pair is counted densely (4096 slots)
+constexpr int kListWords = 256;  // most words per step of the listed mode
 
 // The (word chunk, item) unit of a workgroup or wave: `xcd` = workgroup index mod 8 (the XCD the hardware
 // dispatches it to, round-robin), `local` = its index among that XCD's units. With at least 8 chunks, XCD x takes
@@ -247,10 +248,15 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
                                                       int64_t n_tile_pairs, int64_t n_chunks,
                                                       const uint8_t* __restrict__ status,
                                                       int32_t* __restrict__ counts, int32_t layout) {
-    __shared__ uint64_t sA[kKW][kT + 1];
-    __shared__ uint64_t sB[kKW][kT + 1];
-    __shared__ uint64_t sP[kKW];
-    __shared__ int64_t colA[kT], colB[kT];
+    // Dense mode: A and B tiles [word][column] of kKW words. Listed mode: the live columns compacted into slots,
+    // [word][slot] of up to kListWords words, in the same buffer.
+    __shared__ uint64_t s_tiles[2 * kKW * (kT + 1)];
+    __shared__ uint64_t sP[kListWords];
+    __shared__ int64_t s_col[2 * kT];
+    uint64_t(*sA)[kT + 1] = reinterpret_cast<uint64_t(*)[kT + 1]>(s_tiles);
+    uint64_t(*sB)[kT + 1] = reinterpret_cast<uint64_t(*)[kT + 1]>(s_tiles + kKW * (kT + 1));
+    int64_t* colA = s_col;
+    int64_t* colB = s_col + kT;
     __shared__ uint8_t s_la[kListMax], s_lb[kListMax];
     __shared__ int s_nlist;
 
@@ -346,18 +352,91 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
         __syncthreads();
     }
 
-    // A column outside every live pair stages as zeros without a load: a sparsely live tile-pair reads only the
-    // columns it counts (on a diagonal tile-pair both sides of a pair index the A tile).
-    const uint64_t useA = listed ? (diag ? s_use[0] | s_use[1] : s_use[0]) : ~0ull;
-    const uint64_t useB = listed ? s_use[1] : ~0ull;
+    const int32_t* prow = rows + gs * m;
+    const int64_t w_begin = chunk_id * chunk_words;
+    const int64_t w_end = min(n_words, w_begin + chunk_words);
+
+    if (listed) {
+        // Only the columns of live pairs are staged, compacted into slots (on a diagonal tile-pair both sides of
+        // a pair index the A tile). With fewer slots a step covers more words: the largest power of two up to
+        // kListWords whose [word][slot] block fits the tile buffer, so a sparsely live tile-pair pays its global
+        // load latencies and barriers over several times more words per step.
+        const uint64_t uA = diag ? (s_use[0] | s_use[1]) : s_use[0];
+        const uint64_t uB = diag ? 0ull : s_use[1];
+        const int nA = __popcll(uA), nS = nA + __popcll(uB);
+        const int stride = nS | 1;  // odd: a wave's 64-bit stores down one slot spread over the banks
+        int lg = 5;
+        while ((1 << (lg + 1)) <= kListWords && (stride << (lg + 1)) <= 2 * kKW * (kT + 1)) ++lg;
+        const int kw = 1 << lg;
+        if (tid < kT) {
+            if ((uA >> tid) & 1ull)
+                s_col[__popcll(uA & ((1ull << tid) - 1))] = (int64_t)rows[(gs + ta * kT + tid) * m + np] * pitch;
+        } else if (tid < 2 * kT) {
+            const int c = tid - kT;
+            if ((uB >> c) & 1ull)
+                s_col[nA + __popcll(uB & ((1ull << c) - 1))] = (int64_t)rows[(gs + tb * kT + c) * m + np] * pitch;
+        }
+        // This thread's list entries tid, tid + 256, ... as slots.
+        int la[kListMax / 256], lb[kListMax / 256], sum[kListMax / 256];
+#pragma unroll
+        for (int e = 0; e < kListMax / 256; ++e) {
+            const int li = tid + 256 * e;
+            sum[e] = 0;
+            la[e] = 0;
+            lb[e] = 0;
+            if (li < n_live) {
+                const int a = s_la[li], b = s_lb[li];
+                la[e] = __popcll(uA & ((1ull << a) - 1));
+                lb[e] = diag ? __popcll(uA & ((1ull << b) - 1)) : nA + __popcll(uB & ((1ull << b) - 1));
+            }
+        }
+        __syncthreads();
+        for (int64_t wb = w_begin; wb < w_end; wb += kw) {
+            for (int w = tid; w < kw; w += 256) {
+                uint64_t part = wb + w < w_end ? ~0ull : 0ull;
+                if (wb + w < w_end)
+                    for (int t = 0; t < np; ++t) part &= bv[(int64_t)prow[t] * pitch + wb + w];
+                sP[w] = part;
+            }
+            __syncthreads();
+            const int n_el = nS << lg;
+#pragma unroll 4
+            for (int e = tid; e < n_el; e += 256) {
+                const int c = e >> lg, w = e & (kw - 1);
+                s_tiles[w * stride + c] = wb + w < w_end ? (bv[s_col[c] + wb + w] & sP[w]) : 0ull;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < kListMax / 256; ++e) {
+                if (tid + 256 * e < n_live) {
+                    int acc = sum[e];
+                    const uint64_t* pa = s_tiles + la[e];
+                    const uint64_t* pb = s_tiles + lb[e];
+#pragma unroll 8
+                    for (int w = 0; w < kw; ++w) acc = popc_acc(pa[w * stride] & pb[w * stride], acc);
+                    sum[e] = acc;
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int e = 0; e < kListMax / 256; ++e) {
+            const int li = tid + 256 * e;
+            if (li < n_live && sum[e]) {
+                const int64_t i = ta * kT + s_la[li], j = tb * kT + s_lb[li];
+                atomicAdd(&counts[cbase + pair_slot(i, j, s, layout)], sum[e]);
+            }
+        }
+        return;
+    }
+
     if (tid < kT) {
         const int64_t i = ta * kT + tid;
-        colA[tid] = (i < s && ((useA >> tid) & 1ull)) ? (int64_t)rows[(gs + i) * m + np] * pitch : -1;
+        colA[tid] = i < s ? (int64_t)rows[(gs + i) * m + np] * pitch : -1;
     } else if (tid < 2 * kT) {
         const int64_t j = tb * kT + (tid - kT);
-        colB[tid - kT] = (j < s && ((useB >> (tid - kT)) & 1ull)) ? (int64_t)rows[(gs + j) * m + np] * pitch : -1;
+        colB[tid - kT] = j < s ? (int64_t)rows[(gs + j) * m + np] * pitch : -1;
     }
-    const int32_t* prow = rows + gs * m;
 
     int acc[4][4];
 #pragma unroll
@@ -365,8 +444,6 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[r][c] = 0;
 
-    const int64_t w_begin = chunk_id * chunk_words;
-    const int64_t w_end = min(n_words, w_begin + chunk_words);
     __syncthreads();
     for (int64_t wb = w_begin; wb < w_end; wb += kKW) {
         if (tid < kKW) sP[tid] = (wb + tid < w_end) ? ~0ull : 0ull;
@@ -394,22 +471,6 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
         }
         __syncthreads();
         uint64_t(*B)[kT + 1] = diag ? sA : sB;
-        if (listed) {
-            // Thread t counts list entries t, t + 256, t + 512, t + 768.
-#pragma unroll
-            for (int e = 0; e < kListMax / 256; ++e) {
-                const int li = tid + 256 * e;
-                if (li < n_live) {
-                    const int a = s_la[li], b = s_lb[li];
-                    int sum = acc[0][e];
-#pragma unroll 8
-                    for (int w = 0; w < kKW; ++w) sum = popc_acc(sA[w][a] & B[w][b], sum);
-                    acc[0][e] = sum;
-                }
-            }
-            __syncthreads();
-            continue;
-        }
 #pragma unroll 4
         for (int w = 0; w < kKW; ++w) {
             uint64_t av[4], bw[4];
@@ -425,17 +486,6 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
         __syncthreads();
     }
 
-    if (listed) {
-#pragma unroll
-        for (int e = 0; e < kListMax / 256; ++e) {
-            const int li = tid + 256 * e;
-            if (li < n_live) {
-                const int64_t i = ta * kT + s_la[li], j = tb * kT + s_lb[li];
-                if (acc[0][e]) atomicAdd(&counts[cbase + pair_slot(i, j, s, layout)], acc[0][e]);
-            }
-        }
-        return;
-    }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
 #pragma unroll
