@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
     // Dense mode: A and B tiles [word][column] of kKW words. Listed mode: the live columns compacted into slots,
     // [word][slot] of up to kListWords words, in the same buffer.
     __shared__ uint64_t s_tiles[2 * kKW * (kT + 1)];
-    __shared__ uint64_t sP[kListWords];
+    __shared__ uint64_t sP[kKW];
     __shared__ int64_t s_col[2 * kT];
     uint64_t(*sA)[kT + 1] = reinterpret_cast<uint64_t(*)[kT + 1]>(s_tiles);
     uint64_t(*sB)[kT + 1] = reinterpret_cast<uint64_t(*)[kT + 1]>(s_tiles + kKW * (kT + 1));
@@ -391,20 +391,18 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
             }
         }
         __syncthreads();
+        // kw divides 256, so a thread stages the same word wq of every step, for slots c0, c0 + cstep, ..., and
+        // ANDs its own prefix word in: a step waits on one round of global loads. (Loading the next step's words
+        // while the current one counts measured slower: the registers held across the count cost a wave per
+        // SIMD.)
+        const int wq = tid & (kw - 1), c0 = tid >> lg, cstep = 256 >> lg;
         for (int64_t wb = w_begin; wb < w_end; wb += kw) {
-            for (int w = tid; w < kw; w += 256) {
-                uint64_t part = wb + w < w_end ? ~0ull : 0ull;
-                if (wb + w < w_end)
-                    for (int t = 0; t < np; ++t) part &= bv[(int64_t)prow[t] * pitch + wb + w];
-                sP[w] = part;
-            }
-            __syncthreads();
-            const int n_el = nS << lg;
+            const bool in = wb + wq < w_end;
+            uint64_t p = in ? ~0ull : 0ull;
+            if (in)
+                for (int t = 0; t < np; ++t) p &= bv[(int64_t)prow[t] * pitch + wb + wq];
 #pragma unroll 4
-            for (int e = tid; e < n_el; e += 256) {
-                const int c = e >> lg, w = e & (kw - 1);
-                s_tiles[w * stride + c] = wb + w < w_end ? (bv[s_col[c] + wb + w] & sP[w]) : 0ull;
-            }
+            for (int c = c0; c < nS; c += cstep) s_tiles[wq * stride + c] = in ? (bv[s_col[c] + wb + wq] & p) : 0ull;
             __syncthreads();
 #pragma unroll
             for (int e = 0; e < kListMax / 256; ++e) {
