@@ -32,6 +32,20 @@ hipError_t launch_free(const int32_t* rows, int64_t n_cap, const int64_t* d_n, i
                        const int32_t* prev_rows, const int32_t* prev_counts, const int32_t* prev_table,
                        int64_t table_size, uint8_t* free_out, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
+
+// Level 2 (rows2: n2 lexsorted pairs of column ids < n_cols) as a pair index for the K=3 subset test: a bitmap
+// `bits` (n_cols x words uint64, words = ceil(n_cols / 64)) and `rank` (n_cols x words int32, the row of the first
+// pair in each word). pair_index_bytes is their size.
+int64_t pair_index_bytes(int64_t n_cols);
+hipError_t launch_pair_index(const int32_t* rows2, int64_t n2, int64_t n_cols, uint64_t* bits, int32_t* rank,
+                             hipStream_t s);
+// etm_level_classify with an optional pair index (used when m = 2; null pointers: the hash).
+int level_classify(const int32_t* d_rows, int64_t n, int32_t m, const int32_t* d_prev_counts,
+                   const uint8_t* d_prev_free, const int32_t* d_table, int64_t table_size,
+                   const int64_t* d_group_start, const int64_t* d_cum_pairs, int64_t n_groups, int64_t n_cands,
+                   const int32_t* d_group_list, const int64_t* h_class_counts, int32_t mode, uint8_t* d_status,
+                   int32_t* d_counts, int32_t* d_gid, int64_t* d_tally, const uint64_t* d_pair_bits,
+                   const int32_t* d_pair_rank, int64_t pair_words, void* stream);
 hipError_t launch_zero_i64(int64_t* p, int n, hipStream_t s);
 
 }  // namespace detail

@@ -298,12 +298,25 @@ struct Classifier {
     const int32_t* table;
     int64_t mask;
     int32_t mode;
+    // Level 2 as a pair index (set only when the level holds pairs, m = 2): bit b of bitmap row a is set for
+    // each pair (a, b) of column ids, and pair_rank[a * pair_words + w] is that pair's row when it is the first
+    // of its word, so a K=3 candidate's one subset test is a bitmap word and a rank instead of a hash probe.
+    const uint64_t* pair_bits;
+    const int32_t* pair_rank;
+    int64_t pair_words;
+
+    __device__ int64_t pair_row(int32_t a, int32_t b) const {
+        const int64_t at = (int64_t)a * pair_words + (b >> 6);
+        const uint64_t w = pair_bits[at], bit = 1ull << (b & 63);
+        return (w & bit) ? (int64_t)pair_rank[at] + __popcll(w & (bit - 1)) : -1;
+    }
 
     // Row of the subset of candidate (i, j) of the group at row gs that drops prefix item d, or -1.
     __device__ int64_t probe(int64_t gs, int64_t i, int64_t j, int d) const {
         const int32_t* pre = rows + gs * m;
         const int np = m - 1;
         const int32_t a = rows[(gs + i) * m + np], b = rows[(gs + j) * m + np];
+        if (pair_bits) return pair_row(a, b);
         uint64_t h = item_hash(a) + item_hash(b);
         for (int t = 0; t < np; ++t)
             if (t != d) h += item_hash(pre[t]);
@@ -323,7 +336,15 @@ struct Classifier {
         int32_t lowest = prev_counts ? min(prev_counts[ri], prev_counts[rj]) : 0;
         bool nf = infer && (prev_free[ri] == 0 || prev_free[rj] == 0);
         bool missing = false;
-        if (prune || infer) {
+        if (pair_bits && (prune || infer)) {
+            const int64_t r = pair_row(a, b);
+            if (r < 0) {
+                missing = true;
+            } else if (infer) {
+                lowest = min(lowest, prev_counts[r]);
+                nf = nf || prev_free[r] == 0;
+            }
+        } else if (prune || infer) {
             uint64_t h = item_hash(a) + item_hash(b);
             for (int t = 0; t < np; ++t) h += item_hash(pre[t]);
             for (int d = 0; d < np && !missing; ++d) {
@@ -1089,6 +1110,40 @@ __global__ __launch_bounds__(kOne) void k_select_single(const int32_t* __restric
     if (threadIdx.x == 0) *d_out_n = total;
 }
 
+// The pair index of a level of pairs (rows2, lexsorted): bits first (one atomic OR per pair), then a wave per
+// bitmap row scans its words' popcounts from the row's first pair in rows2 (a binary search on the first item).
+__global__ __launch_bounds__(256) void k_pair_index_bits(const int32_t* __restrict__ rows2, int64_t n2,
+                                                          int64_t words, uint64_t* __restrict__ bits) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n2; p += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t a = rows2[2 * p], b = rows2[2 * p + 1];
+        atomicOr((unsigned long long*)&bits[(int64_t)a * words + (b >> 6)], 1ull << (b & 63));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pair_index_rank(const int32_t* __restrict__ rows2, int64_t n2,
+                                                          int64_t n_cols, int64_t words,
+                                                          const uint64_t* __restrict__ bits,
+                                                          int32_t* __restrict__ rank) {
+    const int lane = threadIdx.x & 63;
+    const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x / 64);
+    for (int64_t a = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); a < n_cols; a += n_waves) {
+        int64_t lo = 0, hi = n2;  // first row whose first item is >= a
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (rows2[2 * mid] < a) lo = mid + 1;
+            else hi = mid;
+        }
+        int32_t run = (int32_t)lo;
+        for (int64_t w0 = 0; w0 < words; w0 += 64) {
+            const int64_t w = w0 + lane;
+            const int32_t v = w < words ? __popcll(bits[a * words + w]) : 0;
+            const int32_t inc = etm::wave_scan_i32(v);
+            if (w < words) rank[a * words + w] = run + inc - v;
+            run += __builtin_amdgcn_readlane(inc, 63);
+        }
+    }
+}
+
 int grid_for(int64_t n, int cap = 16384) {
     int64_t g = (n + 255) / 256;
     if (g < 1) g = 1;
@@ -1217,12 +1272,47 @@ extern "C" int etm_level_hash(const int32_t* d_rows, int64_t n, int32_t m, int32
     return ETM_OK;
 }
 
+namespace etm {
+namespace detail {
+
+int64_t pair_index_bytes(int64_t n_cols) {
+    const int64_t words = (n_cols + 63) / 64;
+    return n_cols * words * 12;
+}
+
+hipError_t launch_pair_index(const int32_t* rows2, int64_t n2, int64_t n_cols, uint64_t* bits, int32_t* rank,
+                             hipStream_t s) {
+    const int64_t words = (n_cols + 63) / 64;
+    hipError_t e = hipMemsetAsync(bits, 0, (size_t)(n_cols * words) * 8, s);
+    if (e != hipSuccess) return e;
+    if (n2 > 0) k_pair_index_bits<<<grid_for(n2), 256, 0, s>>>(rows2, n2, words, bits);
+    e = hipGetLastError();
+    if (e != hipSuccess || n_cols == 0) return e;
+    k_pair_index_rank<<<grid_for(n_cols * 64), 256, 0, s>>>(rows2, n2, n_cols, words, bits, rank);
+    return hipGetLastError();
+}
+
+}  // namespace detail
+}  // namespace etm
+
 extern "C" int etm_level_classify(const int32_t* d_rows, int64_t n, int32_t m, const int32_t* d_prev_counts,
                                   const uint8_t* d_prev_free, const int32_t* d_table, int64_t table_size,
                                   const int64_t* d_group_start, const int64_t* d_cum_pairs, int64_t n_groups,
                                   int64_t n_cands, const int32_t* d_group_list, const int64_t* h_class_counts,
                                   int32_t mode, uint8_t* d_status, int32_t* d_counts, int32_t* d_gid,
                                   int64_t* d_tally, void* stream) {
+    return etm::detail::level_classify(d_rows, n, m, d_prev_counts, d_prev_free, d_table, table_size, d_group_start,
+                                       d_cum_pairs, n_groups, n_cands, d_group_list, h_class_counts, mode, d_status,
+                                       d_counts, d_gid, d_tally, nullptr, nullptr, 0, stream);
+}
+
+int etm::detail::level_classify(const int32_t* d_rows, int64_t n, int32_t m, const int32_t* d_prev_counts,
+                                const uint8_t* d_prev_free, const int32_t* d_table, int64_t table_size,
+                                const int64_t* d_group_start, const int64_t* d_cum_pairs, int64_t n_groups,
+                                int64_t n_cands, const int32_t* d_group_list, const int64_t* h_class_counts,
+                                int32_t mode, uint8_t* d_status, int32_t* d_counts, int32_t* d_gid,
+                                int64_t* d_tally, const uint64_t* d_pair_bits, const int32_t* d_pair_rank,
+                                int64_t pair_words, void* stream) {
     ETM_ARG(d_rows && d_table && d_group_start && d_cum_pairs && d_status && d_counts,
             "etm_level_classify: null pointer");
     ETM_ARG(m >= 2 && m <= 62, "etm_level_classify: 2 <= m <= 62 required, got %d", m);
@@ -1237,7 +1327,8 @@ extern "C" int etm_level_classify(const int32_t* d_rows, int64_t n, int32_t m, c
         int64_t small = 0;
         for (int c = 0; c < 4; ++c) small += h_class_counts[c];
         const int64_t big = h_class_counts[4];
-        Classifier C{d_rows, m, d_prev_counts, d_prev_free, d_table, table_size - 1, mode};
+        Classifier C{d_rows, m, d_prev_counts, d_prev_free, d_table, table_size - 1, mode,
+                     m == 2 ? d_pair_bits : nullptr, d_pair_rank, pair_words};
         if (small > 0) {
             PackedClasses pc{{h_class_counts[0], h_class_counts[1], h_class_counts[2], h_class_counts[3]},
                              std::max(1, m - 1)};

@@ -314,6 +314,8 @@ constexpr double kK2RowsMaxR = 2.0e-4;
 constexpr double kK3RowsMaxR = 2.5e-4;
 constexpr int64_t kK3RowsLongest = 256;
 constexpr int64_t kRowsBitsMax = 512ll << 20;
+// Largest K=3 pair index (12 bytes per column x 64 columns: 75 MB at 20,000 columns).
+constexpr int64_t kPairIndexMaxBytes = 256ll << 20;
 
 void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n_rows, const int64_t* col_counts,
                  int32_t n_cols, const etm_mine_opts& o, etm_levels& out, hipStream_t s,
@@ -459,9 +461,22 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
             gid = a.get<int32_t>(n_cands);
             const bool infer = o.infer_counts && !prune && prev.free != nullptr;
             const int32_t mode = ETM_SUBSET_PRUNE | (infer ? ETM_SUBSET_INFER : 0);
-            check_rc(etm_level_classify(prev.rows, prev.n, m, prev.counts, infer ? prev.free : nullptr, prev.table,
-                                        prev.table_size, g.gs, g.cp, n_groups, n_cands, g.list, classes, mode,
-                                        status, counts, gid, cur_scal + 1, s),
+            // At K=3 the one subset test of a candidate reads a pair index of level 2 (a bitmap word and a rank)
+            // instead of probing its hash, where the index stays small.
+            uint64_t* pair_bits = nullptr;
+            int32_t* pair_rank = nullptr;
+            int64_t pair_words = 0;
+            if (k == 3 && etm::detail::pair_index_bytes(n_cols) <= kPairIndexMaxBytes) {
+                pair_words = ((int64_t)n_cols + 63) / 64;
+                pair_bits = a.get<uint64_t>((int64_t)n_cols * pair_words);
+                pair_rank = a.get<int32_t>((int64_t)n_cols * pair_words);
+                check(etm::detail::launch_pair_index(prev.rows, prev.n, n_cols, pair_bits, pair_rank, s),
+                      "pair index");
+            }
+            check_rc(etm::detail::level_classify(prev.rows, prev.n, m, prev.counts, infer ? prev.free : nullptr,
+                                                 prev.table, prev.table_size, g.gs, g.cp, n_groups, n_cands, g.list,
+                                                 classes, mode, status, counts, gid, cur_scal + 1, pair_bits,
+                                                 pair_rank, pair_words, s),
                      "etm_level_classify");
             // The deduction bounds need levels k-2 and k-3 complete (the inference mode keeps every level whole).
             if (infer && o.deduction != ETM_DEDUCTION_OFF && k <= 24) {
