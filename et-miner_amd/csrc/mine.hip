@@ -181,6 +181,9 @@ struct Seg {
     int64_t n;
     int64_t dst;
 };
+// Largest group (suffixes) counted by the wave-per-group kernels; bigger groups go to the tiled kernel.
+constexpr int32_t kSmallMax = 8;
+
 constexpr int kGatherSegs = 32;
 struct Segs {
     Seg s[kGatherSegs];
@@ -284,13 +287,13 @@ GroupsDev alloc_groups(Arena& a, int64_t n) {
 void build_groups(Arena& a, const Level& lv, const int64_t* d_n, int64_t n_cap, GroupsDev& g, int64_t* d_totals,
                   hipStream_t s) {
     if (n_cap <= etm::detail::kSingleMax) {
-        check(etm::detail::launch_groups_single(lv.rows, d_n, n_cap, lv.m, 16, g.gs, g.cp, g.ct, g.list, d_totals, s),
+        check(etm::detail::launch_groups_single(lv.rows, d_n, n_cap, lv.m, kSmallMax, g.gs, g.cp, g.ct, g.list, d_totals, s),
               "group build");
         return;
     }
     const size_t ws_n = etm_level_groups_ws(n_cap);
     int64_t* ws = a.get<int64_t>((int64_t)(ws_n + 7) / 8);
-    check_rc(etm_level_groups(lv.rows, n_cap, lv.m, 16, g.gs, g.cp, g.ct, g.list, d_totals, ws, ws_n, s),
+    check_rc(etm_level_groups(lv.rows, n_cap, lv.m, kSmallMax, g.gs, g.cp, g.ct, g.list, d_totals, ws, ws_n, s),
              "etm_level_groups");
 }
 
@@ -613,7 +616,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         const bool early = cap <= etm::detail::kSingleMax && k + 1 <= max_len;
         if (early) {
             gn = alloc_groups(a, cap);
-            check(etm::detail::launch_groups_single(cur.rows, d_n, cap, k, 16, gn.gs, gn.cp, gn.ct, gn.list,
+            check(etm::detail::launch_groups_single(cur.rows, d_n, cap, k, kSmallMax, gn.gs, gn.cp, gn.ct, gn.list,
                                                     cur_scal + 8, s),
                   "group build");
         }
