@@ -181,7 +181,9 @@ struct Seg {
     int64_t n;
     int64_t dst;
 };
-// Largest group (suffixes) counted by the wave-per-group kernels; bigger groups go to the tiled kernel.
+// Largest group (suffixes) counted by the wave-per-group kernels; bigger groups go to the tiled kernel, whose
+// listed mode stages only a group's live columns. Measured on MI355X: 8 beat 16 (the 16-suffix kernel holds
+// 120 accumulators a lane, two waves a SIMD) and 4 on deep_k_1m, wide_vocab_2m and the served stream.
 constexpr int32_t kSmallMax = 8;
 
 constexpr int kGatherSegs = 32;
