@@ -28,7 +28,8 @@ __all__ = [
     "DeviceRows", "upload_rows", "count_pairs_k2_rows", "K3Graph", "build_k3_graph", "count_k3_rows",
 ]
 
-#: Groups with at most this many suffixes run the wave-per-group kernel; larger ones the tiled kernel.
+#: Groups with at most this many suffixes run the wave-per-group kernel; larger ones the tiled kernel. The native
+#: loop uses 8, measured faster on MI355X; this driver keeps 16, so the 9-16 suffix kernel stays exercised.
 SMALL_MAX = 16
 #: Largest itemset length (the reference's cap, gpu/kernels/loader.py:25-37).
 MAX_K = 62
