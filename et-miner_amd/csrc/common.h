@@ -180,6 +180,26 @@ __device__ __forceinline__ int64_t upper_index(const int64_t* __restrict__ cum, 
     return lo;
 }
 
+// upper_index for a whole wave with the same n and x (every lane gets the answer): each round the 64 lanes read
+// 64 evenly spaced entries of the range and a ballot keeps the stretch between the last one <= x and the next,
+// so a range of n entries takes ceil(log64 n) dependent loads instead of log2 n.
+__device__ __forceinline__ int64_t wave_upper_index(const int64_t* __restrict__ cum, int64_t n, int64_t x) {
+    const int lane = threadIdx.x & (kWave - 1);
+    int64_t lo = 0, hi = n - 1;  // the answer lies in [lo, hi]; cum[lo] <= x
+    while (lo < hi) {
+        const int64_t span = hi - lo + 1;
+        const int64_t step = span <= kWave ? 1 : (span + kWave - 1) / kWave;
+        const int64_t g = lo + (int64_t)lane * step;
+        const bool le = g <= hi && cum[g] <= x;
+        const int t = __popcll(__ballot(le)) - 1;  // the true lanes are a prefix: cum ascends
+        if (step == 1) return lo + t;
+        const int64_t nlo = lo + (int64_t)t * step;
+        hi = min(hi, nlo + step - 1);
+        lo = nlo;
+    }
+    return lo;
+}
+
 // Commutative 64-bit hash of one item (an itemset hashes to the sum of its items' hashes, so dropping an
 // item is one subtraction).
 __host__ __device__ __forceinline__ uint64_t item_hash(int32_t item) {

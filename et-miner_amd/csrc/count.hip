@@ -264,7 +264,7 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
     // 1-D grid: one workgroup per (word chunk, tile-pair) unit, spread over the 8 XCDs by xcd_unit.
     int64_t chunk_id, tp;
     if (!xcd_unit(blockIdx.x % kXcds, blockIdx.x / kXcds, n_tile_pairs, n_chunks, &chunk_id, &tp)) return;
-    const int64_t g = etm::upper_index(cum_tiles, n_groups, tp);
+    const int64_t g = etm::wave_upper_index(cum_tiles, n_groups, tp);
     const int64_t gs = group_start[g];
     const int64_t s = group_start[g + 1] - gs;
     const int64_t nt = (s + kT - 1) / kT;
