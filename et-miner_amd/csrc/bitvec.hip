@@ -136,8 +136,8 @@ __global__ __launch_bounds__(256) void k_hist_global(const Id* __restrict__ ids,
     }
 }
 
-// The builder: a workgroup owns kTileWords words (64 x kTileWords rows) x up to kTileCols columns in LDS
-// (64 KiB); each entry of its rows sets its bit with an LDS atomic OR, so each row's ids are read once per
+// The builder: a workgroup owns kTileWords words (64 x kTileWords rows) x a slab of up to kTileCols columns in LDS
+// (at most 64 KiB); each entry of its rows sets its bit with an LDS atomic OR, so each row's ids are read once per
 // kTileCols columns and no work is spent on a column a row does not hold. Each column's kTileWords words leave
 // as one contiguous 64-byte run; no atomic reaches HBM. (Measured against a register-mask + wave-ballot
 // transpose of 128/256-column slabs: deep_k_1m 2.85 -> 2.77 ms/step, wide_vocab_2m 17.4 -> 16.4 ms/step.)
@@ -150,11 +150,12 @@ __global__ __launch_bounds__(kBuildThreads) void k_bitvec_build_lds(const int64_
                                                                     int64_t lo, int64_t span,
                                                                     const int32_t* __restrict__ lut, int32_t n_cols,
                                                                     int64_t pitch, const int32_t* __restrict__ order,
-                                                                    uint64_t* __restrict__ out) {
-    __shared__ unsigned long long tile[kTileCols * kTileWords];
+                                                                    int32_t slab_cols, uint64_t* __restrict__ out) {
+    // The slab's tile, sized at launch: slab_cols (at most kTileCols) columns x kTileWords words.
+    extern __shared__ unsigned long long tile[];
     const int64_t w0 = (int64_t)blockIdx.x * kTileWords;
-    const int c0 = blockIdx.y * kTileCols;
-    const int nc = min(kTileCols, n_cols - c0);
+    const int c0 = blockIdx.y * slab_cols;
+    const int nc = min(slab_cols, n_cols - c0);
     for (int t = threadIdx.x; t < nc * kTileWords; t += kBuildThreads) tile[t] = 0;
     __syncthreads();
     // Lane l of wave v takes the rows at positions w0 * 64 + 8 l + (v, v + 4): the 64 lanes of a wave spread
@@ -379,13 +380,20 @@ extern "C" int etm_bitvec_build(const int64_t* d_indptr, const void* d_ids, int 
                 (long long)tiles);
     hipStream_t s = ETM_STREAM(stream);
     {
-        dim3 grid((unsigned)tiles, (unsigned)((n_cols + kTileCols - 1) / kTileCols));
+        // Column slabs of equal width (at most kTileCols): the LDS tile is only as large as a slab, so narrow
+        // column sets leave room for more workgroups per CU (157 columns: 10 KB instead of 64 KB).
+        const int32_t slabs = (n_cols + kTileCols - 1) / kTileCols;
+        const int32_t slab_cols = (n_cols + slabs - 1) / slabs;
+        const size_t lds = (size_t)slab_cols * kTileWords * sizeof(unsigned long long);
+        dim3 grid((unsigned)tiles, (unsigned)slabs);
         if (id_bytes == 4)
-            k_bitvec_build_lds<int32_t><<<grid, kBuildThreads, 0, s>>>(d_indptr, (const int32_t*)d_ids, n_rows, lo,
-                                                                       span, d_lut, n_cols, pitch, d_order, d_out);
+            k_bitvec_build_lds<int32_t><<<grid, kBuildThreads, lds, s>>>(d_indptr, (const int32_t*)d_ids, n_rows, lo,
+                                                                         span, d_lut, n_cols, pitch, d_order,
+                                                                         slab_cols, d_out);
         else
-            k_bitvec_build_lds<int64_t><<<grid, kBuildThreads, 0, s>>>(d_indptr, (const int64_t*)d_ids, n_rows, lo,
-                                                                       span, d_lut, n_cols, pitch, d_order, d_out);
+            k_bitvec_build_lds<int64_t><<<grid, kBuildThreads, lds, s>>>(d_indptr, (const int64_t*)d_ids, n_rows, lo,
+                                                                         span, d_lut, n_cols, pitch, d_order,
+                                                                         slab_cols, d_out);
         ETM_LAUNCHED("k_bitvec_build_lds");
     }
     return ETM_OK;
