@@ -4,21 +4,26 @@
 // lexsorted level, see etm.h):
 //
 // * k_count_small<S>: groups of 2..S suffixes. One wave64 per (group, word
-//   chunk); lane l reads word w0 + l + 64t of every column, so each load
-//   instruction fetches one contiguous 512-byte run per column. Per word the
-//   wave ANDs the prefix columns, ANDs each suffix column with that, and
-//   accumulates popcounts of all S(S-1)/2 suffix pairs in registers. Partial
-//   counts are reduced across the wave with shuffles and added to the count
-//   array with one integer atomic per pair and chunk (exact and
-//   order-independent). HBM-bound: (prefix + suffixes) x 8 bytes per word.
+//   chunk); lane l reads word w0 + l + 64t of every column (or, where every
+//   column is 16-byte aligned, words w0 + 2l + 128t and the next one as one
+//   16-byte load), so each load instruction fetches one contiguous run per
+//   column. Per word the wave ANDs the prefix columns, ANDs each suffix
+//   column with that, and accumulates popcounts of the counted suffix pairs
+//   in registers. Partial counts are reduced across the wave with shuffles and
+//   added to the count array with one integer atomic per pair and chunk
+//   (exact and order-independent). Reads (prefix + suffixes) x 8 bytes per
+//   word, mostly from L2: bound by load issue there, by HBM otherwise.
 //
 // * k_count_tiled: groups with more suffixes (and the K=2 pair space, one
 //   group with an empty prefix). One workgroup per (tile-pair, word chunk) of
-//   64 x 64 suffixes. Per 32-word step it stages the prefix AND and the two
-//   tiles' suffix columns (already ANDed with the prefix) in LDS, transposed
-//   to [word][suffix] so the compute loop reads are conflict-free; each of
-//   256 threads accumulates a 4 x 4 register block of pairs. VALU-bound: 4
-//   instructions (2 AND + 2 BCNT) per candidate-word.
+//   64 x 64 suffixes. Dense mode: per 32-word step it stages the prefix AND
+//   and the two tiles' suffix columns (already ANDed with the prefix) in LDS,
+//   transposed to [word][suffix] so the compute loop reads are conflict-free;
+//   each of 256 threads accumulates a 4 x 4 register block of pairs
+//   (VALU-bound: 4 instructions, 2 AND + 2 BCNT, per candidate-word). Listed
+//   mode (at most kListMax live pairs): only the live pairs' columns are
+//   staged, compacted into LDS slots, up to 256 words per step, and a thread
+//   counts up to four listed pairs.
 //
 // A tile-pair or group none of whose candidates has status ETM_CAND_COUNT
 // returns before reading any bitvector.
@@ -370,8 +375,8 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
     __syncthreads();
     if (listed) {
         // The list grouped by the pair's A column (a counting sort over the tile's 64 A columns): the lanes of a
-        // wave then mostly read the same A word (an LDS broadcast) instead of scattered ones. In arrival order
-        // 37% of the listed mode's LDS-active cycles were bank conflicts; grouped, 20%.
+        // wave then mostly read the same A word (an LDS broadcast) instead of scattered ones (with 128 columns
+        // staged, arrival order measured 37% of the LDS-active cycles in bank conflicts, grouped 20%).
         __shared__ int s_at[kT];
         if (tid < kT) s_at[tid] = 0;
         __syncthreads();
