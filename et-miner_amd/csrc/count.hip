@@ -798,10 +798,11 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
     int64_t base = 0;
     for (int c = 0; c < 4; ++c) {
         if (cls[c] == 0) continue;
-        // About 4096 waves per class and chunks of at least 256 words: classes of many groups keep long chunks
+        // About 2048 waves per class and chunks of at least 256 words: classes of many groups keep long chunks
         // (fewer partial sums, chunk-major L2 reuse per XCD), classes of few groups get enough waves (measured
-        // on MI355X over 256-4096-word floors and 4K-64K wave targets).
-        int64_t chunk = word_chunk > 0 ? word_chunk : pick_chunk(n_words, cls[c], 4096, 64, 256);
+        // on MI355X over 256-4096-word floors and 2K-64K wave targets; with 16-byte loads 2048 beat 4096 and
+        // 8192 on deep_k_1m).
+        int64_t chunk = word_chunk > 0 ? word_chunk : pick_chunk(n_words, cls[c], 2048, 64, 256);
         // The tile skip reads one 64-bit occupancy window per column: at most 64 tiles of 64 words per chunk.
         if (d_occ) chunk = std::min<int64_t>(ceil_div(chunk, 64) * 64, 4096);
         hipError_t e;
