@@ -4,6 +4,7 @@
 // free flags. Nothing here returns to the host except the scalar totals the
 // caller reads to size the next buffers.
 #include <algorithm>
+#include <mutex>
 
 #include "common.h"
 #include "internal.h"
@@ -1415,6 +1416,76 @@ extern "C" int etm_level_compact(const int32_t* d_rows, int32_t m, const int64_t
     WriteSurvivor w{d_rows,       m,        d_group_start, d_cum_pairs, n_groups, d_counts,
                     d_out_rows,   d_out_counts, d_out_cand, d_gid};
     ETM_HIP(etm::scan_finish(f, w, n_cands, (const int64_t*)d_ws, s));
+    return ETM_OK;
+}
+
+// ── compaction without allocation ───────────────────────────────────────
+//
+// etm_level_compact_noalloc keeps the scan's block sums in `scratch`, a file-scope device array: each device
+// that runs the library holds its own copy until the process ends, at its own address. Host code never passes
+// the array's name to a kernel (on the host it is a stand-in, not a device address): the first call on a device
+// resolves that device's address with hipGetSymbolAddress, and every later use goes through it.
+
+namespace {
+
+constexpr int64_t kNoAllocScratchWords = (1 << 20) / 8;  // 1 MiB of int64 block sums
+__device__ __align__(256) int64_t scratch[kNoAllocScratchWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int64_t* g_scratch_ptr[kMaxDevices] = {};
+
+// Survivor writer that drops every survivor at or past the output capacity.
+struct WriteSurvivorCapped {
+    WriteSurvivor w;
+    int64_t cap;
+    __device__ void operator()(int64_t c, int64_t v, int64_t p) const {
+        if (v && p < cap) w(c, v, p);
+    }
+};
+
+}  // namespace
+
+extern "C" int64_t etm_level_compact_noalloc_max(void) {
+    return (kNoAllocScratchWords - 1) * etm::kScanItems;
+}
+
+extern "C" int etm_level_compact_noalloc(const int32_t* d_rows, int32_t m, const int64_t* d_group_start,
+                                         const int64_t* d_cum_pairs, int64_t n_groups, int64_t n_cands,
+                                         const uint8_t* d_status, const int32_t* d_counts, const int32_t* d_gid,
+                                         int64_t min_count, int32_t* d_out_rows, int32_t* d_out_counts,
+                                         int64_t* d_out_cand, int64_t out_cap, int64_t* d_out_n, void* stream) {
+    ETM_ARG(d_rows && d_group_start && d_cum_pairs && d_counts && d_out_n,
+            "etm_level_compact_noalloc: null pointer");
+    ETM_ARG(out_cap >= 0 && (out_cap == 0 || (d_out_rows && d_out_counts)),
+            "etm_level_compact_noalloc: outputs must hold out_cap >= 0 survivors");
+    ETM_ARG(m >= 1 && m < 62, "etm_level_compact_noalloc: 1 <= m < 62 required, got %d", m);
+    ETM_ARG(n_cands >= 0 && n_cands <= etm_level_compact_noalloc_max(),
+            "etm_level_compact_noalloc: %lld candidates exceed the static scratch's limit of %lld",
+            (long long)n_cands, (long long)etm_level_compact_noalloc_max());
+    int dev = 0;
+    ETM_HIP(hipGetDevice(&dev));
+    ETM_ARG(dev >= 0 && dev < kMaxDevices, "etm_level_compact_noalloc: device %d out of range", dev);
+    int64_t* ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (g_scratch_ptr[dev] == nullptr) {
+            void* p = nullptr;
+            ETM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            g_scratch_ptr[dev] = (int64_t*)p;
+        }
+        ptr = g_scratch_ptr[dev];
+    }
+    hipStream_t s = ETM_STREAM(stream);
+    // Every block sum the scan reads is written first by this call's reduce (and the total by its block scan),
+    // so nothing left by an earlier call is read.
+    Survives f{d_status, d_counts, min_count};
+    ETM_HIP(etm::scan_prepare(f, n_cands, ptr, d_out_n, s));
+    if (out_cap == 0) return ETM_OK;
+    WriteSurvivorCapped w{WriteSurvivor{d_rows, m, d_group_start, d_cum_pairs, n_groups, d_counts, d_out_rows,
+                                        d_out_counts, d_out_cand, d_gid},
+                          out_cap};
+    ETM_HIP(etm::scan_finish(f, w, n_cands, ptr, s));
     return ETM_OK;
 }
 

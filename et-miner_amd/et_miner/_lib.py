@@ -68,6 +68,9 @@ SIGNATURES: dict = {
                                 c_int32, c_int32, P, c_int64, P]),
     "etm_level_compact_ws": (c_size_t, [c_int64]),
     "etm_level_compact": (c_int, [P, c_int32, P, P, c_int64, c_int64, P, P, P, c_int64, P, P, P, P, P, c_size_t, P]),
+    "etm_level_compact_noalloc_max": (c_int64, []),
+    "etm_level_compact_noalloc": (c_int, [P, c_int32, P, P, c_int64, c_int64, P, P, P, c_int64, P, P, P, c_int64, P,
+                                          P]),
     "etm_level_free": (c_int, [P, c_int64, c_int32, P, P, P, P, c_int64, P, P]),
     "etm_select_rows": (c_int, [P, c_int64, c_int32, P, P, P, P, P, P, c_size_t, P]),
     "etm_count_itemsets": (c_int, [P, c_int64, c_int64, P, P, c_int64, P, c_int32, P]),

@@ -216,6 +216,22 @@ int etm_level_compact(const int32_t* d_rows, int32_t m, const int64_t* d_group_s
                       const int32_t* d_gid, int64_t min_count, int32_t* d_out_rows, int32_t* d_out_counts, int64_t* d_out_cand,
                       int64_t* d_out_n, void* d_ws, size_t ws_bytes, void* stream);
 
+/* etm_level_compact in one call that allocates nothing: no workspace argument, no device or host allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The scan's block sums live in a 1 MiB static device array,
+ * one per GPU, shared by every call on that GPU: calls that overlap on one GPU (other streams, other threads)
+ * race, and must be ordered by the caller. Writes the survivor count to d_out_n[0] and the first
+ * min(count, out_cap) survivors, in candidate order, to d_out_rows (out_cap x (m+1)), d_out_counts and, when
+ * not NULL, d_out_cand; a count above out_cap means the outputs were too small (nothing past them is written).
+ * out_cap == 0 only counts. n_cands above etm_level_compact_noalloc_max() (134,216,704) is rejected with
+ * ETM_EINVAL. */
+int64_t etm_level_compact_noalloc_max(void);
+int etm_level_compact_noalloc(const int32_t* d_rows, int32_t m, const int64_t* d_group_start,
+                              const int64_t* d_cum_pairs, int64_t n_groups, int64_t n_cands, const uint8_t* d_status,
+                              const int32_t* d_counts, const int32_t* d_gid, int64_t min_count, int32_t* d_out_rows,
+                              int32_t* d_out_counts, int64_t* d_out_cand, int64_t out_cap, int64_t* d_out_n,
+                              void* stream);
+
 /* Free flags of a level of k-itemsets against the hash of its complete previous level: d_free[x] = 1 when
  * no (k-1)-subset of x has x's count. Every subset must be present in the previous level.
  * Replaces _prune_non_free_mask (gpu/mining.py:81-121, rust_ext/src/core/groups.rs:250-308). */

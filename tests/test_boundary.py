@@ -21,7 +21,7 @@ def _header_functions() -> dict:
     text = (ROOT / "include" / "etm.h").read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     out = {}
-    for m in re.finditer(r"\b(?:int|size_t)\s+(etm_\w+)\s*\(([^)]*)\)\s*;", text):
+    for m in re.finditer(r"\b(?:int|int64_t|size_t)\s+(etm_\w+)\s*\(([^)]*)\)\s*;", text):
         params = [p for p in m.group(2).split(",") if p.strip() and p.strip() != "void"]
         out[m.group(1)] = len(params)
     return out
