@@ -372,13 +372,17 @@ def run_plain(jobs: list, time_scale: float = 1.0, resident: bool = True) -> tup
 
 
 def jobs_from_specs(specs: list, workers: int = 8) -> list:
-    """The data of served-job specs (``et_miner.synthetic.served_jobs``), generated in a process pool."""
+    """The data of served-job specs (``et_miner.synthetic.served_jobs``), generated in a process pool.
+
+    The pool's workers are spawned, not forked, so a caller that has already initialised the GPU hands no device
+    state to them."""
+    import multiprocessing
     from concurrent.futures import ProcessPoolExecutor
 
     if workers <= 1:
         datas = [s.workload.generate() for s in specs]
     else:
-        with ProcessPoolExecutor(max_workers=workers) as pool:
+        with ProcessPoolExecutor(max_workers=workers, mp_context=multiprocessing.get_context("spawn")) as pool:
             datas = list(pool.map(_generate, [s.workload for s in specs]))
     return [Job(s.job_id, d.indptr, d.indices, s.workload.min_support, s.workload.max_length,
                 arrival_s=s.arrival_s, dataset=s.workload.name) for s, d in zip(specs, datas)]

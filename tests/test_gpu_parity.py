@@ -201,44 +201,6 @@ def test_workloads_match_oracle(name, n_rows, engine):
     assert got == want
 
 
-@pytest.mark.parametrize("name", ["deep_k_1m", "wide_vocab_2m"])
-def test_full_size_workloads_match_oracle(name):
-    """The benchmarked configurations at their full sizes (1,000,000 and 2,000,000 rows): every itemset and
-    every count equal the oracle's (about 5 s and 30 s of oracle time)."""
-    from et_miner.synthetic import workload
-
-    w = workload(name)
-    data = w.generate()
-    want = miner.mine(data.indptr, data.indices, w.min_support, w.max_length).as_dict()
-    got = _gpu_mine(data.indptr, data.indices, w.min_support, w.max_length, "native",
-                    use_generator_pruning=w.use_generator_pruning)
-    assert got == want
-
-
-def test_son_equals_resident_at_full_size():
-    """deep_sparse_20m at its full 20,000,000 rows: SON in 2,000,000-row chunks equals the resident run (both
-    exact; the oracle would need minutes here), and the planted itemsets are all found with at least their
-    planted support."""
-    from et_miner.core.apriori import mine_csr
-    from et_miner.core.result import as_dict
-    from et_miner.streaming.son import son_csr
-    from et_miner.synthetic import workload
-
-    w = workload("deep_sparse_20m")
-    data = w.generate()
-    ids = data.indices.astype(np.int64)
-    resident = as_dict(mine_csr(data.indptr, ids, w.min_support, w.max_length, use_generator_pruning=True,
-                                with_counts=True))
-    son = as_dict(son_csr(data.indptr, ids, w.min_support, w.max_length, w.chunk_size, use_generator_pruning=True,
-                          with_counts=True))
-    assert son == resident
-    assert len(resident) > 1_000_000
-    for itemset, planted in data.planted:
-        key = tuple(sorted(int(x) for x in itemset))
-        if len(key) <= (w.max_length or len(key)):
-            assert resident.get(key, 0) >= planted, (key, resident.get(key), planted)
-
-
 @pytest.mark.parametrize("k2_kernel", ["valu", "mfma"])
 def test_k2_kernels_mine_identically(k2_kernel):
     """The int8 MFMA Gram and the VALU tiles give the same lattice (wide vocabulary: hundreds of columns)."""
