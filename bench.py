@@ -1,32 +1,43 @@
-"""Benchmark of the GPU counting path: candidate counts/s on one MI355X.
+"""Benchmark of the GPU counting path: itemset-candidate counts/s on one MI355X.
 
-A step is one complete mining pass of the hot path over the deep_k_1m workload (1,000,000 transactions over 200
-item ids, baskets of 1 + Poisson(6) Zipf(0.9) items, 20 planted itemsets of 8-12 items in 3% of rows with 10%
-of insertions missing one member, min_support 0.02, no max_length -> levels to K=12). The transactions' CSR is
-resident in HBM before timing; a step runs the device input layer (id range, histogram, bitvector build, K=1
-popcount) and the level loop (prefix groups, subset test, AND + popcount counting, compaction) to exhaustion,
-and copies every frequent itemset and count back to the host.
+A step is one complete mining pass of the hot path over the deep_sparse_20m workload (20,000,000 transactions
+over 5,000 item ids, baskets of 1 + Poisson(4) Zipf(1.2) items, 50 planted itemsets of 6-16 items each in 0.5%
+of rows, min_support 0.002, max_length 16, generator pruning -> levels to K=16), the largest configuration that
+runs on one GPU. The transactions' CSR is resident in HBM before timing; a step runs the device input layer (id
+range, histogram, bitvector build, K=1 popcount) and the level loop (prefix groups, subset test, Pascal inference
+and deduction bounds, AND + popcount counting, compaction) to exhaustion, and copies every frequent itemset and
+count back to the host.
 
 value = itemset candidates evaluated per second (counted + inferred, all levels; K=1 columns included),
-whole-job, one GPU. The roofline object covers the counting launches of levels K >= 3 (the dominant kernel
-family); their algorithmic bytes are the prefix-group decomposition's compulsory traffic,
-sum over groups of (prefix items + suffixes) x words x 8 B + 4 B per candidate, and their duration is measured
-with HIP events recorded on the stream the kernels run on, over the timed steps.
+whole-job, one GPU.
 
-cpu_baseline: the oracle (numpy/scipy restatement of the reference's CPU route, one thread) on the same
-workload, in the same run, on this host; its output is checked equal to the GPU's.
+roofline: the counting launches of levels K >= 3 (k_count_small<2|4|8> and k_count_tiled, the step's dominant
+kernel family), timed with HIP events recorded by the native loop on the stream the kernels run on, over the
+timed steps. Their algorithmic bytes are what the kernels must read for the candidates left counted by the
+subset test, Pascal inference and the deduction bounds: per level, (prefix items x groups with a counted pair +
+suffix rows in a counted pair) x words x 8 B + 4 B per counted candidate (computed on the device). The full
+prefix-group decomposition's bytes (every group, inferred candidates included) are reported beside them, with
+the ratio of the two. traffic is the HBM bytes per step the same dispatches move, from rocprofv3 FETCH_SIZE and
+WRITE_SIZE (profiles/deep_sparse_20m_pmc_traffic.md).
+
+cpu_baseline: the oracle (numpy/scipy restatement of the reference's CPU route) on a bounded sample of the same
+workload (its first 1,000,000 transactions, same law and threshold), on this host, with one thread and with one
+thread per host CPU, the median of three runs each; its lattice is checked equal to the GPU's on that sample.
+
+legs (secondary, not the headline): plain_60k, deep_k_1m (its 19.7 MB of bitvectors stay in the caches, so its
+rate is set by cache bandwidth, not HBM) and wide_vocab_2m with each K=2 kernel (VALU AND+popcount tiles, int8
+MFMA Gram), each checked against the oracle's committed per-level digests.
 
 served: the served_jobs stream (512 independent jobs, seed 4: transactions log-uniform in 10K-2M, vocabulary
 500-20K, Zipf 0.9-1.3, a coin between 1 + Poisson(4|6|9) and lognormal(2.3, 0.6) baskets, min_support
 log-uniform in 0.001-0.05, max_length 3-10), every dataset resident in HBM before timing. The plain form mines
-the jobs one after another on one stream; the served form runs them through the request loop (8 workers, one
-stream each, footprint admission, K=2 kernel autotuned per shape bucket) after one warm-up pass. Throughput is
-jobs / makespan with every job queued at t=0; latency (completion - arrival) is measured by replaying Poisson
-arrivals at twice the plain form's measured jobs/s. Every job's itemsets and counts are checked identical in
-both forms.
+the jobs one after another on one stream; the served form runs them through the request loop after one warm-up
+pass. Throughput is jobs / makespan with every job queued at t=0; latency (completion - arrival) is measured by
+replaying Poisson arrivals at twice the plain form's measured jobs/s. Every job's itemsets and counts are
+checked identical in both forms.
 
-streaming: deep_sparse_20m (20,000,000 transactions over 5,000 ids, s = 0.002, max_length 16, generator pruning)
-mined from host arrays resident and by SON in 2,000,000-row chunks; wall seconds and identical results.
+streaming: deep_sparse_20m mined from host arrays resident and by SON in 2,000,000-row chunks; wall seconds and
+identical results.
 """
 
 from __future__ import annotations
@@ -34,6 +45,8 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import statistics
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -41,9 +54,15 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "et-miner_amd"))
 sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_PROFILE = ROOT / "profiles" / "deep_k_1m_pmc_traffic.md"
+VALU_CW_PEAK = 9.8e12  # candidate-words/s: 256 CUs x 64 lanes x 2.4 GHz / 4 VALU ops per word
+# candidate-words/s of the int8 MFMA Gram: 5.0 POP/s dense int8 / 128 ops per word (64 bit products, each a
+# multiply and an add)
+MFMA_I8_CW_PEAK = 5.0e15 / 128
+CPU_SAMPLE_ROWS = 1_000_000
+DIGESTS = ROOT / "tests" / "golden" / "workload_digests.json"
 
 
 def measured_traffic(workload_name: str):
@@ -51,56 +70,45 @@ def measured_traffic(workload_name: str):
     WRITE_SIZE, each scaled by a plain copy's reading); None when no summary covers the workload."""
     import re
 
-    if workload_name != "deep_k_1m" or not TRAFFIC_PROFILE.exists():
+    path = ROOT / "profiles" / f"{workload_name}_pmc_traffic.md"
+    if not path.exists():
         return None
-    m = re.search(r"^traffic_bytes_per_step = (\d+)$", TRAFFIC_PROFILE.read_text(), re.M)
+    m = re.search(r"^traffic_bytes_per_step = (\d+)$", path.read_text(), re.M)
     return int(m.group(1)) if m else None
-VALU_CW_PEAK = 9.8e12  # candidate-words/s: 256 CUs x 64 lanes x 2.4 GHz / 4 VALU ops per word
-# candidate-words/s of the int8 MFMA Gram: 5.0 POP/s dense int8 / 128 ops per word (64 bit products, each a
-# multiply and an add)
-MFMA_I8_CW_PEAK = 5.0e15 / 128
 
 
-class EventTimer:
-    """HIP events around each level's counting launches, on the current (kernel) stream."""
+def committed_digests(name: str):
+    import json as _json
 
-    def __init__(self, torch):
-        self.torch = torch
-        self.records = []
-        self.open = None
-        self.enabled = True
+    if not DIGESTS.exists():
+        return None
+    entry = _json.loads(DIGESTS.read_text())["workloads"].get(name)
+    return None if entry is None else [{"k": d["k"], "n": d["n"], "sha256": d["sha256"]} for d in entry["levels"]]
 
-    def begin(self, k):
-        if not self.enabled:
-            return
-        ev = self.torch.cuda.Event(enable_timing=True)
-        ev.record()
-        self.open = (k, ev)
 
-    def end(self, k, stats):
-        if not self.enabled or self.open is None:
-            return
-        ev = self.torch.cuda.Event(enable_timing=True)
-        ev.record()
-        self.records.append((k, self.open[1], ev, stats))
-        self.open = None
-
-    def summary(self, n_words):
-        out = {"k2": [0.0, 0, 0], "k3p": [0.0, 0, 0]}  # ms, bytes, candidate-words
-        for k, a, b, st in self.records:
-            key = "k2" if k == 2 else "k3p"
-            out[key][0] += a.elapsed_time(b)
-            out[key][1] += st.bytes_alg
-            out[key][2] += st.n_counted * n_words
-        return out
+def host_cpu() -> dict:
+    """lscpu's model, sockets, cores and threads, and the thread-pool settings of this process."""
+    info = {"os_cpu_count": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            key, _, val = line.partition(":")
+            key = key.strip()
+            if key in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[key] = val.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    for var in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        info[var] = os.environ.get(var)
+    return info
 
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="deep_k_1m")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="deep_sparse_20m")
     ap.add_argument("--rows", type=int, default=None, help="override the workload's row count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--engine", default="native", choices=["native", "python"],
@@ -109,11 +117,12 @@ def main(argv=None) -> int:
                     help="bitvector row order of the native loop (counts never depend on it)")
     ap.add_argument("--served-jobs", type=int, default=512, help="jobs of the served stream (0: skip that leg)")
     ap.add_argument("--served-workers", type=int, default=8)
-    ap.add_argument("--no-streaming", action="store_true",
-                    help="skip the streaming leg (deep_sparse_20m resident vs SON in 2,000,000-row chunks)")
+    ap.add_argument("--no-streaming", action="store_true", help="skip the streaming leg (resident vs SON)")
+    ap.add_argument("--no-legs", action="store_true", help="skip the secondary workloads")
+    ap.add_argument("--levels", action="store_true", help="print the per-level statistics of the headline")
     args = ap.parse_args(argv)
     if args.gpus != 1:
-        print(json.dumps({"error": f"--gpus {args.gpus}: this build runs on one GPU; multi-GPU counting "
+        print(json.dumps({"error": f"--gpus {args.gpus}: this build runs on one GPU; counting over several GPUs "
                                    "(row-split with RCCL) is not built"}))
         return 2
 
@@ -121,20 +130,20 @@ def main(argv=None) -> int:
     import torch
 
     from et_miner.core.apriori import mine_csr
-    from et_miner.core.result import as_dict, _mining_min_count
+    from et_miner.core.result import _mining_min_count
     from et_miner.gpu import device as dev
     from et_miner.gpu import kernels as K
     from et_miner.synthetic import workload
+    from golden.digests import table_digests
 
     t_gen = time.perf_counter()
     w = workload(args.workload, args.rows)
     data = w.generate()
-    t_gen = time.perf_counter() - t_gen
-    print(f"[bench] generated {w.name}: {data.n_rows:,} rows, {len(data.indices):,} entries in {t_gen:.1f}s",
-          file=sys.stderr, flush=True)
+    print(f"[bench] generated {w.name}: {data.n_rows:,} rows, {len(data.indices):,} entries in "
+          f"{time.perf_counter() - t_gen:.1f}s", file=sys.stderr, flush=True)
 
     served_jobs_data = None
-    if args.served_jobs > 0:  # generated before the GPU is initialised: the generator forks worker processes
+    if args.served_jobs > 0:  # generated before the GPU is initialised
         from et_miner.serve.server import jobs_from_specs
         from et_miner.synthetic import served_jobs
 
@@ -144,36 +153,34 @@ def main(argv=None) -> int:
               f"{sum(j.n_rows for j in served_jobs_data):,} rows in {time.perf_counter() - t_gen:.1f}s",
               file=sys.stderr, flush=True)
 
-    stream_data = None
-    if not args.no_streaming:
-        t_gen = time.perf_counter()
-        stream_w = workload("deep_sparse_20m")
-        stream_data = stream_w.generate()
-        print(f"[bench] generated {stream_w.name}: {stream_data.n_rows:,} rows in {time.perf_counter() - t_gen:.1f}s",
-              file=sys.stderr, flush=True)
+    legs_data = []
+    if not args.no_legs:
+        for name in ("plain_60k", "deep_k_1m", "wide_vocab_2m"):
+            lw = workload(name)
+            legs_data.append((lw, lw.generate()))
 
     torch.cuda.set_device(0)
     d_csr = (dev.to_device(data.indptr), dev.to_device(data.indices.astype(np.int64)))
     torch.cuda.synchronize()
     n_words = K.words_for(data.n_rows)
-    timer = EventTimer(torch)
 
-    def step(collect=False):
+    def step(collect=False, timed=False, live=False):
         stats = []
         table = mine_csr(None, None, w.min_support, w.max_length, use_generator_pruning=w.use_generator_pruning,
-                         device_csr=d_csr, with_counts=collect, timer=timer, stats_out=stats, engine=args.engine,
-                         time_kernels=timer.enabled, row_order=args.row_order)
+                         device_csr=d_csr, with_counts=collect, stats_out=stats, engine=args.engine,
+                         time_kernels=timed, live_bytes=live, row_order=args.row_order)
         return table, stats
 
-    timer.enabled = False
-    for _ in range(args.warmup):
-        step()
+    # The warm-up steps also measure each level's live bytes (one extra pass over the candidates, kept out of the
+    # timed steps; every step mines the same lattice).
+    live_by_k = {}
+    for _ in range(max(1, args.warmup)):
+        live_by_k = {st.k: st.bytes_live for st in step(timed=True, live=True)[1]}
     torch.cuda.synchronize()
-    timer.enabled = True
     stats_all = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        _, stats = step()
+        _, stats = step(timed=True)
         stats_all.append(stats)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -183,28 +190,35 @@ def main(argv=None) -> int:
     n_gen = sum(s.n_candidates for s in per_step)
     n_itemsets = sum(s.n_frequent for s in per_step)
     value = n_eval * args.steps / elapsed
-    if args.engine == "python":
-        ksum = timer.summary(n_words)
-    else:  # the native loop records the events itself, on the stream its kernels run on
-        ksum = {"k2": [0.0, 0, 0], "k3p": [0.0, 0, 0]}
-        for stats in stats_all:
-            for st in stats:
-                if st.k >= 2:
-                    key = "k2" if st.k == 2 else "k3p"
-                    ksum[key][0] += st.count_ms
-                    ksum[key][1] += st.bytes_alg
-                    ksum[key][2] += st.n_counted * n_words
-    k3_ms, k3_bytes = ksum["k3p"][0] / args.steps, ksum["k3p"][1] / args.steps
-    k2_ms, k2_cw = ksum["k2"][0] / args.steps, ksum["k2"][2] / args.steps
-    k3_launches = sum(1 for s in per_step if s.k >= 3)
-    # the K=2 kernel family the level ran on ("valu" tiles, int8 "mfma" Gram or the CSR "rows" kernel); the
-    # python engine drives the VALU tiles
+    k3 = {"ms": 0.0, "live": 0, "alg": 0, "counted": 0}
+    k2 = {"ms": 0.0, "cw": 0}
+    for stats in stats_all:
+        for st in stats:
+            if st.k == 2:
+                k2["ms"] += st.count_ms
+                k2["cw"] += st.n_counted * n_words
+            elif st.k >= 3 and st.n_counted > 0:
+                st.bytes_live = live_by_k.get(st.k, 0)
+                k3["ms"] += st.count_ms
+                k3["live"] += st.bytes_live
+                k3["alg"] += st.bytes_alg
+                k3["counted"] += st.n_counted
+    k3_ms, k3_live, k3_alg = (k3["ms"] / args.steps, k3["live"] / args.steps, k3["alg"] / args.steps)
+    k3_launch_levels = sum(1 for s in per_step if s.k >= 3 and s.n_counted > 0)
+    k2_ms, k2_cw = k2["ms"] / args.steps, k2["cw"] / args.steps
     k2_kernel = next((s.kernel for s in per_step if s.k == 2 and s.kernel), "valu")
     k2_bound, k2_peak = {"valu": ("valu", VALU_CW_PEAK), "mfma": ("mfma", MFMA_I8_CW_PEAK)}.get(k2_kernel,
                                                                                                ("hbm", None))
-    achieved = (k3_bytes / (k3_ms / 1000) / 1e9) if k3_ms > 0 else 0.0
-    print(f"[bench] {args.steps} steps, {ms_per_step:.2f} ms/step, {n_eval:,} candidates/step, "
-          f"K>=3 counting {k3_ms:.3f} ms/step over {k3_launches} launches", file=sys.stderr, flush=True)
+    achieved = (k3_live / (k3_ms / 1000) / 1e9) if k3_ms > 0 else 0.0
+    print(f"[bench] {args.steps} steps, {ms_per_step:.2f} ms/step, {n_eval:,} candidates/step, K>=3 counting "
+          f"{k3_ms:.3f} ms/step over {k3_launch_levels} levels, {achieved:.0f} GB/s of live bytes",
+          file=sys.stderr, flush=True)
+    if args.levels:
+        for st in per_step:
+            print(f"[bench]   K={st.k:2d} kernel={st.kernel or '-':5s} cands={st.n_candidates:>9,} "
+                  f"counted={st.n_counted:>9,} inferred={st.n_inferred:>9,} skipped={st.n_skipped:>7,} "
+                  f"freq={st.n_frequent:>9,} wall={st.ms:7.3f}ms count={st.count_ms:7.3f}ms "
+                  f"live={st.bytes_live / 1e9:8.3f}GB alg={st.bytes_alg / 1e9:9.3f}GB", file=sys.stderr, flush=True)
 
     # The same step when the caller hands over host arrays: the CSR crosses PCIe (pinned) before mining.
     host_ptr = torch.from_numpy(data.indptr).pin_memory()
@@ -219,32 +233,27 @@ def main(argv=None) -> int:
         del up
     upload_ms = sorted(up_ms)[1]
     csr_bytes = host_ptr.numel() * 8 + host_ids.numel() * 8
+    del host_ptr, host_ids
     pcie = {"upload_ms": upload_ms, "csr_bytes": csr_bytes, "upload_gb_per_s": csr_bytes / upload_ms / 1e6,
             "value": n_eval / ((ms_per_step + upload_ms) / 1000), "unit": "candidates/s",
             "note": "host CSR uploaded from pinned memory each step, then the same resident step"}
 
-    # Parity of the timed configuration with the oracle, and the CPU baseline from the same oracle run.
-    gpu_res = as_dict(step(collect=True)[0])
+    # Parity of the timed configuration with the oracle's committed per-level digests.
+    digests_gpu = table_digests(step(collect=True)[0])
+    want = committed_digests(w.name) if args.rows is None else None
+    parity = None if want is None else digests_gpu == want
+
     cpu = None
-    parity = None
     if not args.no_cpu_baseline:
-        from oracle.miner import mine
+        cpu = _cpu_baseline(w, data, mine_csr, table_digests)
 
-        cpu_stats = []
-        t_c = time.perf_counter()
-        lat = mine(data.indptr, data.indices, w.min_support, w.max_length,
-                   use_generator_pruning=w.use_generator_pruning,
-                   level_callback=lambda k, c, f: cpu_stats.append((k, c, f)))
-        t_c = time.perf_counter() - t_c
-        parity = lat.as_dict() == gpu_res
-        cpu_cands = sum(c for _, c, _ in cpu_stats)
-        cpu = {"value": cpu_cands / t_c, "unit": "candidates/s", "cores": 1, "kind": "port",
-               "sample": f"the full {w.name} workload ({data.n_rows:,} rows), oracle mine() single-threaded "
-                         f"numpy/scipy, {t_c:.2f} s, {cpu_cands:,} candidates, host {os.cpu_count()} cpus"}
-        print(f"[bench] cpu oracle {t_c:.2f}s, parity={parity}", file=sys.stderr, flush=True)
-
+    legs = {}
+    for lw, ld in legs_data:
+        legs.update(_leg(lw, ld, mine_csr, table_digests, committed_digests))
     served = _served_leg(served_jobs_data, args.served_workers) if served_jobs_data else None
-    streaming = _streaming_leg(stream_w, stream_data) if stream_data is not None else None
+    streaming = None
+    if not args.no_streaming and w.chunk_size:
+        streaming = _streaming_leg(w, data)
 
     line = {
         "metric": "itemset-candidate counts/s",
@@ -264,34 +273,48 @@ def main(argv=None) -> int:
             "engine": args.engine,
             "n_transactions": data.n_rows,
             "n_items": w.vocab,
+            "nnz": int(len(data.indices)),
             "min_support": w.min_support,
             "min_count": _mining_min_count(w.min_support, data.n_rows),
             "max_length": w.max_length,
+            "use_generator_pruning": w.use_generator_pruning,
+            "resident": "CSR in HBM before the timed region",
             "candidates_evaluated_per_step": n_eval,
             "candidates_generated_per_step": n_gen,
             "frequent_itemsets": n_itemsets,
             "levels": max(s.k for s in per_step),
             "parity_vs_oracle": parity,
+            "parity_basis": "per-level (n, sha256) of every (itemset, count) against the oracle's committed digests "
+                            "(tests/golden/workload_digests.json)",
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_count_small/k_count_tiled, levels K>=3",
+            "kernel": "K>=3 counting launches (k_count_small<2|4|8>, k_count_tiled)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(w.name) if args.rows is None else None,
-            "traffic_unit": "HBM bytes per step (rocprofv3 FETCH_SIZE + WRITE_SIZE over the same dispatches)",
-            "bytes_alg_per_step": k3_bytes,
+            "traffic_unit": "HBM bytes per step of the same dispatches (rocprofv3 FETCH_SIZE + WRITE_SIZE)",
+            "bytes_alg_per_step": k3_live,
+            "bytes_alg_basis": "what the kernels must read for the counted candidates: (prefix items x groups with "
+                               "a counted pair + suffix rows in a counted pair) x words x 8 + 4 per counted "
+                               "candidate, computed on the device",
+            "bytes_decomposition_per_step": k3_alg,
+            "live_over_decomposition": (k3_live / k3_alg) if k3_alg else None,
+            "counted_candidates_per_step": k3["counted"] / args.steps,
             "kernel_ms_per_step": k3_ms,
-            "launches_per_step": k3_launches,
+            "levels_per_step": k3_launch_levels,
+            "ms_per_level_launch": k3_ms / k3_launch_levels if k3_launch_levels else None,
         },
         "k2_count": {"kernel": k2_kernel, "bound": k2_bound, "kernel_ms_per_step": k2_ms,
                      "candidate_words_per_step": k2_cw,
                      "achieved_cw_per_s": (k2_cw / (k2_ms / 1000)) if k2_ms > 0 else 0.0,
-                     "peak_cw_per_s": k2_peak},
+                     "peak_cw_per_s": k2_peak,
+                     "frac": (k2_cw / (k2_ms / 1000) / k2_peak) if (k2_ms > 0 and k2_peak) else None},
         "cpu_baseline": cpu,
         "pcie_inclusive": pcie,
+        "legs": legs or None,
         "served": served,
         "streaming": streaming,
     }
@@ -299,9 +322,90 @@ def main(argv=None) -> int:
     return 0
 
 
+def _cpu_baseline(w, data, mine_csr, table_digests) -> dict:
+    """The oracle on the workload's first CPU_SAMPLE_ROWS transactions with one thread and with one per host CPU,
+    median of three runs each; its lattice is compared with the GPU's on the same sample."""
+    import numpy as np
+
+    from oracle.miner import mine
+    from golden.digests import lattice_digests
+
+    n = min(CPU_SAMPLE_ROWS, data.n_rows)
+    indptr = np.ascontiguousarray(data.indptr[: n + 1])
+    ids = np.ascontiguousarray(data.indices[: indptr[-1]])
+    threads = os.cpu_count() or 1
+    runs = {}
+    lat = None
+    cands = 0
+    for nj in (1, threads):
+        times = []
+        for _ in range(3):
+            cpu_stats = []
+            t_c = time.perf_counter()
+            lat = mine(indptr, ids, w.min_support, w.max_length, use_generator_pruning=w.use_generator_pruning,
+                       level_callback=lambda k, c, f: cpu_stats.append(c), n_jobs=nj)
+            times.append(time.perf_counter() - t_c)
+            cands = sum(cpu_stats)
+        runs[nj] = statistics.median(times)
+        print(f"[bench] cpu oracle, {nj} thread(s): {runs[nj]:.2f} s (median of 3)", file=sys.stderr, flush=True)
+    gpu = table_digests(mine_csr(indptr, ids.astype(np.int64), w.min_support, w.max_length,
+                                 use_generator_pruning=w.use_generator_pruning, with_counts=True))
+    parity = lattice_digests(lat.levels, lat.items) == gpu
+    best = threads if runs[threads] <= runs[1] else 1
+    return {"value": cands / runs[best], "unit": "candidates/s", "cores": best, "kind": "port",
+            "sample": f"the first {n:,} transactions of {w.name} (same law, min_support {w.min_support}, "
+                      f"max_length {w.max_length}, generator pruning): oracle mine(), numpy/scipy, {cands:,} "
+                      f"candidates, median of 3 runs",
+            "seconds": {"1_thread": runs[1], f"{threads}_threads": runs[threads]},
+            "values": {"1_thread": cands / runs[1], f"{threads}_threads": cands / runs[threads]},
+            "parity_vs_gpu_on_sample": parity, "host": host_cpu()}
+
+
+def _leg(w, data, mine_csr, table_digests, committed) -> dict:
+    """A secondary workload: ms per step and candidates/s over 5 resident steps, checked against the oracle's
+    committed digests; wide_vocab_2m once per K=2 kernel."""
+    import numpy as np
+    import torch
+
+    from et_miner.gpu import device as dev
+
+    d_csr = (dev.to_device(data.indptr), dev.to_device(data.indices.astype(np.int64)))
+    want = committed(w.name)
+    out = {}
+    for kern in (("valu", "mfma") if w.name == "wide_vocab_2m" else ("auto",)):
+        def run(collect=False, stats=None):
+            return mine_csr(None, None, w.min_support, w.max_length, use_generator_pruning=w.use_generator_pruning,
+                            device_csr=d_csr, with_counts=collect, stats_out=stats, k2_kernel=kern,
+                            time_kernels=stats is not None)
+
+        run()
+        torch.cuda.synchronize()
+        steps, stats = 5, []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            stats = []
+            run(stats=stats)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1000 / steps
+        n_eval = sum(s.n_evaluated for s in stats)
+        key = w.name if kern == "auto" else f"{w.name}[{kern}]"
+        k2 = [s for s in stats if s.k == 2]
+        out[key] = {"n_transactions": data.n_rows, "ms_per_step": ms, "candidates_per_s": n_eval / (ms / 1000),
+                    "k2_kernel": k2[0].kernel if k2 else None, "k2_ms": k2[0].count_ms if k2 else None,
+                    "k3plus_count_ms": sum(s.count_ms for s in stats if s.k >= 3),
+                    "parity_vs_oracle": None if want is None else table_digests(run(collect=True)) == want}
+        if w.name == "deep_k_1m":
+            out[key]["note"] = ("bitvectors of 19.7 MB stay in the L2 and the 256 MB Infinity Cache: the counting "
+                                "kernels read from cache, so no HBM fraction is quoted")
+        print(f"[bench] leg {key}: {ms:.2f} ms/step, parity={out[key]['parity_vs_oracle']}", file=sys.stderr,
+              flush=True)
+    del d_csr
+    return out
+
+
 def _streaming_leg(w, data):
-    """deep_sparse_20m (use_generator_pruning) mined resident and by SON in 2,000,000-row chunks, from host
-    arrays (the CSR crosses PCIe in both); wall seconds of one warm run each, results compared by digest."""
+    """The workload mined from host arrays resident and by SON in chunk_size-row chunks (the CSR crosses PCIe in
+    both); wall seconds of one warm run each, results compared by digest."""
     import numpy as np
 
     from et_miner.core.apriori import mine_csr

@@ -34,11 +34,25 @@
 
 namespace {
 
+// Work items (unit x word chunk) the launches aim for, and the shortest chunk (words). Launching every unit:
+// about 2048 items per small class and per tiled level (measured on MI355X over 2K-64K targets and 256-4096-word
+// floors; with 16-byte loads 2048 beat 4096 and 8192 on deep_k_1m). Launching the live units only: 16384 and
+// 32768 items with chunks of at least 1024 and 512 words (deep_sparse_20m's levels K >= 3: 23.1 -> 13.4 ms
+// against the 2048 targets; shorter floors cost deep_k_1m's levels 10%).
+constexpr int64_t kSmallTarget = 2048, kTiledTarget = 2048;
+constexpr int64_t kLiveSmallTarget = 16384, kLiveTiledTarget = 32768;
+constexpr int64_t kLiveSmallMin = 1024, kLiveTiledMin = 512;
+
 constexpr int kT = 64;   // suffixes per tile
 constexpr int kKW = 32;  // words per LDS step
 constexpr int kXcds = 8;  // MI355X: 8 XCDs of 32 CUs, each with its own L2
 constexpr int kListMax = 1024;  // live pairs from which a tile-pair is counted densely (4096 slots)
 constexpr int kListWords = 256;  // most words per step of the listed mode
+
+// Start of each small size class in the class-ordered group list (start[4] = the list's length).
+struct ClassBounds {
+    int64_t start[5];
+};
 
 // The (word chunk, item) unit of a workgroup or wave: `xcd` = workgroup index mod 8 (the XCD the hardware
 // dispatches it to, round-robin), `local` = its index among that XCD's units. With at least 8 chunks, XCD x takes
@@ -56,6 +70,39 @@ __device__ __forceinline__ bool xcd_unit(int64_t xcd, int64_t local, int64_t n_i
         *item = u % n_items;
     }
     return *ch < n_chunks;
+}
+
+__host__ __device__ __forceinline__ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Word chunk (a multiple of `align` words) giving about `target` work items over `units` independent units.
+// The chunk count rounds up to a multiple of the 8 XCDs where the columns allow it: xcd_unit then keeps each
+// XCD on its own chunks, whose words its L2 holds; with fewer chunks every XCD reads every word.
+__host__ __device__ inline int64_t pick_chunk(int64_t n_words, int64_t units, int64_t target, int64_t align,
+                                              int64_t min_words) {
+    if (n_words <= 0) return align;
+    int64_t chunks = units > 0 ? ceil_div(target, units) : 1;
+    const int64_t floor_words = min_words > align ? min_words : align;
+    int64_t max_chunks = n_words / floor_words > 1 ? n_words / floor_words : 1;
+    chunks = chunks < max_chunks ? chunks : max_chunks;
+    if (chunks < 1) chunks = 1;
+    const int64_t aligned = ceil_div(chunks, kXcds) * kXcds;
+    if (aligned <= max_chunks) chunks = aligned;
+    return ceil_div(ceil_div(n_words, chunks), align) * align;
+}
+
+// A wave-uniform 64-bit value (computed by every lane alike) moved to scalar registers: loop bounds derived from
+// it stay uniform, which keeps the counting loops' register allocation that of kernel-argument bounds.
+__device__ __forceinline__ int64_t uniform64(int64_t x) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// pick_chunk on the device, for the kernels whose unit count lives on the device.
+__device__ __forceinline__ void live_chunks(int64_t n_words, int64_t units, int64_t target, int64_t align,
+                                            int64_t min_words, int64_t* chunk, int64_t* n_chunks) {
+    *chunk = uniform64(pick_chunk(n_words, units, target, align, min_words));
+    *n_chunks = uniform64(ceil_div(n_words, *chunk));
 }
 
 // Units per XCD of the mapping above (the grid is 8 x this, rounded up to whole workgroups).
@@ -80,25 +127,18 @@ struct SmallUnroll {
     static constexpr int U = S <= 4 ? 4 : (S <= 8 ? 2 : 1);
 };
 
+// One (group, word chunk) unit of a group of 2..S suffixes, counted by the calling wave.
 template <int S, bool SKIP, bool VEC>
-__global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
-                                                      int64_t chunk_words, int64_t n_chunks,
-                                                      const int32_t* __restrict__ rows, int32_t m,
-                                                      const int64_t* __restrict__ group_start,
-                                                      const int64_t* __restrict__ cum_pairs,
-                                                      const int32_t* __restrict__ list, int64_t n_list,
-                                                      const uint8_t* __restrict__ status,
-                                                      int32_t* __restrict__ counts, int32_t layout,
-                                                      const uint64_t* __restrict__ occ, int64_t occ_pitch) {
+__device__ __forceinline__ void small_unit(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
+                                           int64_t chunk_words, const int32_t* __restrict__ rows, int32_t m,
+                                           const int64_t* __restrict__ group_start,
+                                           const int64_t* __restrict__ cum_pairs, const uint8_t* __restrict__ status,
+                                           int32_t* __restrict__ counts, int32_t layout,
+                                           const uint64_t* __restrict__ occ, int64_t occ_pitch, int64_t g,
+                                           int64_t ch) {
     constexpr int NP = S * (S - 1) / 2;
     constexpr int U = SmallUnroll<S>::U;
     const int lane = threadIdx.x & 63;
-    // One wave per (word chunk, group) unit, spread over the 8 XCDs by xcd_unit.
-    int64_t ch, item;
-    if (!xcd_unit(blockIdx.x % kXcds, (int64_t)(blockIdx.x / kXcds) * 4 + (threadIdx.x >> 6), n_list, n_chunks, &ch,
-                  &item))
-        return;
-    const int64_t g = list[item];
     const int64_t gs = group_start[g];
     const int s = (int)(group_start[g + 1] - gs);
     if (s < 2 || s > S) return;
@@ -297,30 +337,84 @@ __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict_
     }
 }
 
-__global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
-                                                      int64_t chunk_words, const int32_t* __restrict__ rows,
-                                                      int32_t m, const int64_t* __restrict__ group_start,
+// One wave per (word chunk, group) unit of a size class, spread over the 8 XCDs by xcd_unit.
+template <int S, bool SKIP, bool VEC>
+__global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
+                                                      int64_t chunk_words, int64_t n_chunks,
+                                                      const int32_t* __restrict__ rows, int32_t m,
+                                                      const int64_t* __restrict__ group_start,
                                                       const int64_t* __restrict__ cum_pairs,
-                                                      const int64_t* __restrict__ cum_tiles, int64_t n_groups,
-                                                      int64_t n_tile_pairs, int64_t n_chunks,
+                                                      const int32_t* __restrict__ list, int64_t n_list,
                                                       const uint8_t* __restrict__ status,
-                                                      int32_t* __restrict__ counts, int32_t layout) {
-    // Dense mode: A and B tiles [word][column] of kKW words. Listed mode: the live columns compacted into slots,
-    // [word][slot] of up to kListWords words, in the same buffer.
-    __shared__ uint64_t s_tiles[2 * kKW * (kT + 1)];
-    __shared__ uint64_t sP[kKW];
-    __shared__ int64_t s_col[2 * kT];
+                                                      int32_t* __restrict__ counts, int32_t layout,
+                                                      const uint64_t* __restrict__ occ, int64_t occ_pitch) {
+    int64_t ch, item;
+    if (!xcd_unit(blockIdx.x % kXcds, (int64_t)(blockIdx.x / kXcds) * 4 + (threadIdx.x >> 6), n_list, n_chunks, &ch,
+                  &item))
+        return;
+    small_unit<S, SKIP, VEC>(bv, pitch, n_words, chunk_words, rows, m, group_start, cum_pairs, status, counts, layout,
+                             occ, occ_pitch, uniform64(list[uniform64(item)]), uniform64(ch));
+}
+
+// Over a class's live groups (the groups with at least one counted pair, listed on the device by k_live_units):
+// *d_n_live groups, each cut into chunks sized from that count (live_chunks), the (group, chunk) units spread
+// over the 8 XCDs by xcd_unit, one wave each. The grid is sized from an upper bound of the units
+// (live_units_per_xcd); the waves past the actual units return at once.
+template <int S, bool VEC>
+__global__ __launch_bounds__(256) void k_count_small_live(const uint64_t* __restrict__ bv, int64_t pitch,
+                                                           int64_t n_words, const int32_t* __restrict__ rows,
+                                                           int32_t m, const int64_t* __restrict__ group_start,
+                                                           const int64_t* __restrict__ cum_pairs,
+                                                           const int32_t* __restrict__ live,
+                                                           const int64_t* __restrict__ d_n_live,
+                                                           const uint8_t* __restrict__ status,
+                                                           int32_t* __restrict__ counts, int32_t layout,
+                                                           int64_t target, int64_t min_words) {
+    const int64_t n_live = *d_n_live;
+    if (n_live == 0) return;
+    int64_t chunk, n_chunks;
+    live_chunks(n_words, n_live, target, 64, min_words, &chunk, &n_chunks);
+    int64_t ch, item;
+    if (!xcd_unit(blockIdx.x % kXcds, (int64_t)(blockIdx.x / kXcds) * 4 + (threadIdx.x >> 6), n_live, n_chunks, &ch,
+                  &item))
+        return;
+    small_unit<S, false, VEC>(bv, pitch, n_words, chunk, rows, m, group_start, cum_pairs, status, counts, layout,
+                              nullptr, 0, uniform64(live[uniform64(item)]), uniform64(ch));
+}
+
+// LDS of the tiled kernels. Dense mode: A and B tiles [word][column] of kKW words. Listed mode: the live columns
+// compacted into slots, [word][slot] of up to kListWords words, in the same buffer.
+struct TiledShared {
+    uint64_t tiles[2 * kKW * (kT + 1)];
+    uint64_t P[kKW];
+    int64_t col[2 * kT];
+    uint8_t la[kListMax], lb[kListMax];
+    int at[kT];
+    unsigned long long use[2];
+    int nlist;
+};
+
+// One (tile-pair, word chunk) unit, counted by the calling workgroup (every thread calls it with the same unit).
+__device__ __forceinline__ void tiled_unit(TiledShared& sh, const uint64_t* __restrict__ bv, int64_t pitch,
+                                           int64_t n_words, int64_t chunk_words, const int32_t* __restrict__ rows,
+                                           int32_t m, const int64_t* __restrict__ group_start,
+                                           const int64_t* __restrict__ cum_pairs,
+                                           const int64_t* __restrict__ cum_tiles, int64_t n_groups,
+                                           const uint8_t* __restrict__ status, int32_t* __restrict__ counts,
+                                           int32_t layout, int64_t tp, int64_t chunk_id) {
+    uint64_t* s_tiles = sh.tiles;
+    uint64_t* sP = sh.P;
+    int64_t* s_col = sh.col;
     uint64_t(*sA)[kT + 1] = reinterpret_cast<uint64_t(*)[kT + 1]>(s_tiles);
     uint64_t(*sB)[kT + 1] = reinterpret_cast<uint64_t(*)[kT + 1]>(s_tiles + kKW * (kT + 1));
     int64_t* colA = s_col;
     int64_t* colB = s_col + kT;
-    __shared__ uint8_t s_la[kListMax], s_lb[kListMax];
-    __shared__ int s_nlist;
+    uint8_t* s_la = sh.la;
+    uint8_t* s_lb = sh.lb;
+    int& s_nlist = sh.nlist;
+    unsigned long long* s_use = sh.use;
 
     const int tid = threadIdx.x;
-    // 1-D grid: one workgroup per (word chunk, tile-pair) unit, spread over the 8 XCDs by xcd_unit.
-    int64_t chunk_id, tp;
-    if (!xcd_unit(blockIdx.x % kXcds, blockIdx.x / kXcds, n_tile_pairs, n_chunks, &chunk_id, &tp)) return;
     const int64_t g = etm::wave_upper_index(cum_tiles, n_groups, tp);
     const int64_t gs = group_start[g];
     const int64_t s = group_start[g + 1] - gs;
@@ -349,7 +443,6 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
         }
     }
     // Which of the tile's A and B columns take part in a live pair: in listed mode only those are staged.
-    __shared__ unsigned long long s_use[2];
     if (tid == 0) {
         s_nlist = 0;
         s_use[0] = 0;
@@ -377,7 +470,7 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
         // The list grouped by the pair's A column (a counting sort over the tile's 64 A columns): the lanes of a
         // wave then mostly read the same A word (an LDS broadcast) instead of scattered ones (with 128 columns
         // staged, arrival order measured 37% of the LDS-active cycles in bank conflicts, grouped 20%).
-        __shared__ int s_at[kT];
+        int* s_at = sh.at;
         if (tid < kT) s_at[tid] = 0;
         __syncthreads();
 #pragma unroll
@@ -554,6 +647,106 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
     }
 }
 
+// One workgroup per (word chunk, tile-pair) unit, spread over the 8 XCDs by xcd_unit.
+__global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
+                                                      int64_t chunk_words, const int32_t* __restrict__ rows,
+                                                      int32_t m, const int64_t* __restrict__ group_start,
+                                                      const int64_t* __restrict__ cum_pairs,
+                                                      const int64_t* __restrict__ cum_tiles, int64_t n_groups,
+                                                      int64_t n_tile_pairs, int64_t n_chunks,
+                                                      const uint8_t* __restrict__ status,
+                                                      int32_t* __restrict__ counts, int32_t layout) {
+    __shared__ TiledShared sh;
+    int64_t chunk_id, tp;
+    if (!xcd_unit(blockIdx.x % kXcds, blockIdx.x / kXcds, n_tile_pairs, n_chunks, &chunk_id, &tp)) return;
+    tiled_unit(sh, bv, pitch, n_words, chunk_words, rows, m, group_start, cum_pairs, cum_tiles, n_groups, status,
+               counts, layout, tp, chunk_id);
+}
+
+// Over the live tile-pairs (at least one counted pair; listed on the device by k_live_units): *d_n_live of
+// them, each cut into chunks sized from that count, one workgroup per (tile-pair, chunk) unit spread over the 8
+// XCDs by xcd_unit. The grid is sized from an upper bound of the units; the workgroups past them return at once.
+__global__ __launch_bounds__(256) void k_count_tiled_live(const uint64_t* __restrict__ bv, int64_t pitch,
+                                                           int64_t n_words, const int32_t* __restrict__ rows,
+                                                           int32_t m, const int64_t* __restrict__ group_start,
+                                                           const int64_t* __restrict__ cum_pairs,
+                                                           const int64_t* __restrict__ cum_tiles, int64_t n_groups,
+                                                           const int32_t* __restrict__ live,
+                                                           const int64_t* __restrict__ d_n_live,
+                                                           const uint8_t* __restrict__ status,
+                                                           int32_t* __restrict__ counts, int32_t layout,
+                                                           int64_t target, int64_t min_words) {
+    __shared__ TiledShared sh;
+    const int64_t n_live = *d_n_live;
+    if (n_live == 0) return;
+    int64_t chunk, n_chunks;
+    live_chunks(n_words, n_live, target, kKW, min_words, &chunk, &n_chunks);
+    int64_t ch, item;
+    if (!xcd_unit(blockIdx.x % kXcds, blockIdx.x / kXcds, n_live, n_chunks, &ch, &item)) return;
+    tiled_unit(sh, bv, pitch, n_words, chunk, rows, m, group_start, cum_pairs, cum_tiles, n_groups, status, counts,
+               layout, uniform64(live[uniform64(item)]), uniform64(ch));
+}
+
+// ── live units ─────────────────────────────────────────────────────────
+//
+// The groups of the small classes and the tile-pairs of the tiled groups with at least one candidate left
+// ETM_CAND_COUNT, listed so that the persistent counting kernels run over live work only (after the subset test,
+// Pascal inference and the deduction bounds, most units of a deep level have none). Lists are in no particular
+// order; every count is an integer atomic, so the result does not depend on it.
+
+// One launch for both lists. Workgroups [0, small_blocks) take the small groups, a thread per group of
+// small_list[0 .. n_small): class c's live groups go to live_small[start_c ..], counted in n_live[c]. The other
+// workgroups take the tile-pairs, a wave per tile-pair (lane l checks row ta * 64 + l against the rows of tile tb):
+// live tile-pairs go to live_tp, counted in n_live[4]. n_live[0..4] start at zero.
+__global__ __launch_bounds__(256) void k_live_units(const int32_t* __restrict__ small_list, int64_t n_small,
+                                                    ClassBounds cb, int64_t small_blocks,
+                                                    const int64_t* __restrict__ group_start,
+                                                    const int64_t* __restrict__ cum_pairs,
+                                                    const int64_t* __restrict__ cum_tiles, int64_t n_groups,
+                                                    int64_t n_tile_pairs, const uint8_t* __restrict__ status,
+                                                    int32_t* __restrict__ live_small, int32_t* __restrict__ live_tp,
+                                                    int64_t* __restrict__ n_live) {
+    if ((int64_t)blockIdx.x < small_blocks) {
+        for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n_small;
+             e += small_blocks * (int64_t)blockDim.x) {
+            const int32_t g = small_list[e];
+            const int64_t s = group_start[g + 1] - group_start[g];
+            const int64_t base = cum_pairs[g];
+            bool any = false;
+            for (int64_t q = 0; q < s * (s - 1) / 2 && !any; ++q) any = status[base + q] == ETM_CAND_COUNT;
+            if (!any) continue;
+            int c = 0;
+            while (c < 3 && e >= cb.start[c + 1]) ++c;
+            const int64_t at = (int64_t)atomicAdd((unsigned long long*)&n_live[c], 1ull);
+            live_small[cb.start[c] + at] = g;
+        }
+        return;
+    }
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = ((int64_t)gridDim.x - small_blocks) * (blockDim.x >> 6);
+    for (int64_t tp = ((int64_t)blockIdx.x - small_blocks) * (blockDim.x >> 6) + (threadIdx.x >> 6); tp < n_tile_pairs;
+         tp += waves) {
+        const int64_t g = etm::wave_upper_index(cum_tiles, n_groups, tp);
+        const int64_t gs = group_start[g];
+        const int64_t s = group_start[g + 1] - gs;
+        const int64_t nt = (s + kT - 1) / kT;
+        int64_t ta, tb1;
+        etm::lex_pair_decode(tp - cum_tiles[g], nt + 1, &ta, &tb1);
+        const int64_t tb = tb1 - 1;
+        const int64_t i = ta * kT + lane;
+        bool any = false;
+        if (i < s) {
+            const int64_t j0 = max(i + 1, tb * kT), j1 = min(s, tb * kT + kT);
+            const int64_t base = cum_pairs[g] + etm::lex_pair_index(i, j0, s);
+            for (int64_t j = j0; j < j1 && !any; ++j) any = status[base + (j - j0)] == ETM_CAND_COUNT;
+        }
+        if (__ballot(any) != 0 && lane == 0) {
+            const int64_t at = (int64_t)atomicAdd((unsigned long long*)&n_live[4], 1ull);
+            live_tp[at] = (int32_t)tp;
+        }
+    }
+}
+
 // ── int8 MFMA Gram of a single-group level (the K=2 pair space) ───────────────────────────────────────────
 //
 // One workgroup per (128 x 128 tile-pair of columns, word chunk). Per 4-word step (256 transactions = 8 K-steps
@@ -723,20 +916,6 @@ __global__ void k_zero_i64n(int64_t* p, int64_t n) {
         p[i] = 0;
 }
 
-int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
-
-// Word chunk (a multiple of `align` words) giving about `target` work items over `units` independent units.
-// The chunk count rounds up to a multiple of the 8 XCDs where the columns allow it: xcd_unit then keeps each
-// XCD on its own chunks, whose words its L2 holds; with fewer chunks every XCD reads every word.
-int64_t pick_chunk(int64_t n_words, int64_t units, int64_t target, int64_t align, int64_t min_words) {
-    if (n_words <= 0) return align;
-    int64_t chunks = units > 0 ? ceil_div(target, units) : 1;
-    int64_t max_chunks = std::max<int64_t>(1, n_words / std::max<int64_t>(min_words, align));
-    chunks = std::max<int64_t>(1, std::min(chunks, max_chunks));
-    const int64_t aligned = ceil_div(chunks, kXcds) * kXcds;
-    if (aligned <= max_chunks) chunks = aligned;
-    return ceil_div(ceil_div(n_words, chunks), align) * align;
-}
 
 template <int S>
 hipError_t launch_small(const uint64_t* bv, int64_t pitch, int64_t n_words, int64_t chunk, const int32_t* rows,
@@ -768,6 +947,81 @@ hipError_t launch_small(const uint64_t* bv, int64_t pitch, int64_t n_words, int6
     return hipGetLastError();
 }
 
+constexpr int64_t kLiveCounters = 8;  // int64: [0..3] live groups per small class, [4] live tile-pairs
+
+// Units one XCD can be handed by xcd_unit when at most `ub` units are live and live_chunks cuts each into
+// n_chunks <= max(1, ceil(target / n) rounded up to a multiple of 8, capped) chunks: with n_chunks >= 8 an XCD
+// takes n x ceil(n_chunks / 8) <= target / 8 + 2n units, with fewer chunks n x n_chunks / 8 < n.
+int64_t live_units_per_xcd(int64_t ub, int64_t target) { return ceil_div(target, kXcds) + 2 * ub + 1; }
+
+}  // namespace
+
+extern "C" size_t etm_count_level_ws(int64_t n_small, int64_t n_tile_pairs) {
+    return (size_t)kLiveCounters * 8 + (size_t)(std::max<int64_t>(n_small, 0) + std::max<int64_t>(n_tile_pairs, 0)) * 4;
+}
+
+namespace {
+
+// The counted level through its live units: lists of the small-class groups and tile-pairs with a counted pair
+// (k_live_units), then one launch per small class and one for the tiles over those lists. d_counters (8 int64,
+// zero on entry) may be given by the caller; else the workspace's are zeroed first.
+int live_count_impl(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const int32_t* d_rows, int32_t m,
+                    const int64_t* d_group_start, const int64_t* d_cum_pairs, const int64_t* d_cum_tiles,
+                    int64_t n_groups, int64_t n_tile_pairs, const int32_t* d_small_list, const int64_t* cls,
+                     const uint8_t* d_status, int32_t* d_counts, int32_t layout, void* d_ws, int64_t* d_counters,
+                     hipStream_t s) {
+    int64_t* cnt = d_counters != nullptr ? d_counters : (int64_t*)d_ws;
+    int32_t* live_small = (int32_t*)((int64_t*)d_ws + kLiveCounters);
+    ClassBounds cb{};
+    for (int c = 0; c < 4; ++c) cb.start[c + 1] = cb.start[c] + cls[c];
+    const int64_t n_small = cb.start[4];
+    int32_t* live_tp = live_small + n_small;
+    if (d_counters == nullptr) {
+        k_zero_i64n<<<1, 64, 0, s>>>(cnt, kLiveCounters);
+        ETM_LAUNCHED("k_zero_i64n");
+    }
+    const int64_t small_blocks = std::min<int64_t>(ceil_div(n_small, 256), 2048);
+    const int64_t tile_blocks = std::min<int64_t>(ceil_div(n_tile_pairs, 4), 8192);
+    if (small_blocks + tile_blocks > 0) {
+        k_live_units<<<(unsigned)(small_blocks + tile_blocks), 256, 0, s>>>(
+            d_small_list, n_small, cb, small_blocks, d_group_start, d_cum_pairs, d_cum_tiles, n_groups, n_tile_pairs,
+            d_status, live_small, live_tp, cnt);
+        ETM_LAUNCHED("k_live_units");
+    }
+    const bool vec = pitch % 2 == 0 && ((uintptr_t)d_bv & 15) == 0;
+    for (int c = 0; c < 4; ++c) {
+        if (cls[c] == 0) continue;
+        const int64_t blocks = kXcds * ceil_div(live_units_per_xcd(cls[c], kLiveSmallTarget), 4);
+        ETM_REQUIRE(blocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",
+                    (long long)blocks);
+        const unsigned grid = (unsigned)blocks;
+        const int32_t* list = live_small + cb.start[c];
+#define ETM_SMALL_LIVE(S, V)                                                                                        \
+    k_count_small_live<S, V><<<grid, 256, 0, s>>>(d_bv, pitch, n_words, d_rows, m, d_group_start, d_cum_pairs, list, \
+                                                  cnt + c, d_status, d_counts, layout, kLiveSmallTarget,      \
+                                                  kLiveSmallMin)
+        switch (c) {
+            case 0: if (vec) ETM_SMALL_LIVE(2, true); else ETM_SMALL_LIVE(2, false); break;
+            case 1: if (vec) ETM_SMALL_LIVE(4, true); else ETM_SMALL_LIVE(4, false); break;
+            case 2: if (vec) ETM_SMALL_LIVE(8, true); else ETM_SMALL_LIVE(8, false); break;
+            default: ETM_SMALL_LIVE(16, false); break;
+        }
+#undef ETM_SMALL_LIVE
+        ETM_LAUNCHED("k_count_small_live");
+    }
+    if (n_tile_pairs > 0) {
+        const int64_t blocks = kXcds * live_units_per_xcd(n_tile_pairs, kLiveTiledTarget);
+        ETM_REQUIRE(blocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",
+                    (long long)blocks);
+        k_count_tiled_live<<<(unsigned)blocks, 256, 0, s>>>(d_bv, pitch, n_words, d_rows, m, d_group_start,
+                                                            d_cum_pairs, d_cum_tiles, n_groups, live_tp, cnt + 4,
+                                                            d_status, d_counts, layout, kLiveTiledTarget,
+                                                            kLiveTiledMin);
+        ETM_LAUNCHED("k_count_tiled_live");
+    }
+    return ETM_OK;
+}
+
 }  // namespace
 
 extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const int32_t* d_rows, int32_t m,
@@ -775,7 +1029,7 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
                                int64_t n_groups, int64_t n_tile_pairs, const int32_t* d_small_list,
                                const int64_t* h_class_counts, const uint8_t* d_status, int32_t* d_counts,
                                int32_t layout, int32_t word_chunk, int32_t kernel, const uint64_t* d_occ,
-                               int64_t occ_pitch, void* stream) {
+                               int64_t occ_pitch, void* d_ws, size_t ws_bytes, void* stream) {
     ETM_ARG(d_bv && d_rows && d_group_start && d_cum_pairs && d_counts, "etm_count_level: null pointer");
     ETM_ARG(d_occ == nullptr || occ_pitch >= ((n_words + 63) / 64 + 63) / 64 + 1,
             "etm_count_level: occ_pitch %lld too small for %lld words", (long long)occ_pitch, (long long)n_words);
@@ -793,16 +1047,22 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
         }
     const int64_t n_small = cls[0] + cls[1] + cls[2] + cls[3];
     ETM_ARG(n_small == 0 || d_small_list != nullptr, "etm_count_level: null small-group list");
+    ETM_ARG(d_ws == nullptr || ws_bytes >= etm_count_level_ws(n_small, n_tile_pairs),
+            "etm_count_level: workspace %zu < %zu bytes", ws_bytes, etm_count_level_ws(n_small, n_tile_pairs));
+    ETM_ARG(n_tile_pairs <= 0x7FFFFFFF && n_small <= 0x7FFFFFFF, "etm_count_level: %lld tile-pairs exceed int32",
+            (long long)n_tile_pairs);
     if (n_words == 0) return ETM_OK;
     hipStream_t s = ETM_STREAM(stream);
+    // With a workspace and a status array (and no occupancy skip) only the units holding a counted pair run.
+    if (d_ws != nullptr && d_status != nullptr && d_occ == nullptr && word_chunk <= 0)
+        return live_count_impl(d_bv, pitch, n_words, d_rows, m, d_group_start, d_cum_pairs, d_cum_tiles, n_groups,
+                               n_tile_pairs, d_small_list, cls, d_status, d_counts, layout, d_ws, nullptr, s);
     int64_t base = 0;
     for (int c = 0; c < 4; ++c) {
         if (cls[c] == 0) continue;
-        // About 2048 waves per class and chunks of at least 256 words: classes of many groups keep long chunks
-        // (fewer partial sums, chunk-major L2 reuse per XCD), classes of few groups get enough waves (measured
-        // on MI355X over 256-4096-word floors and 2K-64K wave targets; with 16-byte loads 2048 beat 4096 and
-        // 8192 on deep_k_1m).
-        int64_t chunk = word_chunk > 0 ? word_chunk : pick_chunk(n_words, cls[c], 2048, 64, 256);
+        // About kSmallTarget waves per class and chunks of at least 256 words: classes of many groups keep long
+        // chunks (fewer partial sums, chunk-major L2 reuse per XCD), classes of few groups get enough waves.
+        int64_t chunk = word_chunk > 0 ? word_chunk : pick_chunk(n_words, cls[c], kSmallTarget, 64, 256);
         // The tile skip reads one 64-bit occupancy window per column: at most 64 tiles of 64 words per chunk.
         if (d_occ) chunk = std::min<int64_t>(ceil_div(chunk, 64) * 64, 4096);
         hipError_t e;
@@ -848,7 +1108,7 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
         ETM_REQUIRE(n_tile_pairs <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld tile-pairs exceed the grid",
                     (long long)n_tile_pairs);
         const int64_t chunk = word_chunk > 0 ? ceil_div(word_chunk, kKW) * kKW
-                                             : pick_chunk(n_words, n_tile_pairs, 2048, kKW, 128);
+                                             : pick_chunk(n_words, n_tile_pairs, kTiledTarget, kKW, 128);
         const int64_t n_chunks = ceil_div(n_words, chunk);
         const int64_t blocks = kXcds * xcd_units(n_tile_pairs, n_chunks);
         ETM_REQUIRE(blocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",
@@ -859,6 +1119,21 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
         ETM_LAUNCHED("k_count_tiled");
     }
     return ETM_OK;
+}
+
+int etm::detail::count_level_live(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const int32_t* d_rows,
+                                  int32_t m, const int64_t* d_group_start, const int64_t* d_cum_pairs,
+                                  const int64_t* d_cum_tiles, int64_t n_groups, int64_t n_tile_pairs,
+                                  const int32_t* d_small_list, const int64_t* h_class_counts, const uint8_t* d_status,
+                                  int32_t* d_counts, void* d_ws, int64_t* d_counters, void* stream) {
+    ETM_ARG(d_bv && d_rows && d_group_start && d_cum_pairs && d_counts && d_status && d_ws && d_counters,
+            "count_level_live: null pointer");
+    ETM_ARG(pitch >= n_words && n_words >= 0 && m >= 1 && m <= 62, "count_level_live: bad sizes");
+    ETM_ARG(n_tile_pairs <= 0x7FFFFFFF, "count_level_live: %lld tile-pairs exceed int32", (long long)n_tile_pairs);
+    if (n_words == 0) return ETM_OK;
+    return live_count_impl(d_bv, pitch, n_words, d_rows, m, d_group_start, d_cum_pairs, d_cum_tiles, n_groups,
+                           n_tile_pairs, d_small_list, h_class_counts, d_status, d_counts, ETM_LAYOUT_LEX, d_ws,
+                           d_counters, ETM_STREAM(stream));
 }
 
 extern "C" int etm_count_itemsets(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const int32_t* d_items,

@@ -1159,6 +1159,51 @@ __global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
         p[i] = v;
 }
 
+// ── what the counting kernels must read ────────────────────────────────
+
+// Marks, for every candidate left ETM_CAND_COUNT, its group (bit g of group_bits) and its two suffix rows (bits
+// gs + i and gs + j of row_bits), and counts the counted candidates into out[2].
+__global__ __launch_bounds__(256) void k_live_mark(const uint8_t* __restrict__ status, const int32_t* __restrict__ gid,
+                                                   const int64_t* __restrict__ group_start,
+                                                   const int64_t* __restrict__ cum_pairs, int64_t n_groups,
+                                                   int64_t n_cands, unsigned long long* __restrict__ row_bits,
+                                                   unsigned long long* __restrict__ group_bits, int64_t* out) {
+    __shared__ int64_t lds[256 / etm::kWave + 1];
+    int64_t n_live = 0;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n_cands; c += (int64_t)gridDim.x * blockDim.x) {
+        if (status[c] != ETM_CAND_COUNT) continue;
+        ++n_live;
+        const int64_t g = gid ? gid[c] : etm::upper_index(cum_pairs, n_groups, c);
+        const int64_t gs = group_start[g];
+        int64_t i, j;
+        etm::lex_pair_decode(c - cum_pairs[g], group_start[g + 1] - gs, &i, &j);
+        atomicOr(&group_bits[g >> 6], 1ull << (g & 63));
+        atomicOr(&row_bits[(gs + i) >> 6], 1ull << ((gs + i) & 63));
+        atomicOr(&row_bits[(gs + j) >> 6], 1ull << ((gs + j) & 63));
+    }
+    n_live = etm::block_sum<256>(n_live, lds);
+    if (threadIdx.x == 0 && n_live) atomicAdd((unsigned long long*)&out[2], (unsigned long long)n_live);
+}
+
+// out[0] += set bits of row_bits (n_row_words words), out[1] += set bits of group_bits (n_group_words words).
+__global__ __launch_bounds__(256) void k_live_sum(const uint64_t* __restrict__ row_bits, int64_t n_row_words,
+                                                  const uint64_t* __restrict__ group_bits, int64_t n_group_words,
+                                                  int64_t* out) {
+    __shared__ int64_t lds[256 / etm::kWave + 1];
+    int64_t a = 0, b = 0;
+    for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < n_row_words || w < n_group_words;
+         w += (int64_t)gridDim.x * blockDim.x) {
+        if (w < n_row_words) a += __popcll(row_bits[w]);
+        if (w < n_group_words) b += __popcll(group_bits[w]);
+    }
+    a = etm::block_sum<256>(a, lds);
+    b = etm::block_sum<256>(b, lds);
+    if (threadIdx.x == 0) {
+        if (a) atomicAdd((unsigned long long*)&out[0], (unsigned long long)a);
+        if (b) atomicAdd((unsigned long long*)&out[1], (unsigned long long)b);
+    }
+}
+
 }  // namespace
 
 namespace etm {
@@ -1214,6 +1259,23 @@ hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s) {
 
 hipError_t launch_zero_i64(int64_t* p, int n, hipStream_t s) {
     k_zero_i64<<<std::max(1, std::min((n + 255) / 256, 1024)), 256, 0, s>>>(p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_live_bytes(const uint8_t* status, const int32_t* gid, const int64_t* group_start,
+                             const int64_t* cum_pairs, int64_t n_groups, int64_t n_cands, int64_t n_rows,
+                             uint64_t* row_bits, uint64_t* group_bits, int64_t* out, hipStream_t s) {
+    const int64_t rw = (n_rows + 63) / 64, gw = (n_groups + 63) / 64;
+    if (rw + gw > 0x7FFFFFFF) return hipErrorInvalidValue;
+    hipError_t e = launch_zero_i64((int64_t*)row_bits, (int)rw, s);
+    if (e == hipSuccess) e = launch_zero_i64((int64_t*)group_bits, (int)gw, s);
+    if (e != hipSuccess || n_cands == 0) return e;
+    k_live_mark<<<grid_for(n_cands), 256, 0, s>>>(status, gid, group_start, cum_pairs, n_groups, n_cands,
+                                                  (unsigned long long*)row_bits, (unsigned long long*)group_bits,
+                                                  out);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_live_sum<<<grid_for(std::max(rw, gw), 1024), 256, 0, s>>>(row_bits, rw, group_bits, gw, out);
     return hipGetLastError();
 }
 

@@ -9,6 +9,7 @@
 // multi-block scans with one extra sync. Device memory comes from the stream-ordered pool
 // (hipMallocAsync / hipFreeAsync), released when the call returns.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -113,6 +114,8 @@ class Arena {
         }
         h_.used = 0;
     }
+    // Device bytes the current call has taken so far.
+    int64_t bytes() const { return (int64_t)h_.demand; }
     template <class T>
     T* get(int64_t n) {
         const size_t bytes = (std::max<size_t>((size_t)std::max<int64_t>(n, 1) * sizeof(T), 256) + 255) & ~(size_t)255;
@@ -184,7 +187,10 @@ struct Seg {
 // Largest group (suffixes) counted by the wave-per-group kernels; bigger groups go to the tiled kernel, whose
 // listed mode stages only a group's live columns. Measured on MI355X: 8 beat 16 (the 16-suffix kernel holds
 // 120 accumulators a lane, two waves a SIMD) and 4 on deep_k_1m, wide_vocab_2m and the served stream.
-constexpr int32_t kSmallMax = 8;
+constexpr int32_t kSmallMaxDefault = 8;
+// Columns of at least this many words take the live-unit schedule automatically (with Pascal inference, every
+// length does): there a level's units are long to stream, so launching only the live ones pays off.
+constexpr int64_t kLiveMinWords = 65536;
 
 constexpr int kGatherSegs = 32;
 struct Segs {
@@ -256,6 +262,29 @@ void pool_give(int32_t* p, int64_t cap) {
     }
 }
 
+using Clock = std::chrono::steady_clock;
+
+double ms_since(Clock::time_point t) {
+    return std::chrono::duration<double, std::milli>(Clock::now() - t).count();
+}
+
+// Hands a finished level to the caller's callback (when there is one); a nonzero return stops the call.
+void report(const etm_mine_opts& o, const etm_levels& out, const etm_level_stat& st, const int32_t* h_rows,
+            const int32_t* h_counts, int64_t device_bytes, int64_t host_bytes, bool next_level) {
+    if (o.level_cb == nullptr) return;
+    etm_level_info info{};
+    info.stat = st;
+    info.h_rows = h_rows;
+    info.h_counts = h_counts;
+    info.h_col_to_item = out.col_to_item.empty() ? nullptr : out.col_to_item.data();
+    info.n_cols = (int64_t)out.col_counts.size();
+    info.device_bytes = device_bytes;
+    info.host_bytes = host_bytes;
+    info.next_level = next_level ? 1 : 0;
+    if (o.level_cb(&info, o.cb_user) != 0)
+        throw Error(ETM_ECANCELED, "the level callback stopped mining after K=" + std::to_string(st.k));
+}
+
 int64_t pow2_at_least(int64_t x) {
     int64_t v = 1;
     while (v < x) v <<= 1;
@@ -272,9 +301,9 @@ struct Level {
     int32_t m = 0;
 };
 
-// Device scalars of one level: [0] survivors, [1..3] tally, [4] selected (free-set prune), [8..23] totals of the
-// next level's groups.
-constexpr int kScal = 32;
+// Device scalars of one level: [0] survivors, [1..3] tally, [4] selected (free-set prune), [5..7] live bytes,
+// [8..31] totals of the next level's groups, [32..39] live-unit counters of the counting launches.
+constexpr int kScal = 40;
 
 struct GroupsDev {
     int64_t *gs, *cp, *ct;
@@ -287,15 +316,16 @@ GroupsDev alloc_groups(Arena& a, int64_t n) {
 
 // Group build of a level (single workgroup when small and d_n given; else the multi-block scans, which need n).
 void build_groups(Arena& a, const Level& lv, const int64_t* d_n, int64_t n_cap, GroupsDev& g, int64_t* d_totals,
-                  hipStream_t s) {
+                  int32_t small_max, hipStream_t s) {
     if (n_cap <= etm::detail::kSingleMax) {
-        check(etm::detail::launch_groups_single(lv.rows, d_n, n_cap, lv.m, kSmallMax, g.gs, g.cp, g.ct, g.list, d_totals, s),
+        check(etm::detail::launch_groups_single(lv.rows, d_n, n_cap, lv.m, small_max, g.gs, g.cp, g.ct, g.list,
+                                                d_totals, s),
               "group build");
         return;
     }
     const size_t ws_n = etm_level_groups_ws(n_cap);
     int64_t* ws = a.get<int64_t>((int64_t)(ws_n + 7) / 8);
-    check_rc(etm_level_groups(lv.rows, n_cap, lv.m, kSmallMax, g.gs, g.cp, g.ct, g.list, d_totals, ws, ws_n, s),
+    check_rc(etm_level_groups(lv.rows, n_cap, lv.m, small_max, g.gs, g.cp, g.ct, g.list, d_totals, ws, ws_n, s),
              "etm_level_groups");
 }
 
@@ -322,14 +352,23 @@ constexpr int64_t kRowsBitsMax = 512ll << 20;
 // Largest K=3 pair index (12 bytes per column x 64 columns: 75 MB at 20,000 columns).
 constexpr int64_t kPairIndexMaxBytes = 256ll << 20;
 
+// Levels K=1 .. exhaustion of device bitvectors. t_call: the call's start (K=1's wall time runs from it);
+// base_bytes: device bytes the caller's arena already holds for this call (the input layer's).
 void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n_rows, const int64_t* col_counts,
-                 int32_t n_cols, const etm_mine_opts& o, etm_levels& out, hipStream_t s,
+                 int32_t n_cols, const etm_mine_opts& o, etm_levels& out, hipStream_t s, Clock::time_point t_call,
                  const RowsSrc* src = nullptr, bool ordered = false) {
     if (n_rows > 0x7FFFFFFF) throw Error(ETM_EOVERFLOW, "n_rows exceeds int32: the count arrays are int32");
     const int64_t mc = std::max<int64_t>(1, o.min_count);
     const bool prune = o.prune_non_free != 0;
     const bool track_free = prune || o.infer_counts != 0;
     const int64_t max_len = o.max_length > 0 ? std::min<int64_t>(o.max_length, n_cols) : n_cols;
+    if (o.small_max != 0 && o.small_max != 4 && o.small_max != 8 && o.small_max != 16)
+        throw Error(ETM_EINVAL, "small_max must be 0, 4, 8 or 16, got " + std::to_string(o.small_max));
+    if (o.k3_schedule < ETM_SCHEDULE_AUTO || o.k3_schedule > ETM_SCHEDULE_LIVE)
+        throw Error(ETM_EINVAL, "k3_schedule must be one of ETM_SCHEDULE_*");
+    const int32_t small_max = o.small_max ? o.small_max : kSmallMaxDefault;
+    const bool live_schedule = o.k3_schedule == ETM_SCHEDULE_LIVE ||
+                               (o.k3_schedule == ETM_SCHEDULE_AUTO && (o.infer_counts != 0 || n_words >= kLiveMinWords));
 
     // K = 1.
     std::vector<int32_t> k1_rows;
@@ -340,10 +379,11 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         k1_rows.push_back(c);
         k1_counts.push_back((int32_t)col_counts[c]);
     }
+    const bool emit = o.emit_levels != 0;
     out.k1_rows = k1_rows;
     out.k1_counts = k1_counts;
     out.k.push_back(1);
-    out.n.push_back((int64_t)out.k1_counts.size());
+    out.n.push_back(emit ? 0 : (int64_t)out.k1_counts.size());
     out.rows.push_back(out.k1_rows.data());
     out.counts.push_back(out.k1_counts.data());
     etm_level_stat st1{};
@@ -352,10 +392,18 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
     st1.n_counted = n_cols;
     st1.n_frequent = (int64_t)k1_rows.size();
     st1.bytes_alg = (int64_t)n_cols * n_words * 8;
+    st1.bytes_live = st1.bytes_alg;
+    st1.wall_ms = (float)ms_since(t_call);
     out.stats.push_back(st1);
-    if (k1_rows.empty() || max_len < 2) return;
-
     Arena a(s);
+    // Host bytes the kept levels take once copied back (K=1 lives on the host already).
+    int64_t host_bytes = emit ? 0 : (int64_t)k1_rows.size() * 8;
+    const int64_t n1_all = (int64_t)k1_rows.size();
+    report(o, out, st1, emit ? k1_rows.data() : nullptr, emit ? k1_counts.data() : nullptr, a.bytes(), host_bytes,
+           n1_all >= 2 && max_len >= 2);
+    if (k1_rows.empty() || max_len < 2) return;
+    Clock::time_point t_level = Clock::now();
+
     Pinned host;
     const int64_t n1 = (int64_t)k1_rows.size();
     Level prev, prev2, prev3;  // levels k-1, k-2, k-3 of the level being mined
@@ -422,7 +470,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         occ = a.get<uint64_t>((int64_t)n_cols * occ_pitch);
         check_rc(etm_tile_occupancy(d_bv, pitch, n_words, n_cols, occ, occ_pitch, s), "etm_tile_occupancy");
     };
-    build_groups(a, prev, nullptr, n1, g, scal + 8, s);
+    build_groups(a, prev, nullptr, n1, g, scal + 8, small_max, s);
     check(hipMemcpyAsync(host.p, scal, kScal * 8, hipMemcpyDeviceToHost, s), "readback");
     wait_stream(s);
     int64_t tot[24];
@@ -541,14 +589,32 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         } else {
             const bool skip_tiles = ordered && k >= 3 && cls[0] + cls[1] + cls[2] + cls[3] > 0;
             if (skip_tiles) ensure_occ();
-            check_rc(etm_count_level(d_bv, pitch, n_words, prev.rows, m, g.gs, g.cp, g.ct, n_groups, n_tiles, g.list,
-                                     cls, status, counts, ETM_LAYOUT_LEX, 0, kern, skip_tiles ? occ : nullptr,
-                                     occ_pitch, s),
-                     "etm_count_level");
+            if (status != nullptr && !skip_tiles && live_schedule) {
+                // From K=3 on, only the groups and tile-pairs holding a counted candidate are launched; their
+                // counters are the level's scalars [32..39], zeroed with the others.
+                const size_t ws_n = etm_count_level_ws(cls[0] + cls[1] + cls[2] + cls[3], n_tiles);
+                void* ws = a.get<int64_t>((int64_t)(ws_n + 7) / 8);
+                check_rc(etm::detail::count_level_live(d_bv, pitch, n_words, prev.rows, m, g.gs, g.cp, g.ct, n_groups,
+                                                       n_tiles, g.list, cls, status, counts, ws, cur_scal + 32, s),
+                         "etm_count_level");
+            } else {
+                check_rc(etm_count_level(d_bv, pitch, n_words, prev.rows, m, g.gs, g.cp, g.ct, n_groups, n_tiles,
+                                         g.list, cls, status, counts, ETM_LAYOUT_LEX, 0, kern,
+                                         skip_tiles ? occ : nullptr, occ_pitch, nullptr, 0, s),
+                         "etm_count_level");
+            }
         }
         if (o.time_kernels) {
             check(hipEventRecord(e1, s), "event record");
             events.emplace_back(e0, e1);
+        }
+        // What the counting kernels had to read for the candidates left counted (cur_scal[5..7]).
+        if (o.live_bytes && status != nullptr) {
+            uint64_t* row_bits = a.get<uint64_t>((prev.n + 63) / 64);
+            uint64_t* group_bits = a.get<uint64_t>((n_groups + 63) / 64);
+            check(etm::detail::launch_live_bytes(status, gid, g.gs, g.cp, n_groups, n_cands, prev.n, row_bits,
+                                                 group_bits, cur_scal + 5, s),
+                  "live bytes");
         }
         // Survivors, in lexicographic order.
         Level cur;
@@ -618,7 +684,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         const bool early = cap <= etm::detail::kSingleMax && k + 1 <= max_len;
         if (early) {
             gn = alloc_groups(a, cap);
-            check(etm::detail::launch_groups_single(cur.rows, d_n, cap, k, kSmallMax, gn.gs, gn.cp, gn.ct, gn.list,
+            check(etm::detail::launch_groups_single(cur.rows, d_n, cap, k, small_max, gn.gs, gn.cp, gn.ct, gn.list,
                                                     cur_scal + 8, s),
                   "group build");
         }
@@ -629,15 +695,33 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         st.n_inferred = k >= 3 ? host.p[2] : 0;
         st.n_skipped = k >= 3 ? host.p[3] : 0;
         st.n_frequent = cur.n;
+        st.bytes_live = (o.live_bytes && status != nullptr)
+                            ? ((int64_t)(m - 1) * host.p[6] + host.p[5]) * n_words * 8 + 4 * host.p[7]
+                            : (status == nullptr ? st.bytes_alg : 0);
+        const int32_t* h_rows = nullptr;
+        const int32_t* h_counts = nullptr;
+        if (emit && cur.n > 0 && o.level_cb) {
+            int32_t* buf = staging(cur.n * (k + 1));
+            check(hipMemcpyAsync(buf, cur.rows, (size_t)(cur.n * k) * 4, hipMemcpyDeviceToHost, s), "level out");
+            check(hipMemcpyAsync(buf + cur.n * k, cur.counts, (size_t)cur.n * 4, hipMemcpyDeviceToHost, s),
+                  "level out");
+            wait_stream(s);
+            h_rows = buf;
+            h_counts = buf + cur.n * k;
+        }
+        st.wall_ms = (float)ms_since(t_level);
         out.stats.push_back(st);
-        levels.push_back(Pending{cur.rows, cur.counts, cur.n, k});
+        levels.push_back(emit ? Pending{nullptr, nullptr, 0, k} : Pending{cur.rows, cur.counts, cur.n, k});
+        if (!emit) host_bytes += cur.n * (int64_t)(k + 1) * 4;
+        report(o, out, st, h_rows, h_counts, a.bytes(), host_bytes, k + 1 <= max_len && cur.n >= k + 1);
+        t_level = Clock::now();
         if (cur.n == 0) break;
         if (early) {
             std::memcpy(tot, host.p + 8, sizeof tot);
             g = gn;
         } else if (k + 1 <= max_len) {
             g = alloc_groups(a, cur.n);
-            build_groups(a, cur, nullptr, cur.n, g, cur_scal + 8, s);
+            build_groups(a, cur, nullptr, cur.n, g, cur_scal + 8, small_max, s);
             check(hipMemcpyAsync(host.p, cur_scal, kScal * 8, hipMemcpyDeviceToHost, s), "readback");
             wait_stream(s);
             std::memcpy(tot, host.p + 8, sizeof tot);
@@ -720,11 +804,13 @@ extern "C" int etm_mine_bitvecs(const uint64_t* d_bv, int64_t pitch, int64_t n_w
                                 etm_levels** out, void* stream) {
     ETM_ARG(d_bv && opts && out && (n_cols == 0 || h_col_counts), "etm_mine_bitvecs: null pointer");
     ETM_ARG(pitch >= n_words && n_words == (n_rows + 63) / 64, "etm_mine_bitvecs: bad pitch / n_words");
+    ETM_ARG(!opts->emit_levels || opts->level_cb, "etm_mine_bitvecs: emit_levels needs a level callback");
+    const Clock::time_point t_call = Clock::now();
     *out = nullptr;
     auto res = std::make_unique<etm_levels>();
     res->col_counts.assign(h_col_counts, h_col_counts + n_cols);
     const int rc = guarded([&] {
-        mine_levels(d_bv, pitch, n_words, n_rows, h_col_counts, n_cols, *opts, *res, ETM_STREAM(stream));
+        mine_levels(d_bv, pitch, n_words, n_rows, h_col_counts, n_cols, *opts, *res, ETM_STREAM(stream), t_call);
     });
     if (rc == ETM_OK) *out = res.release();
     return rc;
@@ -735,6 +821,8 @@ extern "C" int etm_mine_csr(const int64_t* d_indptr, const void* d_ids, int id_b
     ETM_ARG(d_indptr && opts && out && (nnz == 0 || d_ids), "etm_mine_csr: null pointer");
     ETM_ARG(id_bytes == 4 || id_bytes == 8, "etm_mine_csr: id_bytes must be 4 or 8");
     ETM_ARG(n_rows >= 0 && nnz >= 0, "etm_mine_csr: negative size");
+    ETM_ARG(!opts->emit_levels || opts->level_cb, "etm_mine_csr: emit_levels needs a level callback");
+    const Clock::time_point t_call = Clock::now();
     *out = nullptr;
     auto res = std::make_unique<etm_levels>();
     hipStream_t s = ETM_STREAM(stream);
@@ -806,7 +894,8 @@ extern "C" int etm_mine_csr(const int64_t* d_indptr, const void* d_ids, int id_b
         check(hipMemcpyAsync(res->col_counts.data(), pc, n_cols * 8, hipMemcpyDeviceToHost, s), "readback");
         wait_stream(s);
         const RowsSrc src{d_indptr, d_ids, id_bytes, nnz, lo, span, d_lut};
-        mine_levels(bv, pitch, n_words, n_rows, res->col_counts.data(), n_cols, *opts, *res, s, &src, ordered);
+        mine_levels(bv, pitch, n_words, n_rows, res->col_counts.data(), n_cols, *opts, *res, s, t_call, &src,
+                    ordered);
     });
     if (rc == ETM_OK) *out = res.release();
     return rc;

@@ -17,13 +17,17 @@ from ctypes import POINTER, c_char_p, c_int, c_int32, c_int64, c_size_t, c_void_
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libetm.so"
-ABI_VERSION = 3
+ABI_VERSION = 4
 
-ETM_OK, ETM_EINVAL, ETM_ENOMEM, ETM_EHIP, ETM_EOVERFLOW, ETM_ENODEV = 0, 1, 2, 3, 4, 5
+ETM_OK, ETM_EINVAL, ETM_ENOMEM, ETM_EHIP, ETM_EOVERFLOW, ETM_ENODEV, ETM_ECANCELED = 0, 1, 2, 3, 4, 5, 6
 CAND_COUNT, CAND_SKIP, CAND_INFER = 0, 1, 2
 SUBSET_PRUNE, SUBSET_INFER = 1, 2
 DEDUCTION_AUTO, DEDUCTION_OFF = 0, 1
 ROW_ORDERS = {"auto": 0, "keep": 1, "lex": 2}
+#: etm_mine_opts.k3_schedule (ETM_SCHEDULE_*).
+K3_SCHEDULES = {"auto": 0, "all": 1, "live": 2}
+#: Accepted etm_mine_opts.small_max values (0 = the library's default, 8).
+SMALL_MAXES = (0, 4, 8, 16)
 #: Words per tile of the occupancy bitmap (ETM_OCC_TILE_WORDS).
 OCC_TILE_WORDS = 64
 #: Largest candidate the deduction bounds settle (etm_level_deduce).
@@ -65,7 +69,8 @@ SIGNATURES: dict = {
     "etm_level_deduce": (c_int, [P, c_int32, P, P, c_int64, P, P, P, c_int64, P, P, P, c_int64, c_int64, c_int64,
                                  P, P, c_int64, c_int64, P, P, P, P, P]),
     "etm_count_level": (c_int, [P, c_int64, c_int64, P, c_int32, P, P, P, c_int64, c_int64, P, P, P, P, c_int32,
-                                c_int32, c_int32, P, c_int64, P]),
+                                c_int32, c_int32, P, c_int64, P, c_size_t, P]),
+    "etm_count_level_ws": (c_size_t, [c_int64, c_int64]),
     "etm_level_compact_ws": (c_size_t, [c_int64]),
     "etm_level_compact": (c_int, [P, c_int32, P, P, c_int64, c_int64, P, P, P, c_int64, P, P, P, P, P, c_size_t, P]),
     "etm_level_compact_noalloc_max": (c_int64, []),
@@ -92,23 +97,41 @@ SIGNATURES: dict = {
 }
 
 
-class MineOpts(ctypes.Structure):
-    """etm_mine_opts."""
-
-    _fields_ = [("min_count", c_int64), ("max_length", c_int32), ("prune_non_free", c_int32),
-                ("infer_counts", c_int32), ("time_kernels", c_int32), ("k2_kernel", c_int32),
-                ("k3_kernel", c_int32), ("deduction", c_int32), ("row_order", c_int32)]
-
-
 class LevelStat(ctypes.Structure):
     """etm_level_stat."""
 
     _fields_ = [("k", c_int32), ("kernel", c_int32), ("n_candidates", c_int64), ("n_counted", c_int64),
                 ("n_inferred", c_int64), ("n_skipped", c_int64), ("n_frequent", c_int64), ("bytes_alg", c_int64),
-                ("count_ms", ctypes.c_float), ("pad2", ctypes.c_float)]
+                ("bytes_live", c_int64), ("count_ms", ctypes.c_float), ("wall_ms", ctypes.c_float)]
+
+
+class LevelInfo(ctypes.Structure):
+    """etm_level_info: what the level callback receives as each level completes."""
+
+    _fields_ = [("stat", LevelStat), ("h_rows", POINTER(c_int32)), ("h_counts", POINTER(c_int32)),
+                ("h_col_to_item", POINTER(c_int64)), ("n_cols", c_int64), ("device_bytes", c_int64), ("host_bytes", c_int64), ("next_level", c_int32),
+                ("reserved", c_int32)]
+
+
+#: etm_level_cb: int (*)(const etm_level_info*, void*)
+LEVEL_CB = ctypes.CFUNCTYPE(c_int, POINTER(LevelInfo), c_void_p)
+
+
+class MineOpts(ctypes.Structure):
+    """etm_mine_opts."""
+
+    _fields_ = [("min_count", c_int64), ("max_length", c_int32), ("prune_non_free", c_int32),
+                ("infer_counts", c_int32), ("time_kernels", c_int32), ("k2_kernel", c_int32),
+                ("k3_kernel", c_int32), ("deduction", c_int32), ("row_order", c_int32), ("live_bytes", c_int32),
+                ("emit_levels", c_int32), ("k3_schedule", c_int32), ("small_max", c_int32), ("level_cb", LEVEL_CB),
+                ("cb_user", c_void_p)]
 
 _lock = threading.Lock()
 _lib = None
+
+
+class Cancelled(RuntimeError):
+    """A level callback stopped a mining call (the callback's own exception is re-raised by the caller)."""
 
 
 class LibraryMissing(RuntimeError):
@@ -150,6 +173,8 @@ def call(name: str, *args) -> int:
     if rc == ETM_OK:
         return rc
     msg = f"{name}: {last_error()}"
+    if rc == ETM_ECANCELED:
+        raise Cancelled(msg)
     if rc in (ETM_EINVAL, ETM_EOVERFLOW):
         raise ValueError(msg)
     if rc == ETM_ENOMEM:

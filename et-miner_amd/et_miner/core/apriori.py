@@ -18,7 +18,6 @@ from __future__ import annotations
 
 import math
 import numbers
-import time
 import warnings
 from math import comb
 
@@ -227,7 +226,8 @@ def apriori(
             raise ValueError("Cannot provide both 'transactions' and 'bitvecs'. Use 'bitvecs' for pre-built GPU "
                              "bitvectors or 'transactions' for DataFrame input.")
         return done(_mine_bitvecs(bitvecs, min_support, max_length, prune_equal_support, use_generator_pruning,
-                                  level_callback, session, warn_complexity, output_dir, resume_from_k))
+                                  level_callback, session, warn_complexity, output_dir, resume_from_k, max_ram_gb,
+                                  max_vram_gb))
     if transactions is None:
         raise ValueError("Either 'transactions' or 'bitvecs' must be provided")
     if streaming:
@@ -244,14 +244,16 @@ def apriori(
         session.end_phase(n_transactions=len(indptr) - 1)
     return done(mine_csr(indptr, ids, min_support, max_length, prune_equal_support, use_generator_pruning,
                          level_callback, session, items=items, warn_complexity=warn_complexity,
-                         output_dir=output_dir, resume_from_k=resume_from_k))
+                         output_dir=output_dir, resume_from_k=resume_from_k, max_ram_gb=max_ram_gb,
+                         max_vram_gb=max_vram_gb))
 
 
 def mine_csr(indptr, ids, min_support, max_length=None, prune_equal_support=False, use_generator_pruning=False,
              level_callback=None, session=None, items=None, warn_complexity=False, device_csr=None,
              with_counts=False, timer=None, stats_out=None, engine: str = "native", time_kernels=False,
              k2_kernel: str = "auto", k3_kernel: str = "auto", row_order: str = "auto", output_dir=None,
-             resume_from_k=None):
+             resume_from_k=None, max_ram_gb=None, max_vram_gb=None, live_bytes=False, k3_schedule: str = "auto",
+             small_max: int = 0):
     """The use_gpu route from a host CSR (or ``device_csr=(indptr_d, ids_d)`` already in HBM).
 
     ``engine="native"`` runs the whole level loop inside libetm (one C call); ``engine="python"`` drives the
@@ -261,10 +263,15 @@ def mine_csr(indptr, ids, min_support, max_length=None, prune_equal_support=Fals
     kernel; ``k3_kernel`` ("auto", "valu", "rows") picks K=3's; ``row_order`` ("auto", "keep", "lex") the
     bitvectors' row order (native engine only; counts never depend on it).
 
-    ``output_dir`` writes each level to ``frequent_k{k}.parquet`` there and returns an empty result, as the
-    reference's row-split miner does (``gpu/row_split.py:1172-1178``); with ``resume_from_k`` >= 2 as well, level
-    ``resume_from_k`` is read back from that directory and mining continues above it (the Python level loop;
-    ``gpu/row_split.py:539-606``).
+    ``level_callback(k, n_candidates, n_frequent, duration_ms)`` is called as each level completes, with the
+    level's wall time; ``max_ram_gb`` / ``max_vram_gb`` raise MemoryError between levels as the reference's
+    memory guard does (``gpu/row_split.py:467-487``). ``output_dir`` writes each level to
+    ``frequent_k{k}.parquet`` there as it completes (host memory holds one level at a time) and returns an
+    empty result, as the reference's row-split miner does (``gpu/row_split.py:512-537``, ``:1172-1178``); with
+    ``resume_from_k`` >= 2 as well, level ``resume_from_k`` is read back from that directory and mining
+    continues above it (the Python level loop; ``gpu/row_split.py:539-606``). ``live_bytes`` fills the native
+    loop's ``LevelStats.bytes_live``; ``k3_schedule`` and ``small_max`` pick the native loop's launch schedule of
+    levels K >= 3 (see ``gpu.native.mine_csr_device``).
     """
     from ..gpu import device as dev
 
@@ -273,10 +280,10 @@ def mine_csr(indptr, ids, min_support, max_length=None, prune_equal_support=Fals
     if n_rows <= 0:
         return itemset_table([], np.zeros(0, np.int64), 1, with_counts=True) if with_counts else empty_result()
     min_count = _mining_min_count(min_support, n_rows)
-    if output_dir is not None:
-        return _mine_to_dir(indptr, ids, device_csr, n_rows, min_count, min_support, max_length,
-                            prune_equal_support, use_generator_pruning, level_callback, session, items,
-                            output_dir, resume_from_k, stats_out, k2_kernel, k3_kernel, row_order)
+    if output_dir is not None and resume_from_k is not None and resume_from_k >= 2:
+        return _resume_csr(indptr, ids, device_csr, n_rows, min_count, max_length, prune_equal_support,
+                           use_generator_pruning, level_callback, session, items, output_dir, resume_from_k,
+                           stats_out, max_ram_gb, max_vram_gb)
     if session:
         session.start_phase("input_layer")
     if device_csr is None:
@@ -284,46 +291,49 @@ def mine_csr(indptr, ids, min_support, max_length=None, prune_equal_support=Fals
 
         device_csr = host_csr_to_device(indptr, ids)
     indptr_d, ids_d = device_csr
+    hooks = _LevelHooks(level_callback, session, max_ram_gb, max_vram_gb, output_dir, n_rows, items)
     if engine == "native":
         from ..gpu.native import mine_csr_device
 
+        if session:
+            session.end_phase()
         res = mine_csr_device(indptr_d, ids_d, n_rows, min_count, max_length, prune_equal_support,
                               use_generator_pruning, time_kernels=time_kernels, levels=False, k2_kernel=k2_kernel,
-                              k3_kernel=k3_kernel, row_order=row_order)
-        if session:
-            session.end_phase(n_columns=len(res.col_to_item))
+                              k3_kernel=k3_kernel, row_order=row_order, on_level=hooks.native(),
+                              emit_levels=hooks.emit, live_bytes=live_bytes, k3_schedule=k3_schedule,
+                              small_max=small_max)
         if warn_complexity and res.stats:
             _warn_complexity(res.stats[0].n_frequent, min_support)
-        cb = _timed(level_callback, session)
-        if cb:
-            for st in res.stats:
-                cb(st.k, st.n_candidates, st.n_frequent, st.count_ms)
         if stats_out is not None:
             stats_out.extend(res.stats)
+        if hooks.emit:
+            return empty_result()
         table = res.table(n_rows, with_counts)
         if items is not None:
             flat = table.column("itemset").combine_chunks()
             table = pa.table({"itemset": pa.LargeListArray.from_arrays(flat.offsets, items.take(flat.values)),
                               "support": table.column("support")})
         return table if with_counts else to_frame(table)
-    else:
-        from ..gpu.input_layer import build_device_input
-        from ..gpu.row_split import mine_levels
+    from ..gpu.input_layer import build_device_input
+    from ..gpu.row_split import mine_levels
 
-        built = build_device_input(indptr_d, ids_d, n_rows, min_count)
-        if session:
-            session.end_phase(n_columns=0 if built is None else len(built.col_to_item))
-        if built is None:
-            levels, stats, col_to_item = [], [], np.zeros(0, np.int64)
-        else:
-            if warn_complexity:
-                _warn_complexity(int((built.col_counts >= min_count).sum()), min_support)
-            res = mine_levels(built.bitvecs, built.n_words, n_rows, built.col_counts, min_count, max_length,
-                              prune_equal_support, use_generator_pruning, _timed(level_callback, session),
-                              timer=timer)
-            levels, stats, col_to_item = res.levels, res.stats, built.col_to_item
+    built = build_device_input(indptr_d, ids_d, n_rows, min_count)
+    if session:
+        session.end_phase(n_columns=0 if built is None else len(built.col_to_item))
+    if built is None:
+        levels, stats, col_to_item = [], [], np.zeros(0, np.int64)
+    else:
+        if warn_complexity:
+            _warn_complexity(int((built.col_counts >= min_count).sum()), min_support)
+        hooks.col_to_item = np.asarray(built.col_to_item, dtype=np.int64)
+        res = mine_levels(built.bitvecs, built.n_words, n_rows, built.col_counts, min_count, max_length,
+                          prune_equal_support, use_generator_pruning, hooks.python(), timer=timer,
+                          guard=hooks.guard_python, level_sink=hooks.sink() if hooks.emit else None)
+        levels, stats, col_to_item = res.levels, res.stats, built.col_to_item
     if stats_out is not None:
         stats_out.extend(stats)
+    if hooks.emit:
+        return empty_result()
     levels = [(r, c) for r, c in levels if len(r)]
     if with_counts:
         return itemset_table(levels, col_to_item, n_rows, with_counts=True)
@@ -332,25 +342,115 @@ def mine_csr(indptr, ids, min_support, max_length=None, prune_equal_support=Fals
     return _frame(levels, col_to_item, n_rows, items)
 
 
-def _mine_to_dir(indptr, ids, device_csr, n_rows, min_count, min_support, max_length, prune_equal_support,
-                 use_generator_pruning, level_callback, session, items, output_dir, resume_from_k, stats_out,
-                 k2_kernel, k3_kernel, row_order):
-    """The output_dir route: every level to parquet (or the levels above a resumed one), an empty result back."""
-    from ..io import flush
+class _LevelHooks:
+    """The per-level work of a mining call, run as each level completes: the reference's ``level_callback``
+    (``k, n_candidates, n_frequent, duration_ms``; ``gpu/row_split.py:855-856, 1140-1141``), the profile
+    session's phase per level, the memory guards (``gpu/row_split.py:467-487``: checked when another level
+    follows) and the per-level parquet flush of ``output_dir`` (``gpu/row_split.py:512-537``).
 
-    if resume_from_k is None or resume_from_k < 2:
-        table = mine_csr(indptr, ids, min_support, max_length, prune_equal_support, use_generator_pruning,
-                         level_callback, session, items=items, device_csr=device_csr, with_counts=True,
-                         stats_out=stats_out, k2_kernel=k2_kernel, k3_kernel=k3_kernel, row_order=row_order)
-        if items is not None:
-            table = pa.table({"itemset": table.column("itemset"), "support": table.column("support")})
-        flush.write_levels(table, output_dir, n_rows)
-        return empty_result()
+    ``items`` (the sorted distinct values of non-integer items, or None) and ``col_to_item`` (set by the route,
+    or taken from the native loop's report) map column ids to what the files hold."""
+
+    def __init__(self, level_callback, session, max_ram_gb, max_vram_gb, output_dir, n_rows, items=None,
+                 col_to_item=None, sort_rows=False):
+        self.level_callback = level_callback
+        self.session = session
+        self.max_ram_gb, self.max_vram_gb = max_ram_gb, max_vram_gb
+        self.output_dir = output_dir
+        self.n_rows = n_rows
+        self.items = items
+        self.col_to_item = col_to_item
+        self.sort_rows = sort_rows
+        self.emit = output_dir is not None
+        if self.emit:
+            from ..io import flush
+
+            flush._check_local(output_dir)
+
+    @property
+    def active(self) -> bool:
+        return bool(self.level_callback or self.session or self.emit or self.max_ram_gb is not None
+                    or self.max_vram_gb is not None)
+
+    def _record(self, k, n_cand, n_freq, ms):
+        if self.session:
+            self.session.record_phase(f"k{k}", ms, n_candidates=n_cand, n_frequent=n_freq)
+        if self.level_callback:
+            self.level_callback(k, n_cand, n_freq, ms)
+
+    def _write(self, k, rows, counts):
+        from ..io import flush
+
+        if len(rows) == 0:  # the reference writes no file for an empty level (gpu/row_split.py:1131-1133)
+            return
+        item_rows = np.asarray(self.col_to_item, dtype=np.int64)[rows] if len(rows) else np.empty((0, k), np.int64)
+        if self.sort_rows:
+            item_rows = np.sort(item_rows, axis=1)
+        if self.items is not None:
+            item_rows = np.asarray(self.items.take(pa.array(item_rows.reshape(-1))).to_numpy(zero_copy_only=False),
+                                   dtype=object).reshape(item_rows.shape)
+        flush.write_level(self.output_dir, k, item_rows, np.asarray(counts, dtype=np.float64) / float(self.n_rows))
+
+    def guard(self, k, device_bytes, host_bytes) -> None:
+        if self.max_ram_gb is None and self.max_vram_gb is None:
+            return
+        import psutil
+
+        ram = (psutil.Process().memory_info().rss + host_bytes) / float(1 << 30)
+        vram = (device_bytes + _torch_allocated()) / float(1 << 30)
+        for used, limit, what, remedy in (
+            (ram, self.max_ram_gb, "RAM", "pass output_dir so each level is flushed as it completes"),
+            (vram, self.max_vram_gb, "VRAM", "raise min_support, or mine with streaming=True to count in chunks"),
+        ):
+            if limit is not None and used > limit:
+                raise MemoryError(
+                    f"Memory guard tripped after K={k}: {what}={used:.1f}GB exceeds max_{what.lower()}_gb={limit}GB. "
+                    "The lattice is INCOMPLETE at this point, so it is not returned. "
+                    f"Raise max_{what.lower()}_gb, lower max_length, or {remedy}.")
+
+    def native(self):
+        """The native loop's on_level (None when nothing is asked for)."""
+        if not self.active:
+            return None
+
+        def on_level(rep):
+            st = rep.stats
+            if self.col_to_item is None and rep.col_to_item is not None:
+                self.col_to_item = rep.col_to_item
+            if self.emit:
+                self._write(st.k, rep.rows, rep.counts)
+            self._record(st.k, st.n_candidates, st.n_frequent, st.ms)
+            if rep.next_level:
+                self.guard(st.k, rep.device_bytes, rep.host_bytes)
+
+        return on_level
+
+    def python(self):
+        return self._record if (self.level_callback or self.session) else None
+
+    def guard_python(self, k) -> None:
+        self.guard(k, 0, 0)
+
+    def sink(self):
+        return self._write
+
+
+def _torch_allocated() -> int:
+    """Device bytes torch holds in this process (the resident CSR, the caller's bitvectors)."""
+    from ..gpu import device as dev
+
+    t = dev.torch()
+    return int(t.cuda.memory_allocated()) if t.cuda.is_available() else 0
+
+
+def _resume_csr(indptr, ids, device_csr, n_rows, min_count, max_length, prune_equal_support, use_generator_pruning,
+                level_callback, session, items, output_dir, resume_from_k, stats_out, max_ram_gb, max_vram_gb):
+    """output_dir + resume_from_k: level ``resume_from_k`` read back, the levels above it mined and written there
+    one at a time; an empty result back."""
     if items is not None:
         raise ValueError("resume_from_k needs integer item ids (the flushed levels hold ids)")
     from ..gpu import device as dev
     from ..gpu.input_layer import build_device_input, host_csr_to_device
-    from ..gpu.row_split import mine_levels
 
     if device_csr is None:
         device_csr = host_csr_to_device(indptr, ids)
@@ -358,46 +458,32 @@ def _mine_to_dir(indptr, ids, device_csr, n_rows, min_count, min_support, max_le
     built = build_device_input(indptr_d, ids_d, n_rows, min_count)
     if built is None:
         return empty_result()
-    res = _resume_levels(mine_levels, built.bitvecs, built.n_words, n_rows, built.col_counts, built.col_to_item,
-                         min_count, max_length, prune_equal_support, use_generator_pruning, level_callback, session,
-                         output_dir, resume_from_k, None)
+    hooks = _LevelHooks(level_callback, session, max_ram_gb, max_vram_gb, output_dir, n_rows,
+                        col_to_item=np.asarray(built.col_to_item, dtype=np.int64))
+    res = _resume_levels(built.bitvecs, built.n_words, n_rows, built.col_counts, min_count, max_length,
+                         prune_equal_support, use_generator_pruning, hooks, resume_from_k, None)
     if stats_out is not None:
         stats_out.extend(res.stats)
     dev.synchronize()
     return empty_result()
 
 
-def _resume_levels(mine_levels, bv, n_words, n_rows, col_counts, col_to_item, min_count, max_length, prune, infer,
-                   level_callback, session, output_dir, resume_from_k, pitch):
-    """Level ``resume_from_k`` back from ``output_dir``, the levels above it mined and written there."""
+def _resume_levels(bv, n_words, n_rows, col_counts, min_count, max_length, prune, infer, hooks, resume_from_k, pitch):
+    """Level ``resume_from_k`` back from ``hooks.output_dir``; the levels above it mined on the Python level loop
+    and written there as each completes (``gpu/row_split.py:539-606``)."""
+    from ..gpu.row_split import mine_levels
     from ..io import flush
 
-    items_rows, counts = flush.read_level(output_dir, resume_from_k, n_rows)
-    cols, counts = flush.lexsorted(flush.columns_of(items_rows, np.asarray(col_to_item, dtype=np.int64)), counts)
-    res = mine_levels(bv, n_words, n_rows, col_counts, min_count, max_length, prune, infer,
-                      _timed(level_callback, session), pitch=pitch, start=(cols, counts))
-    c2i = np.asarray(col_to_item, dtype=np.int64)
-    levels = [(np.sort(c2i[r], axis=1), c) for r, c in res.levels if len(r)]
-    flush.write_levels(itemset_table(levels, None, n_rows, with_counts=True), output_dir, n_rows,
-                       min_k=resume_from_k + 1)
-    return res
-
-
-def _timed(level_callback, session):
-    if level_callback is None and session is None:
-        return None
-
-    def cb(k, n_cand, n_freq, ms):
-        if session:
-            session.record_phase(f"k{k}", ms, n_candidates=n_cand, n_frequent=n_freq)
-        if level_callback:
-            level_callback(k, n_cand, n_freq, ms)
-
-    return cb
+    items_rows, counts = flush.read_level(hooks.output_dir, resume_from_k, n_rows)
+    cols, counts = flush.lexsorted(flush.columns_of(items_rows, np.asarray(hooks.col_to_item, dtype=np.int64)),
+                                   counts)
+    hooks.sort_rows = True
+    return mine_levels(bv, n_words, n_rows, col_counts, min_count, max_length, prune, infer, hooks.python(),
+                       pitch=pitch, start=(cols, counts), guard=hooks.guard_python, level_sink=hooks.sink())
 
 
 def _mine_bitvecs(bitvecs, min_support, max_length, prune_equal_support, use_generator_pruning, level_callback,
-                  session, warn_complexity, output_dir=None, resume_from_k=None):
+                  session, warn_complexity, output_dir=None, resume_from_k=None, max_ram_gb=None, max_vram_gb=None):
     """The bitvecs= route: validation as the reference (``core/apriori.py:477-527``), then the level loop."""
     from ..gpu import device as dev
     from ..gpu import kernels as K
@@ -431,32 +517,25 @@ def _mine_bitvecs(bitvecs, min_support, max_length, prune_equal_support, use_gen
     counts = dev.to_host(K.column_popcounts(view, arr.shape[1])).astype(np.int64)
     if warn_complexity:
         _warn_complexity(int((counts >= max(1, min_count)).sum()), min_support)
+    hooks = _LevelHooks(level_callback, session, max_ram_gb, max_vram_gb, output_dir, n_trans, col_to_item=mapping,
+                        sort_rows=not order_ok)
     if output_dir is not None and resume_from_k is not None and resume_from_k >= 2:
-        from ..gpu.row_split import mine_levels
-
-        _resume_levels(mine_levels, view, arr.shape[1], n_trans, counts, mapping, max(1, min_count), max_length,
-                       prune_equal_support, use_generator_pruning, level_callback, session, output_dir,
-                       resume_from_k, arr.shape[1])
+        _resume_levels(view, arr.shape[1], n_trans, counts, max(1, min_count), max_length, prune_equal_support,
+                       use_generator_pruning, hooks, resume_from_k, arr.shape[1])
         dev.synchronize()
         return empty_result()
     from ..gpu.native import mine_bitvecs_device
 
     res = mine_bitvecs_device(arr.ptr, arr.shape[1], arr.shape[1], n_trans, counts, max(1, min_count), max_length,
-                              prune_equal_support, use_generator_pruning)
-    cb = _timed(level_callback, session)
-    if cb:
-        for st in res.stats:
-            cb(st.k, st.n_candidates, st.n_frequent, st.count_ms)
+                              prune_equal_support, use_generator_pruning, on_level=hooks.native(),
+                              emit_levels=hooks.emit)
+    if hooks.emit:
+        return empty_result()
     levels = [(r, c) for r, c in res.levels if len(r)]
     if not order_ok:
         # Item ids that do not ascend with the columns: map each level, then sort every itemset's items.
         levels = [(np.sort(mapping[rows], axis=1), cnt) for rows, cnt in levels]
         mapping = None
-    if output_dir is not None:
-        from ..io import flush
-
-        flush.write_levels(itemset_table(levels, mapping, n_trans, with_counts=True), output_dir, n_trans)
-        return empty_result()
     if mapping is None:
         return to_frame(itemset_table(levels, None, n_trans))
     return _frame(levels, mapping, n_trans, None)

@@ -305,19 +305,26 @@ def level_deduce(prev, older2, older3, n_total: int, min_count: int, groups: Gro
 
 def count_level(bv, n_words: int, rows, m: int, groups: Groups, status, counts, layout: int = LAYOUT_LEX,
                 word_chunk: int = 0, pitch: int | None = None, kernel: str = "auto",
-                occ: Occupancy | None = None) -> None:
+                occ: Occupancy | None = None, live: bool = True) -> None:
     """Adds the AND + popcount counts of the counted candidates into ``counts``.
 
     ``kernel`` ("auto", "valu", "mfma") picks the K=2 pair space's kernel (ignored for other levels). With
-    ``occ`` (``tile_occupancy`` of ``bv``) the group kernels skip the tiles where their AND is zero."""
+    ``occ`` (``tile_occupancy`` of ``bv``) the group kernels skip the tiles where their AND is zero. With a
+    ``status`` array and ``live`` (and neither ``occ`` nor ``word_chunk``), only the groups and tile-pairs
+    holding a counted candidate are launched, by the persistent kernels."""
     if kernel not in ("auto", "valu", "mfma"):
         raise ValueError(f"count_level: kernel must be 'auto', 'valu' or 'mfma', got {kernel!r}")
     pitch = int(bv.shape[1]) if pitch is None else int(pitch)
     cls = (ctypes.c_int64 * 4)(*groups.class_counts[:4])
+    ws, ws_n = None, 0
+    if live and status is not None and occ is None and word_chunk <= 0:
+        ws_n = int(_lib.lib().etm_count_level_ws(sum(groups.class_counts[:4]), groups.n_tile_pairs))
+        ws = dev.empty((ws_n + 7) // 8, "int64")
     call("etm_count_level", bv.data_ptr(), pitch, n_words, rows.data_ptr(), m, groups.group_start.data_ptr(),
          groups.cum_pairs.data_ptr(), groups.cum_tiles.data_ptr(), groups.n_groups, groups.n_tile_pairs,
          groups.small_list.data_ptr(), cls, dev.ptr(status), counts.data_ptr(), layout, word_chunk,
-         _lib.K2_KERNELS[kernel], dev.ptr(occ.bits) if occ is not None else None, occ.pitch if occ else 0, _s())
+         _lib.K2_KERNELS[kernel], dev.ptr(occ.bits) if occ is not None else None, occ.pitch if occ else 0,
+         dev.ptr(ws), ws_n, _s())
 
 
 def level_compact(rows, m: int, groups: Groups, status, counts, min_count: int, with_cand: bool = False, gid=None):

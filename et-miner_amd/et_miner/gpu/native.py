@@ -39,17 +39,84 @@ class NativeResult:
         return pa.table(cols)
 
 
+def _stats_of(st) -> LevelStats:
+    return LevelStats(st.k, st.n_candidates, st.n_frequent, st.n_counted, st.n_inferred, st.n_skipped, st.bytes_alg,
+                      float(st.wall_ms), float(st.count_ms), _lib.KERNEL_NAMES.get(st.kernel, ""), st.bytes_live)
+
+
+@dataclass
+class LevelReport:
+    """A level as the native loop hands it over on completion: its statistics, its rows (column ids int32 (n, k))
+    and counts (int64) when the levels are emitted, and the memory the call holds."""
+
+    stats: LevelStats
+    rows: np.ndarray | None
+    counts: np.ndarray | None
+    device_bytes: int
+    host_bytes: int
+    next_level: bool
+    col_to_item: np.ndarray | None = None  # the item id of each column (etm_mine_csr)
+
+
+class _Callback:
+    """The ctypes level callback around a Python function of a LevelReport. An exception it raises stops the
+    mining call (the callback returns nonzero) and is re-raised by ``run``."""
+
+    def __init__(self, fn, emit: bool):
+        self.fn, self.emit, self.error = fn, emit, None
+        self.c = _lib.LEVEL_CB(self._call)
+
+    def _call(self, info_p, _user) -> int:
+        try:
+            info = info_p.contents
+            st = _stats_of(info.stat)
+            rows = counts = None
+            if self.emit and st.n_frequent:
+                n, k = st.n_frequent, st.k
+                rows = np.ctypeslib.as_array(info.h_rows, shape=(n * k,)).reshape(n, k).copy()
+                counts = np.ctypeslib.as_array(info.h_counts, shape=(n,)).astype(np.int64)
+            elif self.emit:
+                rows, counts = np.empty((0, st.k), np.int32), np.empty(0, np.int64)
+            c2i = None
+            if info.h_col_to_item and info.n_cols:
+                c2i = np.ctypeslib.as_array(info.h_col_to_item, shape=(info.n_cols,)).copy()
+            self.fn(LevelReport(st, rows, counts, int(info.device_bytes), int(info.host_bytes),
+                                bool(info.next_level), c2i))
+            return 0
+        except BaseException as e:  # noqa: BLE001 - handed back to the caller after the C call returns
+            self.error = e
+            return 1
+
+    def run(self, name, *args):
+        try:
+            _lib.call(name, *args)
+        except _lib.Cancelled:
+            if self.error is not None:
+                raise self.error
+            raise
+
+
 def _opts(min_count, max_length, prune, infer, time_kernels, k2_kernel="auto", k3_kernel="auto",
-          row_order="auto") -> _lib.MineOpts:
+          row_order="auto", live_bytes=False, callback: _Callback | None = None, k3_schedule="auto",
+          small_max=0) -> _lib.MineOpts:
     if k2_kernel not in _lib.K2_KERNELS:
         raise ValueError(f"k2_kernel must be one of {sorted(_lib.K2_KERNELS)}, got {k2_kernel!r}")
     if k3_kernel not in _lib.K3_KERNELS:
         raise ValueError(f"k3_kernel must be one of {sorted(_lib.K3_KERNELS)}, got {k3_kernel!r}")
     if row_order not in _lib.ROW_ORDERS:
         raise ValueError(f"row_order must be one of {sorted(_lib.ROW_ORDERS)}, got {row_order!r}")
-    return _lib.MineOpts(int(max(1, min_count)), int(max_length or 0), int(bool(prune)), int(bool(infer)),
+    if k3_schedule not in _lib.K3_SCHEDULES:
+        raise ValueError(f"k3_schedule must be one of {sorted(_lib.K3_SCHEDULES)}, got {k3_schedule!r}")
+    if small_max not in _lib.SMALL_MAXES:
+        raise ValueError(f"small_max must be one of {_lib.SMALL_MAXES}, got {small_max!r}")
+    opts = _lib.MineOpts(int(max(1, min_count)), int(max_length or 0), int(bool(prune)), int(bool(infer)),
                          int(bool(time_kernels)), _lib.K2_KERNELS[k2_kernel], _lib.K3_KERNELS[k3_kernel],
-                         _lib.deduction_setting(), _lib.ROW_ORDERS[row_order])
+                         _lib.deduction_setting(), _lib.ROW_ORDERS[row_order], int(bool(live_bytes)), 0,
+                         _lib.K3_SCHEDULES[k3_schedule], int(small_max))
+    if callback is not None:
+        opts.emit_levels = int(callback.emit)
+        opts.level_cb = callback.c
+    return opts
 
 
 def _collect(handle, levels: bool = True) -> NativeResult:
@@ -77,9 +144,7 @@ def _collect(handle, levels: bool = True) -> NativeResult:
                 res.levels.append((rows, lv_counts))
             st = _lib.LevelStat()
             _lib.call("etm_levels_stats", handle, lv, ctypes.byref(st))
-            res.stats.append(LevelStats(st.k, st.n_candidates, st.n_frequent, st.n_counted, st.n_inferred,
-                                        st.n_skipped, st.bytes_alg, 0.0, st.count_ms,
-                                        _lib.KERNEL_NAMES.get(st.kernel, "")))
+            res.stats.append(_stats_of(st))
         n_cols = ctypes.c_int32()
         _lib.call("etm_levels_columns", handle, ctypes.byref(n_cols), None, None)
         c2i = np.empty(n_cols.value, dtype=np.int64)
@@ -93,30 +158,55 @@ def _collect(handle, levels: bool = True) -> NativeResult:
 
 def mine_csr_device(indptr_d, ids_d, n_rows: int, min_count: int, max_length=None, prune_non_free=False,
                     infer_counts=False, time_kernels=False, levels: bool = True,
-                    k2_kernel: str = "auto", k3_kernel: str = "auto", row_order: str = "auto") -> NativeResult:
+                    k2_kernel: str = "auto", k3_kernel: str = "auto", row_order: str = "auto",
+                    on_level=None, emit_levels: bool = False, live_bytes: bool = False, k3_schedule: str = "auto",
+                    small_max: int = 0) -> NativeResult:
     """Every frequent itemset of a device CSR of item ids (int32 or int64), mined by the native loop.
 
     ``k2_kernel``: "auto", "valu", "mfma" or "rows"; ``k3_kernel``: "auto", "valu" (bitvector kernels) or
     "rows" (row-wise, which needs rows of strictly ascending distinct ids); ``row_order``: "lex" (rows sorted by
-    their most frequent items, with the tile skip), "keep", or "auto" (keeps the rows)."""
+    their most frequent items, with the tile skip), "keep", or "auto" (keeps the rows). ``on_level(report)`` is
+    called with a LevelReport as each level completes (K=1 included); with ``emit_levels`` the report carries the
+    level's rows and counts and the result keeps no itemsets (host memory holds one level at a time). An
+    exception raised by ``on_level`` stops mining and propagates. ``live_bytes`` fills ``LevelStats.bytes_live``.
+    ``k3_schedule`` ("auto", "all", "live") and ``small_max`` (0, 4, 8, 16) choose how levels K >= 3 launch their
+    counting (etm.h ETM_SCHEDULE_*, etm_mine_opts.small_max); neither changes a count."""
     torch = dev.torch()
     id_bytes = 8 if ids_d.dtype == torch.int64 else 4
     if ids_d.dtype not in (torch.int64, torch.int32):
         raise ValueError(f"item ids must be int32 or int64, got {ids_d.dtype}")
+    if emit_levels and on_level is None:
+        raise ValueError("emit_levels needs on_level")
     handle = ctypes.c_void_p()
-    opts = _opts(min_count, max_length, prune_non_free, infer_counts, time_kernels, k2_kernel, k3_kernel, row_order)
-    _lib.call("etm_mine_csr", indptr_d.data_ptr(), ids_d.data_ptr() if ids_d.numel() else None, id_bytes, n_rows,
-              ids_d.numel(), SPAN_LIMIT, ctypes.byref(opts), ctypes.byref(handle), dev.stream_ptr())
+    cb = _Callback(on_level, emit_levels) if on_level is not None else None
+    opts = _opts(min_count, max_length, prune_non_free, infer_counts, time_kernels, k2_kernel, k3_kernel, row_order,
+                 live_bytes, cb, k3_schedule, small_max)
+    args = (indptr_d.data_ptr(), ids_d.data_ptr() if ids_d.numel() else None, id_bytes, n_rows, ids_d.numel(),
+            SPAN_LIMIT, ctypes.byref(opts), ctypes.byref(handle), dev.stream_ptr())
+    if cb is None:
+        _lib.call("etm_mine_csr", *args)
+    else:
+        cb.run("etm_mine_csr", *args)
     return _collect(handle, levels)
 
 
 def mine_bitvecs_device(bv_ptr: int, pitch: int, n_words: int, n_rows: int, col_counts: np.ndarray, min_count: int,
                         max_length=None, prune_non_free=False, infer_counts=False,
-                        time_kernels=False, k2_kernel: str = "auto") -> NativeResult:
-    """Every frequent itemset of device bitvectors (read only), from the columns' K=1 counts."""
+                        time_kernels=False, k2_kernel: str = "auto", on_level=None,
+                        emit_levels: bool = False, live_bytes: bool = False) -> NativeResult:
+    """Every frequent itemset of device bitvectors (read only), from the columns' K=1 counts; ``on_level``,
+    ``emit_levels`` and ``live_bytes`` as in ``mine_csr_device``."""
     counts = np.ascontiguousarray(col_counts, dtype=np.int64)
+    if emit_levels and on_level is None:
+        raise ValueError("emit_levels needs on_level")
     handle = ctypes.c_void_p()
-    opts = _opts(min_count, max_length, prune_non_free, infer_counts, time_kernels, k2_kernel)
-    _lib.call("etm_mine_bitvecs", bv_ptr, pitch, n_words, n_rows, counts.ctypes.data, len(counts),
-              ctypes.byref(opts), ctypes.byref(handle), dev.stream_ptr())
+    cb = _Callback(on_level, emit_levels) if on_level is not None else None
+    opts = _opts(min_count, max_length, prune_non_free, infer_counts, time_kernels, k2_kernel, live_bytes=live_bytes,
+                 callback=cb)
+    args = (bv_ptr, pitch, n_words, n_rows, counts.ctypes.data, len(counts), ctypes.byref(opts),
+            ctypes.byref(handle), dev.stream_ptr())
+    if cb is None:
+        _lib.call("etm_mine_bitvecs", *args)
+    else:
+        cb.run("etm_mine_bitvecs", *args)
     return _collect(handle)

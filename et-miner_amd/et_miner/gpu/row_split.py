@@ -35,7 +35,9 @@ from . import kernels as K
 @dataclass
 class LevelStats:
     """One level: candidates generated by the prefix join, and of them counted / inferred / skipped by the
-    subset test; ``bytes_alg`` is the compulsory HBM traffic of the prefix-group decomposition."""
+    subset test; ``bytes_alg`` is the compulsory HBM traffic of the prefix-group decomposition over every group,
+    ``bytes_live`` the same over what the counting kernels must read for the candidates left counted (native
+    loop, when asked for); ``ms`` is the level's wall time."""
 
     k: int
     n_candidates: int
@@ -47,6 +49,7 @@ class LevelStats:
     ms: float = 0.0
     count_ms: float = 0.0  # event-timed duration of the level's counting launches (native loop, when timed)
     kernel: str = ""  # the native loop's counting kernel family for the level: "valu", "mfma" or "rows"
+    bytes_live: int = 0
 
     @property
     def n_evaluated(self) -> int:
@@ -98,6 +101,8 @@ def mine_levels(
     timer=None,
     k2_kernel: str = "auto",
     start: tuple | None = None,
+    guard=None,
+    level_sink=None,
 ) -> MineResult:
     """Mines every frequent itemset (or free-set) of device bitvectors ``bv`` (n_cols, pitch).
 
@@ -107,7 +112,9 @@ def mine_levels(
     are called on the host around them, on the stream they run on. ``start`` = (rows int32 (n, k0) of column
     ids, lexsorted; counts int64) resumes from a level a previous run wrote: mining continues at k0 + 1 and the
     result holds the levels above k0 only. The loaded level's free flags are unknown, so its first successor
-    runs without inference (``gpu/row_split.py:594-601``).
+    runs without inference (``gpu/row_split.py:594-601``). ``guard(k)`` is called after level k when another
+    level follows (the reference's memory guard, ``gpu/row_split.py:1143-1144``); ``level_sink(k, rows, counts)``
+    receives each level (host arrays) as it completes, and the result then keeps no levels.
     """
     if n_rows > np.iinfo(np.int32).max:
         raise ValueError(f"n_transactions={n_rows:,} exceeds int32 max: the count arrays are int32")
@@ -115,30 +122,36 @@ def mine_levels(
     n_cols = len(col_counts)
     if start is not None:
         return _mine_from(bv, n_words, n_rows, n_cols, min_count, max_length, prune_non_free, infer_counts,
-                          level_callback, pitch, keep_device, timer, k2_kernel, start, out)
+                          level_callback, pitch, keep_device, timer, k2_kernel, start, out, guard, level_sink)
     t0 = time.perf_counter()
     freq = np.flatnonzero(col_counts >= min_count).astype(np.int32)
     free1 = col_counts[freq] < n_rows
     gen_cols = freq[free1] if prune_non_free else freq
     k1_rows = gen_cols.reshape(-1, 1)
     k1_counts = col_counts[gen_cols].astype(np.int64)
-    out.levels.append((k1_rows, k1_counts))
+    if level_sink is None:
+        out.levels.append((k1_rows, k1_counts))
+    else:
+        level_sink(1, k1_rows, k1_counts)
     out.stats.append(LevelStats(1, n_cols, len(gen_cols), n_counted=n_cols, ms=(time.perf_counter() - t0) * 1000))
     if level_callback:
         level_callback(1, n_cols, len(gen_cols), out.stats[-1].ms)
     if len(gen_cols) == 0:
         return out
     eml = min(max_length if max_length else math.inf, n_cols)
+    if guard is not None and eml >= 2 and len(gen_cols) >= 2:
+        guard(1)
     track_free = prune_non_free or infer_counts
     prev = _Level(dev.to_device(k1_rows), dev.to_device(k1_counts.astype(np.int32)))
     if track_free:
         prev.free = dev.to_device((k1_counts < n_rows).astype(np.uint8))
     return _level_loop(bv, n_words, n_rows, min_count, eml, prune_non_free, infer_counts, level_callback,
-                       pitch, keep_device, timer, k2_kernel, prev, 2, [], out, first_host=out.levels[0])
+                       pitch, keep_device, timer, k2_kernel, prev, 2, [], out,
+                       first_host=out.levels[0] if out.levels else None, guard=guard, level_sink=level_sink)
 
 
 def _mine_from(bv, n_words, n_rows, n_cols, min_count, max_length, prune_non_free, infer_counts, level_callback,
-               pitch, keep_device, timer, k2_kernel, start, out) -> MineResult:
+               pitch, keep_device, timer, k2_kernel, start, out, guard=None, level_sink=None) -> MineResult:
     rows0, counts0 = start
     rows0 = np.ascontiguousarray(rows0, dtype=np.int32)
     if rows0.ndim != 2 or rows0.shape[1] < 1 or len(rows0) != len(counts0):
@@ -149,11 +162,13 @@ def _mine_from(bv, n_words, n_rows, n_cols, min_count, max_length, prune_non_fre
         return out
     prev = _Level(dev.to_device(rows0), dev.to_device(np.asarray(counts0, dtype=np.int32)))
     return _level_loop(bv, n_words, n_rows, min_count, eml, prune_non_free, infer_counts, level_callback, pitch,
-                       keep_device, timer, k2_kernel, prev, k0 + 1, [], out, first_host=None)
+                       keep_device, timer, k2_kernel, prev, k0 + 1, [], out, first_host=None, guard=guard,
+                       level_sink=level_sink)
 
 
 def _level_loop(bv, n_words, n_rows, min_count, eml, prune_non_free, infer_counts, level_callback, pitch,
-                keep_device, timer, k2_kernel, prev, k, older, out, first_host) -> MineResult:
+                keep_device, timer, k2_kernel, prev, k, older, out, first_host, guard=None,
+                level_sink=None) -> MineResult:
     """Levels k, k+1, ... from ``prev`` (level k-1 on the device); ``older`` holds levels k-3, k-2 when known."""
     track_free = prune_non_free or infer_counts
     dev_levels = []
@@ -201,7 +216,10 @@ def _level_loop(bv, n_words, n_rows, min_count, eml, prune_non_free, infer_count
                 cur.rows, cur.counts = K.select_rows(cur.rows, k, cur.counts, cur.free)
                 cur.free = dev.empty(cur.n, "uint8").fill_(1) if cur.n else None
         n_freq = cur.n
-        dev_levels.append((cur.rows, cur.counts))
+        if level_sink is None:
+            dev_levels.append((cur.rows, cur.counts))
+        else:
+            level_sink(k, dev.to_host(cur.rows), dev.to_host(cur.counts).astype(np.int64))
         dt = (time.perf_counter() - t_k) * 1000
         st.n_frequent, st.ms = n_freq, dt
         out.stats.append(st)
@@ -209,6 +227,8 @@ def _level_loop(bv, n_words, n_rows, min_count, eml, prune_non_free, infer_count
             level_callback(k, groups.n_cands, n_freq, dt)
         if n_freq == 0:
             break
+        if guard is not None and k + 1 <= eml and n_freq >= k + 1:
+            guard(k)
         older = (older + [prev])[-2:]
         prev = cur
         k += 1

@@ -27,6 +27,23 @@ def _check_local(output_dir: str) -> None:
         raise ValueError(f"output_dir={output_dir!r}: only local directories are served (no remote upload)")
 
 
+def write_level(output_dir: str, k: int, item_rows: np.ndarray, support: np.ndarray) -> str:
+    """Writes one level, ``frequent_k{k}.parquet``: ``item_rows`` (n, k) item ids ascending within each row,
+    ``support`` float64 (n,). Returns the path. The reference's per-level flush (``gpu/row_split.py:512-537``)."""
+    _check_local(output_dir)
+    os.makedirs(output_dir, exist_ok=True)
+    item_rows = np.asarray(item_rows)
+    n = len(item_rows)
+    flat = item_rows.reshape(-1)
+    if flat.dtype.kind in "iu":
+        narrow = len(flat) == 0 or (flat.min() >= -(2**31) and flat.max() < 2**31)
+        flat = flat.astype(np.int32 if narrow else np.int64)
+    lst = pa.LargeListArray.from_arrays(pa.array(np.arange(n + 1, dtype=np.int64) * k), pa.array(flat))
+    path = os.path.join(output_dir, f"frequent_k{k}.parquet")
+    pq.write_table(pa.table({"itemset": lst, "support": pa.array(np.asarray(support, dtype=np.float64))}), path)
+    return path
+
+
 def write_levels(table: pa.Table, output_dir: str, n_rows: int, min_k: int = 1) -> list:
     """Writes the itemsets of ``table`` (``itemset`` lists, ``count`` or ``support``) with at least ``min_k``
     items, one file per length; returns the paths written. Levels are written in length order."""

@@ -44,7 +44,8 @@ enum {
     ETM_ENOMEM = 2,    /* device allocation failed: MemoryError */
     ETM_EHIP = 3,      /* HIP runtime or launch error: RuntimeError */
     ETM_EOVERFLOW = 4, /* a size exceeds what the layout holds: ValueError */
-    ETM_ENODEV = 5     /* no usable gfx950 device: RuntimeError */
+    ETM_ENODEV = 5,    /* no usable gfx950 device: RuntimeError */
+    ETM_ECANCELED = 6  /* a level callback stopped the call: the callback's own error */
 };
 
 /* Candidate status values written by etm_level_classify. */
@@ -53,18 +54,24 @@ enum { ETM_CAND_COUNT = 0, ETM_CAND_SKIP = 1, ETM_CAND_INFER = 2 };
 enum { ETM_SUBSET_PRUNE = 1, ETM_SUBSET_INFER = 2 };
 /* Pair layouts of the K=2 count array: lexicographic (i-major), or the reference's b(b-1)/2 + a. */
 enum { ETM_LAYOUT_LEX = 0, ETM_LAYOUT_TRI = 1 };
-/* Kernel choice for the pair space of a single-group level (K=2): automatic (int8 MFMA Gram from 128 columns,
- * else VALU AND+popcount tiles), VALU tiles only, or the MFMA Gram regardless of size. */
+/* Kernel choice for the pair space of a single-group level (K=2): automatic (the int8 MFMA Gram for at least 256
+ * columns over at least 4,096 words, else the VALU AND+popcount tiles), VALU tiles only, the MFMA Gram regardless
+ * of size, or (etm_mine_* only) the row-wise kernel. */
 enum { ETM_KERNEL_AUTO = 0, ETM_KERNEL_VALU = 1, ETM_KERNEL_MFMA = 2, ETM_KERNEL_ROWS = 3 };
 
 /* etm_mine_opts.deduction: the bounds are on by default in the inference mode, as the reference's
  * ET_MINER_DEDUCTION unset or "1" (gpu/row_split.py:627); OFF is its "0". */
 enum { ETM_DEDUCTION_AUTO = 0, ETM_DEDUCTION_OFF = 1 };
+/* etm_mine_opts.k3_schedule: every group and tile-pair launched (those with nothing counted return at once), only
+ * the ones holding a counted candidate (listed on the device first; see etm_count_level_ws), or automatic (the
+ * live units under Pascal inference or over columns of at least 65,536 words, where most units of deep levels
+ * have nothing to count; all units otherwise). */
+enum { ETM_SCHEDULE_AUTO = 0, ETM_SCHEDULE_ALL = 1, ETM_SCHEDULE_LIVE = 2 };
 /* etm_mine_opts.row_order: the reference's row_order ("keep" / "lex", gpu/input_layer.py:469-529). */
 enum { ETM_ROW_ORDER_AUTO = 0, ETM_ROW_ORDER_KEEP = 1, ETM_ROW_ORDER_LEX = 2 };
 
 /* ABI version of this header (bumped on any signature or struct-meaning change). */
-#define ETM_ABI_VERSION 3
+#define ETM_ABI_VERSION 4
 int etm_abi_version(void);
 
 /* Copies the calling thread's last error message (NUL-terminated, truncated to n). */
@@ -197,8 +204,16 @@ int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const 
                     const int64_t* d_group_start, const int64_t* d_cum_pairs, const int64_t* d_cum_tiles,
                     int64_t n_groups, int64_t n_tile_pairs, const int32_t* d_small_list,
                     const int64_t* h_class_counts, const uint8_t* d_status, int32_t* d_counts, int32_t layout,
-                    int32_t word_chunk, int32_t kernel, const uint64_t* d_occ, int64_t occ_pitch,
-                    void* stream);
+                    int32_t word_chunk, int32_t kernel, const uint64_t* d_occ, int64_t occ_pitch, void* d_ws,
+                    size_t ws_bytes, void* stream);
+
+/* Workspace bytes of etm_count_level's live-unit path for a level of n_small small-class groups (the four class
+ * sizes summed) and n_tile_pairs tile-pairs. With that workspace, a status array, no occupancy bitmap and
+ * word_chunk <= 0, etm_count_level first lists on the device the groups and tile-pairs holding at least one
+ * ETM_CAND_COUNT candidate, then counts only those with persistent kernels (a fixed grid taking (unit, word
+ * chunk) items from one queue per XCD), the chunk length sized on the device from the number of live units.
+ * Without it, every group and tile-pair is launched and those with nothing to count return at once. */
+size_t etm_count_level_ws(int64_t n_small, int64_t n_tile_pairs);
 
 /* Workspace bytes etm_level_compact needs for n_cands candidates. */
 size_t etm_level_compact_ws(int64_t n_cands);
@@ -301,6 +316,45 @@ int etm_union_insert(const int32_t* d_rows, const int32_t* d_counts, int64_t n, 
 
 /* ---- The native level loop --------------------------------------------------------------------------- */
 
+/* Per-level statistics: candidates generated by the prefix join and, of them, counted / inferred / skipped by
+ * the subset test; survivors; bytes; the counting launches' event-timed duration (when time_kernels); the
+ * level's wall time on the host, from the end of the previous level (K=1: from the call's start) to the sync
+ * that ends this one.
+ *   bytes_alg:  the prefix-group decomposition's compulsory bytes over every group of >= 2 rows,
+ *               ((prefix items) x groups + suffix rows) x n_words x 8 + 4 per candidate;
+ *   bytes_live: the same over what the counting kernels must read for the candidates left counted by the subset
+ *               test, Pascal inference and the deduction bounds: (prefix items x groups with a counted pair +
+ *               suffix rows in a counted pair) x n_words x 8 + 4 per counted candidate (K=2: = bytes_alg). */
+typedef struct etm_level_stat {
+    int32_t k;
+    int32_t kernel; /* ETM_KERNEL_* that counted the level (K >= 2; VALU also covers the group kernels) */
+    int64_t n_candidates;
+    int64_t n_counted;
+    int64_t n_inferred;
+    int64_t n_skipped;
+    int64_t n_frequent;
+    int64_t bytes_alg;
+    int64_t bytes_live;
+    float count_ms;
+    float wall_ms;
+} etm_level_stat;
+
+/* What a level callback receives as each level completes (pointers valid until the callback returns). */
+typedef struct etm_level_info {
+    etm_level_stat stat;
+    const int32_t* h_rows;   /* with emit_levels: the level's n_frequent x k column ids, else NULL */
+    const int32_t* h_counts; /* with emit_levels: their counts, else NULL */
+    const int64_t* h_col_to_item; /* etm_mine_csr: the item id of each column (n_cols), else NULL */
+    int64_t n_cols;
+    int64_t device_bytes;    /* device memory the call holds (its arena, bitvectors included for etm_mine_csr) */
+    int64_t host_bytes;      /* host memory of the levels kept for the result so far */
+    int32_t next_level;      /* 1 when another level follows */
+    int32_t reserved;
+} etm_level_info;
+
+/* A nonzero return stops the mining call, which then fails with ETM_ECANCELED. Called on the calling thread. */
+typedef int (*etm_level_cb)(const etm_level_info* info, void* user);
+
 /* Options of a mining call. min_count is the absolute threshold (>= 1 is enforced); max_length 0 = none. */
 typedef struct etm_mine_opts {
     int64_t min_count;
@@ -313,23 +367,14 @@ typedef struct etm_mine_opts {
     int32_t deduction;      /* ETM_DEDUCTION_AUTO: the deduction bounds with infer_counts (k <= 24); ETM_DEDUCTION_OFF */
     int32_t row_order;      /* etm_mine_csr: ETM_ROW_ORDER_LEX (rows sorted by their 64 most frequent columns,
                                and the tile skip), ETM_ROW_ORDER_KEEP, or ETM_ROW_ORDER_AUTO (keeps the rows) */
+    int32_t live_bytes;     /* compute etm_level_stat.bytes_live (one extra pass over each level's candidates) */
+    int32_t emit_levels;    /* hand each level to level_cb as it completes and keep none for the result (host
+                               memory then holds one level at a time), as the reference's per-level flush */
+    int32_t k3_schedule;    /* ETM_SCHEDULE_*: how levels K >= 3 launch their counting units */
+    int32_t small_max;      /* largest group (suffixes) counted by the wave-per-group kernels: 4, 8 or 16; 0 = 8 */
+    etm_level_cb level_cb;  /* called after every level (K=1 included) when not NULL */
+    void* cb_user;
 } etm_mine_opts;
-
-/* Per-level statistics: candidates generated by the prefix join and, of them, counted / inferred / skipped by
- * the subset test; survivors; the prefix-group decomposition's compulsory bytes; the counting launches'
- * event-timed duration (when time_kernels). */
-typedef struct etm_level_stat {
-    int32_t k;
-    int32_t kernel; /* ETM_KERNEL_* that counted the level (K >= 2; VALU also covers the group kernels) */
-    int64_t n_candidates;
-    int64_t n_counted;
-    int64_t n_inferred;
-    int64_t n_skipped;
-    int64_t n_frequent;
-    int64_t bytes_alg;
-    float count_ms;
-    float pad2;
-} etm_level_stat;
 
 /* Host-side result of a mining call: levels of (rows of column ids int32 (n, k), counts), the columns. */
 typedef struct etm_levels etm_levels;

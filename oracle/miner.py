@@ -217,15 +217,30 @@ def popcount_rows(a: np.ndarray) -> np.ndarray:
     return np.bitwise_count(a).sum(axis=1, dtype=np.int64)
 
 
-def count_candidates(cands: np.ndarray, k: int, bv: np.ndarray) -> np.ndarray:
+def count_candidates(cands: np.ndarray, k: int, bv: np.ndarray, n_jobs: int = 1) -> np.ndarray:
     """Counts of lexsorted k-candidates, one prefix group at a time (core/cpu_miner.py:706-760).
 
     Per group: AND the prefix's k-2 columns, keep its nonzero words, AND each suffix column with it, then
-    popcount each pair's AND."""
+    popcount each pair's AND. ``n_jobs`` > 1 counts disjoint runs of groups on that many threads (numpy releases
+    the GIL inside these array operations), as the reference's CPU route spreads groups over its workers."""
     out = np.empty(len(cands), dtype=np.int64)
     starts = _group_starts(cands, k - 2)
     ends = np.r_[starts[1:], len(cands)]
-    for s, e in zip(starts.tolist(), ends.tolist()):
+    if n_jobs > 1 and len(starts) > 1:
+        from concurrent.futures import ThreadPoolExecutor
+
+        # Runs of whole groups with about equal candidate counts, one task each (several per thread).
+        bounds = np.searchsorted(starts, np.linspace(0, len(cands), 4 * n_jobs + 1)[1:-1].astype(np.int64))
+        runs = [r for r in np.split(np.arange(len(starts)), np.unique(bounds)) if len(r)]
+        with ThreadPoolExecutor(n_jobs) as pool:
+            list(pool.map(lambda r: _count_groups(cands, k, bv, starts[r], ends[r], out), runs))
+        return out
+    _count_groups(cands, k, bv, starts, ends, out)
+    return out
+
+
+def _count_groups(cands, k, bv, starts, ends, out) -> None:
+    for s, e in zip(np.asarray(starts).tolist(), np.asarray(ends).tolist()):
         a, b = cands[s:e, k - 2], cands[s:e, k - 1]
         suffix = np.union1d(a, b)
         ia, ib = np.searchsorted(suffix, a), np.searchsorted(suffix, b)
@@ -244,7 +259,6 @@ def count_candidates(cands: np.ndarray, k: int, bv: np.ndarray) -> np.ndarray:
         for c0 in range(0, len(ia), step):
             c1 = min(c0 + step, len(ia))
             out[s + c0 : s + c1] = popcount_rows(sub[ia[c0:c1]] & sub[ib[c0:c1]])
-    return out
 
 
 def count_per_candidate(cands: np.ndarray, bv: np.ndarray) -> np.ndarray:
@@ -267,13 +281,24 @@ class _Level:
     free: np.ndarray | None
     index: LevelIndex | None = None
 
-    def subset_positions(self, cands: np.ndarray, base: int) -> np.ndarray:
+    def subset_positions(self, cands: np.ndarray, base: int, n_jobs: int = 1) -> np.ndarray:
         if self.index is None:
             self.index = LevelIndex(self.sets, base)
         k = cands.shape[1]
         out = np.empty((len(cands), k), dtype=np.int64)
-        for j in range(k):
-            out[:, j] = self.index.find(cands[:, [c for c in range(k) if c != j]])
+
+        def fill(lo, hi):
+            for j in range(k):
+                out[lo:hi, j] = self.index.find(cands[lo:hi][:, [c for c in range(k) if c != j]])
+
+        if n_jobs > 1 and len(cands) >= 4096:
+            from concurrent.futures import ThreadPoolExecutor
+
+            cuts = np.linspace(0, len(cands), 4 * n_jobs + 1).astype(np.int64)
+            with ThreadPoolExecutor(n_jobs) as pool:
+                list(pool.map(lambda i: fill(cuts[i], cuts[i + 1]), range(len(cuts) - 1)))
+        else:
+            fill(0, len(cands))
         return out
 
 
@@ -285,11 +310,13 @@ def mine(
     prune_equal_support: bool = False,
     use_generator_pruning: bool = False,
     level_callback=None,
+    n_jobs: int = 1,
 ) -> Lattice:
     """Every frequent itemset (or free-set) of a CSR of item ids, with exact counts (core/cpu_miner.py:984-1161).
 
     ``indptr`` (N+1) and ``ids`` are the transactions; rows may repeat items. Levels come back lexsorted over
-    column ids, which ascend with item ids.
+    column ids, which ascend with item ids. ``n_jobs`` threads count the prefix groups of each level (the result
+    does not depend on it).
     """
     indptr = np.asarray(indptr, dtype=np.int64)
     n_rows = len(indptr) - 1
@@ -328,12 +355,12 @@ def mine(
                 cc = np.empty(len(cands), dtype=np.int64)
                 needs = np.ones(len(cands), dtype=bool)
                 if infer:
-                    pos = prev.subset_positions(cands, n_cols)
+                    pos = prev.subset_positions(cands, n_cols, n_jobs)
                     licensed = np.any(~prev.free[pos], axis=1)
                     cc[licensed] = prev.counts[pos[licensed]].min(axis=1)
                     needs = ~licensed
                 if needs.any():
-                    cc[needs] = count_candidates(cands[needs], k, bv)
+                    cc[needs] = count_candidates(cands[needs], k, bv, n_jobs)
                 keep = cc >= mc
                 parts_s.append(cands[keep])
                 parts_c.append(cc[keep])
@@ -344,7 +371,7 @@ def mine(
         level_free = None
         if track_free:
             if len(sets):
-                pos = prev.subset_positions(sets, n_cols)
+                pos = prev.subset_positions(sets, n_cols, n_jobs)
                 level_free = ~np.any(prev.counts[pos] == counts[:, None], axis=1)
             else:
                 level_free = np.zeros(0, dtype=bool)

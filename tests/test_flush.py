@@ -92,3 +92,39 @@ def test_output_dir_route_validation():
 
     with pytest.raises(ValueError, match="requires the level-loop miner"):
         apriori([[1, 2]], 0.5, use_gpu=True, streaming=True, output_dir="/tmp/x")
+
+
+def test_level_hooks_write_then_report_and_guard(tmp_path):
+    """The per-level hook of the native loop, driven with host reports: each level is written before the level
+    callback runs, an empty level writes no file, and the memory guards raise the reference's MemoryError only
+    when another level follows."""
+    from et_miner.core.apriori import _LevelHooks
+    from et_miner.gpu.native import LevelReport
+    from et_miner.gpu.row_split import LevelStats
+
+    seen = []
+    hooks = _LevelHooks(lambda k, c, f, ms: seen.append((k, f, ms, sorted(p.name for p in tmp_path.iterdir()))),
+                        None, None, None, str(tmp_path), n_rows=10, col_to_item=np.array([7, 8, 9], np.int64))
+    on_level = hooks.native()
+    on_level(LevelReport(LevelStats(1, 3, 3, ms=1.5), np.array([[0], [1], [2]], np.int32),
+                         np.array([5, 6, 7], np.int64), 0, 0, True))
+    on_level(LevelReport(LevelStats(2, 3, 1, ms=2.5), np.array([[0, 2]], np.int32), np.array([4], np.int64), 0, 0,
+                         False))
+    on_level(LevelReport(LevelStats(3, 0, 0, ms=0.5), np.empty((0, 3), np.int32), np.empty(0, np.int64), 0, 0,
+                         False))
+    assert seen == [(1, 3, 1.5, ["frequent_k1.parquet"]),
+                    (2, 1, 2.5, ["frequent_k1.parquet", "frequent_k2.parquet"]),
+                    (3, 0, 0.5, ["frequent_k1.parquet", "frequent_k2.parquet"])]
+    t = pq.read_table(tmp_path / "frequent_k2.parquet").to_pydict()
+    assert t["itemset"] == [[7, 9]] and t["support"] == [0.4]
+
+    tight = _LevelHooks(None, None, 1e-9, None, None, n_rows=10)
+    report = LevelReport(LevelStats(2, 3, 3), None, None, 0, 0, True)
+    with pytest.raises(MemoryError, match=r"Memory guard tripped after K=2: RAM=.*max_ram_gb=1e-09GB"):
+        tight.native()(report)
+    tight.native()(LevelReport(LevelStats(2, 3, 3), None, None, 0, 0, False))  # the last level: no guard
+    vram = _LevelHooks(None, None, None, 1.0, None, n_rows=10)
+    vram.native()(LevelReport(LevelStats(2, 3, 3), None, None, 1 << 29, 0, True))
+    with pytest.raises(MemoryError, match="VRAM=2.0GB exceeds max_vram_gb=1.0GB"):
+        vram.native()(LevelReport(LevelStats(2, 3, 3), None, None, 2 << 30, 0, True))
+    assert _LevelHooks(None, None, None, None, None, n_rows=10).native() is None

@@ -198,3 +198,16 @@ def test_deduction_bounds_bracket_every_count(k, seed, mc):
         if kind == "skip":
             assert true < mc, (x, true)
     assert seen["skip"] + seen["infer"] > 0, seen
+
+
+def test_thread_count_does_not_change_the_lattice():
+    """The oracle's n_jobs (threads over prefix groups and subset lookups) leaves every itemset and count as
+    with one thread, in each mining mode."""
+    from et_miner.synthetic import workload
+
+    w = workload("deep_k_1m", 60_000)
+    d = w.generate()
+    for kw in ({}, {"use_generator_pruning": True}, {"prune_equal_support": True}):
+        one = miner.mine(d.indptr, d.indices, w.min_support, None, **kw).as_dict()
+        many = miner.mine(d.indptr, d.indices, w.min_support, None, n_jobs=6, **kw).as_dict()
+        assert one == many and len(one) > 1000, kw
