@@ -11,9 +11,9 @@ count back to the host.
 value = itemset candidates evaluated per second (counted + inferred, all levels; K=1 columns included),
 whole-job, one GPU.
 
-roofline: the counting launches of levels K >= 3 (k_count_small<2|4|8> and k_count_tiled, the step's dominant
-kernel family), timed with HIP events recorded by the native loop on the stream the kernels run on, over the
-timed steps. Their algorithmic bytes are what the kernels must read for the candidates left counted by the
+roofline: the counting launches of levels K >= 3 (the live-unit lists and the wave-per-group and tiled counting
+kernels, the step's dominant kernel family), timed with HIP events recorded by the native loop on the stream the
+kernels run on, over the timed steps and every level K >= 3. Their algorithmic bytes are what the kernels must read for the candidates left counted by the
 subset test, Pascal inference and the deduction bounds: per level, (prefix items x groups with a counted pair +
 suffix rows in a counted pair) x words x 8 B + 4 B per counted candidate (computed on the device). The full
 prefix-group decomposition's bytes (every group, inferred candidates included) are reported beside them, with
@@ -31,10 +31,11 @@ MFMA Gram), each checked against the oracle's committed per-level digests.
 served: the served_jobs stream (512 independent jobs, seed 4: transactions log-uniform in 10K-2M, vocabulary
 500-20K, Zipf 0.9-1.3, a coin between 1 + Poisson(4|6|9) and lognormal(2.3, 0.6) baskets, min_support
 log-uniform in 0.001-0.05, max_length 3-10), every dataset resident in HBM before timing. The plain form mines
-the jobs one after another on one stream; the served form runs them through the request loop after one warm-up
-pass. Throughput is jobs / makespan with every job queued at t=0; latency (completion - arrival) is measured by
-replaying Poisson arrivals at twice the plain form's measured jobs/s. Every job's itemsets and counts are
-checked identical in both forms.
+the jobs one after another on one stream; the served form runs them through the request loop (8 workers, one
+stream each, footprint admission, kernel settings tuned per shape bucket beforehand) after one warm-up pass.
+Throughput is jobs / makespan with every job queued at t=0; latency (completion - arrival) is measured by
+replaying Poisson arrivals at twice the plain form's measured jobs/s. Every job's itemsets and counts are checked
+identical in both forms.
 
 streaming: deep_sparse_20m mined from host arrays resident and by SON in 2,000,000-row chunks; wall seconds and
 identical results.
@@ -197,14 +198,14 @@ def main(argv=None) -> int:
             if st.k == 2:
                 k2["ms"] += st.count_ms
                 k2["cw"] += st.n_counted * n_words
-            elif st.k >= 3 and st.n_counted > 0:
+            elif st.k >= 3:
                 st.bytes_live = live_by_k.get(st.k, 0)
                 k3["ms"] += st.count_ms
                 k3["live"] += st.bytes_live
                 k3["alg"] += st.bytes_alg
                 k3["counted"] += st.n_counted
     k3_ms, k3_live, k3_alg = (k3["ms"] / args.steps, k3["live"] / args.steps, k3["alg"] / args.steps)
-    k3_launch_levels = sum(1 for s in per_step if s.k >= 3 and s.n_counted > 0)
+    k3_launch_levels = sum(1 for s in per_step if s.k >= 3)
     k2_ms, k2_cw = k2["ms"] / args.steps, k2["cw"] / args.steps
     k2_kernel = next((s.kernel for s in per_step if s.k == 2 and s.kernel), "valu")
     k2_bound, k2_peak = {"valu": ("valu", VALU_CW_PEAK), "mfma": ("mfma", MFMA_I8_CW_PEAK)}.get(k2_kernel,
@@ -289,7 +290,8 @@ def main(argv=None) -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "K>=3 counting launches (k_count_small<2|4|8>, k_count_tiled)",
+            "kernel": "K>=3 counting launches (k_live_units, k_count_small_live<2|4|8>, k_count_tiled_live; "
+                      "k_count_small<S> and k_count_tiled where every unit is launched)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -448,7 +450,8 @@ def _served_leg(jobs, workers):
     print(f"[bench] plain: {ps['jobs_per_s']:.1f} jobs/s", file=sys.stderr, flush=True)
     srv = Server(workers=workers, autotune=True)
     try:
-        serve(zero, server=srv, time_scale=0.0)  # warm-up: the workers' arenas sized, K=2 buckets tuned
+        tuned = srv.tune(zero)  # the shape buckets' kernel settings, timed with the GPU otherwise idle
+        serve(zero, server=srv, time_scale=0.0)  # warm-up: the workers' arenas sized
         mixed, ss = serve(zero, server=srv, time_scale=0.0)
         print(f"[bench] served: {ss['jobs_per_s']:.1f} jobs/s", file=sys.stderr, flush=True)
         rate = 2.0 * ps["jobs_per_s"]
@@ -468,6 +471,8 @@ def _served_leg(jobs, workers):
         "plain": {k: ps[k] for k in keep},
         "served": {k: ss[k] for k in keep} | {"workers": workers, "peak_concurrent_jobs": ss["peak_concurrent_jobs"]},
         "served_over_plain": ss["jobs_per_s"] / ps["jobs_per_s"],
+        "tuned_buckets": len(tuned),
+        "tuned_changed": sum(1 for t in tuned.values() if (t.k3_schedule, t.small_max) != ("auto", 0)),
         "poisson_rate_per_s": rate,
         "poisson_served": {k: sp[k] for k in keep},
         "poisson_plain": {k: pp[k] for k in keep},

@@ -260,13 +260,23 @@ class Server:
 
         d_ptr, d_ids = self.store.register(job.dataset, job.indptr, job.ids, self.device)
         mc = _mining_min_count(job.min_support, job.n_rows)
-        k2 = "auto"
+        kw = {}
         if self.autotune:
-            from .autotune import k2_kernel_for
+            from .autotune import setting_for
 
-            k2 = k2_kernel_for(job, mc, d_ptr, d_ids, self.store.profile(job.dataset, job.ids))
+            kw = setting_for(job, mc, d_ptr, d_ids, self.store.profile(job.dataset, job.ids)).kwargs()
         return mine_csr_device(d_ptr, d_ids, job.n_rows, mc, job.max_length,
-                               infer_counts=job.use_generator_pruning, levels=False, k2_kernel=k2)
+                               infer_counts=job.use_generator_pruning, levels=False, **kw)
+
+    def tune(self, jobs: list) -> dict:
+        """Tunes the kernel settings of the jobs' shape buckets serially, before they are served (needs
+        ``autotune``; the workers must be idle)."""
+        if self.executor is not None or not self.autotune:
+            return {}
+        from .autotune import tune
+
+        with self.torch.cuda.device(self.device):
+            return tune(jobs, self.store, self.device)
 
     def submit(self, job: Job, arrival_s: float | None = None) -> Future:
         """Queues a job; its future resolves to a JobResult (or raises the job's error)."""

@@ -25,4 +25,6 @@ def test_traffic_profile_is_readable():
 
     traffic = bench.measured_traffic("deep_k_1m")
     assert isinstance(traffic, int) and traffic > 0
+    headline = bench.measured_traffic("deep_sparse_20m")
+    assert isinstance(headline, int) and 10e9 < headline < 40e9
     assert bench.measured_traffic("wide_vocab_2m") is None  # no PMC traffic profile for this workload

@@ -170,7 +170,7 @@ def test_served_equals_alone_and_oracle():
 @pytest.mark.gpu
 def test_served_under_tight_budget_and_arrivals():
     """A budget that admits about two jobs at a time, Poisson arrivals, a dataset shared by repeated jobs, and
-    the K=2 autotuner: results unchanged."""
+    the autotuner (K=2 kernel, K>=3 schedule and class split per shape bucket): results unchanged."""
     import dataclasses
 
     from et_miner.serve import footprint_bytes, run_plain, serve
@@ -179,7 +179,16 @@ def test_served_under_tight_budget_and_arrivals():
     jobs += [dataclasses.replace(j, job_id=100 + j.job_id, min_support=j.min_support * 2) for j in jobs[:4]]
     budget = 2 * max(footprint_bytes(j) for j in jobs)
     alone, _ = run_plain(jobs, time_scale=0.0)
+    from et_miner.serve import autotune
+
+    autotune.reset()
     mixed, summary = serve(jobs, workers=4, time_scale=0.2, memory_budget_bytes=budget, autotune=True)
     for a, b in zip(alone, mixed):
         assert a.digest == b.digest, a.job_id
+    # Every tuned bucket holds a valid setting of each tuned choice.
+    tuned = autotune.decisions()
+    assert tuned
+    for s in tuned.values():
+        assert s.k2_kernel in ("valu", "mfma", "rows")
+        assert s.k3_schedule in ("auto", "all", "live") and s.small_max in (0, 4, 8, 16)
     assert summary["latency_p99_s"] >= summary["latency_p50_s"] > 0
