@@ -54,6 +54,20 @@ struct ClassBounds {
     int64_t start[5];
 };
 
+// A sparse live tile-pair: at most kSparseMax counted pairs (ta * 64 + a[p], tb * 64 + b[p]) of its group, with at
+// most kSparseRatio pairs per column in them. One wave counts it from registers (k_count_sparse_live): no LDS
+// staging, no barriers, each column read straight from the caches per pair. Measured on deep_sparse_20m's levels
+// K >= 3: 32 pairs and 4 per column 10.9 ms; 32 and 2: 11.3 ms; 16 and 2: 11.4 ms; 64 and 4: 11.9 ms; 64 and 8:
+// 12.7 ms (12.7 ms without the sparse kernel).
+constexpr int kSparseMax = 32;
+constexpr int kSparseRatio = 4;
+struct SparseTP {
+    int32_t tp;
+    int32_t n;
+    uint8_t a[kSparseMax];
+    uint8_t b[kSparseMax];
+};
+
 // The (word chunk, item) unit of a workgroup or wave: `xcd` = workgroup index mod 8 (the XCD the hardware
 // dispatches it to, round-robin), `local` = its index among that XCD's units. With at least 8 chunks, XCD x takes
 // chunks x, x+8, ..., and every item of a chunk in turn, so the waves of one XCD share chunks in its L2; with
@@ -687,6 +701,98 @@ __global__ __launch_bounds__(256) void k_count_tiled_live(const uint64_t* __rest
                layout, uniform64(live[uniform64(item)]), uniform64(ch));
 }
 
+// One wave per (sparse tile-pair, word chunk): lane p < n holds pair p's two column offsets and its count slot;
+// per word, the prefix AND, then for each pair the AND of its two columns with it (the pair's offsets broadcast
+// from lane p), popcounts accumulated per lane; a wave sum and one atomic per pair at the end.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_count_sparse_live(const uint64_t* __restrict__ bv, int64_t pitch,
+                                                            int64_t n_words, const int32_t* __restrict__ rows,
+                                                            int32_t m, const int64_t* __restrict__ group_start,
+                                                            const int64_t* __restrict__ cum_pairs,
+                                                            const int64_t* __restrict__ cum_tiles, int64_t n_groups,
+                                                            const SparseTP* __restrict__ sparse,
+                                                            const int64_t* __restrict__ d_n_live,
+                                                            int32_t* __restrict__ counts, int64_t target,
+                                                            int64_t min_words) {
+    const int64_t n_live = *d_n_live;
+    if (n_live == 0) return;
+    int64_t chunk, n_chunks;
+    live_chunks(n_words, n_live, target, 64, min_words, &chunk, &n_chunks);
+    int64_t ch, item;
+    if (!xcd_unit(blockIdx.x % kXcds, (int64_t)(blockIdx.x / kXcds) * 4 + (threadIdx.x >> 6), n_live, n_chunks, &ch,
+                  &item))
+        return;
+    ch = uniform64(ch);
+    const SparseTP* e = sparse + uniform64(item);
+    const int lane = threadIdx.x & 63;
+    const int64_t tp = uniform64(e->tp);
+    const int n = (int)uniform64(e->n);
+    const int64_t g = etm::wave_upper_index(cum_tiles, n_groups, tp);
+    const int64_t gs = group_start[g];
+    const int64_t s = group_start[g + 1] - gs;
+    const int64_t nt = (s + kT - 1) / kT;
+    int64_t ta, tb1;
+    etm::lex_pair_decode(tp - cum_tiles[g], nt + 1, &ta, &tb1);
+    const int64_t tb = tb1 - 1;
+    const int np = m - 1;
+    int64_t ca = 0, cb = 0, slot = 0;
+    if (lane < n) {
+        const int64_t i = ta * kT + e->a[lane], j = tb * kT + e->b[lane];
+        ca = (int64_t)rows[(gs + i) * m + np] * pitch;
+        cb = (int64_t)rows[(gs + j) * m + np] * pitch;
+        slot = cum_pairs[g] + etm::lex_pair_index(i, j, s);
+    }
+    const int32_t* prow = rows + gs * m;
+    int acc[kSparseMax];
+#pragma unroll
+    for (int p = 0; p < kSparseMax; ++p) acc[p] = 0;
+    const int64_t w_begin = ch * chunk;
+    const int64_t w_end = min(n_words, w_begin + chunk);
+    // Two words a lane per iteration: one 16-byte load per column (VEC), or two 8-byte loads 64 words apart.
+    for (int64_t w0 = w_begin + (VEC ? 2 * lane : lane); w0 < w_end; w0 += 128) {
+        const int64_t w1 = VEC ? w0 + 1 : w0 + 64;
+        const bool in0 = true, in1 = w1 < w_end;
+        uint64_t pre0 = in0 ? ~0ull : 0ull, pre1 = in1 ? ~0ull : 0ull;
+        for (int t = 0; t < np; ++t) {
+            const uint64_t* c = bv + (int64_t)prow[t] * pitch;
+            if (VEC) {
+                const ulonglong2 x = *reinterpret_cast<const ulonglong2*>(c + w0);
+                pre0 &= x.x;
+                pre1 &= x.y;
+            } else {
+                pre0 &= c[w0];
+                pre1 &= in1 ? c[w1] : 0ull;
+            }
+        }
+        if (__ballot((pre0 | pre1) != 0) == 0) continue;  // the prefix is empty on the wave's words
+#pragma unroll
+        for (int p = 0; p < kSparseMax; ++p) {
+            if (p >= n) break;
+            const int64_t oa = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(ca >> 32), p) << 32) |
+                                         (uint32_t)__builtin_amdgcn_readlane((int)ca, p));
+            const int64_t ob = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(cb >> 32), p) << 32) |
+                                         (uint32_t)__builtin_amdgcn_readlane((int)cb, p));
+            uint64_t x0, x1;
+            if (VEC) {
+                const ulonglong2 a = *reinterpret_cast<const ulonglong2*>(bv + oa + w0);
+                const ulonglong2 b = *reinterpret_cast<const ulonglong2*>(bv + ob + w0);
+                x0 = a.x & b.x;
+                x1 = a.y & b.y;
+            } else {
+                x0 = bv[oa + w0] & bv[ob + w0];
+                x1 = in1 ? (bv[oa + w1] & bv[ob + w1]) : 0ull;
+            }
+            acc[p] = popc_acc(x1 & pre1, popc_acc(x0 & pre0, acc[p]));
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < kSparseMax; ++p) {
+        if (p >= n) break;
+        const int tot = etm::wave_sum(acc[p]);
+        if (lane == p && tot) atomicAdd(&counts[slot], tot);
+    }
+}
+
 // ── live units ─────────────────────────────────────────────────────────
 //
 // The groups of the small classes and the tile-pairs of the tiled groups with at least one candidate left
@@ -697,7 +803,8 @@ __global__ __launch_bounds__(256) void k_count_tiled_live(const uint64_t* __rest
 // One launch for both lists. Workgroups [0, small_blocks) take the small groups, a thread per group of
 // small_list[0 .. n_small): class c's live groups go to live_small[start_c ..], counted in n_live[c]. The other
 // workgroups take the tile-pairs, a wave per tile-pair (lane l checks row ta * 64 + l against the rows of tile tb):
-// live tile-pairs go to live_tp, counted in n_live[4]. n_live[0..4] start at zero.
+// sparse live tile-pairs go to `sparse` with their pair lists, counted in n_live[5], the other live ones to
+// live_tp, counted in n_live[4]. n_live[0..5] start at zero.
 __global__ __launch_bounds__(256) void k_live_units(const int32_t* __restrict__ small_list, int64_t n_small,
                                                     ClassBounds cb, int64_t small_blocks,
                                                     const int64_t* __restrict__ group_start,
@@ -705,7 +812,7 @@ __global__ __launch_bounds__(256) void k_live_units(const int32_t* __restrict__ 
                                                     const int64_t* __restrict__ cum_tiles, int64_t n_groups,
                                                     int64_t n_tile_pairs, const uint8_t* __restrict__ status,
                                                     int32_t* __restrict__ live_small, int32_t* __restrict__ live_tp,
-                                                    int64_t* __restrict__ n_live) {
+                                                    SparseTP* __restrict__ sparse, int64_t* __restrict__ n_live) {
     if ((int64_t)blockIdx.x < small_blocks) {
         for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n_small;
              e += small_blocks * (int64_t)blockDim.x) {
@@ -734,13 +841,38 @@ __global__ __launch_bounds__(256) void k_live_units(const int32_t* __restrict__ 
         etm::lex_pair_decode(tp - cum_tiles[g], nt + 1, &ta, &tb1);
         const int64_t tb = tb1 - 1;
         const int64_t i = ta * kT + lane;
-        bool any = false;
+        // Bit j - tb * 64 of `mine`: pair (i, j) is counted.
+        uint64_t mine = 0;
         if (i < s) {
             const int64_t j0 = max(i + 1, tb * kT), j1 = min(s, tb * kT + kT);
             const int64_t base = cum_pairs[g] + etm::lex_pair_index(i, j0, s);
-            for (int64_t j = j0; j < j1 && !any; ++j) any = status[base + (j - j0)] == ETM_CAND_COUNT;
+            for (int64_t j = j0; j < j1; ++j)
+                if (status[base + (j - j0)] == ETM_CAND_COUNT) mine |= 1ull << (j - tb * kT);
         }
-        if (__ballot(any) != 0 && lane == 0) {
+        const int cnt = __popcll(mine);
+        const int n = etm::wave_sum(cnt);
+        if (n == 0) continue;
+        // Columns in a counted pair: the A rows with a bit, the B columns of any bit (one set on the diagonal).
+        uint64_t used_b = mine;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) used_b |= __shfl_xor(used_b, off, 64);
+        const uint64_t used_a = __ballot(mine != 0);
+        const int u = ta == tb ? __popcll(used_a | used_b) : __popcll(used_a) + __popcll(used_b);
+        if (n <= kSparseMax && n <= kSparseRatio * u) {
+            int64_t at = 0;
+            if (lane == 0) at = (int64_t)atomicAdd((unsigned long long*)&n_live[5], 1ull);
+            at = __shfl(at, 0, 64);
+            SparseTP* e = sparse + at;
+            int off = etm::wave_scan_i32(cnt) - cnt;  // this lane's first slot in the entry's list
+            for (uint64_t x = mine; x; x &= x - 1, ++off) {
+                e->a[off] = (uint8_t)lane;
+                e->b[off] = (uint8_t)__builtin_ctzll(x);
+            }
+            if (lane == 0) {
+                e->tp = (int32_t)tp;
+                e->n = n;
+            }
+        } else if (lane == 0) {
             const int64_t at = (int64_t)atomicAdd((unsigned long long*)&n_live[4], 1ull);
             live_tp[at] = (int32_t)tp;
         }
@@ -750,17 +882,15 @@ __global__ __launch_bounds__(256) void k_live_units(const int32_t* __restrict__ 
 // ── int8 MFMA Gram of a single-group level (the K=2 pair space) ───────────────────────────────────────────
 //
 // One workgroup per (128 x 128 tile-pair of columns, word chunk). Per 4-word step (256 transactions = 8 K-steps
-// of 32 rows) each thread stages one column's four raw words in LDS ([word][column], 8 KB per side), then each of
-// the 4 waves runs a 2 x 2 block of v_mfma_i32_32x32x32_i8 on its 64 x 64 quarter of the output, expanding its
-// operands from the raw bits in registers: operand lane l carries column l % 32 and rows 16 (l / 32) .. +15 of
-// the K-step (one 16-bit LDS read, four 4-bit-to-4-byte spreads) for both A and B, so the product is the exact
-// Gram sum whatever the instruction's K order within a lane. LDS carries the bits once, not the 8x larger bytes.
-// Counts accumulate in int32 (exact for fewer than 2^31 transactions) and leave with one integer atomic per pair.
+// of 32 rows) the bits are expanded once into int8 0/1 operands in LDS and the 4 waves run v_mfma_i32_32x32x32_i8
+// on them: operand lane l carries column l % 32 and rows 16 (l / 32) .. +15 of the K-step for both A and B, so the
+// product is the exact Gram sum whatever the instruction's K order within a lane. Counts accumulate in int32
+// (exact for fewer than 2^31 transactions) and leave with one integer atomic per pair.
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int kMT = 128;  // columns per MFMA tile
-constexpr int kMW = 4;    // words per LDS step
+constexpr int kMW = 4;    // words per step: 256 rows, 8 K-steps of 32
 
 // Bits 4q..4q+3 of x as four 0/1 bytes (bit t -> byte t).
 __device__ __forceinline__ int spread4(uint32_t x, int q) {
@@ -777,111 +907,82 @@ __device__ __forceinline__ v4i spread16(uint32_t x) {
     return v;
 }
 
-// BM x BM blocks of 32 x 32 MFMA tiles per wave; a workgroup of 4 waves covers a (64 BM) x (64 BM) tile-pair
-// (BM = 2: 128-column tiles, 8 KB of LDS per step).
-template <int BM>
-__global__ __launch_bounds__(256) void k_gram_mfma(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
-                                                    int64_t chunk_words, const int32_t* __restrict__ cols, int64_t s,
-                                                    int64_t n_tp, int64_t n_chunks, int32_t* __restrict__ counts,
-                                                    int32_t layout) {
-    constexpr int T = 64 * BM;  // columns per side of the tile-pair
-    __shared__ uint64_t rawA[2][kMW][T];
-    __shared__ uint64_t rawB[2][kMW][T];
-    __shared__ int64_t colA[T], colB[T];
+// K=2 pair counts of a wide column set as an int8 Gram on the matrix cores. A workgroup of 4 waves covers a
+// tile-pair of 128 x 128 columns over a chunk of words. Thread t owns column t of the tile-pair (A columns 0..127,
+// then B): per step of kMW words it expands its column's words (loaded into registers during the previous step)
+// into int8 0/1 bytes in LDS, [column][row] with rows contiguous, so an MFMA operand (16 rows of one column) is one
+// 16-byte LDS read; then each wave runs its 2 x 2 block of v_mfma_i32_32x32x32_i8 over the step's K-steps. Each
+// column is expanded once per workgroup (waves that expand the operands they use would expand each one twice).
+constexpr int kGRow = 64 * kMW + 16;  // LDS bytes per column: kMW words of rows + 16 of padding across banks
+
+__global__ __launch_bounds__(256) void k_gram_mfma(const uint64_t* __restrict__ bv, int64_t pitch,
+                                                           int64_t n_words, int64_t chunk_words,
+                                                           const int32_t* __restrict__ cols, int64_t s, int64_t n_tp,
+                                                           int64_t n_chunks, int32_t* __restrict__ counts,
+                                                           int32_t layout) {
+    constexpr int T = kMT;  // columns per side
+    __shared__ __align__(16) uint8_t ops[2 * T * kGRow];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int64_t chunk_id, tp;
     if (!xcd_unit(blockIdx.x % kXcds, blockIdx.x / kXcds, n_tp, n_chunks, &chunk_id, &tp)) return;
     const int64_t nt = (s + T - 1) / T;
     int64_t ta, tb1;
-    etm::lex_pair_decode(tp, nt + 1, &ta, &tb1);
-    const int64_t tb = tb1 - 1;
-    for (int t = tid; t < 2 * T; t += 256) {
-        if (t < T) {
-            const int64_t i = ta * T + t;
-            colA[t] = i < s ? (int64_t)cols[i] * pitch : -1;
-        } else {
-            const int64_t j = tb * T + (t - T);
-            colB[t - T] = j < s ? (int64_t)cols[j] * pitch : -1;
-        }
-    }
+    etm::lex_pair_decode(uniform64(tp), nt + 1, &ta, &tb1);
+    ta = uniform64(ta);
+    const int64_t tb = uniform64(tb1) - 1;
+    // This thread's column: A column ta * T + tid, or B column tb * T + tid - T.
+    const int64_t mycol = tid < T ? ta * T + tid : tb * T + (tid - T);
+    const int64_t base = mycol < s ? (int64_t)cols[mycol] * pitch : -1;
     const int wm = wave >> 1, wn = wave & 1;
-    v16i acc[BM][BM];
+    v16i acc[2][2];
 #pragma unroll
-    for (int a = 0; a < BM; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < BM; ++b)
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
-    const int64_t w_begin = chunk_id * chunk_words;
+    const int64_t w_begin = uniform64(chunk_id) * chunk_words;
     const int64_t w_end = min(n_words, w_begin + chunk_words);
-    __syncthreads();
-    // Staging: thread t holds the columns t, t + 256, ... of the concatenated sides A | B.
-    constexpr int PER = 2 * T / 256;  // columns staged per thread per step
-    int64_t base[PER];
+    uint64_t next[kMW];
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int t = tid + 256 * q;
-        base[q] = t < T ? colA[t] : colB[t - T];
-    }
+    for (int w = 0; w < kMW; ++w) next[w] = (base >= 0 && w_begin + w < w_end) ? bv[base + w_begin + w] : 0ull;
+    uint8_t* mine = ops + tid * kGRow;
     const int sub = 16 * (lane >> 5);
-    uint64_t next[PER][kMW];
-#pragma unroll
-    for (int q = 0; q < PER; ++q)
+    for (int64_t wb = w_begin; wb < w_end; wb += kMW) {
+        // Expand this column's kMW words: byte r of the column's run = bit r % 64 of word r / 64.
 #pragma unroll
         for (int w = 0; w < kMW; ++w)
-            next[q][w] = (base[q] >= 0 && w_begin + w < w_end) ? bv[base[q] + w_begin + w] : 0ull;
-    int buf = 0;
-    for (int64_t wb = w_begin; wb < w_end; wb += kMW, buf ^= 1) {
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int t = tid + 256 * q;
-#pragma unroll
-            for (int w = 0; w < kMW; ++w) {
-                if (t < T) rawA[buf][w][t] = next[q][w];
-                else rawB[buf][w][t - T] = next[q][w];
-            }
-        }
+            for (int q = 0; q < 4; ++q)
+                *reinterpret_cast<v4i*>(mine + 64 * w + 16 * q) = spread16((uint32_t)(next[w] >> (16 * q)) & 0xFFFFu);
         __syncthreads();
         const int64_t nb = wb + kMW;
 #pragma unroll
-        for (int q = 0; q < PER; ++q)
+        for (int w = 0; w < kMW; ++w) next[w] = (base >= 0 && nb + w < w_end) ? bv[base + nb + w] : 0ull;
 #pragma unroll
-            for (int w = 0; w < kMW; ++w)
-                next[q][w] = (base[q] >= 0 && nb + w < w_end) ? bv[base[q] + nb + w] : 0ull;
+        for (int ks = 0; ks < 2 * kMW; ++ks) {
+            v4i a[2], b[2];
 #pragma unroll
-        for (int w = 0; w < kMW; ++w) {
-            uint64_t xa[BM], xb[BM];
-#pragma unroll
-            for (int t = 0; t < BM; ++t) {
-                xa[t] = rawA[buf][w][wm * 32 * BM + 32 * t + (lane & 31)];
-                xb[t] = rawB[buf][w][wn * 32 * BM + 32 * t + (lane & 31)];
+            for (int t = 0; t < 2; ++t) {
+                a[t] = *reinterpret_cast<const v4i*>(ops + (wm * 64 + 32 * t + (lane & 31)) * kGRow + 32 * ks + sub);
+                b[t] = *reinterpret_cast<const v4i*>(ops + (T + wn * 64 + 32 * t + (lane & 31)) * kGRow + 32 * ks + sub);
             }
 #pragma unroll
-            for (int hw = 0; hw < 2; ++hw) {
-                const int sh = 32 * hw + sub;
-                v4i a[BM], b[BM];
+            for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-                for (int t = 0; t < BM; ++t) {
-                    a[t] = spread16((uint32_t)(xa[t] >> sh) & 0xFFFFu);
-                    b[t] = spread16((uint32_t)(xb[t] >> sh) & 0xFFFFu);
-                }
-#pragma unroll
-                for (int mi = 0; mi < BM; ++mi)
-#pragma unroll
-                    for (int ni = 0; ni < BM; ++ni)
-                        acc[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[mi], b[ni], acc[mi][ni], 0, 0, 0);
-            }
+                for (int ni = 0; ni < 2; ++ni)
+                    acc[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[mi], b[ni], acc[mi][ni], 0, 0, 0);
         }
+        __syncthreads();  // the next step's expansion overwrites the operands
     }
-    // acc[mi][ni][r]: row 8 (r / 4) + 4 (lane / 32) + r % 4 of the A block, column lane % 32 of the B block.
 #pragma unroll
-    for (int mi = 0; mi < BM; ++mi) {
+    for (int mi = 0; mi < 2; ++mi) {
 #pragma unroll
-        for (int ni = 0; ni < BM; ++ni) {
+        for (int ni = 0; ni < 2; ++ni) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int64_t i = ta * T + wm * 32 * BM + mi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                const int64_t j = tb * T + wn * 32 * BM + ni * 32 + (lane & 31);
+                const int64_t i = ta * T + wm * 64 + mi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int64_t j = tb * T + wn * 64 + ni * 32 + (lane & 31);
                 if (i < j && j < s && acc[mi][ni][r]) atomicAdd(&counts[pair_slot(i, j, s, layout)], acc[mi][ni][r]);
             }
         }
@@ -956,8 +1057,16 @@ int64_t live_units_per_xcd(int64_t ub, int64_t target) { return ceil_div(target,
 
 }  // namespace
 
+// Workspace layout: kLiveCounters int64, the small list (n_small int32), the dense tile-pair list (n_tile_pairs
+// int32), then, 8-byte aligned, the sparse tile-pair entries (n_tile_pairs SparseTP).
+int64_t ws_sparse_offset(int64_t n_small, int64_t n_tile_pairs) {
+    return (kLiveCounters * 8 + (n_small + n_tile_pairs) * 4 + 7) / 8 * 8;
+}
+
 extern "C" size_t etm_count_level_ws(int64_t n_small, int64_t n_tile_pairs) {
-    return (size_t)kLiveCounters * 8 + (size_t)(std::max<int64_t>(n_small, 0) + std::max<int64_t>(n_tile_pairs, 0)) * 4;
+    n_small = std::max<int64_t>(n_small, 0);
+    n_tile_pairs = std::max<int64_t>(n_tile_pairs, 0);
+    return (size_t)(ws_sparse_offset(n_small, n_tile_pairs) + n_tile_pairs * (int64_t)sizeof(SparseTP));
 }
 
 namespace {
@@ -976,6 +1085,7 @@ int live_count_impl(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const 
     for (int c = 0; c < 4; ++c) cb.start[c + 1] = cb.start[c] + cls[c];
     const int64_t n_small = cb.start[4];
     int32_t* live_tp = live_small + n_small;
+    SparseTP* sparse = (SparseTP*)((char*)d_ws + ws_sparse_offset(n_small, n_tile_pairs));
     if (d_counters == nullptr) {
         k_zero_i64n<<<1, 64, 0, s>>>(cnt, kLiveCounters);
         ETM_LAUNCHED("k_zero_i64n");
@@ -985,7 +1095,7 @@ int live_count_impl(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const 
     if (small_blocks + tile_blocks > 0) {
         k_live_units<<<(unsigned)(small_blocks + tile_blocks), 256, 0, s>>>(
             d_small_list, n_small, cb, small_blocks, d_group_start, d_cum_pairs, d_cum_tiles, n_groups, n_tile_pairs,
-            d_status, live_small, live_tp, cnt);
+            d_status, live_small, live_tp, sparse, cnt);
         ETM_LAUNCHED("k_live_units");
     }
     const bool vec = pitch % 2 == 0 && ((uintptr_t)d_bv & 15) == 0;
@@ -1018,6 +1128,20 @@ int live_count_impl(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const 
                                                             d_status, d_counts, layout, kLiveTiledTarget,
                                                             kLiveTiledMin);
         ETM_LAUNCHED("k_count_tiled_live");
+        const int64_t sblocks = kXcds * ceil_div(live_units_per_xcd(n_tile_pairs, kLiveSmallTarget), 4);
+        ETM_REQUIRE(sblocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",
+                    (long long)sblocks);
+        if (vec)
+            k_count_sparse_live<true><<<(unsigned)sblocks, 256, 0, s>>>(d_bv, pitch, n_words, d_rows, m,
+                                                                        d_group_start, d_cum_pairs, d_cum_tiles,
+                                                                        n_groups, sparse, cnt + 5, d_counts,
+                                                                        kLiveSmallTarget, kLiveSmallMin);
+        else
+            k_count_sparse_live<false><<<(unsigned)sblocks, 256, 0, s>>>(d_bv, pitch, n_words, d_rows, m,
+                                                                         d_group_start, d_cum_pairs, d_cum_tiles,
+                                                                         n_groups, sparse, cnt + 5, d_counts,
+                                                                         kLiveSmallTarget, kLiveSmallMin);
+        ETM_LAUNCHED("k_count_sparse_live");
     }
     return ETM_OK;
 }
@@ -1054,7 +1178,7 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
     if (n_words == 0) return ETM_OK;
     hipStream_t s = ETM_STREAM(stream);
     // With a workspace and a status array (and no occupancy skip) only the units holding a counted pair run.
-    if (d_ws != nullptr && d_status != nullptr && d_occ == nullptr && word_chunk <= 0)
+    if (d_ws != nullptr && d_status != nullptr && d_occ == nullptr && word_chunk <= 0 && layout == ETM_LAYOUT_LEX)
         return live_count_impl(d_bv, pitch, n_words, d_rows, m, d_group_start, d_cum_pairs, d_cum_tiles, n_groups,
                                n_tile_pairs, d_small_list, cls, d_status, d_counts, layout, d_ws, nullptr, s);
     int64_t base = 0;
@@ -1098,8 +1222,8 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
             const int64_t blocks = kXcds * xcd_units(n_tp, n_chunks);
             ETM_REQUIRE(blocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",
                         (long long)blocks);
-            k_gram_mfma<2><<<(unsigned)blocks, 256, 0, s>>>(d_bv, pitch, n_words, chunk, d_rows, s_cols, n_tp, n_chunks,
-                                                            d_counts, layout);
+            k_gram_mfma<<<(unsigned)blocks, 256, 0, s>>>(d_bv, pitch, n_words, chunk, d_rows, s_cols, n_tp, n_chunks,
+                                                         d_counts, layout);
             ETM_LAUNCHED("k_gram_mfma");
             return ETM_OK;
         }

@@ -6,9 +6,10 @@
 // (survivors, counted/inferred/skipped tallies, the next level's group totals). Levels of up to
 // kSingleMax rows (or candidates) take single-workgroup group-build and compaction kernels that read their
 // sizes on the device, so nothing between two syncs waits on the host. Larger levels fall back to the
-// multi-block scans with one extra sync. Device memory comes from the stream-ordered pool
-// (hipMallocAsync / hipFreeAsync), released when the call returns.
+// multi-block scans with one extra sync. Device memory comes from a per-thread bump allocator over blocks
+// kept across calls (DeviceHeap below), so the levels make no allocator calls between their syncs.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <exception>
@@ -16,6 +17,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -953,24 +955,49 @@ extern "C" int etm_levels_flat(const etm_levels* lv, int64_t* h_totals, int64_t*
     h_totals[0] = n_sets;
     h_totals[1] = n_vals;
     if (!h_values || !h_offsets || !h_counts) return ETM_OK;
-    const bool map = !lv->col_to_item.empty();
-    int64_t at = 0, row = 0;
-    h_offsets[0] = 0;
-    const int64_t* c2i = map ? lv->col_to_item.data() : nullptr;
+    const int64_t* c2i = lv->col_to_item.empty() ? nullptr : lv->col_to_item.data();
+    // Each level's rows [lo, hi) land at fixed places (itemsets before the level, items before them), so ranges of
+    // rows are written by separate threads when the result is large.
+    struct Range {
+        size_t l;
+        int64_t lo, hi, row0, at0;
+    };
+    std::vector<Range> ranges;
+    const int64_t kRowsPerTask = 1 << 18;
+    int64_t row = 0, at = 0;
     for (size_t l = 0; l < lv->k.size(); ++l) {
         const int64_t k = lv->k[l], n = lv->n[l];
-        const int32_t* r = lv->rows[l];
-        const int32_t* c = lv->counts[l];
-        if (c2i)
-            for (int64_t i = 0; i < n * k; ++i) h_values[at + i] = c2i[r[i]];
-        else
-            for (int64_t i = 0; i < n * k; ++i) h_values[at + i] = r[i];
-        for (int64_t i = 0; i < n; ++i) {
-            h_counts[row] = c[i];
-            at += k;
-            h_offsets[++row] = at;
+        for (int64_t lo = 0; lo < n; lo += kRowsPerTask) {
+            const int64_t hi = std::min(n, lo + kRowsPerTask);
+            ranges.push_back(Range{l, lo, hi, row + lo, at + lo * k});
         }
+        row += n;
+        at += n * k;
     }
+    h_offsets[0] = 0;
+    auto fill = [&](const Range& g) {
+        const int64_t k = lv->k[g.l];
+        const int32_t* r = lv->rows[g.l];
+        const int32_t* c = lv->counts[g.l];
+        int64_t* v = h_values + g.at0;
+        for (int64_t i = g.lo * k; i < g.hi * k; ++i) *v++ = c2i ? c2i[r[i]] : r[i];
+        for (int64_t i = g.lo; i < g.hi; ++i) {
+            h_counts[g.row0 + (i - g.lo)] = c[i];
+            h_offsets[g.row0 + (i - g.lo) + 1] = g.at0 + (i - g.lo + 1) * k;
+        }
+    };
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (ranges.size() <= 1 || hw == 1) {
+        for (const Range& g : ranges) fill(g);
+        return ETM_OK;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < std::min<size_t>(hw, ranges.size()); ++t)
+        pool.emplace_back([&] {
+            for (size_t i; (i = next.fetch_add(1)) < ranges.size();) fill(ranges[i]);
+        });
+    for (auto& th : pool) th.join();
     return ETM_OK;
 }
 

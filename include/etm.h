@@ -208,10 +208,11 @@ int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const 
                     size_t ws_bytes, void* stream);
 
 /* Workspace bytes of etm_count_level's live-unit path for a level of n_small small-class groups (the four class
- * sizes summed) and n_tile_pairs tile-pairs. With that workspace, a status array, no occupancy bitmap and
- * word_chunk <= 0, etm_count_level first lists on the device the groups and tile-pairs holding at least one
- * ETM_CAND_COUNT candidate, then counts only those with persistent kernels (a fixed grid taking (unit, word
- * chunk) items from one queue per XCD), the chunk length sized on the device from the number of live units.
+ * sizes summed) and n_tile_pairs tile-pairs. With that workspace, a status array, the LEX layout, no occupancy
+ * bitmap and word_chunk <= 0, etm_count_level first lists on the device the groups and tile-pairs holding at
+ * least one ETM_CAND_COUNT candidate (tile-pairs with at most 32 of them, and at most 4 per column in them, with
+ * their pair lists), then counts only those, the chunk length sized on the device from the number of live units:
+ * the small groups and sparse tile-pairs one wave per (unit, chunk), the other tile-pairs one workgroup each.
  * Without it, every group and tile-pair is launched and those with nothing to count return at once. */
 size_t etm_count_level_ws(int64_t n_small, int64_t n_tile_pairs);
 
