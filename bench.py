@@ -134,7 +134,7 @@ def main(argv=None) -> int:
     from et_miner.core.result import _mining_min_count
     from et_miner.gpu import device as dev
     from et_miner.gpu import kernels as K
-    from et_miner.synthetic import workload
+    from workloads import workload
     from golden.digests import table_digests
 
     t_gen = time.perf_counter()
@@ -146,7 +146,7 @@ def main(argv=None) -> int:
     served_jobs_data = None
     if args.served_jobs > 0:  # generated before the GPU is initialised
         from et_miner.serve.server import jobs_from_specs
-        from et_miner.synthetic import served_jobs
+        from workloads import served_jobs
 
         t_gen = time.perf_counter()
         served_jobs_data = jobs_from_specs(served_jobs(512)[:args.served_jobs], workers=min(16, os.cpu_count() or 1))

@@ -382,7 +382,8 @@ def run_plain(jobs: list, time_scale: float = 1.0, resident: bool = True) -> tup
 
 
 def jobs_from_specs(specs: list, workers: int = 8) -> list:
-    """The data of served-job specs (``et_miner.synthetic.served_jobs``), generated in a process pool.
+    """The data of served-job specs (each with ``job_id``, ``arrival_s`` and a ``workload`` whose ``generate()``
+    returns CSR arrays), generated in a process pool.
 
     The pool's workers are spawned, not forked, so a caller that has already initialised the GPU hands no device
     state to them."""

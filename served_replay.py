@@ -1,7 +1,7 @@
-"""Replays the served-jobs stream (``et_miner.synthetic.served_jobs``) on one GPU, plain or served.
+"""Replays the served-jobs stream (``workloads.served_jobs``) on one GPU, plain or served.
 
-    python -m et_miner.serve.replay --mode plain --jobs 512
-    python -m et_miner.serve.replay --mode served --workers 8 --jobs 512
+    python served_replay.py --mode plain --jobs 512
+    python served_replay.py --mode served --workers 8 --jobs 512
 
 Every job's dataset is generated on the host (before the GPU is touched) and made resident in HBM. The
 stream runs ``--passes`` times with every job queued at t=0 (the first pass warms the per-thread arenas); the
@@ -14,6 +14,11 @@ import argparse
 import json
 import os
 import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "et-miner_amd"))
+sys.path.insert(0, str(ROOT))
 
 
 def main(argv=None) -> int:
@@ -24,8 +29,8 @@ def main(argv=None) -> int:
     ap.add_argument("--passes", type=int, default=2)
     args = ap.parse_args(argv)
 
-    from ..synthetic import served_jobs
-    from .server import Server, jobs_from_specs, run_plain, serve
+    from workloads import served_jobs
+    from et_miner.serve.server import Server, jobs_from_specs, run_plain, serve
 
     jobs = jobs_from_specs(served_jobs(512)[: args.jobs], workers=min(16, os.cpu_count() or 1))
     import torch

@@ -65,7 +65,7 @@ def _job(spec) -> tuple:
 def main(argv=None) -> int:
     import numpy as np
 
-    from et_miner.synthetic import served_jobs, workload
+    from workloads import served_jobs, workload
 
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--only", nargs="*", default=None, help="workload names and/or 'served_jobs'")

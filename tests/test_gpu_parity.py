@@ -158,7 +158,7 @@ def test_random_datasets_match_oracle(seed, engine):
 
 @pytest.mark.parametrize("engine", ENGINES)
 def test_free_set_mode_matches_oracle(engine):
-    from et_miner.synthetic import PRESETS, generate_csr
+    from workloads import PRESETS, generate_csr
 
     data = generate_csr(PRESETS["smoke"])
     want = miner.mine(data.indptr, data.indices, 0.01, prune_equal_support=True).as_dict()
@@ -176,7 +176,7 @@ def test_free_set_mode_matches_oracle(engine):
 @pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("name", ["smoke-C1-base", "deepk-C1-base", "skew-C1-base"])
 def test_reference_lattices_on_gpu(name, engine):
-    from et_miner.synthetic import PRESETS, generate_csr
+    from workloads import PRESETS, generate_csr
 
     gold, params = load_lattice(name)
     data = generate_csr(PRESETS[params["dataset"]])
@@ -191,7 +191,7 @@ def test_reference_lattices_on_gpu(name, engine):
 @pytest.mark.parametrize("name,n_rows", [("plain_60k", None), ("deep_k_1m", 200_000), ("wide_vocab_2m", 100_000),
                                          ("deep_sparse_20m", 400_000)])
 def test_workloads_match_oracle(name, n_rows, engine):
-    from et_miner.synthetic import workload
+    from workloads import workload
 
     w = workload(name, n_rows)
     data = w.generate()
@@ -204,7 +204,7 @@ def test_workloads_match_oracle(name, n_rows, engine):
 @pytest.mark.parametrize("k2_kernel", ["valu", "mfma"])
 def test_k2_kernels_mine_identically(k2_kernel):
     """The int8 MFMA Gram and the VALU tiles give the same lattice (wide vocabulary: hundreds of columns)."""
-    from et_miner.synthetic import workload
+    from workloads import workload
 
     w = workload("wide_vocab_2m", 150_000)
     data = w.generate()
@@ -215,7 +215,7 @@ def test_k2_kernels_mine_identically(k2_kernel):
 def test_son_matches_in_core():
     from et_miner.core.result import as_dict
     from et_miner.streaming.son import son_csr
-    from et_miner.synthetic import workload
+    from workloads import workload
 
     for name, n_rows, chunk in [("plain_60k", None, 7_000), ("deep_k_1m", 100_000, 30_000)]:
         w = workload(name, n_rows)
@@ -248,7 +248,7 @@ def test_bitvecs_route_is_read_only_and_exact():
 
     from et_miner import apriori
     from et_miner.core.result import as_dict
-    from et_miner.synthetic import PRESETS, generate_csr
+    from workloads import PRESETS, generate_csr
 
     data = generate_csr(PRESETS["smoke"])
     gold, _ = load_lattice("smoke-C1-base")

@@ -86,7 +86,7 @@ def test_rows_with_inference_and_free_sets(infer):
 def test_rows_match_bitvector_kernels_on_a_wide_sparse_level():
     """A wide vocabulary at a low threshold: the automatic choice takes the rows; every count equals the
     bitvector kernels' (VALU tiles / MFMA Gram)."""
-    from et_miner.synthetic import workload
+    from workloads import workload
 
     w = workload("wide_vocab_2m", 60_000)
     d = w.generate()

@@ -23,7 +23,7 @@ def _want(name):
 
 @pytest.fixture(scope="module")
 def data():
-    from et_miner.synthetic import workload
+    from workloads import workload
 
     return {n: workload(n).generate() for n in ("plain_60k", "deep_k_1m")}
 
@@ -34,7 +34,7 @@ def data():
 @pytest.mark.parametrize("infer", [False, True])
 def test_schedules_match_oracle(data, name, k3_schedule, small_max, infer):
     from et_miner.core.apriori import mine_csr
-    from et_miner.synthetic import workload
+    from workloads import workload
 
     w = workload(name)
     d = data[name]

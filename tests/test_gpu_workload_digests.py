@@ -41,7 +41,7 @@ def _mine(data, w, **kw):
 @pytest.mark.parametrize("name,k2_kernel", [("plain_60k", "auto"), ("deep_k_1m", "auto"), ("wide_vocab_2m", "valu"),
                                             ("wide_vocab_2m", "mfma")])
 def test_full_size_workload_equals_oracle(name, k2_kernel):
-    from et_miner.synthetic import workload
+    from workloads import workload
 
     w = workload(name)
     entry = DIGESTS["workloads"][name]
@@ -55,7 +55,7 @@ def test_full_size_workload_equals_oracle(name, k2_kernel):
 def test_deep_sparse_20m_resident_and_son_equal_oracle():
     """20,000,000 rows: the resident run and SON in 2,000,000-row chunks both give the oracle's lattice."""
     from et_miner.streaming.son import son_csr
-    from et_miner.synthetic import workload
+    from workloads import workload
 
     w = workload("deep_sparse_20m")
     entry = DIGESTS["workloads"]["deep_sparse_20m"]
@@ -77,7 +77,7 @@ def test_served_sample_alone_and_mixed_equal_oracle():
     the mix."""
     from et_miner.serve import run_plain, serve
     from et_miner.serve.server import jobs_from_specs
-    from et_miner.synthetic import served_jobs
+    from workloads import served_jobs
 
     sample = DIGESTS["served_jobs"]
     specs = served_jobs(sample["n_jobs"], seed=sample["seed"])

@@ -107,7 +107,7 @@ def test_count_itemsets_equals_brute_force():
 
 @pytest.mark.parametrize("name", ["smoke-C1-base", "deepk-C1-base", "skew-C1-base"])
 def test_reference_lattices(name):
-    from et_miner.synthetic import PRESETS, generate_csr
+    from workloads import PRESETS, generate_csr
 
     gold, params = load_lattice(name)
     data = generate_csr(PRESETS[params["dataset"]])
@@ -119,7 +119,7 @@ def test_reference_lattices(name):
 
 
 def test_planted_motifs_recovered():
-    from et_miner.synthetic import PRESETS, generate_csr
+    from workloads import PRESETS, generate_csr
 
     data = generate_csr(PRESETS["smoke"])
     got = miner.mine(data.indptr, data.indices, 0.01).as_dict()
@@ -130,7 +130,7 @@ def test_planted_motifs_recovered():
 
 
 def test_son_equals_in_core():
-    from et_miner.synthetic import workload
+    from workloads import workload
 
     data = workload("plain_60k", n_rows=30_000).generate()
     core = miner.mine(data.indptr, data.indices, 0.01).as_dict()
@@ -203,7 +203,7 @@ def test_deduction_bounds_bracket_every_count(k, seed, mc):
 def test_thread_count_does_not_change_the_lattice():
     """The oracle's n_jobs (threads over prefix groups and subset lookups) leaves every itemset and count as
     with one thread, in each mining mode."""
-    from et_miner.synthetic import workload
+    from workloads import workload
 
     w = workload("deep_k_1m", 60_000)
     d = w.generate()

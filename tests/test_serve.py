@@ -146,7 +146,7 @@ def test_footprint_counts_frequent_columns():
 
 def _served_sample(n_jobs, rows_range, seed=4):
     from et_miner.serve.server import jobs_from_specs
-    from et_miner.synthetic import served_jobs
+    from workloads import served_jobs
 
     return jobs_from_specs(served_jobs(n_jobs, seed=seed, rows_range=rows_range), workers=1)
 

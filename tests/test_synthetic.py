@@ -4,7 +4,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from et_miner.synthetic import PRESETS, generate_csr, served_jobs, workload
+from workloads import PRESETS, generate_csr, served_jobs, workload
 
 
 def _rows_ok(data):

@@ -38,7 +38,7 @@ def test_digest_is_order_free_and_sensitive():
 
 
 def test_committed_entries_reproduce():
-    from et_miner.synthetic import served_jobs, workload
+    from workloads import served_jobs, workload
 
     w = workload("plain_60k")
     data = w.generate()

@@ -1,6 +1,7 @@
-"""Seeded synthetic transaction data, as CSR arrays of item ids.
+"""Seeded synthetic transaction data, as CSR arrays of item ids: the inputs of the benchmark and of the tests.
 
-Two families live here:
+This module is measurement and test infrastructure, not part of the et_miner package: nothing in the package
+imports it. Two families live here:
 
 * ``PRESETS`` / ``generate_csr``: the reference's own presets
   (et_miner/synthetic.py:105-195, :335-449 in the reference). Their lattices
