@@ -372,9 +372,9 @@ __global__ __launch_bounds__(256) void k_count_small(const uint64_t* __restrict_
 
 // Over a class's live groups (the groups with at least one counted pair, listed on the device by k_live_units):
 // *d_n_live groups, each cut into chunks sized from that count (live_chunks), the (group, chunk) units spread
-// over the 8 XCDs by xcd_unit, one wave each. The grid is sized from an upper bound of the units
-// (live_units_per_xcd); the waves past the actual units return at once.
-template <int S, bool VEC>
+// over the 8 XCDs by xcd_unit, one wave each at a time. The grid is sized from an upper bound of the units, capped
+// (live_grid_per_xcd): each wave takes every stride-th unit of its XCD and returns past the last.
+template <int S, bool VEC, bool LOOP>
 __global__ __launch_bounds__(256) void k_count_small_live(const uint64_t* __restrict__ bv, int64_t pitch,
                                                            int64_t n_words, const int32_t* __restrict__ rows,
                                                            int32_t m, const int64_t* __restrict__ group_start,
@@ -388,12 +388,14 @@ __global__ __launch_bounds__(256) void k_count_small_live(const uint64_t* __rest
     if (n_live == 0) return;
     int64_t chunk, n_chunks;
     live_chunks(n_words, n_live, target, 64, min_words, &chunk, &n_chunks);
-    int64_t ch, item;
-    if (!xcd_unit(blockIdx.x % kXcds, (int64_t)(blockIdx.x / kXcds) * 4 + (threadIdx.x >> 6), n_live, n_chunks, &ch,
-                  &item))
-        return;
-    small_unit<S, false, VEC>(bv, pitch, n_words, chunk, rows, m, group_start, cum_pairs, status, counts, layout,
-                              nullptr, 0, uniform64(live[uniform64(item)]), uniform64(ch));
+    const int64_t stride = (int64_t)(gridDim.x / kXcds) * 4;
+    for (int64_t u = (int64_t)(blockIdx.x / kXcds) * 4 + (threadIdx.x >> 6);; u += stride) {
+        int64_t ch, item;
+        if (!xcd_unit(blockIdx.x % kXcds, u, n_live, n_chunks, &ch, &item)) return;
+        small_unit<S, false, VEC>(bv, pitch, n_words, chunk, rows, m, group_start, cum_pairs, status, counts, layout,
+                                  nullptr, 0, uniform64(live[uniform64(item)]), uniform64(ch));
+        if (!LOOP) return;
+    }
 }
 
 // LDS of the tiled kernels. Dense mode: A and B tiles [word][column] of kKW words. Listed mode: the live columns
@@ -679,7 +681,9 @@ __global__ __launch_bounds__(256) void k_count_tiled(const uint64_t* __restrict_
 
 // Over the live tile-pairs (at least one counted pair; listed on the device by k_live_units): *d_n_live of
 // them, each cut into chunks sized from that count, one workgroup per (tile-pair, chunk) unit spread over the 8
-// XCDs by xcd_unit. The grid is sized from an upper bound of the units; the workgroups past them return at once.
+// XCDs by xcd_unit. The grid is sized from a capped upper bound of the units (live_grid_per_xcd); each workgroup
+// takes every stride-th unit of its XCD.
+template <bool LOOP>
 __global__ __launch_bounds__(256) void k_count_tiled_live(const uint64_t* __restrict__ bv, int64_t pitch,
                                                            int64_t n_words, const int32_t* __restrict__ rows,
                                                            int32_t m, const int64_t* __restrict__ group_start,
@@ -695,35 +699,27 @@ __global__ __launch_bounds__(256) void k_count_tiled_live(const uint64_t* __rest
     if (n_live == 0) return;
     int64_t chunk, n_chunks;
     live_chunks(n_words, n_live, target, kKW, min_words, &chunk, &n_chunks);
-    int64_t ch, item;
-    if (!xcd_unit(blockIdx.x % kXcds, blockIdx.x / kXcds, n_live, n_chunks, &ch, &item)) return;
-    tiled_unit(sh, bv, pitch, n_words, chunk, rows, m, group_start, cum_pairs, cum_tiles, n_groups, status, counts,
-               layout, uniform64(live[uniform64(item)]), uniform64(ch));
+    for (int64_t u = blockIdx.x / kXcds;; u += gridDim.x / kXcds) {
+        int64_t ch, item;
+        if (!xcd_unit(blockIdx.x % kXcds, u, n_live, n_chunks, &ch, &item)) return;
+        tiled_unit(sh, bv, pitch, n_words, chunk, rows, m, group_start, cum_pairs, cum_tiles, n_groups, status,
+                   counts, layout, uniform64(live[uniform64(item)]), uniform64(ch));
+        if (!LOOP) return;
+        __syncthreads();  // the next unit's set-up overwrites this one's LDS
+    }
 }
 
 // One wave per (sparse tile-pair, word chunk): lane p < n holds pair p's two column offsets and its count slot;
 // per word, the prefix AND, then for each pair the AND of its two columns with it (the pair's offsets broadcast
 // from lane p), popcounts accumulated per lane; a wave sum and one atomic per pair at the end.
 template <bool VEC>
-__global__ __launch_bounds__(256) void k_count_sparse_live(const uint64_t* __restrict__ bv, int64_t pitch,
-                                                            int64_t n_words, const int32_t* __restrict__ rows,
-                                                            int32_t m, const int64_t* __restrict__ group_start,
-                                                            const int64_t* __restrict__ cum_pairs,
-                                                            const int64_t* __restrict__ cum_tiles, int64_t n_groups,
-                                                            const SparseTP* __restrict__ sparse,
-                                                            const int64_t* __restrict__ d_n_live,
-                                                            int32_t* __restrict__ counts, int64_t target,
-                                                            int64_t min_words) {
-    const int64_t n_live = *d_n_live;
-    if (n_live == 0) return;
-    int64_t chunk, n_chunks;
-    live_chunks(n_words, n_live, target, 64, min_words, &chunk, &n_chunks);
-    int64_t ch, item;
-    if (!xcd_unit(blockIdx.x % kXcds, (int64_t)(blockIdx.x / kXcds) * 4 + (threadIdx.x >> 6), n_live, n_chunks, &ch,
-                  &item))
-        return;
-    ch = uniform64(ch);
-    const SparseTP* e = sparse + uniform64(item);
+__device__ __forceinline__ void sparse_unit(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
+                                            const int32_t* __restrict__ rows, int32_t m,
+                                            const int64_t* __restrict__ group_start,
+                                            const int64_t* __restrict__ cum_pairs,
+                                            const int64_t* __restrict__ cum_tiles, int64_t n_groups,
+                                            const SparseTP* __restrict__ e, int64_t chunk, int64_t ch,
+                                            int32_t* __restrict__ counts) {
     const int lane = threadIdx.x & 63;
     const int64_t tp = uniform64(e->tp);
     const int n = (int)uniform64(e->n);
@@ -793,6 +789,30 @@ __global__ __launch_bounds__(256) void k_count_sparse_live(const uint64_t* __res
     }
 }
 
+template <bool VEC, bool LOOP>
+__global__ __launch_bounds__(256) void k_count_sparse_live(const uint64_t* __restrict__ bv, int64_t pitch,
+                                                            int64_t n_words, const int32_t* __restrict__ rows,
+                                                            int32_t m, const int64_t* __restrict__ group_start,
+                                                            const int64_t* __restrict__ cum_pairs,
+                                                            const int64_t* __restrict__ cum_tiles, int64_t n_groups,
+                                                            const SparseTP* __restrict__ sparse,
+                                                            const int64_t* __restrict__ d_n_live,
+                                                            int32_t* __restrict__ counts, int64_t target,
+                                                            int64_t min_words) {
+    const int64_t n_live = *d_n_live;
+    if (n_live == 0) return;
+    int64_t chunk, n_chunks;
+    live_chunks(n_words, n_live, target, 64, min_words, &chunk, &n_chunks);
+    const int64_t stride = (int64_t)(gridDim.x / kXcds) * 4;
+    for (int64_t u = (int64_t)(blockIdx.x / kXcds) * 4 + (threadIdx.x >> 6);; u += stride) {
+        int64_t ch, item;
+        if (!xcd_unit(blockIdx.x % kXcds, u, n_live, n_chunks, &ch, &item)) return;
+        sparse_unit<VEC>(bv, pitch, n_words, rows, m, group_start, cum_pairs, cum_tiles, n_groups,
+                         sparse + uniform64(item), chunk, uniform64(ch), counts);
+        if (!LOOP) return;
+    }
+}
+
 // ── live units ─────────────────────────────────────────────────────────
 //
 // The groups of the small classes and the tile-pairs of the tiled groups with at least one candidate left
@@ -804,7 +824,8 @@ __global__ __launch_bounds__(256) void k_count_sparse_live(const uint64_t* __res
 // small_list[0 .. n_small): class c's live groups go to live_small[start_c ..], counted in n_live[c]. The other
 // workgroups take the tile-pairs, a wave per tile-pair (lane l checks row ta * 64 + l against the rows of tile tb):
 // sparse live tile-pairs go to `sparse` with their pair lists, counted in n_live[5], the other live ones to
-// live_tp, counted in n_live[4]. n_live[0..5] start at zero.
+// live_tp, counted in n_live[4]; tile-pairs of groups flagged in gl_flag (when given) are left to the
+// group-listed kernel. n_live[0..5] start at zero.
 __global__ __launch_bounds__(256) void k_live_units(const int32_t* __restrict__ small_list, int64_t n_small,
                                                     ClassBounds cb, int64_t small_blocks,
                                                     const int64_t* __restrict__ group_start,
@@ -812,7 +833,8 @@ __global__ __launch_bounds__(256) void k_live_units(const int32_t* __restrict__ 
                                                     const int64_t* __restrict__ cum_tiles, int64_t n_groups,
                                                     int64_t n_tile_pairs, const uint8_t* __restrict__ status,
                                                     int32_t* __restrict__ live_small, int32_t* __restrict__ live_tp,
-                                                    SparseTP* __restrict__ sparse, int64_t* __restrict__ n_live) {
+                                                    SparseTP* __restrict__ sparse, int64_t* __restrict__ n_live,
+                                                    const uint8_t* __restrict__ gl_flag) {
     if ((int64_t)blockIdx.x < small_blocks) {
         for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n_small;
              e += small_blocks * (int64_t)blockDim.x) {
@@ -834,6 +856,7 @@ __global__ __launch_bounds__(256) void k_live_units(const int32_t* __restrict__ 
     for (int64_t tp = ((int64_t)blockIdx.x - small_blocks) * (blockDim.x >> 6) + (threadIdx.x >> 6); tp < n_tile_pairs;
          tp += waves) {
         const int64_t g = etm::wave_upper_index(cum_tiles, n_groups, tp);
+        if (gl_flag != nullptr && gl_flag[g]) continue;  // counted by the group-listed kernel
         const int64_t gs = group_start[g];
         const int64_t s = group_start[g + 1] - gs;
         const int64_t nt = (s + kT - 1) / kT;
@@ -875,6 +898,294 @@ __global__ __launch_bounds__(256) void k_live_units(const int32_t* __restrict__ 
         } else if (lane == 0) {
             const int64_t at = (int64_t)atomicAdd((unsigned long long*)&n_live[4], 1ull);
             live_tp[at] = (int32_t)tp;
+        }
+    }
+}
+
+// ── group-listed counting of the tiled groups ───────────────────────────────────────────────────────
+//
+// A tiled group (more than small_max suffixes) cut into 64 x 64 tile-pairs re-reads each suffix column once per
+// tile-pair it appears in, and every (tile-pair, chunk) unit re-derives its live pairs from 4096 statuses. Where
+// a group's counted pairs are few enough (kGlSparseMax < pairs <= kGlPairsMax) over few enough columns (at most
+// kGlColsMax), k_gl_scan lists them once per level, and k_count_glist counts each (group, word chunk) unit with
+// all of the group's used columns staged in LDS: each column is read once per chunk. The other tiled groups keep
+// the tile-pair kernels (k_live_units skips the flagged ones).
+constexpr int kGlSparseMax = kSparseMax;   // fewer counted pairs: the tile-pair path's sparse wave kernel
+constexpr int kGlPairsMax = 4096;          // counted pairs of a group-listed group: 16 per thread
+constexpr int kGlColsMax = 2048;           // its used suffix columns (LDS slots)
+constexpr int64_t kGlScanMax = 1 << 22;    // candidates of a group that one workgroup scans
+constexpr int kGlPer = 16;                 // words a thread stages per step
+constexpr int kGlLdsWords = 256 * kGlPer;  // staging buffer of k_count_glist (32 KiB)
+constexpr int kGlMaxWords = 256;           // most words per staging step (a divisor of 256)
+constexpr int kGlPreMax = 4;               // prefix words loaded ahead with the columns (more: read in step)
+constexpr int kGlScanPer = 16;             // candidates per thread per step of k_gl_scan
+
+// Words per staging step of a group-listed group of n_used columns: the largest power of two (at most
+// kGlMaxWords) whose [word][slot] block (odd stride) fits kGlLdsWords.
+__device__ __forceinline__ int gl_step_log2(int n_used) {
+    const int stride = n_used | 1;
+    int lg = 0;
+    while ((1 << (lg + 1)) <= kGlMaxWords && stride * (1 << (lg + 1)) <= kGlLdsWords) ++lg;
+    return lg;
+}
+// Groups whose steps hold at most this many words have long units (many steps per chunk): they are listed first
+// and dispatched first, so that none of them starts late and sets the launch's tail.
+constexpr int kGlHeavyWords = 32;
+
+struct GlGroup {
+    int64_t pair_start, col_start;
+    int32_t g, n_pairs, n_used, pad;
+};
+
+struct GlistWs {
+    uint8_t* flag;         // [n_groups] 1: the group is group-listed
+    GlGroup* groups;       // [n_tiled]
+    uint32_t* pair_slots;  // [pair_cap] slot(i) | slot(j) << 16
+    uint32_t* pair_ij;     // [pair_cap] i | j << 16
+    int32_t* cols;         // [col_cap] column ids of the used suffixes, by slot
+};
+
+// Counters of the mode: [0] heavy group-listed groups (listed from the front of ws.groups), [1] pair slots
+// taken, [2] column slots taken, [3] light ones (listed from the back).
+__global__ __launch_bounds__(256) void k_gl_scan(const int32_t* __restrict__ tiled_list, int64_t n_tiled,
+                                                 const int32_t* __restrict__ rows, int32_t m,
+                                                 const int64_t* __restrict__ group_start,
+                                                 const int64_t* __restrict__ cum_pairs,
+                                                 const uint8_t* __restrict__ status, GlistWs ws,
+                                                 int64_t* __restrict__ gl_cnt) {
+    __shared__ uint64_t used[65536 / 64];
+    __shared__ int32_t wpre[65536 / 64];
+    __shared__ int64_t lds[8];
+    __shared__ int64_t s_base[3];
+    const int tid = threadIdx.x;
+    const int32_t g = tiled_list[blockIdx.x];
+    const int64_t gs = group_start[g];
+    const int64_t s = group_start[g + 1] - gs;
+    const int64_t P = s * (s - 1) / 2;
+    const int64_t cbase = cum_pairs[g];
+    if (s > 65535 || P > kGlScanMax) {
+        if (tid == 0) ws.flag[g] = 0;
+        return;
+    }
+    const int nw = (int)((s + 63) / 64);
+    for (int w = tid; w < nw; w += 256) used[w] = 0;
+    __syncthreads();
+    // Thread t takes candidates [t * 16, t * 16 + 16) of each step of 4096, decoded once and then walked in
+    // candidate order ((i, j) -> (i, j + 1), or (i + 1, i + 2) at the end of row i).
+    constexpr int kStep = 256 * kGlScanPer;
+    // Pass 1: the counted pairs and the suffixes they use.
+    int64_t n_pairs = 0;
+    for (int64_t p0 = (int64_t)tid * kGlScanPer; p0 < P; p0 += kStep) {
+        uint32_t live = 0;
+#pragma unroll
+        for (int q = 0; q < kGlScanPer; ++q)
+            if (p0 + q < P && status[cbase + p0 + q] == ETM_CAND_COUNT) live |= 1u << q;
+        if (!live) continue;
+        int64_t i, j;
+        etm::lex_pair_decode(p0, s, &i, &j);
+        for (int q = 0; q < kGlScanPer; ++q) {
+            if ((live >> q) & 1u) {
+                atomicOr((unsigned long long*)&used[i >> 6], 1ull << (i & 63));
+                atomicOr((unsigned long long*)&used[j >> 6], 1ull << (j & 63));
+                ++n_pairs;
+            }
+            if (++j == s) {
+                ++i;
+                j = i + 1;
+            }
+        }
+    }
+    n_pairs = etm::block_sum<256>(n_pairs, lds);
+    // Slots: exclusive popcount prefix of the used bitmap, four words a thread.
+    int32_t cnt4 = 0;
+    for (int w = 4 * tid; w < min(nw, 4 * tid + 4); ++w) cnt4 += __popcll(used[w]);
+    int64_t n_used = 0;
+    const int64_t pre4 = etm::block_exclusive_scan<256>(cnt4, lds, &n_used);
+    {
+        int32_t at = (int32_t)pre4;
+        for (int w = 4 * tid; w < min(nw, 4 * tid + 4); ++w) {
+            wpre[w] = at;
+            at += __popcll(used[w]);
+        }
+    }
+    const bool listed = n_pairs > kGlSparseMax && n_pairs <= kGlPairsMax && n_used <= kGlColsMax;
+    if (tid == 0) {
+        ws.flag[g] = listed ? 1 : 0;
+        if (listed) {
+            const bool heavy = (1 << gl_step_log2((int)n_used)) <= kGlHeavyWords;
+            const int64_t q = heavy ? (int64_t)atomicAdd((unsigned long long*)&gl_cnt[0], 1ull)
+                                    : n_tiled - 1 - (int64_t)atomicAdd((unsigned long long*)&gl_cnt[3], 1ull);
+            s_base[0] = (int64_t)atomicAdd((unsigned long long*)&gl_cnt[1], (unsigned long long)n_pairs);
+            s_base[1] = (int64_t)atomicAdd((unsigned long long*)&gl_cnt[2], (unsigned long long)n_used);
+            ws.groups[q] = GlGroup{s_base[0], s_base[1], g, (int32_t)n_pairs, (int32_t)n_used, 0};
+        }
+    }
+    __syncthreads();
+    if (!listed) return;
+    const int64_t pair_start = s_base[0], col_start = s_base[1];
+    const int np = m - 1;
+    auto slot_of = [&](int64_t x) {
+        const uint64_t w = used[x >> 6];
+        return (uint32_t)(wpre[x >> 6] + __popcll(w & ((1ull << (x & 63)) - 1)));
+    };
+    for (int64_t i = tid; i < s; i += 256)
+        if ((used[i >> 6] >> (i & 63)) & 1ull) ws.cols[col_start + slot_of(i)] = rows[(gs + i) * m + np];
+    // Pass 2: the pairs in candidate order (their A slots ascend, so a wave's A reads mostly broadcast).
+    int64_t run = 0;
+    for (int64_t s0 = 0; s0 < P; s0 += kStep) {
+        const int64_t p0 = s0 + (int64_t)tid * kGlScanPer;
+        uint32_t live = 0;
+#pragma unroll
+        for (int q = 0; q < kGlScanPer; ++q)
+            if (p0 + q < P && status[cbase + p0 + q] == ETM_CAND_COUNT) live |= 1u << q;
+        int64_t total = 0;
+        int64_t at = run + etm::block_exclusive_scan<256>(__popc(live), lds, &total);
+        if (live) {
+            int64_t i, j;
+            etm::lex_pair_decode(p0, s, &i, &j);
+            for (int q = 0; q < kGlScanPer; ++q) {
+                if ((live >> q) & 1u) {
+                    ws.pair_slots[pair_start + at] = slot_of(i) | (slot_of(j) << 16);
+                    ws.pair_ij[pair_start + at] = (uint32_t)i | ((uint32_t)j << 16);
+                    ++at;
+                }
+                if (++j == s) {
+                    ++i;
+                    j = i + 1;
+                }
+            }
+        }
+        run += total;
+    }
+}
+
+// One workgroup per (group-listed group, word chunk) unit. Per step of kw words the group's used columns are
+// staged [word][slot] (odd stride) in LDS, ANDed with the group's prefix; the next step's words are loaded into
+// registers while the current step counts, so a step costs its counting, not a global load round trip. Pairs:
+// with at most 256 of them, tpp = 256 / pow2ceil(pairs) threads share a pair's words (one accumulator each,
+// summed over the tpp lanes at the end); with more, thread t takes pairs t, t + 256, ... over every word. One
+// atomic per pair at the end. kw is the largest power of two (at most kGlMaxWords) whose block fits the buffer.
+__global__ __launch_bounds__(256) void k_count_glist(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
+                                                     const int32_t* __restrict__ rows, int32_t m,
+                                                     const int64_t* __restrict__ group_start,
+                                                     const int64_t* __restrict__ cum_pairs, GlistWs ws,
+                                                     int64_t n_tiled, const int64_t* __restrict__ gl_cnt,
+                                                     int32_t* __restrict__ counts, int64_t target,
+                                                     int64_t min_words) {
+    __shared__ uint64_t buf[kGlLdsWords];
+    __shared__ int32_t col_id[kGlColsMax];
+    const int64_t n_heavy = gl_cnt[0], n_light = gl_cnt[3];
+    const int64_t n_gl = n_heavy + n_light;
+    if (n_gl == 0) return;
+    int64_t chunk, n_chunks;
+    live_chunks(n_words, n_gl, target, kGlMaxWords, min_words, &chunk, &n_chunks);
+    // Units of the heavy groups first (chunk-major), then those of the light ones (listed from the back).
+    int64_t ch, idx;
+    const int64_t u = blockIdx.x;
+    if (u < n_heavy * n_chunks) {
+        ch = u / n_heavy;
+        idx = u % n_heavy;
+    } else {
+        const int64_t v = u - n_heavy * n_chunks;
+        if (v >= n_light * n_chunks) return;
+        ch = v / n_light;
+        idx = n_tiled - 1 - v % n_light;
+    }
+    const GlGroup e = ws.groups[uniform64(idx)];
+    const int tid = threadIdx.x;
+    const int64_t gs = group_start[e.g];
+    const int64_t s = group_start[e.g + 1] - gs;
+    const int np = m - 1;
+    const int32_t* prow = rows + gs * m;
+    const int n_used = e.n_used, n_pairs = e.n_pairs;
+    for (int c = tid; c < n_used; c += 256) col_id[c] = ws.cols[e.col_start + c];
+    const int stride = n_used | 1;
+    const int lg = gl_step_log2(n_used);
+    const int kw = 1 << lg;
+    // Pair work: tpp lanes per pair (tpp = 1: up to kGlPairsMax / 256 pairs a thread).
+    int tpp = 1;
+    while (tpp < 4 && n_pairs * (2 * tpp) <= 256) tpp <<= 1;
+    const int sub = tid & (tpp - 1);
+    uint32_t slot[kGlPairsMax / 256];
+    int acc[kGlPairsMax / 256];
+#pragma unroll
+    for (int q = 0; q < kGlPairsMax / 256; ++q) {
+        const int pi = tpp > 1 ? (q == 0 ? tid / tpp : n_pairs) : tid + 256 * q;
+        slot[q] = pi < n_pairs ? ws.pair_slots[e.pair_start + pi] : 0u;
+        acc[q] = 0;
+    }
+    __syncthreads();
+    const int64_t w_begin = uniform64(ch) * chunk;
+    const int64_t w_end = min(n_words, w_begin + chunk);
+    // kw divides 256: thread t stages word t % kw of every step, for slots t / kw + q * (256 / kw), q < kGlPer.
+    const int wq = tid & (kw - 1), c0 = tid >> lg, cstep = 256 >> lg;
+    uint64_t nxt[kGlPer];
+    uint64_t pw[kGlPreMax];
+    auto load_step = [&](int64_t wb) {
+        const bool in = wb + wq < w_end;
+#pragma unroll
+        for (int t = 0; t < kGlPreMax; ++t)
+            pw[t] = (in && t < np) ? bv[(int64_t)prow[t] * pitch + wb + wq] : ~0ull;
+#pragma unroll
+        for (int q = 0; q < kGlPer; ++q) {
+            const int c = c0 + q * cstep;
+            nxt[q] = (in && c < n_used) ? bv[(int64_t)col_id[c] * pitch + wb + wq] : 0ull;
+        }
+    };
+    load_step(w_begin);
+    for (int64_t wb = w_begin; wb < w_end; wb += kw) {
+        uint64_t pre = wb + wq < w_end ? ~0ull : 0ull;
+#pragma unroll
+        for (int t = 0; t < kGlPreMax; ++t) pre &= pw[t];
+        for (int t = kGlPreMax; t < np; ++t) pre &= bv[(int64_t)prow[t] * pitch + wb + wq];
+#pragma unroll
+        for (int q = 0; q < kGlPer; ++q) {
+            const int c = c0 + q * cstep;
+            if (c < n_used) buf[wq * stride + c] = nxt[q] & pre;
+        }
+        __syncthreads();
+        if (wb + kw < w_end) load_step(wb + kw);
+        if (tpp > 1) {
+            if (tid / tpp < n_pairs) {
+                const uint64_t* pa = buf + (slot[0] & 0xFFFFu);
+                const uint64_t* pb = buf + (slot[0] >> 16);
+                int a = acc[0];
+#pragma unroll 8
+                for (int w = sub; w < kw; w += tpp) a = popc_acc(pa[w * stride] & pb[w * stride], a);
+                acc[0] = a;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < kGlPairsMax / 256; ++q) {
+                if (tid + 256 * q < n_pairs) {
+                    const uint64_t* pa = buf + (slot[q] & 0xFFFFu);
+                    const uint64_t* pb = buf + (slot[q] >> 16);
+                    int a = acc[q];
+#pragma unroll 8
+                    for (int w = 0; w < kw; ++w) a = popc_acc(pa[w * stride] & pb[w * stride], a);
+                    acc[q] = a;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const int64_t cbase = cum_pairs[e.g];
+    if (tpp > 1) {
+        int a = acc[0];
+        for (int off = tpp >> 1; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+        const int pi = tid / tpp;
+        if (sub == 0 && pi < n_pairs && a) {
+            const uint32_t ij = ws.pair_ij[e.pair_start + pi];
+            atomicAdd(&counts[cbase + etm::lex_pair_index(ij & 0xFFFFu, ij >> 16, s)], a);
+        }
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < kGlPairsMax / 256; ++q) {
+        const int pi = tid + 256 * q;
+        if (pi < n_pairs && acc[q]) {
+            const uint32_t ij = ws.pair_ij[e.pair_start + pi];
+            atomicAdd(&counts[cbase + etm::lex_pair_index(ij & 0xFFFFu, ij >> 16, s)], acc[q]);
         }
     }
 }
@@ -1048,12 +1359,24 @@ hipError_t launch_small(const uint64_t* bv, int64_t pitch, int64_t n_words, int6
     return hipGetLastError();
 }
 
-constexpr int64_t kLiveCounters = 8;  // int64: [0..3] live groups per small class, [4] live tile-pairs
+constexpr int64_t kLiveCounters = 16;  // int64: [0..3] live groups per small class, [4] live tile-pairs,
+                                       // [5] sparse tile-pairs, [8..11] the group-listed mode's counters
 
 // Units one XCD can be handed by xcd_unit when at most `ub` units are live and live_chunks cuts each into
 // n_chunks <= max(1, ceil(target / n) rounded up to a multiple of 8, capped) chunks: with n_chunks >= 8 an XCD
 // takes n x ceil(n_chunks / 8) <= target / 8 + 2n units, with fewer chunks n x n_chunks / 8 < n.
 int64_t live_units_per_xcd(int64_t ub, int64_t target) { return ceil_div(target, kXcds) + 2 * ub + 1; }
+
+// Waves (or workgroups) per XCD of a live launch: the units' upper bound, or, when that bound exceeds 6x the
+// target's share (levels of very many groups: most deep levels, which hold few or no live units), 1.5x the
+// target's share with each wave taking every stride-th unit of its XCD (*loop). The looping form holds more
+// registers (the compiler keeps the unit's set-up live across the loop), so bounds below that use the one-unit
+// form.
+int64_t live_grid_per_xcd(int64_t ub, int64_t target, bool* loop) {
+    const int64_t units = live_units_per_xcd(ub, target), cap = 3 * ceil_div(target, 2 * kXcds);
+    *loop = units > 4 * cap;
+    return *loop ? cap : units;
+}
 
 }  // namespace
 
@@ -1078,7 +1401,7 @@ int live_count_impl(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const 
                     const int64_t* d_group_start, const int64_t* d_cum_pairs, const int64_t* d_cum_tiles,
                     int64_t n_groups, int64_t n_tile_pairs, const int32_t* d_small_list, const int64_t* cls,
                      const uint8_t* d_status, int32_t* d_counts, int32_t layout, void* d_ws, int64_t* d_counters,
-                     hipStream_t s) {
+                     const GlistWs* gl, int64_t n_tiled, hipStream_t s) {
     int64_t* cnt = d_counters != nullptr ? d_counters : (int64_t*)d_ws;
     int32_t* live_small = (int32_t*)((int64_t*)d_ws + kLiveCounters);
     ClassBounds cb{};
@@ -1090,58 +1413,95 @@ int live_count_impl(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const 
         k_zero_i64n<<<1, 64, 0, s>>>(cnt, kLiveCounters);
         ETM_LAUNCHED("k_zero_i64n");
     }
+    const bool glist = gl != nullptr && n_tiled > 0;
+    if (glist) {
+        ETM_REQUIRE(n_tiled <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld tiled groups exceed the grid",
+                    (long long)n_tiled);
+        k_gl_scan<<<(unsigned)n_tiled, 256, 0, s>>>(d_small_list + n_small, n_tiled, d_rows, m, d_group_start,
+                                                    d_cum_pairs, d_status, *gl, cnt + 8);
+        ETM_LAUNCHED("k_gl_scan");
+    }
     const int64_t small_blocks = std::min<int64_t>(ceil_div(n_small, 256), 2048);
     const int64_t tile_blocks = std::min<int64_t>(ceil_div(n_tile_pairs, 4), 8192);
     if (small_blocks + tile_blocks > 0) {
         k_live_units<<<(unsigned)(small_blocks + tile_blocks), 256, 0, s>>>(
             d_small_list, n_small, cb, small_blocks, d_group_start, d_cum_pairs, d_cum_tiles, n_groups, n_tile_pairs,
-            d_status, live_small, live_tp, sparse, cnt);
+            d_status, live_small, live_tp, sparse, cnt, glist ? gl->flag : nullptr);
         ETM_LAUNCHED("k_live_units");
     }
     const bool vec = pitch % 2 == 0 && ((uintptr_t)d_bv & 15) == 0;
     for (int c = 0; c < 4; ++c) {
         if (cls[c] == 0) continue;
-        const int64_t blocks = kXcds * ceil_div(live_units_per_xcd(cls[c], kLiveSmallTarget), 4);
+        bool loop = false;
+        const int64_t blocks = kXcds * ceil_div(live_grid_per_xcd(cls[c], kLiveSmallTarget, &loop), 4);
         ETM_REQUIRE(blocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",
                     (long long)blocks);
         const unsigned grid = (unsigned)blocks;
         const int32_t* list = live_small + cb.start[c];
-#define ETM_SMALL_LIVE(S, V)                                                                                        \
-    k_count_small_live<S, V><<<grid, 256, 0, s>>>(d_bv, pitch, n_words, d_rows, m, d_group_start, d_cum_pairs, list, \
-                                                  cnt + c, d_status, d_counts, layout, kLiveSmallTarget,      \
-                                                  kLiveSmallMin)
+#define ETM_SMALL_LIVE_L(S, V, L)                                                                                   \
+    k_count_small_live<S, V, L><<<grid, 256, 0, s>>>(d_bv, pitch, n_words, d_rows, m, d_group_start, d_cum_pairs,    \
+                                                     list, cnt + c, d_status, d_counts, layout, kLiveSmallTarget,   \
+                                                     kLiveSmallMin)
+#define ETM_SMALL_LIVE(S, V)                 \
+    do {                                         \
+        if (loop) ETM_SMALL_LIVE_L(S, V, true);  \
+        else ETM_SMALL_LIVE_L(S, V, false);      \
+    } while (0)
         switch (c) {
-            case 0: if (vec) ETM_SMALL_LIVE(2, true); else ETM_SMALL_LIVE(2, false); break;
-            case 1: if (vec) ETM_SMALL_LIVE(4, true); else ETM_SMALL_LIVE(4, false); break;
-            case 2: if (vec) ETM_SMALL_LIVE(8, true); else ETM_SMALL_LIVE(8, false); break;
+            case 0:
+                if (vec) ETM_SMALL_LIVE(2, true);
+                else ETM_SMALL_LIVE(2, false);
+                break;
+            case 1:
+                if (vec) ETM_SMALL_LIVE(4, true);
+                else ETM_SMALL_LIVE(4, false);
+                break;
+            case 2:
+                if (vec) ETM_SMALL_LIVE(8, true);
+                else ETM_SMALL_LIVE(8, false);
+                break;
             default: ETM_SMALL_LIVE(16, false); break;
         }
 #undef ETM_SMALL_LIVE
+#undef ETM_SMALL_LIVE_L
         ETM_LAUNCHED("k_count_small_live");
     }
     if (n_tile_pairs > 0) {
-        const int64_t blocks = kXcds * live_units_per_xcd(n_tile_pairs, kLiveTiledTarget);
+        bool loop = false;
+        const int64_t blocks = kXcds * live_grid_per_xcd(n_tile_pairs, kLiveTiledTarget, &loop);
         ETM_REQUIRE(blocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",
                     (long long)blocks);
-        k_count_tiled_live<<<(unsigned)blocks, 256, 0, s>>>(d_bv, pitch, n_words, d_rows, m, d_group_start,
-                                                            d_cum_pairs, d_cum_tiles, n_groups, live_tp, cnt + 4,
-                                                            d_status, d_counts, layout, kLiveTiledTarget,
-                                                            kLiveTiledMin);
+#define ETM_TILED_LIVE(L)                                                                                          \
+    k_count_tiled_live<L><<<(unsigned)blocks, 256, 0, s>>>(d_bv, pitch, n_words, d_rows, m, d_group_start,         \
+                                                           d_cum_pairs, d_cum_tiles, n_groups, live_tp, cnt + 4,   \
+                                                           d_status, d_counts, layout, kLiveTiledTarget,           \
+                                                           kLiveTiledMin)
+        if (loop) ETM_TILED_LIVE(true);
+        else ETM_TILED_LIVE(false);
+#undef ETM_TILED_LIVE
         ETM_LAUNCHED("k_count_tiled_live");
-        const int64_t sblocks = kXcds * ceil_div(live_units_per_xcd(n_tile_pairs, kLiveSmallTarget), 4);
+        const int64_t sblocks = kXcds * ceil_div(live_grid_per_xcd(n_tile_pairs, kLiveSmallTarget, &loop), 4);
         ETM_REQUIRE(sblocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",
                     (long long)sblocks);
-        if (vec)
-            k_count_sparse_live<true><<<(unsigned)sblocks, 256, 0, s>>>(d_bv, pitch, n_words, d_rows, m,
-                                                                        d_group_start, d_cum_pairs, d_cum_tiles,
-                                                                        n_groups, sparse, cnt + 5, d_counts,
-                                                                        kLiveSmallTarget, kLiveSmallMin);
-        else
-            k_count_sparse_live<false><<<(unsigned)sblocks, 256, 0, s>>>(d_bv, pitch, n_words, d_rows, m,
-                                                                         d_group_start, d_cum_pairs, d_cum_tiles,
-                                                                         n_groups, sparse, cnt + 5, d_counts,
-                                                                         kLiveSmallTarget, kLiveSmallMin);
+#define ETM_SPARSE_LIVE(V, L)                                                                                       \
+    k_count_sparse_live<V, L><<<(unsigned)sblocks, 256, 0, s>>>(d_bv, pitch, n_words, d_rows, m, d_group_start,     \
+                                                                d_cum_pairs, d_cum_tiles, n_groups, sparse, cnt + 5, \
+                                                                d_counts, kLiveSmallTarget, kLiveSmallMin)
+        if (vec && loop) ETM_SPARSE_LIVE(true, true);
+        else if (vec) ETM_SPARSE_LIVE(true, false);
+        else if (loop) ETM_SPARSE_LIVE(false, true);
+        else ETM_SPARSE_LIVE(false, false);
+#undef ETM_SPARSE_LIVE
         ETM_LAUNCHED("k_count_sparse_live");
+    }
+    if (glist) {
+        const int64_t blocks = kXcds * live_units_per_xcd(n_tiled, kLiveTiledTarget);
+        ETM_REQUIRE(blocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",
+                    (long long)blocks);
+        k_count_glist<<<(unsigned)blocks, 256, 0, s>>>(d_bv, pitch, n_words, d_rows, m, d_group_start, d_cum_pairs,
+                                                       *gl, n_tiled, cnt + 8, d_counts, kLiveTiledTarget,
+                                                       kLiveTiledMin);
+        ETM_LAUNCHED("k_count_glist");
     }
     return ETM_OK;
 }
@@ -1180,7 +1540,8 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
     // With a workspace and a status array (and no occupancy skip) only the units holding a counted pair run.
     if (d_ws != nullptr && d_status != nullptr && d_occ == nullptr && word_chunk <= 0 && layout == ETM_LAYOUT_LEX)
         return live_count_impl(d_bv, pitch, n_words, d_rows, m, d_group_start, d_cum_pairs, d_cum_tiles, n_groups,
-                               n_tile_pairs, d_small_list, cls, d_status, d_counts, layout, d_ws, nullptr, s);
+                               n_tile_pairs, d_small_list, cls, d_status, d_counts, layout, d_ws, nullptr, nullptr, 0,
+                               s);
     int64_t base = 0;
     for (int c = 0; c < 4; ++c) {
         if (cls[c] == 0) continue;
@@ -1245,19 +1606,51 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
     return ETM_OK;
 }
 
+namespace {
+// The group-listed workspace of a level: flags, group entries, the pair lists and the used columns.
+struct GlistSizes {
+    int64_t flag, groups, pairs, cols;
+    GlistSizes(int64_t n_groups, int64_t n_tiled, int64_t n_cands, int64_t n_rows) {
+        const int64_t pair_cap = std::min<int64_t>(n_cands, n_tiled * kGlPairsMax);
+        const int64_t col_cap = std::min<int64_t>(n_rows, n_tiled * kGlColsMax);
+        flag = (n_groups + 255) / 256 * 256;
+        groups = n_tiled * (int64_t)sizeof(GlGroup);
+        pairs = (pair_cap * 4 + 255) / 256 * 256;
+        cols = col_cap * 4;
+    }
+    int64_t total() const { return flag + groups + 2 * pairs + cols; }
+};
+}  // namespace
+
+size_t etm::detail::count_glist_ws(int64_t n_groups, int64_t n_tiled, int64_t n_cands, int64_t n_rows) {
+    return n_tiled > 0 ? (size_t)GlistSizes(n_groups, n_tiled, n_cands, n_rows).total() : 0;
+}
+
 int etm::detail::count_level_live(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const int32_t* d_rows,
                                   int32_t m, const int64_t* d_group_start, const int64_t* d_cum_pairs,
                                   const int64_t* d_cum_tiles, int64_t n_groups, int64_t n_tile_pairs,
                                   const int32_t* d_small_list, const int64_t* h_class_counts, const uint8_t* d_status,
-                                  int32_t* d_counts, void* d_ws, int64_t* d_counters, void* stream) {
+                                  int32_t* d_counts, void* d_ws, int64_t* d_counters, int64_t n_tiled, int64_t n_cands,
+                                  int64_t n_rows, void* d_gl_ws, void* stream) {
     ETM_ARG(d_bv && d_rows && d_group_start && d_cum_pairs && d_counts && d_status && d_ws && d_counters,
             "count_level_live: null pointer");
     ETM_ARG(pitch >= n_words && n_words >= 0 && m >= 1 && m <= 62, "count_level_live: bad sizes");
     ETM_ARG(n_tile_pairs <= 0x7FFFFFFF, "count_level_live: %lld tile-pairs exceed int32", (long long)n_tile_pairs);
+    ETM_ARG(n_tiled == 0 || d_gl_ws != nullptr, "count_level_live: tiled groups need the group-listed workspace");
     if (n_words == 0) return ETM_OK;
+    GlistWs gl{};
+    if (n_tiled > 0) {
+        const GlistSizes z(n_groups, n_tiled, n_cands, n_rows);
+        char* p = (char*)d_gl_ws;
+        gl.flag = (uint8_t*)p;
+        gl.groups = (GlGroup*)(p + z.flag);
+        gl.pair_slots = (uint32_t*)(p + z.flag + z.groups);
+        gl.pair_ij = (uint32_t*)(p + z.flag + z.groups + z.pairs);
+        gl.cols = (int32_t*)(p + z.flag + z.groups + 2 * z.pairs);
+    }
     return live_count_impl(d_bv, pitch, n_words, d_rows, m, d_group_start, d_cum_pairs, d_cum_tiles, n_groups,
                            n_tile_pairs, d_small_list, h_class_counts, d_status, d_counts, ETM_LAYOUT_LEX, d_ws,
-                           d_counters, ETM_STREAM(stream));
+                           d_counters, n_tiled > 0 ? &gl : nullptr, n_tiled, ETM_STREAM(stream));
 }
 
 extern "C" int etm_count_itemsets(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const int32_t* d_items,

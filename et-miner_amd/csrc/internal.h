@@ -47,13 +47,19 @@ int level_classify(const int32_t* d_rows, int64_t n, int32_t m, const int32_t* d
                    int32_t* d_counts, int32_t* d_gid, int64_t* d_tally, const uint64_t* d_pair_bits,
                    const int32_t* d_pair_rank, int64_t pair_words, void* stream);
 hipError_t launch_zero_i64(int64_t* p, int n, hipStream_t s);
-// etm_count_level's live-unit path (LEX layout, a status array) with the caller's live counters d_counters[0..7],
-// zero on entry (so no zeroing launch is needed); d_ws holds etm_count_level_ws(...) bytes.
+// etm_count_level's live-unit path (LEX layout, a status array) with the caller's live counters d_counters[0..15],
+// zero on entry (so no zeroing launch is needed); d_ws holds etm_count_level_ws(...) bytes. With n_tiled > 0 (the
+// tiled class's groups, listed after the four small classes in d_small_list), the tiled groups whose counted
+// pairs are few go to the group-listed kernel; d_gl_ws holds count_glist_ws(...) bytes.
 int count_level_live(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const int32_t* d_rows, int32_t m,
                      const int64_t* d_group_start, const int64_t* d_cum_pairs, const int64_t* d_cum_tiles,
                      int64_t n_groups, int64_t n_tile_pairs, const int32_t* d_small_list,
                      const int64_t* h_class_counts, const uint8_t* d_status, int32_t* d_counts, void* d_ws,
-                     int64_t* d_counters, void* stream);
+                     int64_t* d_counters, int64_t n_tiled, int64_t n_cands, int64_t n_rows, void* d_gl_ws,
+                     void* stream);
+// Workspace bytes of count_level_live's group-listed mode for a level of n_rows rows in n_groups groups, n_tiled
+// of them in the tiled class, with n_cands candidates (0 without tiled groups).
+size_t count_glist_ws(int64_t n_groups, int64_t n_tiled, int64_t n_cands, int64_t n_rows);
 // What the counting kernels read for the candidates left ETM_CAND_COUNT (status of n_cands candidates of a level
 // of n_rows rows in n_groups prefix groups; gid optional): adds the suffix rows in a counted pair to out[0], the
 // groups with a counted pair to out[1] and the counted candidates to out[2]. row_bits and group_bits are scratch

@@ -106,13 +106,30 @@ __device__ __forceinline__ int size_class(int64_t s, int32_t small_max) {
     return s <= 2 ? 0 : s <= 4 ? 1 : s <= 8 ? 2 : 3;
 }
 
-// Pass 1: class sizes and the totals; pass 2 (place): each small group at its class's offset.
+// Pass 1 (classes): class sizes and the totals; pass 2 (place): each group at its class's offset, ascending
+// within a class. Both passes cut the groups into the same gridDim.x contiguous ranges, one per workgroup. Pass 1
+// writes each range's tallies to a partial row (no contended atomics: thousands of device-scope atomics on a
+// few counters serialise) and the last workgroup to finish sums the rows into the totals and
+// turns the class columns into exclusive offsets per range; pass 2 then places a range's groups from its
+// offsets with a workgroup-wide ballot scan per 256 groups.
+constexpr int kGroupBlocks = 256;  // most workgroups of the two passes
+constexpr int kPartCols = 3 + kClasses;  // n2, rows2, smax, then the class sizes
+constexpr int kGroupDone = 23;           // totals[] slot of pass 1's completion counter
+
+__device__ __forceinline__ void group_range(int64_t n_groups, int64_t* lo, int64_t* hi) {
+    const int64_t per = (n_groups + gridDim.x - 1) / gridDim.x;
+    *lo = min(n_groups, (int64_t)blockIdx.x * per);
+    *hi = min(n_groups, *lo + per);
+}
+
 __global__ __launch_bounds__(256) void k_group_classes(const int64_t* __restrict__ group_start, int64_t n,
-                                                        int32_t small_max, int64_t* __restrict__ totals) {
-    const int64_t n_groups = totals[0];
+                                                        int32_t small_max, int64_t* __restrict__ totals,
+                                                        int64_t* __restrict__ part) {
+    const int64_t n_groups = min(totals[0], n);
+    int64_t lo, hi;
+    group_range(n_groups, &lo, &hi);
     int64_t n2 = 0, rows2 = 0, smax = 0, cls[kClasses] = {0, 0, 0, 0, 0};
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n_groups && g < n;
-         g += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t g = lo + threadIdx.x; g < hi; g += blockDim.x) {
         const int64_t s = group_start[g + 1] - group_start[g];
         const int c = size_class(s, small_max);
         if (c >= 0) cls[c] += 1;
@@ -123,49 +140,98 @@ __global__ __launch_bounds__(256) void k_group_classes(const int64_t* __restrict
         smax = s > smax ? s : smax;
     }
     __shared__ int64_t lds[5];
-    n2 = etm::block_sum<256>(n2, lds);
-    rows2 = etm::block_sum<256>(rows2, lds);
-    for (int c = 0; c < kClasses; ++c) cls[c] = etm::block_sum<256>(cls[c], lds);
+    __shared__ int last;
+    int64_t v[kPartCols] = {n2, rows2, 0, cls[0], cls[1], cls[2], cls[3], cls[4]};
+    for (int c = 0; c < kPartCols; ++c)
+        if (c != 2) v[c] = etm::block_sum<256>(v[c], lds);
     smax = etm::wave_max(smax);
-    if ((threadIdx.x & 63) == 0 && smax) atomicMax((unsigned long long*)&totals[4], (unsigned long long)smax);
+    __shared__ int64_t wmax[4];
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = smax;
+    __syncthreads();
     if (threadIdx.x == 0) {
-        if (n2) atomicAdd((unsigned long long*)&totals[5], (unsigned long long)n2);
-        if (rows2) atomicAdd((unsigned long long*)&totals[6], (unsigned long long)rows2);
-        for (int c = 0; c < kClasses; ++c)
-            if (cls[c]) {
-                atomicAdd((unsigned long long*)&totals[8 + c], (unsigned long long)cls[c]);
-                if (c < 4) atomicAdd((unsigned long long*)&totals[3], (unsigned long long)cls[c]);
-            }
+        v[2] = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+        for (int c = 0; c < kPartCols; ++c) part[(int64_t)blockIdx.x * kPartCols + c] = v[c];
+        __threadfence();
+        const unsigned long long done = __hip_atomic_fetch_add((unsigned long long*)&totals[kGroupDone], 1ull,
+                                                               __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = done == gridDim.x - 1;
     }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    // The last workgroup, a thread per range: column sums into the totals, then the class columns as exclusive
+    // offsets per range (class c's list starts after the classes before it), by a scan over the ranges in LDS.
+    __shared__ int64_t scan[kClasses][kGroupBlocks];
+    __shared__ int64_t sums[kClasses];
+    const unsigned b = threadIdx.x;
+    const bool have = b < gridDim.x;
+    int64_t row[kPartCols];
+    for (int c = 0; c < kPartCols; ++c) row[c] = have ? part[(int64_t)b * kPartCols + c] : 0;
+    const int64_t t_n2 = etm::block_sum<256>(row[0], lds);
+    const int64_t t_rows2 = etm::block_sum<256>(row[1], lds);
+    const int64_t w_max = etm::wave_max(row[2]);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = w_max;
+    for (int c = 0; c < kClasses; ++c) scan[c][b] = row[3 + c];
+    __syncthreads();
+    for (int off = 1; off < kGroupBlocks; off <<= 1) {  // inclusive scans of the five class columns
+        int64_t add[kClasses];
+        for (int c = 0; c < kClasses; ++c) add[c] = b >= (unsigned)off ? scan[c][b - off] : 0;
+        __syncthreads();
+        for (int c = 0; c < kClasses; ++c) scan[c][b] += add[c];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int64_t at = 0;
+        for (int c = 0; c < kClasses; ++c) {
+            sums[c] = at;  // the class's first slot
+            const int64_t total = scan[c][kGroupBlocks - 1];
+            totals[8 + c] = total;
+            at += total;
+            if (c == kClasses - 2) totals[3] = at;  // the groups of the four wave-per-group classes
+        }
+        totals[4] = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+        totals[5] = t_n2;
+        totals[6] = t_rows2;
+    }
+    __syncthreads();
+    if (have)
+        for (int c = 0; c < kClasses; ++c) part[(int64_t)b * kPartCols + 3 + c] = sums[c] + scan[c][b] - row[3 + c];
+    if (threadIdx.x == 0) totals[kGroupDone] = 0;
 }
 
 __global__ __launch_bounds__(256) void k_group_place(const int64_t* __restrict__ group_start, int64_t n,
                                                       int32_t small_max, int32_t* __restrict__ small_list,
-                                                      int64_t* __restrict__ totals) {
-    const int64_t n_groups = totals[0];
-    const int64_t lim = n_groups < n ? n_groups : n;
-    const int lane = threadIdx.x & 63;
+                                                      const int64_t* __restrict__ totals,
+                                                      const int64_t* __restrict__ part) {
+    const int64_t n_groups = min(totals[0], n);
+    int64_t lo, hi;
+    group_range(n_groups, &lo, &hi);
+    __shared__ int64_t at[kClasses];
+    __shared__ int wcount[4][kClasses];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-    // Whole waves walk the groups (the loop bound is wave-uniform), so each class takes one cursor atomic per
-    // wave: the lanes of a class get consecutive slots in lane order.
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t g0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); g0 < lim; g0 += stride) {
-        const int64_t g = g0 + lane;
-        const int c = g < lim ? size_class(group_start[g + 1] - group_start[g], small_max) : -1;
+    if (threadIdx.x < kClasses) at[threadIdx.x] = part[(int64_t)blockIdx.x * kPartCols + 3 + threadIdx.x];
+    for (int64_t g0 = lo; g0 < hi; g0 += blockDim.x) {
+        const int64_t g = g0 + threadIdx.x;
+        const int c = g < hi ? size_class(group_start[g + 1] - group_start[g], small_max) : -1;
+        uint64_t mine = 0;
 #pragma unroll
         for (int q = 0; q < kClasses; ++q) {
             const uint64_t mask = __ballot(c == q);
-            if (mask == 0) continue;
-            const int leader = __ffsll((long long)mask) - 1;
-            unsigned long long at = 0;
-            if (lane == leader) at = atomicAdd((unsigned long long*)&totals[16 + q], (unsigned long long)__popcll(mask));
-            at = __shfl(at, leader, 64);
-            if (c == q) {
-                int64_t base = 0;
-                for (int t = 0; t < q; ++t) base += totals[8 + t];
-                small_list[base + (int64_t)at + __popcll(mask & below)] = (int32_t)g;
-            }
+            if (c == q) mine = mask;
+            if (lane == 0) wcount[wave][q] = __popcll(mask);
         }
+        __syncthreads();
+        if (c >= 0) {
+            int64_t pos = at[c] + __popcll(mine & below);
+            for (int w = 0; w < wave; ++w) pos += wcount[w][c];
+            small_list[pos] = (int32_t)g;
+        }
+        __syncthreads();
+        if (threadIdx.x < kClasses)
+            at[threadIdx.x] += wcount[0][threadIdx.x] + wcount[1][threadIdx.x] + wcount[2][threadIdx.x] +
+                               wcount[3][threadIdx.x];
+        __syncthreads();
     }
 }
 
@@ -615,10 +681,20 @@ __global__ __launch_bounds__(256) void k_deduce(Classifier C, DeepLevel d2, Deep
     int32_t* c1 = x + k;
     int32_t* s2 = c1 + k;
     int64_t n_inf = 0, n_skip = 0;
-    const int64_t step = (int64_t)gridDim.x * per_block;
-    for (int64_t base = (int64_t)blockIdx.x * per_block; base < n_cands; base += step) {
-        const int64_t c = base + sub;
-        const bool act = c < n_cands && status[c] == ETM_CAND_COUNT;
+    // Each wave scans 64 candidates at a time and runs the bounds only for those still ETM_CAND_COUNT, 64 / L
+    // of them per round (sub-wave w takes the w-th left): on deep levels of inferred candidates most waves
+    // find none and move on after one coalesced status load.
+    const int lane = threadIdx.x & 63, per_wave = 64 / L, sub_w = lane / L;
+    const int64_t wave_step = (int64_t)gridDim.x * (blockDim.x >> 6) * 64;
+    for (int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64; base < n_cands;
+         base += wave_step) {
+      uint64_t left = __ballot(base + lane < n_cands && status[base + lane] == ETM_CAND_COUNT);
+      while (left) {
+        uint64_t mine = left;
+        for (int t = 0; t < sub_w && mine; ++t) mine &= mine - 1;
+        for (int t = 0; t < per_wave && left; ++t) left &= left - 1;
+        const int64_t c = mine ? base + __ffsll((long long)mine) - 1 : -1;
+        const bool act = c >= 0;
         int64_t gs = 0, i = 0, j = 0;
         if (act) {
             const int64_t g = gid ? (int64_t)gid[c] : etm::upper_index(cum_pairs, n_groups, c);
@@ -695,6 +771,7 @@ __global__ __launch_bounds__(256) void k_deduce(Classifier C, DeepLevel d2, Deep
             }
         }
         wave_lds_sync();  // the next candidate's items overwrite this one's
+      }
     }
     if (tally) {
         n_inf = etm::block_sum<256>(n_inf, lds);
@@ -1283,7 +1360,14 @@ hipError_t launch_live_bytes(const uint8_t* status, const int32_t* gid, const in
 }  // namespace etm
 
 
-extern "C" size_t etm_level_groups_ws(int64_t n) { return etm::scan_ws_bytes(n + 1); }
+namespace {
+// The class passes' partial rows follow the scans' workspace.
+size_t group_part_offset(int64_t n) { return (etm::scan_ws_bytes(n + 1) + 255) / 256 * 256; }
+}  // namespace
+
+extern "C" size_t etm_level_groups_ws(int64_t n) {
+    return group_part_offset(n) + (size_t)kGroupBlocks * kPartCols * sizeof(int64_t);
+}
 
 extern "C" int etm_level_groups(const int32_t* d_rows, int64_t n, int32_t m, int32_t small_max,
                                 int64_t* d_group_start, int64_t* d_cum_pairs, int64_t* d_cum_tiles,
@@ -1311,9 +1395,12 @@ extern "C" int etm_level_groups(const int32_t* d_rows, int64_t n, int32_t m, int
     GroupTiles tiles{d_group_start, d_totals, small_max};
     ETM_HIP(etm::scan_prepare(tiles, n + 1, ws, &d_totals[2], s));
     ETM_HIP(etm::scan_finish(tiles, WritePrefix{d_cum_tiles}, n + 1, ws, s));
-    k_group_classes<<<grid_for(n, 1024), 256, 0, s>>>(d_group_start, n, small_max, d_totals);
+    // Both passes take the same ranges: one launch size for both, from the row count (groups <= rows).
+    const int blocks = grid_for(n, kGroupBlocks);
+    int64_t* part = (int64_t*)((char*)d_ws + group_part_offset(n));
+    k_group_classes<<<blocks, 256, 0, s>>>(d_group_start, n, small_max, d_totals, part);
     ETM_LAUNCHED("k_group_classes");
-    k_group_place<<<grid_for(n), 256, 0, s>>>(d_group_start, n, small_max, d_small_list, d_totals);
+    k_group_place<<<blocks, 256, 0, s>>>(d_group_start, n, small_max, d_small_list, d_totals, part);
     ETM_LAUNCHED("k_group_place");
     return ETM_OK;
 }
@@ -1448,7 +1535,7 @@ extern "C" int etm_level_deduce(const int32_t* d_rows, int32_t m, const int32_t*
     Classifier C{d_rows, m, d_prev_counts, nullptr, d_table, table_size - 1, ETM_SUBSET_INFER};
     DeepLevel d2{d_rows2, d_counts2, d_table2, table_size2 - 1};
     DeepLevel d3{d_rows3, d_counts3, d_table3, k == 3 ? 0 : table_size3 - 1};
-    const int64_t blocks = std::min<int64_t>((n_cands + per_block - 1) / per_block, 8192);
+    const int64_t blocks = std::min<int64_t>((n_cands + 255) / 256, 8192);
     k_deduce<<<(unsigned)blocks, 256, dyn, s>>>(C, d2, d3, n_total, min_count, d_group_start, d_cum_pairs, n_groups,
                                                 d_gid, n_cands, L, stride8, d_status, d_counts, d_tally);
     ETM_LAUNCHED("k_deduce");

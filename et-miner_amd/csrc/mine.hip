@@ -304,8 +304,8 @@ struct Level {
 };
 
 // Device scalars of one level: [0] survivors, [1..3] tally, [4] selected (free-set prune), [5..7] live bytes,
-// [8..31] totals of the next level's groups, [32..39] live-unit counters of the counting launches.
-constexpr int kScal = 40;
+// [8..31] totals of the next level's groups, [32..47] live-unit counters of the counting launches.
+constexpr int kScal = 48;
 
 struct GroupsDev {
     int64_t *gs, *cp, *ct;
@@ -592,12 +592,17 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
             const bool skip_tiles = ordered && k >= 3 && cls[0] + cls[1] + cls[2] + cls[3] > 0;
             if (skip_tiles) ensure_occ();
             if (status != nullptr && !skip_tiles && live_schedule) {
-                // From K=3 on, only the groups and tile-pairs holding a counted candidate are launched; their
-                // counters are the level's scalars [32..39], zeroed with the others.
+                // From K=3 on, only the groups and tile-pairs holding a counted candidate are launched, and the
+                // tiled groups with few counted pairs are counted group-listed; their counters are the level's
+                // scalars [32..47], zeroed with the others.
                 const size_t ws_n = etm_count_level_ws(cls[0] + cls[1] + cls[2] + cls[3], n_tiles);
                 void* ws = a.get<int64_t>((int64_t)(ws_n + 7) / 8);
+                const int64_t n_tiled = tot[12];
+                const size_t gl_n = etm::detail::count_glist_ws(n_groups, n_tiled, n_cands, prev.n);
+                void* gl_ws = gl_n ? a.get<int64_t>((int64_t)(gl_n + 7) / 8) : nullptr;
                 check_rc(etm::detail::count_level_live(d_bv, pitch, n_words, prev.rows, m, g.gs, g.cp, g.ct, n_groups,
-                                                       n_tiles, g.list, cls, status, counts, ws, cur_scal + 32, s),
+                                                       n_tiles, g.list, cls, status, counts, ws, cur_scal + 32,
+                                                       n_tiled, n_cands, prev.n, gl_ws, s),
                          "etm_count_level");
             } else {
                 check_rc(etm_count_level(d_bv, pitch, n_words, prev.rows, m, g.gs, g.cp, g.ct, n_groups, n_tiles,
@@ -986,8 +991,11 @@ extern "C" int etm_levels_flat(const etm_levels* lv, int64_t* h_totals, int64_t*
             h_offsets[g.row0 + (i - g.lo) + 1] = g.at0 + (i - g.lo + 1) * k;
         }
     };
+    // Threads pay off from a few million item ids (starting one costs tens of microseconds; one thread writes
+    // about a billion ids a second).
+    const int64_t kThreadedMinValues = 1 << 22;
     const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    if (ranges.size() <= 1 || hw == 1) {
+    if (ranges.size() <= 1 || hw == 1 || n_vals < kThreadedMinValues) {
         for (const Range& g : ranges) fill(g);
         return ETM_OK;
     }

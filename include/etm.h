@@ -137,11 +137,11 @@ size_t etm_level_groups_ws(int64_t n);
  *                                than small_max suffixes; 0 for the others),
  *   d_small_list[n_groups]       ids of the groups with >= 2 suffixes, by size class: first the groups of
  *                                2 suffixes, then 3-4, 5-8, 9-16 (the wave-per-group classes up to small_max),
- *                                then those with more than small_max (the tiled kernel's); unordered within a
- *                                class from the multi-block path, ascending from the single-workgroup path,
+ *                                then those with more than small_max (the tiled kernel's); ascending within
+ *                                a class,
  *   d_totals[24]                 {n_groups, n_candidates, n_tile_pairs, groups in the first four classes,
  *                                 max_suffixes, groups with >= 2 rows, rows in those groups, rows,
- *                                 class sizes (2, 3-4, 5-8, 9-16, > small_max), 0, 0, 0, 8 internal cursors}.
+ *                                 class sizes (2, 3-4, 5-8, 9-16, > small_max), 0, 0, 0, 8 internal slots}.
  * Capacities: n + 1 for the first three, n for d_small_list.
  * Replaces build_k3plus_groups_from_flat (gpu/kernels/k3plus.py:243-376, rust_ext/src/core/groups.rs:59-205)
  * and the kernel split of group_kernels (gpu/row_split_chunks.py:96-121). */
