@@ -1254,9 +1254,13 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const uint64_t* __restrict__ 
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
     const int64_t w_begin = uniform64(chunk_id) * chunk_words;
     const int64_t w_end = min(n_words, w_begin + chunk_words);
-    uint64_t next[kMW];
+    // Two steps of this column's words in flight: the words of step s + 2 are requested while step s counts.
+    uint64_t next[kMW], after[kMW];
 #pragma unroll
-    for (int w = 0; w < kMW; ++w) next[w] = (base >= 0 && w_begin + w < w_end) ? bv[base + w_begin + w] : 0ull;
+    for (int w = 0; w < kMW; ++w) {
+        next[w] = (base >= 0 && w_begin + w < w_end) ? bv[base + w_begin + w] : 0ull;
+        after[w] = (base >= 0 && w_begin + kMW + w < w_end) ? bv[base + w_begin + kMW + w] : 0ull;
+    }
     uint8_t* mine = ops + tid * kGRow;
     const int sub = 16 * (lane >> 5);
     for (int64_t wb = w_begin; wb < w_end; wb += kMW) {
@@ -1267,9 +1271,12 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const uint64_t* __restrict__ 
             for (int q = 0; q < 4; ++q)
                 *reinterpret_cast<v4i*>(mine + 64 * w + 16 * q) = spread16((uint32_t)(next[w] >> (16 * q)) & 0xFFFFu);
         __syncthreads();
-        const int64_t nb = wb + kMW;
+        const int64_t nb = wb + 2 * kMW;
 #pragma unroll
-        for (int w = 0; w < kMW; ++w) next[w] = (base >= 0 && nb + w < w_end) ? bv[base + nb + w] : 0ull;
+        for (int w = 0; w < kMW; ++w) {
+            next[w] = after[w];
+            after[w] = (base >= 0 && nb + w < w_end) ? bv[base + nb + w] : 0ull;
+        }
 #pragma unroll
         for (int ks = 0; ks < 2 * kMW; ++ks) {
             v4i a[2], b[2];
@@ -1298,6 +1305,29 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const uint64_t* __restrict__ 
             }
         }
     }
+}
+
+// Word chunk of the Gram: the workgroups (tile-pairs x chunks) fill whole rounds of the resident slots (two
+// 69.6 KiB-LDS workgroups per CU) as closely as possible over 2..8 rounds, the chunks per tile-pair rounded
+// down to a multiple of the 8 XCDs where at least 8 (xcd_unit then keeps each XCD on its own chunks). A last
+// round a third full costs the launch a third of a round.
+int64_t gram_chunk(int64_t n_words, int64_t n_tp) {
+    constexpr int64_t kSlots = 2 * 256;
+    const int64_t max_chunks = std::max<int64_t>(1, n_words / 512);
+    int64_t best = 1;
+    double best_fill = -1.0;
+    for (int64_t rounds = 2; rounds <= 8; ++rounds) {
+        int64_t ch = std::max<int64_t>(1, rounds * kSlots / std::max<int64_t>(n_tp, 1));
+        if (ch >= kXcds) ch = ch / kXcds * kXcds;
+        ch = std::min(ch, max_chunks);
+        const int64_t units = ch * n_tp;
+        const double fill = (double)units / (double)(ceil_div(units, kSlots) * kSlots);
+        if (fill > best_fill + 0.02) {
+            best_fill = fill;
+            best = ch;
+        }
+    }
+    return ceil_div(ceil_div(n_words, best), kMW) * kMW;
 }
 
 // One wave per (itemset, word chunk): AND of the itemset's columns, popcount, wave sum, one 64-bit add.
@@ -1577,8 +1607,7 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
             // wave per SIMD), which waste more edge pairs and hide load latency worse.
             const int64_t nt = ceil_div(s_cols, kMT);
             const int64_t n_tp = nt * (nt + 1) / 2;
-            const int64_t chunk = word_chunk > 0 ? ceil_div(word_chunk, kMW) * kMW
-                                                 : pick_chunk(n_words, n_tp, 1024, kMW, 512);
+            const int64_t chunk = word_chunk > 0 ? ceil_div(word_chunk, kMW) * kMW : gram_chunk(n_words, n_tp);
             const int64_t n_chunks = ceil_div(n_words, chunk);
             const int64_t blocks = kXcds * xcd_units(n_tp, n_chunks);
             ETM_REQUIRE(blocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",

@@ -267,20 +267,27 @@ struct Query {
         if (t < kept) return pre[(skip < np && t >= skip) ? t + 1 : t];
         return t == kept ? a : b;
     }
+    // Whether a level row of size() items equals the query: the kept prefix items against `pre`, then a and b
+    // as values (an item() loop would select between a global and a private address per item).
+    __device__ __forceinline__ bool match(const int32_t* row, int32_t) const {
+        const int kept = np - (skip < np ? 1 : 0);
+        for (int t = 0; t < kept; ++t)
+            if (row[t] != pre[t + (t >= skip ? 1 : 0)]) return false;
+        if (row[kept] != a) return false;
+        return !use_b || row[kept + 1] == b;
+    }
 };
 
-// Row id of the level row equal to q (q.item(0..m)), or -1.
+// Row id of the level row equal to q (q.item(0..m)), or -1. Inlined with q by value, so the query stays in
+// registers (a query passed by reference to an outlined call lives in scratch memory).
 template <class Q>
-__device__ int64_t hash_find(const int32_t* __restrict__ rows, int32_t m, const int32_t* __restrict__ table,
-                             int64_t mask, uint64_t h, const Q& q) {
+__device__ __forceinline__ int64_t hash_find(const int32_t* __restrict__ rows, int32_t m,
+                                             const int32_t* __restrict__ table, int64_t mask, uint64_t h, const Q q) {
     int64_t slot = (int64_t)(h & (uint64_t)mask);
     for (;;) {
         const int32_t r = table[slot];
         if (r < 0) return -1;
-        const int32_t* row = rows + (int64_t)r * m;
-        bool eq = true;
-        for (int t = 0; t < m && eq; ++t) eq = row[t] == q.item(t);
-        if (eq) return r;
+        if (q.match(rows + (int64_t)r * m, m)) return r;
         slot = (slot + 1) & mask;
     }
 }
@@ -433,6 +440,13 @@ struct Classifier {
     }
 };
 
+// LDS written by one wave's lanes, made visible to the same wave's other lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Small groups packed into waves, a lane per (candidate, dropped prefix item): a candidate's k-2 subset probes
 // run in parallel lanes (their latency does not add up on deep levels) and several groups share a wave when
 // they have few (candidate, probe) pairs. Size class c (groups of at most 2, 4, 8, 16 rows: at most
@@ -462,8 +476,17 @@ __global__ __launch_bounds__(256) void k_classify_packed(Classifier C, const int
                                                           const int32_t* __restrict__ list, PackedClasses pc,
                                                           uint8_t* __restrict__ status, int32_t* __restrict__ counts,
                                                           int32_t* __restrict__ gid, int64_t* __restrict__ tally) {
+    // Per wave: its G groups' bounds and prefix items (with their hashes), and each candidate slot's two suffix
+    // items, staged in LDS in three rounds of independent loads; a probe lane then reads only the hash table and
+    // the row it points to (a lane re-deriving all of this from the group list costs some eight dependent global
+    // loads per probe).
     __shared__ int32_t s_min[4][kPackedSlots];
     __shared__ int32_t s_flags[4][kPackedSlots];  // bit 0: a subset is missing, bit 1: a subset is not free
+    __shared__ int32_t s_a[4][kPackedSlots], s_b[4][kPackedSlots];
+    __shared__ int64_t s_gs[4][64], s_cb[4][64];
+    __shared__ int32_t s_sz[4][64], s_g[4][64];
+    __shared__ int32_t s_pre[4][64];
+    __shared__ uint64_t s_hpre[4][64];
     __shared__ int64_t lds[5];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const bool infer = (C.mode & ETM_SUBSET_INFER) != 0;
@@ -480,41 +503,69 @@ __global__ __launch_bounds__(256) void k_classify_packed(Classifier C, const int
     int64_t n_count = 0, n_infer = 0, n_skip = 0;
     if (c < 4) {
         const int P = PackedClasses::per_group(c), G = pc.groups(c);
-        const int np = C.m - 1;
+        const int m = C.m, np = m - 1;
         const int slots = G * P;
-        // Seed each candidate slot from its two parents.
-        for (int sl = lane; sl < slots; sl += 64) {
-            const int64_t li = wave * G + sl / P;
-            const int p = sl % P;
-            int32_t mn = 0, fl = 4;  // bit 2: no such candidate
+        // Round 1: the groups.
+        if (lane < G) {
+            const int64_t li = wave * G + lane;
+            int32_t g = -1;
+            int64_t gs = 0, sz = 0, cb = 0;
             if (li < pc.n[c]) {
-                const int64_t g = list[base + li];
-                const int64_t gs = group_start[g];
-                const int64_t sz = group_start[g + 1] - gs;
-                if (p < sz * (sz - 1) / 2) {
-                    int64_t i, j;
-                    etm::lex_pair_decode(p, sz, &i, &j);
-                    const int64_t ri = gs + i, rj = gs + j;
-                    mn = C.prev_counts ? min(C.prev_counts[ri], C.prev_counts[rj]) : 0;
-                    fl = (infer && (C.prev_free[ri] == 0 || C.prev_free[rj] == 0)) ? 2 : 0;
-                }
+                g = list[base + li];
+                gs = group_start[g];
+                sz = group_start[g + 1] - gs;
+                cb = cum_pairs[g];
             }
+            s_g[wv][lane] = g;
+            s_gs[wv][lane] = gs;
+            s_sz[wv][lane] = (int32_t)sz;
+            s_cb[wv][lane] = cb;
+        }
+        wave_lds_sync();
+        // Round 2: each group's prefix items (G x np <= 64) and each slot's suffix items and parent counts.
+        for (int t = lane; t < G * np; t += 64) {
+            const int gi = t / np, q = t % np;
+            const int32_t v = s_sz[wv][gi] >= 2 ? C.rows[s_gs[wv][gi] * m + q] : 0;
+            s_pre[wv][t] = v;
+            s_hpre[wv][t] = item_hash(v);
+        }
+        for (int sl = lane; sl < slots; sl += 64) {
+            const int gi = sl / P, p = sl % P;
+            const int64_t sz = s_sz[wv][gi];
+            int32_t mn = 0, fl = 4, a = 0, b = 0;  // fl bit 2: no such candidate
+            if (p < sz * (sz - 1) / 2) {
+                int64_t i, j;
+                etm::lex_pair_decode(p, sz, &i, &j);
+                const int64_t ri = s_gs[wv][gi] + i, rj = s_gs[wv][gi] + j;
+                a = C.rows[ri * m + np];
+                b = C.rows[rj * m + np];
+                mn = C.prev_counts ? min(C.prev_counts[ri], C.prev_counts[rj]) : 0;
+                fl = (infer && (C.prev_free[ri] == 0 || C.prev_free[rj] == 0)) ? 2 : 0;
+            }
+            s_a[wv][sl] = a;
+            s_b[wv][sl] = b;
             s_min[wv][sl] = mn;
             s_flags[wv][sl] = fl;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_lds_sync();
+        // Round 3: the probes, a lane per (slot, dropped prefix item).
         if (prune || infer) {
             for (int t = lane; t < slots * np; t += 64) {
                 const int sl = t / np, d = t % np;
                 if (s_flags[wv][sl] & 4) continue;
-                const int64_t g = list[base + wave * G + sl / P];
-                const int64_t gs = group_start[g];
-                const int64_t sz = group_start[g + 1] - gs;
-                int64_t i, j;
-                etm::lex_pair_decode(sl % P, sz, &i, &j);
-                const int64_t r = C.probe(gs, i, j, d);
+                const int gi = sl / P;
+                const int32_t a = s_a[wv][sl], b = s_b[wv][sl];
+                int64_t r;
+                if (C.pair_bits) {
+                    r = C.pair_row(a, b);
+                } else {
+                    const int32_t* pre = &s_pre[wv][gi * np];
+                    const uint64_t* hp = &s_hpre[wv][gi * np];
+                    uint64_t h = item_hash(a) + item_hash(b);
+                    for (int q = 0; q < np; ++q)
+                        if (q != d) h += hp[q];
+                    r = hash_find(C.rows, m, C.table, C.mask, h, Query{pre, np, d, a, b, true});
+                }
                 if (r < 0) {
                     atomicOr(&s_flags[wv][sl], 1);
                 } else if (infer) {
@@ -523,20 +574,18 @@ __global__ __launch_bounds__(256) void k_classify_packed(Classifier C, const int
                 }
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_lds_sync();
         for (int sl = lane; sl < slots; sl += 64) {
             const int f = s_flags[wv][sl];
             if (f & 4) continue;
-            const int64_t g = list[base + wave * G + sl / P];
+            const int gi = sl / P;
             uint8_t st = ETM_CAND_COUNT;
             if ((f & 1) && prune) st = ETM_CAND_SKIP;
             else if (!(f & 1) && infer && (f & 2)) st = ETM_CAND_INFER;
-            const int64_t cand = cum_pairs[g] + sl % P;
+            const int64_t cand = s_cb[wv][gi] + sl % P;
             status[cand] = st;
             counts[cand] = st == ETM_CAND_COUNT ? 0 : (st == ETM_CAND_INFER ? s_min[wv][sl] : -1);
-            if (gid) gid[cand] = (int32_t)g;
+            if (gid) gid[cand] = s_g[wv][gi];
             n_count += st == ETM_CAND_COUNT;
             n_infer += st == ETM_CAND_INFER;
             n_skip += st == ETM_CAND_SKIP;
@@ -618,6 +667,11 @@ struct DropQuery {
         t += t >= d2;
         return x[t];
     }
+    __device__ __forceinline__ bool match(const int32_t* row, int32_t m) const {
+        for (int t = 0; t < m; ++t)
+            if (row[t] != item(t)) return false;
+        return true;
+    }
 };
 
 // A level the bounds read: rows (n, m), counts, and its hash.
@@ -652,12 +706,6 @@ __device__ __forceinline__ int64_t sub_max(int64_t v, int L) {
 __device__ __forceinline__ int64_t sub_min(int64_t v, int L) {
     for (int off = L >> 1; off > 0; off >>= 1) v = min(v, (int64_t)__shfl_xor((long long)v, off, 64));
     return v;
-}
-
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // A sub-wave of L lanes (a power of two <= 64) per candidate still ETM_CAND_COUNT: lanes over the k (k-1)-
@@ -1000,7 +1048,7 @@ __device__ __forceinline__ void groups_single_body(const int32_t* __restrict__ r
         smax = z > smax ? z : smax;
     };
     int32_t c4[5] = {0, 0, 0, 0, 0};
-    for (int64_t base = 0; base < n_groups; base += (int64_t)kOne * 4) stripe_pass<5, 4>(base, n_groups, vals, put, c4, lds);
+    for (int64_t base = 0; base < n_groups; base += (int64_t)kOne * 2) stripe_pass<5, 2>(base, n_groups, vals, put, c4, lds);
     const int64_t cp = c4[0], ct = c4[1];
     int64_t cls[kClasses];
     cls[0] = c4[2] & 0xffff;
@@ -1040,7 +1088,7 @@ __device__ __forceinline__ void groups_single_body(const int32_t* __restrict__ r
         if (at < n_groups) small_list[at] = (int32_t)g;
     };
     int32_t c3[3] = {0, 0, 0};
-    for (int64_t base = 0; base < n_groups; base += (int64_t)kOne * 8) stripe_pass<3, 8>(base, n_groups, cvals, cput, c3, lds);
+    for (int64_t base = 0; base < n_groups; base += (int64_t)kOne * 2) stripe_pass<3, 2>(base, n_groups, cvals, cput, c3, lds);
     if (tid == 0) {
         totals[0] = n_groups;
         totals[1] = cp;
