@@ -11,9 +11,9 @@ count back to the host.
 value = itemset candidates evaluated per second (counted + inferred, all levels; K=1 columns included),
 whole-job, one GPU.
 
-roofline: the counting launches of levels K >= 3 (the live-unit lists and the wave-per-group and tiled counting
-kernels, the step's dominant kernel family), timed with HIP events recorded by the native loop on the stream the
-kernels run on, over the timed steps and every level K >= 3. Their algorithmic bytes are what the kernels must read for the candidates left counted by the
+roofline: the counting launches of levels K >= 3 (the live-unit and group-listed scans, and the wave-per-group,
+tiled, sparse tile-pair and group-listed counting kernels, the step's dominant kernel family), timed with HIP
+events recorded by the native loop on the stream the kernels run on, over the timed steps and every level K >= 3. Their algorithmic bytes are what the kernels must read for the candidates left counted by the
 subset test, Pascal inference and the deduction bounds: per level, (prefix items x groups with a counted pair +
 suffix rows in a counted pair) x words x 8 B + 4 B per counted candidate (computed on the device). The full
 prefix-group decomposition's bytes (every group, inferred candidates included) are reported beside them, with
@@ -290,8 +290,9 @@ def main(argv=None) -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "K>=3 counting launches (k_live_units, k_count_small_live<2|4|8>, k_count_tiled_live; "
-                      "k_count_small<S> and k_count_tiled where every unit is launched)",
+            "kernel": "K>=3 counting launches (k_gl_scan, k_live_units, k_count_small_live<2|4|8>, "
+                      "k_count_tiled_live, k_count_sparse_live, k_count_glist; k_count_small<S> and k_count_tiled "
+                      "where every unit is launched)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
