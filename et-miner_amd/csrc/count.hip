@@ -1408,7 +1408,150 @@ int64_t live_grid_per_xcd(int64_t ub, int64_t target, bool* loop) {
     return *loop ? cap : units;
 }
 
+// ── batches of jobs in lockstep rounds (batch.h) ──────────────────────────
+
+using etm::batch::Job;
+using etm::batch::Queue;
+
+// Units a job's round aims for per queue (small groups, tile-pairs), and the persistent grids that drain the
+// queues: every wave (workgroup) takes units until the queue's index passes its total, so these fixed shapes
+// serve any mix of jobs.
+constexpr int64_t kBatchSmallTarget = 2048, kBatchTiledTarget = 2048;
+constexpr int kBatchSmallBlocks = 1024, kBatchTiledBlocks = 512;
+
+// One workgroup, a thread per job: zeroes the job's round tallies, sizes its hash (a power of two >= 2 x the
+// previous level's rows) and its word chunks, and lays out the round's units per queue (a small class's groups
+// or the tile-pairs, x chunks, chunk-major) as exclusive prefix sums over the jobs.
+__global__ __launch_bounds__(256) void k_batch_plan(Job* __restrict__ jobs, int n_jobs, Queue* __restrict__ q) {
+    __shared__ int64_t lds[256 / etm::kWave + 1];
+    const int j = threadIdx.x;
+    int64_t units[etm::batch::kQueues] = {0, 0, 0, 0};
+    int64_t act = 0;
+    if (j < n_jobs) {
+        Job& J = jobs[j];
+        for (int t = 0; t < 4; ++t) J.tally[t] = 0;
+        J.new_n = 0;
+        if (J.state == etm::batch::kActive) {
+            act = 1;
+            const int64_t* T = J.tot[J.buf];
+            int64_t size = 64;
+            while (size < 2 * J.prev_n) size <<= 1;
+            J.table_mask = size - 1;
+            const int64_t n_small = T[8] + T[9] + T[10];
+            J.s_chunk = pick_chunk(J.n_words, n_small, kBatchSmallTarget, 64, kLiveSmallMin);
+            J.s_nch = ceil_div(J.n_words, J.s_chunk);
+            J.t_chunk = pick_chunk(J.n_words, T[2], kBatchTiledTarget, kKW, kLiveTiledMin);
+            J.t_nch = ceil_div(J.n_words, J.t_chunk);
+            for (int c = 0; c < etm::batch::kSmallQueues; ++c) units[c] = T[8 + c] * J.s_nch;
+            units[etm::batch::kTiledQueue] = T[2] * J.t_nch;
+        }
+    }
+    for (int c = 0; c < etm::batch::kQueues; ++c) {
+        int64_t total = 0;
+        const int64_t p = etm::block_exclusive_scan<256>(units[c], lds, &total);
+        if (j < n_jobs) q->pre[c][j] = p;
+        if (j == 0) {
+            q->pre[c][n_jobs] = total;
+            q->head[c] = 0;
+        }
+    }
+    const int64_t n_act = etm::block_sum<256>(act, lds);
+    if (j == 0) {
+        q->active = n_act;
+        q->active_after = 0;
+    }
+}
+
+// The job holding unit u: pre[j] <= u < pre[j + 1] (pre[0] = 0 <= u < pre[n_jobs]; jobs without units are passed
+// over).
+__device__ __forceinline__ int batch_job_of(const int64_t* __restrict__ pre, int n_jobs, int64_t u) {
+    int lo = 0, hi = n_jobs;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= u) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// The small groups of size class C (S = 2, 4, 8 suffixes at most) of every job: each wave takes the next unit
+// index of the class's queue (one atomic add by lane 0) until the index passes the total, and counts that
+// (group, word chunk) with the wave-per-group body.
+template <int S, int C>
+__global__ __launch_bounds__(256) void k_batch_count_small(const Job* __restrict__ jobs, int n_jobs,
+                                                            Queue* __restrict__ q) {
+    const int lane = threadIdx.x & 63;
+    const int64_t* pre = q->pre[C];
+    const int64_t total = pre[n_jobs];
+    for (;;) {
+        unsigned long long got = 0;
+        if (lane == 0) got = atomicAdd(&q->head[C], 1ull);
+        const int64_t u = uniform64((int64_t)got);
+        if (u >= total) return;
+        const int j = batch_job_of(pre, n_jobs, u);
+        const Job& J = jobs[j];
+        const int64_t r = u - pre[j];
+        const int b = J.buf;
+        const int64_t* T = J.tot[b];
+        const int64_t nc = T[8 + C];
+        int64_t start = 0;
+        for (int t = 0; t < C; ++t) start += T[8 + t];
+        small_unit<S, false, true>(J.bv, J.pitch, J.n_words, J.s_chunk, J.prev_rows, J.k - 1, J.gs[b], J.cp[b],
+                                   J.status, J.counts, ETM_LAYOUT_LEX, nullptr, 0,
+                                   uniform64(J.list[b][start + r % nc]), uniform64(r / nc));
+    }
+}
+
+// Tile-pair units of every job (groups of more than 8 suffixes, and K=2's pair space): each workgroup takes the
+// next unit index from the tiled queue and counts that (tile-pair, word chunk) with the tiled body.
+__global__ __launch_bounds__(256) void k_batch_count_tiled(const Job* __restrict__ jobs, int n_jobs,
+                                                            Queue* __restrict__ q) {
+    __shared__ TiledShared sh;
+    __shared__ unsigned long long s_u;
+    const int64_t* pre = q->pre[etm::batch::kTiledQueue];
+    const int64_t total = pre[n_jobs];
+    for (;;) {
+        if (threadIdx.x == 0) s_u = atomicAdd(&q->head[etm::batch::kTiledQueue], 1ull);
+        __syncthreads();
+        const int64_t u = uniform64((int64_t)s_u);
+        __syncthreads();
+        if (u >= total) return;
+        const int j = batch_job_of(pre, n_jobs, u);
+        const Job& J = jobs[j];
+        const int64_t r = u - pre[j];
+        const int b = J.buf;
+        const int64_t n_tp = J.tot[b][2];
+        tiled_unit(sh, J.bv, J.pitch, J.n_words, J.t_chunk, J.prev_rows, J.k - 1, J.gs[b], J.cp[b], J.ct[b],
+                   J.tot[b][0], J.status, J.counts, ETM_LAYOUT_LEX, uniform64(r % n_tp), uniform64(r / n_tp));
+    }
+}
+
 }  // namespace
+
+namespace etm {
+namespace detail {
+
+hipError_t launch_batch_plan(etm::batch::Job* jobs, int n_jobs, etm::batch::Queue* q, hipStream_t s) {
+    k_batch_plan<<<1, 256, 0, s>>>(jobs, n_jobs, q);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_count(const etm::batch::Job* jobs, int n_jobs, etm::batch::Queue* q, hipStream_t s) {
+    k_batch_count_small<2, 0><<<kBatchSmallBlocks, 256, 0, s>>>(jobs, n_jobs, q);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_batch_count_small<4, 1><<<kBatchSmallBlocks, 256, 0, s>>>(jobs, n_jobs, q);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_batch_count_small<8, 2><<<kBatchSmallBlocks, 256, 0, s>>>(jobs, n_jobs, q);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_batch_count_tiled<<<kBatchTiledBlocks, 256, 0, s>>>(jobs, n_jobs, q);
+    return hipGetLastError();
+}
+
+}  // namespace detail
+}  // namespace etm
 
 // Workspace layout: kLiveCounters int64, the small list (n_small int32), the dense tile-pair list (n_tile_pairs
 // int32), then, 8-byte aligned, the sparse tile-pair entries (n_tile_pairs SparseTP).

@@ -2,6 +2,7 @@
 // and returns the launch status. A non-null d_n means "the count lives on the device at d_n, at most n_cap".
 #pragma once
 
+#include "batch.h"
 #include "common.h"
 
 namespace etm {
@@ -67,6 +68,16 @@ size_t count_glist_ws(int64_t n_groups, int64_t n_tiled, int64_t n_cands, int64_
 hipError_t launch_live_bytes(const uint8_t* status, const int32_t* gid, const int64_t* group_start,
                              const int64_t* cum_pairs, int64_t n_groups, int64_t n_cands, int64_t n_rows,
                              uint64_t* row_bits, uint64_t* group_bits, int64_t* out, hipStream_t s);
+
+// Rounds of a batch of jobs (batch.h): each launch covers n_jobs job records, reads every size from them, and
+// skips the jobs that are not active. The plan launch (count.hip) must precede the hash, classification and
+// counting launches of a round; compaction and the next groups end it.
+hipError_t launch_batch_plan(etm::batch::Job* jobs, int n_jobs, etm::batch::Queue* q, hipStream_t s);
+hipError_t launch_batch_hash(etm::batch::Job* jobs, int n_jobs, hipStream_t s);
+hipError_t launch_batch_classify(etm::batch::Job* jobs, int n_jobs, hipStream_t s);
+hipError_t launch_batch_count(const etm::batch::Job* jobs, int n_jobs, etm::batch::Queue* q, hipStream_t s);
+hipError_t launch_batch_compact(etm::batch::Job* jobs, int n_jobs, hipStream_t s);
+hipError_t launch_batch_groups(etm::batch::Job* jobs, int n_jobs, etm::batch::Queue* q, bool init, hipStream_t s);
 
 }  // namespace detail
 }  // namespace etm

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <mutex>
 
+#include "batch.h"
 #include "common.h"
 #include "internal.h"
 #include "scan.h"
@@ -1329,6 +1330,168 @@ __global__ __launch_bounds__(256) void k_live_sum(const uint64_t* __restrict__ r
     }
 }
 
+// ── batches of jobs in lockstep rounds (batch.h) ──────────────────────────
+//
+// blockIdx.y (or blockIdx.x for the workgroup-per-job kernels) is the job; a job that is not active returns at
+// once, so a round's launches keep one shape whatever the jobs' states.
+
+using etm::batch::Job;
+
+constexpr int kBatchFillBlocks = 16;    // workgroups per job clearing its hash slots
+constexpr int kBatchInsertBlocks = 32;  // workgroups per job inserting its previous level (32 x 1024 rows)
+constexpr int kBatchClassifyBlocks = etm::batch::kCapCands / 256;  // a thread per candidate slot
+
+__global__ __launch_bounds__(256) void k_batch_hash_fill(const Job* __restrict__ jobs) {
+    const Job& J = jobs[blockIdx.y];
+    if (J.state != etm::batch::kActive || J.k < 3) return;
+    int32_t* t = J.table;
+    const int64_t size = J.table_mask + 1;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < size; i += (int64_t)gridDim.x * 256) t[i] = -1;
+}
+
+__global__ __launch_bounds__(256) void k_batch_hash_insert(const Job* __restrict__ jobs) {
+    const Job& J = jobs[blockIdx.y];
+    if (J.state != etm::batch::kActive || J.k < 3) return;
+    const int32_t m = J.k - 1;
+    const int64_t mask = J.table_mask;
+    const int32_t* rows = J.prev_rows;
+    int32_t* table = J.table;
+    for (int64_t r = blockIdx.x * 256 + threadIdx.x; r < J.prev_n; r += (int64_t)gridDim.x * 256) {
+        uint64_t h = 0;
+        for (int t = 0; t < m; ++t) h += item_hash(row_item(rows, r, m, t));
+        int64_t slot = (int64_t)(h & (uint64_t)mask);
+        while (atomicCAS(&table[slot], -1, (int32_t)r) != -1) slot = (slot + 1) & mask;
+    }
+}
+
+// Status, count slot and group of every candidate of the round: at K=2 every pair is counted; from K=3 a
+// candidate with a (k-1)-subset missing from the previous level is skipped (the subset test of the plain loop,
+// by hash probes).
+__global__ __launch_bounds__(256) void k_batch_classify(Job* __restrict__ jobs) {
+    __shared__ int64_t lds[5];
+    Job& J = jobs[blockIdx.y];
+    if (J.state != etm::batch::kActive) return;
+    const int b = J.buf;
+    const int32_t m = J.k - 1;
+    const int64_t n_cands = J.tot[b][1], n_groups = J.tot[b][0];
+    const int64_t* gs = J.gs[b];
+    const int64_t* cp = J.cp[b];
+    const Classifier C{J.prev_rows, m, nullptr, nullptr, J.table, J.table_mask, ETM_SUBSET_PRUNE, nullptr, nullptr, 0};
+    int64_t n_count = 0, n_skip = 0;
+    for (int64_t c = blockIdx.x * 256 + threadIdx.x; c < n_cands; c += (int64_t)gridDim.x * 256) {
+        const int64_t g = n_groups == 1 ? 0 : etm::upper_index(cp, n_groups, c);
+        uint8_t st = ETM_CAND_COUNT;
+        int32_t cnt = 0;
+        if (m >= 2) {
+            const int64_t g0 = gs[g];
+            int64_t i, j;
+            etm::lex_pair_decode(c - cp[g], gs[g + 1] - g0, &i, &j);
+            C.classify(g0, i, j, &st, &cnt);
+        }
+        J.status[c] = st;
+        J.counts[c] = cnt;
+        J.gid[c] = (int32_t)g;
+        n_count += st == ETM_CAND_COUNT;
+        n_skip += st == ETM_CAND_SKIP;
+    }
+    n_count = etm::block_sum<256>(n_count, lds);
+    n_skip = etm::block_sum<256>(n_skip, lds);
+    if (threadIdx.x == 0) {
+        if (n_count) atomicAdd((unsigned long long*)&J.tally[0], (unsigned long long)n_count);
+        if (n_skip) atomicAdd((unsigned long long*)&J.tally[2], (unsigned long long)n_skip);
+    }
+}
+
+// A workgroup per job: the round's survivors (count >= min_count, not skipped) in lexicographic order into the
+// job's level store, rows then counts, recorded as the job's next level.
+__global__ __launch_bounds__(kOne) void k_batch_compact(Job* __restrict__ jobs) {
+    __shared__ int32_t lds[kOneWaves];
+    Job& J = jobs[blockIdx.x];
+    if (J.state != etm::batch::kActive) return;
+    const int b = J.buf;
+    const int32_t k = J.k, m = k - 1;
+    const int64_t n_cands = J.tot[b][1], n_groups = J.tot[b][0];
+    const int64_t rows_off = J.out_used, counts_off = rows_off + n_cands * k;
+    if (counts_off + n_cands > J.out_cap || J.n_levels >= etm::batch::kMaxLevels) {
+        if (threadIdx.x == 0) J.state = etm::batch::kOverflow;
+        return;
+    }
+    const int64_t n = compact_single_body(J.status, J.counts, J.min_count, J.gid, J.prev_rows, m, J.gs[b], J.cp[b],
+                                          n_groups, n_cands, J.out + rows_off, J.out + counts_off, nullptr, lds);
+    if (threadIdx.x == 0) {
+        etm::batch::Level& L = J.lv[J.n_levels];
+        L.n = n;
+        L.rows_off = rows_off;
+        L.counts_off = counts_off;
+        L.cands = n_cands;
+        L.counted = J.tally[0];
+        L.skipped = J.tally[2];
+        L.bytes_alg = ((int64_t)(m - 1) * J.tot[b][5] + J.tot[b][6]) * J.n_words * 8 + 4 * n_cands;
+        L.t_end = 0;
+        J.new_n = n;
+        J.out_used = counts_off + n;
+        J.n_levels += 1;
+    }
+}
+
+// A workgroup per job. With `init`: the prefix groups of the job's previous level (level 1 before the first
+// round). Otherwise the groups of the level the round just recorded, after which the job advances to it, or
+// ends: done when no level can follow (no survivors, max_len reached, fewer rows than the next size, or no
+// candidates), overflow when the next level's candidates pass kCapCands.
+__global__ __launch_bounds__(kOne) void k_batch_groups(Job* __restrict__ jobs, etm::batch::Queue* __restrict__ q,
+                                                       int init) {
+    __shared__ int32_t lds[5 * kOneWaves];
+    __shared__ int64_t red[kOneWaves + 1];
+    Job& J = jobs[blockIdx.x];
+    if (J.state != etm::batch::kActive) return;
+    const int32_t k = J.k;
+    const int32_t* rows;
+    const int32_t* counts = nullptr;
+    int64_t n;
+    int32_t m;
+    int nb;
+    if (init) {
+        rows = J.prev_rows;
+        n = J.prev_n;
+        m = k - 1;
+        nb = J.buf;
+    } else {
+        const etm::batch::Level& L = J.lv[J.n_levels - 1];
+        rows = J.out + L.rows_off;
+        counts = J.out + L.counts_off;
+        n = L.n;
+        m = k;
+        nb = J.buf ^ 1;
+        if (n == 0 || k + 1 > J.max_len || n < k + 1) {
+            if (threadIdx.x == 0) {
+                J.lv[J.n_levels - 1].t_end = wall_clock64();
+                J.state = etm::batch::kDone;
+            }
+            return;
+        }
+    }
+    groups_single_body(rows, n, m, J.small_max, J.gs[nb], J.cp[nb], J.ct[nb], J.list[nb], J.tot[nb], lds, red);
+    if (threadIdx.x == 0) {
+        const int64_t cands = J.tot[nb][1];
+        const int32_t st = cands == 0 ? etm::batch::kDone
+                                      : (cands > etm::batch::kCapCands ? etm::batch::kOverflow : etm::batch::kActive);
+        if (init) {
+            J.t_init = wall_clock64();
+        } else {
+            J.lv[J.n_levels - 1].t_end = wall_clock64();
+            if (st == etm::batch::kActive) atomicAdd((unsigned long long*)&q->active_after, 1ull);
+            if (st == etm::batch::kActive) {
+                J.prev_rows = rows;
+                J.prev_counts = counts;
+                J.prev_n = n;
+                J.buf = nb;
+                J.k = k + 1;
+            }
+        }
+        J.state = st;
+    }
+}
+
 }  // namespace
 
 namespace etm {
@@ -1379,6 +1542,29 @@ hipError_t launch_free(const int32_t* rows, int64_t n_cap, const int64_t* d_n, i
 hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s) {
     if (n == 0) return hipSuccess;
     k_fill_u8<<<grid_for(n), 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_hash(etm::batch::Job* jobs, int n_jobs, hipStream_t s) {
+    k_batch_hash_fill<<<dim3(kBatchFillBlocks, n_jobs), 256, 0, s>>>(jobs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_batch_hash_insert<<<dim3(kBatchInsertBlocks, n_jobs), 256, 0, s>>>(jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_classify(etm::batch::Job* jobs, int n_jobs, hipStream_t s) {
+    k_batch_classify<<<dim3(kBatchClassifyBlocks, n_jobs), 256, 0, s>>>(jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_compact(etm::batch::Job* jobs, int n_jobs, hipStream_t s) {
+    k_batch_compact<<<n_jobs, kOne, 0, s>>>(jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_groups(etm::batch::Job* jobs, int n_jobs, etm::batch::Queue* q, bool init, hipStream_t s) {
+    k_batch_groups<<<n_jobs, kOne, 0, s>>>(jobs, q, init ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -37,6 +38,7 @@ struct etm_levels {
     std::vector<etm_level_stat> stats;
     std::vector<int64_t> col_to_item;
     std::vector<int64_t> col_counts;
+    std::vector<int32_t> owned;  // host copy of levels K >= 2 of a batched job (etm_mine_batch)
     ~etm_levels();
 };
 
@@ -787,6 +789,427 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
     }
 }
 
+// ── batches of jobs (etm_mine_batch) ──────────────────────────────────────
+//
+// The input layers of all jobs run phase by phase (id ranges, histograms, bitvectors and K=1 counts), each phase
+// enqueued for every job before one host sync. The jobs whose K=1 level is narrow enough then advance in lockstep
+// rounds (batch.h): one round is a fixed launch sequence over the job records, replayed from a HIP graph
+// captured once per (records, queue, batch width) on the thread, with up to two rounds in flight and one
+// scalar read back per round. The other jobs, and any that outgrow the rounds, take the per-job loop.
+
+namespace bt = etm::batch;
+
+// A thread's pinned host buffer for the batch phases (grown on demand, kept for the process). Every phase takes
+// it once, for all it needs, and ends with a stream wait before the next phase takes it again.
+void* batch_pinned(size_t bytes) {
+    static thread_local void* buf = nullptr;
+    static thread_local size_t cap = 0;
+    if (bytes > cap) {
+        if (buf) (void)hipHostFree(buf);
+        buf = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(bytes, 1 << 20);
+        check(hipHostMalloc(&buf, want, hipHostMallocDefault), "hipHostMalloc");
+        cap = want;
+    }
+    return buf;
+}
+
+// Ints of a batched job's level store: room for about six full levels (32,768 candidates of up to 10 items);
+// a job that needs more ends its rounds as an overflow and takes the per-job loop.
+constexpr int64_t kBatchOutInts = 2ll << 20;
+// Jobs of a round graph's width come in multiples of this (the slots past the batch are marked done), so a
+// few captured graphs serve every batch size.
+constexpr int kBatchWidthStep = 16;
+constexpr int kBatchInFlight = 2;  // rounds enqueued ahead of the host's read of the active count
+constexpr int kBatchRoundsMax = bt::kMaxLevels + 2;
+
+void enqueue_round(bt::Job* jobs, int nb, bt::Queue* q, hipStream_t s) {
+    check(etm::detail::launch_batch_plan(jobs, nb, q, s), "batch plan");
+    check(etm::detail::launch_batch_hash(jobs, nb, s), "batch hash");
+    check(etm::detail::launch_batch_classify(jobs, nb, s), "batch classify");
+    check(etm::detail::launch_batch_count(jobs, nb, q, s), "batch count");
+    check(etm::detail::launch_batch_compact(jobs, nb, s), "batch compaction");
+    check(etm::detail::launch_batch_groups(jobs, nb, q, false, s), "batch groups");
+}
+
+// Whether rounds replay a captured graph (ETM_BATCH_GRAPH=0: plain launches, for comparison).
+bool batch_graphs() {
+    static const bool on = [] {
+        const char* v = std::getenv("ETM_BATCH_GRAPH");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
+struct RoundGraph {
+    int dev;
+    const void* jobs;
+    const void* q;
+    int nb;
+    hipGraphExec_t exec;
+};
+
+// The executable graph of one round over (jobs, nb, q) on the current device: captured on a private stream of
+// this thread the first time, then reused (the arena gives the records and the queue the same addresses call
+// after call). At most eight are kept per thread.
+hipGraphExec_t round_graph(bt::Job* jobs, int nb, bt::Queue* q) {
+    static thread_local std::vector<RoundGraph> cache;
+    static thread_local hipStream_t cap_stream = nullptr;
+    static thread_local int cap_dev = -1;
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    for (const RoundGraph& g : cache)
+        if (g.dev == dev && g.jobs == jobs && g.q == q && g.nb == nb) return g.exec;
+    if (cap_dev != dev) {
+        check(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking), "capture stream");
+        cap_dev = dev;
+    }
+    check(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    try {
+        enqueue_round(jobs, nb, q, cap_stream);
+    } catch (...) {
+        hipGraph_t broken = nullptr;
+        (void)hipStreamEndCapture(cap_stream, &broken);
+        if (broken) (void)hipGraphDestroy(broken);
+        throw;
+    }
+    hipGraph_t graph = nullptr;
+    check(hipStreamEndCapture(cap_stream, &graph), "end capture");
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    check(e, "graph instantiate");
+    if (cache.size() >= 8) {
+        (void)hipGraphExecDestroy(cache.front().exec);
+        cache.erase(cache.begin());
+    }
+    cache.push_back(RoundGraph{dev, jobs, q, nb, exec});
+    return exec;
+}
+
+// Device wall-clock ticks per millisecond (hipDeviceAttributeWallClockRate is in kHz).
+double wall_ticks_per_ms() {
+    static thread_local int dev_seen = -1;
+    static thread_local double rate = 1e5;
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev != dev_seen) {
+        int khz = 0;
+        check(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "wall clock rate");
+        rate = khz > 0 ? (double)khz : 1e5;
+        dev_seen = dev;
+    }
+    return rate;
+}
+
+// What a batch keeps per job on the host.
+struct BatchHost {
+    int64_t lo = 0, span = 0, hist_off = 0, n_words = 0, pitch = 0, pc_off = 0, mc = 1, max_len = 0;
+    int32_t n_cols = 0;
+    uint64_t* bv = nullptr;
+    std::vector<int32_t> k1;
+};
+
+// Level 1 of a batched job into its result, as the per-job loop writes it (no free-set mode here).
+void batch_level_one(etm_levels& out, const BatchHost& h, Clock::time_point t_call) {
+    out.k1_rows = h.k1;
+    out.k1_counts.resize(h.k1.size());
+    for (size_t t = 0; t < h.k1.size(); ++t) out.k1_counts[t] = (int32_t)out.col_counts[(size_t)h.k1[t]];
+    out.k.push_back(1);
+    out.n.push_back((int64_t)out.k1_rows.size());
+    out.rows.push_back(out.k1_rows.data());
+    out.counts.push_back(out.k1_counts.data());
+    etm_level_stat st1{};
+    st1.k = 1;
+    st1.n_candidates = h.n_cols;
+    st1.n_counted = h.n_cols;
+    st1.n_frequent = (int64_t)h.k1.size();
+    st1.bytes_alg = (int64_t)h.n_cols * h.n_words * 8;
+    st1.bytes_live = st1.bytes_alg;
+    st1.wall_ms = (float)ms_since(t_call);
+    out.stats.push_back(st1);
+}
+
+// Rounds over the batched jobs ids[0..nj): on return each job's levels K >= 2 are in its result, or the job is
+// listed in `overflow`.
+void batch_rounds(Arena& a, bt::Job* d_jobs, bt::Queue* d_q, const std::vector<int>& ids,
+                  std::vector<BatchHost>& hj, std::vector<std::unique_ptr<etm_levels>>& res,
+                  std::vector<int>& overflow, hipStream_t s) {
+    const int nj = (int)ids.size();
+    const int nb = std::min(bt::kMaxJobs, (nj + kBatchWidthStep - 1) / kBatchWidthStep * kBatchWidthStep);
+    const int64_t cap = bt::kCapCands;
+    int64_t l1_ints = 0;
+    for (int i : ids) l1_ints += 2 * (int64_t)hj[(size_t)i].k1.size();
+    const size_t rec_bytes = (size_t)nb * sizeof(bt::Job);
+    const size_t l1_bytes = ((size_t)l1_ints * 4 + 255) & ~(size_t)255;
+    char* hp = (char*)batch_pinned(rec_bytes + l1_bytes + 8 * kBatchRoundsMax);
+    bt::Job* hr = (bt::Job*)hp;
+    int32_t* h_l1 = (int32_t*)(hp + rec_bytes);
+    int64_t* h_act = (int64_t*)(hp + rec_bytes + l1_bytes);
+    std::memset(hr, 0, rec_bytes);
+    int32_t* d_l1 = a.get<int32_t>(l1_ints);
+    int64_t at = 0;
+    for (int t = 0; t < nj; ++t) {
+        const BatchHost& h = hj[(size_t)ids[(size_t)t]];
+        bt::Job& J = hr[t];
+        J.bv = h.bv;
+        J.pitch = h.pitch;
+        J.n_words = h.n_words;
+        J.min_count = h.mc;
+        J.max_len = (int32_t)std::min<int64_t>(h.max_len, bt::kMaxLevels + 1);
+        J.small_max = 8;
+        J.table = a.get<int32_t>(bt::kTableSize);
+        J.status = a.get<uint8_t>(cap);
+        J.counts = a.get<int32_t>(cap);
+        J.gid = a.get<int32_t>(cap);
+        for (int q = 0; q < 2; ++q) {
+            J.gs[q] = a.get<int64_t>(cap + 1);
+            J.cp[q] = a.get<int64_t>(cap + 1);
+            J.ct[q] = a.get<int64_t>(cap + 1);
+            J.list[q] = a.get<int32_t>(cap);
+        }
+        J.out_cap = kBatchOutInts;
+        J.out = a.get<int32_t>(kBatchOutInts);
+        const int64_t n1 = (int64_t)h.k1.size();
+        for (int64_t c = 0; c < n1; ++c) {
+            h_l1[at + c] = h.k1[(size_t)c];
+            h_l1[at + n1 + c] = (int32_t)res[(size_t)ids[(size_t)t]]->col_counts[(size_t)h.k1[(size_t)c]];
+        }
+        J.prev_rows = d_l1 + at;
+        J.prev_counts = d_l1 + at + n1;
+        J.prev_n = n1;
+        at += 2 * n1;
+        J.k = 2;
+        J.state = bt::kActive;
+    }
+    for (int t = nj; t < nb; ++t) hr[t].state = bt::kDone;
+    check(hipMemcpyAsync(d_jobs, hr, rec_bytes, hipMemcpyHostToDevice, s), "upload job records");
+    if (l1_ints) check(hipMemcpyAsync(d_l1, h_l1, (size_t)l1_ints * 4, hipMemcpyHostToDevice, s), "upload level 1");
+    check(etm::detail::launch_batch_groups(d_jobs, nb, d_q, true, s), "batch groups");
+    hipGraphExec_t exec = batch_graphs() ? round_graph(d_jobs, nb, d_q) : nullptr;
+    int launched = 0, seen = 0;
+    bool more = true;
+    while (seen < launched || (more && launched < kBatchRoundsMax)) {
+        if (more && launched < kBatchRoundsMax && launched - seen < kBatchInFlight) {
+            if (exec) check(hipGraphLaunch(exec, s), "round graph");
+            else enqueue_round(d_jobs, nb, d_q, s);
+            check(hipMemcpyAsync(h_act + launched, &d_q->active_after, 8, hipMemcpyDeviceToHost, s), "readback");
+            check(hipEventRecord(pooled_event((size_t)(launched % (2 * kBatchInFlight))), s), "event record");
+            ++launched;
+            continue;
+        }
+        const hipEvent_t ev = pooled_event((size_t)(seen % (2 * kBatchInFlight)));
+        for (;;) {
+            const hipError_t e = hipEventQuery(ev);
+            if (e == hipSuccess) break;
+            if (e != hipErrorNotReady) check(e, "round event");
+            __builtin_ia32_pause();
+        }
+        if (h_act[seen] == 0) more = false;
+        ++seen;
+    }
+    // The records back, then every job's level store in one gathered copy.
+    bt::Job* back = (bt::Job*)batch_pinned(rec_bytes);
+    check(hipMemcpyAsync(back, d_jobs, rec_bytes, hipMemcpyDeviceToHost, s), "readback job records");
+    wait_stream(s);
+    std::vector<bt::Job> rec(back, back + nj);
+    int64_t total = 0;
+    std::vector<int64_t> base((size_t)nj, 0);
+    for (int t = 0; t < nj; ++t) {
+        if (rec[(size_t)t].state != bt::kDone) continue;
+        base[(size_t)t] = total;
+        total += rec[(size_t)t].out_used;
+    }
+    int32_t* h_out = nullptr;
+    if (total > 0) {
+        int32_t* dev_stage = a.get<int32_t>(total);
+        Segs segs{};
+        int ns = 0;
+        int64_t longest = 0;
+        auto flush = [&]() {
+            if (ns == 0) return;
+            const int64_t bx = std::min<int64_t>(std::max<int64_t>((longest + 255) / 256, 1), 1024);
+            k_gather<<<dim3((unsigned)bx, (unsigned)ns), 256, 0, s>>>(segs, dev_stage);
+            check(hipGetLastError(), "gather");
+            segs = Segs{};
+            ns = 0;
+            longest = 0;
+        };
+        for (int t = 0; t < nj; ++t) {
+            const bt::Job& J = rec[(size_t)t];
+            if (J.state != bt::kDone || J.out_used == 0) continue;
+            segs.s[ns++] = Seg{J.out, J.out_used, base[(size_t)t]};
+            longest = std::max(longest, J.out_used);
+            if (ns == kGatherSegs) flush();
+        }
+        flush();
+        h_out = (int32_t*)batch_pinned((size_t)total * 4);
+        check(hipMemcpyAsync(h_out, dev_stage, (size_t)total * 4, hipMemcpyDeviceToHost, s), "copy out");
+        wait_stream(s);
+    }
+    const double tpm = wall_ticks_per_ms();
+    for (int t = 0; t < nj; ++t) {
+        const bt::Job& J = rec[(size_t)t];
+        const int i = ids[(size_t)t];
+        if (J.state != bt::kDone) {
+            overflow.push_back(i);
+            continue;
+        }
+        etm_levels& out = *res[(size_t)i];
+        out.owned.assign(h_out + base[(size_t)t], h_out + base[(size_t)t] + J.out_used);
+        int64_t t_prev = J.t_init;
+        for (int l = 0; l < J.n_levels; ++l) {
+            const bt::Level& L = J.lv[l];
+            const int32_t k = l + 2;
+            out.k.push_back(k);
+            out.n.push_back(L.n);
+            out.rows.push_back(out.owned.data() + L.rows_off);
+            out.counts.push_back(out.owned.data() + L.counts_off);
+            etm_level_stat st{};
+            st.k = k;
+            st.kernel = ETM_KERNEL_VALU;
+            st.n_candidates = L.cands;
+            st.n_counted = L.counted;
+            st.n_skipped = L.skipped;
+            st.n_frequent = L.n;
+            st.bytes_alg = L.bytes_alg;
+            st.bytes_live = k == 2 ? L.bytes_alg : 0;
+            st.wall_ms = (float)((double)(L.t_end - t_prev) / tpm);
+            t_prev = L.t_end;
+            out.stats.push_back(st);
+        }
+    }
+}
+
+void mine_batch(const etm_batch_job* in, int n, int64_t span_limit, const etm_mine_opts& o,
+                std::vector<std::unique_ptr<etm_levels>>& res, hipStream_t s, Clock::time_point t_call) {
+    Arena a(s);
+    // The records and the queue first: their addresses then repeat from call to call on this thread, and so does
+    // the round graph.
+    bt::Job* d_jobs = a.get<bt::Job>(bt::kMaxJobs);
+    bt::Queue* d_q = a.get<bt::Queue>(1);
+    std::vector<BatchHost> hj((size_t)n);
+    auto has_data = [&](int i) { return in[i].nnz > 0 && in[i].n_rows > 0; };
+    // Id ranges.
+    int64_t* d_mm = a.get<int64_t>(2 * (int64_t)n);
+    for (int i = 0; i < n; ++i)
+        if (has_data(i)) check_rc(etm_ids_minmax(in[i].d_ids, in[i].id_bytes, in[i].nnz, d_mm + 2 * i, s), "etm_ids_minmax");
+    int64_t total_span = 0;
+    {
+        int64_t* h = (int64_t*)batch_pinned((size_t)n * 16);
+        check(hipMemcpyAsync(h, d_mm, (size_t)n * 16, hipMemcpyDeviceToHost, s), "readback");
+        wait_stream(s);
+        for (int i = 0; i < n; ++i) {
+            if (!has_data(i)) continue;
+            BatchHost& b = hj[(size_t)i];
+            b.lo = h[2 * i];
+            b.span = h[2 * i + 1] - h[2 * i] + 1;
+            if (b.span > span_limit)
+                throw Error(ETM_EINVAL, "job " + std::to_string(i) + ": item ids span " + std::to_string(b.span) +
+                                            " values: code them to dense ids first");
+            b.hist_off = total_span;
+            total_span += b.span;
+        }
+    }
+    // Histograms down, LUTs up; the same pinned block later takes the K=1 counts (at most one per id).
+    int32_t* d_hist = a.get<int32_t>(total_span);
+    int32_t* d_lut = a.get<int32_t>(total_span);
+    for (int i = 0; i < n; ++i) {
+        const BatchHost& b = hj[(size_t)i];
+        if (has_data(i))
+            check_rc(etm_ids_hist(in[i].d_ids, in[i].id_bytes, in[i].nnz, b.lo, b.span, d_hist + b.hist_off, s),
+                     "etm_ids_hist");
+    }
+    int32_t* h = (int32_t*)batch_pinned((size_t)std::max<int64_t>(total_span, 1) * 16);
+    int32_t* h_hist = h;
+    int32_t* h_lut = h + total_span;
+    int64_t* h_pc = (int64_t*)(h + 2 * total_span);
+    if (total_span) {
+        check(hipMemcpyAsync(h_hist, d_hist, (size_t)total_span * 4, hipMemcpyDeviceToHost, s), "readback");
+        wait_stream(s);
+    }
+    int64_t total_cols = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!has_data(i)) continue;
+        BatchHost& b = hj[(size_t)i];
+        b.mc = std::max<int64_t>(1, in[i].min_count);
+        etm_levels& out = *res[(size_t)i];
+        for (int64_t v = 0; v < b.span; ++v) {
+            h_lut[b.hist_off + v] = -1;
+            if (h_hist[b.hist_off + v] >= b.mc) {
+                h_lut[b.hist_off + v] = (int32_t)out.col_to_item.size();
+                out.col_to_item.push_back(b.lo + v);
+            }
+        }
+        b.n_cols = (int32_t)out.col_to_item.size();
+        b.pc_off = total_cols;
+        total_cols += b.n_cols;
+    }
+    if (total_span) check(hipMemcpyAsync(d_lut, h_lut, (size_t)total_span * 4, hipMemcpyHostToDevice, s), "upload lut");
+    // Bitvectors and K=1 counts.
+    int64_t* d_pc = a.get<int64_t>(total_cols);
+    for (int i = 0; i < n; ++i) {
+        BatchHost& b = hj[(size_t)i];
+        if (!has_data(i) || b.n_cols == 0) continue;
+        b.n_words = (in[i].n_rows + 63) / 64;
+        b.pitch = std::max<int64_t>(64, (b.n_words + 63) / 64 * 64);
+        b.bv = a.get<uint64_t>((int64_t)b.n_cols * b.pitch);
+        check_rc(etm_bitvec_build(in[i].d_indptr, in[i].d_ids, in[i].id_bytes, in[i].n_rows, b.lo, b.span,
+                                  d_lut + b.hist_off, b.n_cols, b.pitch, nullptr, b.bv, s),
+                 "etm_bitvec_build");
+        check_rc(etm_column_popcounts(b.bv, b.n_cols, b.pitch, b.n_words, d_pc + b.pc_off, s), "etm_column_popcounts");
+    }
+    if (total_cols) {
+        check(hipMemcpyAsync(h_pc, d_pc, (size_t)total_cols * 8, hipMemcpyDeviceToHost, s), "readback");
+        wait_stream(s);
+    }
+    // K=1, and which jobs the rounds take.
+    std::vector<int> batched, single;
+    for (int i = 0; i < n; ++i) {
+        BatchHost& b = hj[(size_t)i];
+        if (!has_data(i) || b.n_cols == 0) continue;
+        etm_levels& out = *res[(size_t)i];
+        out.col_counts.assign(h_pc + b.pc_off, h_pc + b.pc_off + b.n_cols);
+        for (int32_t c = 0; c < b.n_cols; ++c)
+            if (out.col_counts[(size_t)c] >= b.mc) b.k1.push_back(c);
+        b.max_len = in[i].max_length > 0 ? std::min<int64_t>(in[i].max_length, b.n_cols) : b.n_cols;
+        const int64_t n1 = (int64_t)b.k1.size();
+        if (n1 >= 2 && n1 <= bt::kMaxCols && b.max_len >= 2 && in[i].n_rows <= 0x7FFFFFFF) batched.push_back(i);
+        else single.push_back(i);
+    }
+    std::vector<int> overflow;
+    for (size_t b0 = 0; b0 < batched.size(); b0 += bt::kMaxJobs) {
+        const size_t b1 = std::min(batched.size(), b0 + bt::kMaxJobs);
+        std::vector<int> ids(batched.begin() + (std::ptrdiff_t)b0, batched.begin() + (std::ptrdiff_t)b1);
+        for (int i : ids) batch_level_one(*res[(size_t)i], hj[(size_t)i], t_call);
+        batch_rounds(a, d_jobs, d_q, ids, hj, res, overflow, s);
+    }
+    // The per-job loop for the rest, on the bitvectors built above.
+    for (int i : overflow) {
+        etm_levels& out = *res[(size_t)i];
+        out.k.clear();
+        out.n.clear();
+        out.rows.clear();
+        out.counts.clear();
+        out.stats.clear();
+        out.owned.clear();
+        single.push_back(i);
+    }
+    for (int i : single) {
+        const BatchHost& b = hj[(size_t)i];
+        etm_mine_opts oi = o;
+        oi.min_count = b.mc;
+        oi.max_length = in[i].max_length;
+        const RowsSrc src{in[i].d_indptr, in[i].d_ids, in[i].id_bytes, in[i].nnz, b.lo, b.span, d_lut + b.hist_off};
+        etm_levels& out = *res[(size_t)i];
+        mine_levels(b.bv, b.pitch, b.n_words, in[i].n_rows, out.col_counts.data(), b.n_cols, oi, out, s, t_call, &src,
+                    false);
+    }
+    // Nothing of this call stays queued behind the return (the LUT upload reads the thread's pinned buffer).
+    wait_stream(s);
+}
+
 template <class F>
 int guarded(F&& f) {
     try {
@@ -1014,4 +1437,30 @@ etm_levels::~etm_levels() { pool_give(block, block_cap); }
 extern "C" int etm_levels_free(etm_levels* lv) {
     delete lv;
     return ETM_OK;
+}
+
+extern "C" int etm_mine_batch(const etm_batch_job* jobs, int32_t n_jobs, int64_t span_limit, const etm_mine_opts* opts,
+                              etm_levels** outs, void* stream) {
+    ETM_ARG(opts && outs && (n_jobs == 0 || jobs), "etm_mine_batch: null pointer");
+    ETM_ARG(n_jobs >= 0, "etm_mine_batch: negative job count");
+    ETM_ARG(!opts->prune_non_free && !opts->infer_counts && !opts->emit_levels && opts->level_cb == nullptr,
+            "etm_mine_batch: the free-set and inference modes and the level callback are mined per job "
+            "(etm_mine_csr)");
+    for (int32_t i = 0; i < n_jobs; ++i) {
+        const etm_batch_job& j = jobs[i];
+        ETM_ARG(j.d_indptr && (j.nnz == 0 || j.d_ids), "etm_mine_batch: job %d: null pointer", i);
+        ETM_ARG(j.id_bytes == 4 || j.id_bytes == 8, "etm_mine_batch: job %d: id_bytes must be 4 or 8", i);
+        ETM_ARG(j.n_rows >= 0 && j.nnz >= 0 && j.max_length >= 0, "etm_mine_batch: job %d: negative size", i);
+        outs[i] = nullptr;
+    }
+    const Clock::time_point t_call = Clock::now();
+    std::vector<std::unique_ptr<etm_levels>> res;
+    res.reserve((size_t)n_jobs);
+    for (int32_t i = 0; i < n_jobs; ++i) res.push_back(std::make_unique<etm_levels>());
+    const int rc = guarded([&] {
+        if (n_jobs > 0) mine_batch(jobs, n_jobs, span_limit, *opts, res, ETM_STREAM(stream), t_call);
+    });
+    if (rc == ETM_OK)
+        for (int32_t i = 0; i < n_jobs; ++i) outs[i] = res[(size_t)i].release();
+    return rc;
 }

@@ -17,7 +17,7 @@ from ctypes import POINTER, c_char_p, c_int, c_int32, c_int64, c_size_t, c_void_
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libetm.so"
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 ETM_OK, ETM_EINVAL, ETM_ENOMEM, ETM_EHIP, ETM_EOVERFLOW, ETM_ENODEV, ETM_ECANCELED = 0, 1, 2, 3, 4, 5, 6
 CAND_COUNT, CAND_SKIP, CAND_INFER = 0, 1, 2
@@ -87,6 +87,7 @@ SIGNATURES: dict = {
     "etm_union_insert": (c_int, [P, P, c_int64, c_int32, P, c_int64, P, c_int64, P, P, P, P, c_int64, P, P]),
     "etm_mine_bitvecs": (c_int, [P, c_int64, c_int64, c_int64, P, c_int32, P, P, P]),
     "etm_mine_csr": (c_int, [P, P, c_int, c_int64, c_int64, c_int64, P, P, P]),
+    "etm_mine_batch": (c_int, [P, c_int32, c_int64, P, P, P]),
     "etm_levels_count": (c_int, [P, P]),
     "etm_levels_shape": (c_int, [P, c_int32, P, P]),
     "etm_levels_copy": (c_int, [P, c_int32, P, P]),
@@ -115,6 +116,13 @@ class LevelInfo(ctypes.Structure):
 
 #: etm_level_cb: int (*)(const etm_level_info*, void*)
 LEVEL_CB = ctypes.CFUNCTYPE(c_int, POINTER(LevelInfo), c_void_p)
+
+
+class BatchJob(ctypes.Structure):
+    """etm_batch_job: one job of etm_mine_batch."""
+
+    _fields_ = [("d_indptr", c_void_p), ("d_ids", c_void_p), ("n_rows", c_int64), ("nnz", c_int64),
+                ("min_count", c_int64), ("id_bytes", c_int32), ("max_length", c_int32)]
 
 
 class MineOpts(ctypes.Structure):

@@ -190,6 +190,39 @@ def mine_csr_device(indptr_d, ids_d, n_rows: int, min_count: int, max_length=Non
     return _collect(handle, levels)
 
 
+def mine_batch_device(jobs, k2_kernel: str = "auto", k3_kernel: str = "auto", levels: bool = True) -> list:
+    """Many independent jobs mined in one call (``etm_mine_batch``): ``jobs`` holds (indptr_d, ids_d, n_rows,
+    min_count, max_length) per job, device CSRs as for ``mine_csr_device``. Returns one NativeResult per job, the
+    same itemsets and counts as ``mine_csr_device`` gives that job alone. Jobs whose K=1 level has at most 256
+    columns advance together, one level per round; the rest (and any whose levels outgrow the rounds) are mined
+    by the per-job loop with ``k2_kernel`` / ``k3_kernel``."""
+    torch = dev.torch()
+    n = len(jobs)
+    arr = (_lib.BatchJob * max(n, 1))()
+    keep = []
+    for i, (indptr_d, ids_d, n_rows, min_count, max_length) in enumerate(jobs):
+        if ids_d.dtype not in (torch.int64, torch.int32):
+            raise ValueError(f"item ids must be int32 or int64, got {ids_d.dtype}")
+        keep.append((indptr_d, ids_d))
+        arr[i] = _lib.BatchJob(indptr_d.data_ptr(), ids_d.data_ptr() if ids_d.numel() else None, int(n_rows),
+                               int(ids_d.numel()), int(max(1, min_count)), 8 if ids_d.dtype == torch.int64 else 4,
+                               int(max_length or 0))
+    opts = _opts(1, None, False, False, False, k2_kernel, k3_kernel)
+    handles = (ctypes.c_void_p * max(n, 1))()
+    _lib.call("etm_mine_batch", ctypes.byref(arr), n, SPAN_LIMIT, ctypes.byref(opts), ctypes.byref(handles),
+              dev.stream_ptr())
+    out = []
+    try:
+        for i in range(n):
+            h, handles[i] = handles[i], None
+            out.append(_collect(ctypes.c_void_p(h), levels))
+    finally:
+        for i in range(n):
+            if handles[i]:
+                _lib.lib().etm_levels_free(ctypes.c_void_p(handles[i]))
+    return out
+
+
 def mine_bitvecs_device(bv_ptr: int, pitch: int, n_words: int, n_rows: int, col_counts: np.ndarray, min_count: int,
                         max_length=None, prune_non_free=False, infer_counts=False,
                         time_kernels=False, k2_kernel: str = "auto", on_level=None,

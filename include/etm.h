@@ -71,7 +71,7 @@ enum { ETM_SCHEDULE_AUTO = 0, ETM_SCHEDULE_ALL = 1, ETM_SCHEDULE_LIVE = 2 };
 enum { ETM_ROW_ORDER_AUTO = 0, ETM_ROW_ORDER_KEEP = 1, ETM_ROW_ORDER_LEX = 2 };
 
 /* ABI version of this header (bumped on any signature or struct-meaning change). */
-#define ETM_ABI_VERSION 4
+#define ETM_ABI_VERSION 5
 int etm_abi_version(void);
 
 /* Copies the calling thread's last error message (NUL-terminated, truncated to n). */
@@ -393,6 +393,32 @@ int etm_mine_bitvecs(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64
  * (gpu/input_layer.py:469-544) followed by the level loop. */
 int etm_mine_csr(const int64_t* d_indptr, const void* d_ids, int id_bytes, int64_t n_rows, int64_t nnz,
                  int64_t span_limit, const etm_mine_opts* opts, etm_levels** out, void* stream);
+
+/* One job of etm_mine_batch: a device CSR of item ids (as etm_mine_csr) with its own threshold and length cap
+ * (max_length 0 = none). */
+typedef struct etm_batch_job {
+    const int64_t* d_indptr;
+    const void* d_ids;
+    int64_t n_rows;
+    int64_t nnz;
+    int64_t min_count;
+    int32_t id_bytes;
+    int32_t max_length;
+} etm_batch_job;
+
+/* Mines n_jobs independent jobs in one call, with the same result per job as etm_mine_csr (outs[i], freed with
+ * etm_levels_free). The input layers of all jobs share their host syncs; then every job whose K=1 level has at
+ * most 256 columns advances in lockstep rounds, one level per round, where each round is one fixed sequence of
+ * launches covering every job (a work-queue-fed persistent counting kernel, workgroup-per-job group build and
+ * compaction), replayed from a captured HIP graph with no host read between its launches. A job whose level
+ * outgrows the rounds' limits (32,768 candidates), or whose K=1 level is wider, is mined by the per-job loop
+ * of etm_mine_csr on the same bitvectors. opts gives the kernel settings of that loop; its min_count and
+ * max_length are ignored (each job has its own). The free-set and inference modes and the level callback are not
+ * supported here (ETM_EINVAL): mine those jobs with etm_mine_csr.
+ * No reference counterpart: the reference mines one job per call (core/apriori.py:303-341); this is the batched
+ * form of the served path, for callers holding many small jobs at once. */
+int etm_mine_batch(const etm_batch_job* jobs, int32_t n_jobs, int64_t span_limit, const etm_mine_opts* opts,
+                   etm_levels** outs, void* stream);
 
 /* Accessors of a result: number of levels; level shape; copy of a level (h_rows n*k int32, h_counts n int64,
  * either may be NULL); the columns (count, item id of each column -- empty for etm_mine_bitvecs -- and K=1
