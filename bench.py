@@ -31,8 +31,10 @@ MFMA Gram), each checked against the oracle's committed per-level digests.
 served: the served_jobs stream (512 independent jobs, seed 4: transactions log-uniform in 10K-2M, vocabulary
 500-20K, Zipf 0.9-1.3, a coin between 1 + Poisson(4|6|9) and lognormal(2.3, 0.6) baskets, min_support
 log-uniform in 0.001-0.05, max_length 3-10), every dataset resident in HBM before timing. The plain form mines
-the jobs one after another on one stream; the served form runs them through the request loop (8 workers, one
-stream each, footprint admission, kernel settings tuned per shape bucket beforehand) after one warm-up pass.
+the jobs one after another on one stream; the served form runs them through the request loop (8 per-job workers,
+one stream each, footprint admission, kernel settings tuned per shape bucket beforehand; 2 batch workers that mine
+the narrow jobs, up to 64 at a time, in batched calls whose levels advance in shared rounds) after one warm-up
+pass.
 Throughput is jobs / makespan with every job queued at t=0; latency (completion - arrival) is measured by
 replaying Poisson arrivals at twice the plain form's measured jobs/s. Every job's itemsets and counts are checked
 identical in both forms.
@@ -117,7 +119,10 @@ def main(argv=None) -> int:
     ap.add_argument("--row-order", default="auto", choices=["auto", "keep", "lex"],
                     help="bitvector row order of the native loop (counts never depend on it)")
     ap.add_argument("--served-jobs", type=int, default=512, help="jobs of the served stream (0: skip that leg)")
-    ap.add_argument("--served-workers", type=int, default=8)
+    ap.add_argument("--served-workers", type=int, default=8, help="per-job workers of the served form")
+    ap.add_argument("--served-batch-workers", type=int, default=2,
+                    help="batch workers of the served form (narrow jobs mined in batched calls; 0: none)")
+    ap.add_argument("--served-max-batch", type=int, default=64, help="most jobs per batched call")
     ap.add_argument("--no-streaming", action="store_true", help="skip the streaming leg (resident vs SON)")
     ap.add_argument("--no-legs", action="store_true", help="skip the secondary workloads")
     ap.add_argument("--levels", action="store_true", help="print the per-level statistics of the headline")
@@ -251,7 +256,8 @@ def main(argv=None) -> int:
     legs = {}
     for lw, ld in legs_data:
         legs.update(_leg(lw, ld, mine_csr, table_digests, committed_digests))
-    served = _served_leg(served_jobs_data, args.served_workers) if served_jobs_data else None
+    served = (_served_leg(served_jobs_data, args.served_workers, args.served_batch_workers, args.served_max_batch)
+              if served_jobs_data else None)
     streaming = None
     if not args.no_streaming and w.chunk_size:
         streaming = _streaming_leg(w, data)
@@ -439,7 +445,7 @@ def _streaming_leg(w, data):
     return out
 
 
-def _served_leg(jobs, workers):
+def _served_leg(jobs, workers, batch_workers, max_batch):
     """Plain vs served jobs/s on the served stream, latency under Poisson arrivals, identical results."""
     import dataclasses
 
@@ -449,7 +455,7 @@ def _served_leg(jobs, workers):
     run_plain(zero, time_scale=0.0)  # warm-up: uploads cached, per-thread arenas sized
     plain, ps = run_plain(zero, time_scale=0.0)
     print(f"[bench] plain: {ps['jobs_per_s']:.1f} jobs/s", file=sys.stderr, flush=True)
-    srv = Server(workers=workers, autotune=True)
+    srv = Server(workers=workers, autotune=True, batch_workers=batch_workers, max_batch=max_batch)
     try:
         tuned = srv.tune(zero)  # the shape buckets' kernel settings, timed with the GPU otherwise idle
         serve(zero, server=srv, time_scale=0.0)  # warm-up: the workers' arenas sized
@@ -470,7 +476,10 @@ def _served_leg(jobs, workers):
         "workload": f"served_jobs: {len(jobs)} jobs, {sum(j.n_rows for j in jobs):,} transactions, seed 4",
         "unit": "jobs/s",
         "plain": {k: ps[k] for k in keep},
-        "served": {k: ss[k] for k in keep} | {"workers": workers, "peak_concurrent_jobs": ss["peak_concurrent_jobs"]},
+        "served": {k: ss[k] for k in keep} | {"workers": workers, "batch_workers": batch_workers,
+                                               "max_batch": max_batch, "batches": ss["batches"],
+                                               "batched_jobs": ss["batched_jobs"],
+                                               "peak_concurrent_jobs": ss["peak_concurrent_jobs"]},
         "served_over_plain": ss["jobs_per_s"] / ps["jobs_per_s"],
         "tuned_buckets": len(tuned),
         "tuned_changed": sum(1 for t in tuned.values() if (t.k3_schedule, t.small_max) != ("auto", 0)),

@@ -9,6 +9,13 @@ several jobs -- their kernels and their per-level host syncs -- interleave on th
 while others are on the GPU starts as soon as a worker and memory free up. Datasets stay resident in HBM in a
 byte-bounded store (least recently used out first), so jobs on a registered dataset skip the upload.
 
+Jobs narrow enough for the batched rounds (plain mode, at most 256 frequent columns: most small jobs) wait in a
+second queue. Batch workers take every such job queued when they come free (up to ``max_batch``, within the
+memory budget) and mine them in one call (``etm_mine_batch``): the input layers share their host syncs, and the
+jobs' levels advance together, one round of shared launches (work-queue-fed counting kernels, a workgroup per job
+for group build and compaction, replayed from a HIP graph) per level, instead of a dozen launches and a host sync
+per job and level.
+
 The plain form (``run_plain``) runs the same jobs one after another on one stream, as one caller of
 ``apriori(use_gpu=True)`` per job would. Both report jobs/s and latency percentiles (completion - arrival), and
 every job's result is the same in either form (``JobResult.digest`` over every itemset and count).
@@ -168,15 +175,42 @@ class DatasetStore:
         return d_ptr, d_ids
 
 
+#: K=1 columns of a job the batched rounds take (etm_mine_batch): K=2 then has at most 32,640 candidates.
+BATCH_MAX_COLS = 256
+#: Device bytes per batched job beside its CSR and bitvectors: the round buffers (hash, per-candidate status,
+#: counts and group ids, two group buffers) and the 8 MiB level store.
+BATCH_JOB_BYTES = 11 << 20
+
+
+def frequent_columns(job: Job, profile: np.ndarray) -> int:
+    """Item ids of a job occurring at least min_count times (its K=1 columns before duplicate rows are merged)."""
+    mc = _mining_min_count(job.min_support, max(1, job.n_rows))
+    return int(len(profile) - np.searchsorted(profile, mc, side="left"))
+
+
+def batchable(job: Job, profile: np.ndarray) -> bool:
+    """Whether the batched rounds take the job: the plain mode (no generator pruning) and at most
+    BATCH_MAX_COLS frequent columns."""
+    return not job.use_generator_pruning and frequent_columns(job, profile) <= BATCH_MAX_COLS
+
+
 class Server:
     """Continuous batching of mining jobs on one GPU (see the module docstring)."""
 
     def __init__(self, workers: int = 8, memory_budget_bytes: int | None = None,
-                 dataset_budget_bytes: int | None = None, autotune: bool = False, executor=None):
+                 dataset_budget_bytes: int | None = None, autotune: bool = False, executor=None,
+                 batch_workers: int = 0, max_batch: int = 64, batch_executor=None):
         """``executor(job) -> (values, offsets, counts, n_levels, n_candidates)`` replaces the device run; it
-        exists to test the scheduling on a host without a GPU and then needs ``memory_budget_bytes``."""
+        exists to test the scheduling on a host without a GPU and then needs ``memory_budget_bytes``.
+
+        ``batch_workers`` threads take the batchable jobs (``batchable``), up to ``max_batch`` of those queued at
+        a time, and mine each group in one batched call (``mine_batch_device``): their levels advance together,
+        one round of shared launches per level. ``batch_executor(jobs) -> [executor results]`` replaces that
+        call in host-only tests."""
         self.executor = executor
+        self.batch_executor = batch_executor
         self.autotune = autotune
+        self.max_batch = max(1, int(max_batch))
         if executor is None:
             import torch
 
@@ -197,6 +231,9 @@ class Server:
         self.order: list = []  # job ids in start order
         self.cv = threading.Condition()
         self.queue: deque = deque()  # (job, future, arrival_s, footprint)
+        self.bqueue: deque = deque()  # the same, for the batch workers
+        self.batches = 0
+        self.batched_jobs = 0
         self.used = 0
         self.running = 0
         self.peak_running = 0
@@ -204,6 +241,9 @@ class Server:
         self.t0 = time.perf_counter()
         self.workers = [threading.Thread(target=self._worker, daemon=True, name=f"etm-serve-{i}")
                         for i in range(workers)]
+        self.workers += [threading.Thread(target=self._batch_worker, daemon=True, name=f"etm-batch-{i}")
+                         for i in range(batch_workers)]
+        self.n_batch_workers = batch_workers
         for t in self.workers:
             t.start()
 
@@ -225,6 +265,76 @@ class Server:
                 elif self.closed:
                     return None
                 self.cv.wait()
+
+    def _take_batch(self):
+        """The batch queue's head and the jobs behind it (up to max_batch) that fit the memory budget; the head
+        alone when nothing else runs and it exceeds the budget."""
+        with self.cv:
+            while True:
+                if self.bqueue:
+                    taken = []
+                    need = 0
+                    while self.bqueue and len(taken) < self.max_batch:
+                        job, fut, arrival, n = self.bqueue[0]
+                        if self.used + need + n > self.budget and (taken or self.running > 0):
+                            break
+                        self.bqueue.popleft()
+                        taken.append((job, fut, arrival, n))
+                        need += n
+                    if taken:
+                        for job, _, _, _ in taken:
+                            self.order.append(job.job_id)
+                        self.running += len(taken)
+                        self.peak_running = max(self.peak_running, self.running)
+                        self.used += need
+                        return taken
+                elif self.closed:
+                    return None
+                self.cv.wait()
+
+    def _batch_worker(self) -> None:
+        torch = self.torch
+        stream = torch.cuda.Stream(device=self.device) if torch is not None else None
+        while True:
+            items = self._take_batch()
+            if items is None:
+                return
+            jobs = [it[0] for it in items]
+            try:
+                start = self.now()
+                if stream is None:
+                    outs = self.batch_executor(jobs)
+                else:
+                    with torch.cuda.stream(stream):
+                        res = self._execute_batch(jobs)
+                    stream.synchronize()
+                    outs = [(*r.flat, len(r.stats), sum(st.n_evaluated for st in r.stats)) for r in res]
+                end = self.now()
+                for (job, fut, arrival, _), (values, offsets, counts, n_levels, n_cands) in zip(items, outs):
+                    fut.set_result(JobResult(job.job_id, (values, offsets, counts), arrival, start, end, n_levels,
+                                             n_cands))
+                with self.cv:
+                    self.batches += 1
+                    self.batched_jobs += len(items)
+            except BaseException as e:  # every job of the failing batch reports it; the server keeps serving
+                for _, fut, _, _ in items:
+                    if not fut.done():
+                        fut.set_exception(e)
+            finally:
+                with self.cv:
+                    self.running -= len(items)
+                    self.used -= sum(it[3] for it in items)
+                    self.cv.notify_all()
+
+    def _execute_batch(self, jobs: list):
+        from ..gpu.native import mine_batch_device
+
+        dev_jobs = []
+        for job in jobs:
+            d_ptr, d_ids = self.store.register(job.dataset, job.indptr, job.ids, self.device)
+            dev_jobs.append((d_ptr, d_ids, job.n_rows, _mining_min_count(job.min_support, job.n_rows),
+                             job.max_length))
+        return mine_batch_device(dev_jobs, levels=False)
 
     def _worker(self) -> None:
         torch = self.torch
@@ -282,11 +392,16 @@ class Server:
         """Queues a job; its future resolves to a JobResult (or raises the job's error)."""
         fut: Future = Future()
         prof = self.store.profile(job.dataset, job.ids) if self.store is not None else None
-        need = min(footprint_bytes(job, prof), self.budget)
+        if prof is None:
+            prof = occurrence_profile(job.ids)
+        to_batch = self.n_batch_workers > 0 and batchable(job, prof)
+        need = footprint_bytes(job, prof) + (BATCH_JOB_BYTES if to_batch else 0)
+        need = min(need, self.budget)
         with self.cv:
             if self.closed:
                 raise RuntimeError("server is closed")
-            self.queue.append((job, fut, self.now() if arrival_s is None else arrival_s, need))
+            (self.bqueue if to_batch else self.queue).append(
+                (job, fut, self.now() if arrival_s is None else arrival_s, need))
             self.cv.notify_all()
         return fut
 
@@ -337,6 +452,7 @@ def serve(jobs: list, workers: int = 8, time_scale: float = 1.0, resident: bool 
     t0 = time.perf_counter()
     srv.t0 = t0
     srv.peak_running = srv.running
+    batches0, batched0 = srv.batches, srv.batched_jobs
     for job in sorted(jobs, key=lambda j: j.arrival_s):
         due = job.arrival_s * time_scale
         _pace(t0, due)
@@ -346,8 +462,11 @@ def serve(jobs: list, workers: int = 8, time_scale: float = 1.0, resident: bool 
         srv.close()
     makespan = max(r.end_s for r in results) if results else 0.0
     out = summarize(results, makespan)
-    out["workers"] = len(srv.workers)
+    out["workers"] = len(srv.workers) - srv.n_batch_workers
+    out["batch_workers"] = srv.n_batch_workers
     out["peak_concurrent_jobs"] = srv.peak_running
+    out["batches"] = srv.batches - batches0
+    out["batched_jobs"] = srv.batched_jobs - batched0
     return sorted(results, key=lambda r: r.job_id), out
 
 

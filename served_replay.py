@@ -2,6 +2,7 @@
 
     python served_replay.py --mode plain --jobs 512
     python served_replay.py --mode served --workers 8 --jobs 512
+    python served_replay.py --mode served --workers 6 --batch-workers 2 --max-batch 64 --jobs 512
 
 Every job's dataset is generated on the host (before the GPU is touched) and made resident in HBM. The
 stream runs ``--passes`` times with every job queued at t=0 (the first pass warms the per-thread arenas); the
@@ -26,6 +27,8 @@ def main(argv=None) -> int:
     ap.add_argument("--mode", choices=["plain", "served"], default="served")
     ap.add_argument("--jobs", type=int, default=512)
     ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--batch-workers", type=int, default=0, help="threads mining narrow jobs in batched calls")
+    ap.add_argument("--max-batch", type=int, default=64, help="most jobs per batched call")
     ap.add_argument("--passes", type=int, default=2)
     args = ap.parse_args(argv)
 
@@ -41,7 +44,7 @@ def main(argv=None) -> int:
         for _ in range(args.passes):
             _, summary = run_plain(jobs, time_scale=0.0)
     else:
-        srv = Server(workers=args.workers)
+        srv = Server(workers=args.workers, batch_workers=args.batch_workers, max_batch=args.max_batch)
         try:
             for _ in range(args.passes):
                 _, summary = serve(jobs, server=srv, time_scale=0.0)

@@ -132,6 +132,55 @@ def test_oversized_job_runs_alone():
     assert srv.peak_running == 1
 
 
+def test_batch_workers_take_narrow_jobs_in_groups():
+    """Narrow jobs (at most 256 frequent columns, no generator pruning) go to the batch workers, several per
+    call, within the memory budget; the others to the per-job workers. Every job gets its own result, and a
+    failing batch reports its error on each of its jobs."""
+    from et_miner.serve import Server
+    from et_miner.serve.server import batchable, occurrence_profile
+
+    narrow = [_tiny_job(i, n_rows=40, seed=i) for i in range(20)]
+    wide_job = _tiny_job(100, n_rows=60, seed=100, min_support=0.1)
+    wide_job.use_generator_pruning = True  # generator pruning is mined per job
+    assert all(batchable(j, occurrence_profile(j.ids)) for j in narrow)
+    assert not batchable(wide_job, occurrence_profile(wide_job.ids))
+    calls, per_job = [], []
+    gate = threading.Event()
+
+    def batch_executor(jobs):
+        gate.wait(5)
+        calls.append([j.job_id for j in jobs])
+        if any(j.job_id == 13 for j in jobs):
+            raise RuntimeError("batch with job 13 failed")
+        return [_oracle_flat(j) + (1, 0) for j in jobs]
+
+    def executor(job):
+        per_job.append(job.job_id)
+        return _oracle_flat(job) + (1, 0)
+
+    srv = Server(workers=1, memory_budget_bytes=1 << 40, executor=executor, batch_workers=2, max_batch=8,
+                 batch_executor=batch_executor)
+    futs = [srv.submit(j, arrival_s=0.0) for j in narrow + [wide_job]]
+    gate.set()
+    results = {}
+    for j, f in zip(narrow + [wide_job], futs):
+        exc = f.exception(10)
+        results[j.job_id] = exc if exc is not None else f.result()
+    srv.close()
+    from et_miner.serve import result_digest
+
+    assert per_job == [100]
+    assert sorted(i for c in calls for i in c) == list(range(20))
+    assert all(len(c) <= 8 for c in calls) and max(len(c) for c in calls) > 1
+    failed = next(c for c in calls if 13 in c)
+    for j in narrow + [wide_job]:
+        r = results[j.job_id]
+        if j.job_id in failed:
+            assert isinstance(r, RuntimeError) and "job 13" in str(r)
+        else:
+            assert r.digest == result_digest(*_oracle_flat(j))
+
+
 def test_footprint_counts_frequent_columns():
     from et_miner.serve import Job
     from et_miner.serve.server import footprint_bytes
