@@ -83,5 +83,21 @@ struct Queue {
     int64_t active_after;  // jobs still active at its end (the host stops the rounds at 0)
 };
 
+// One job of the batched input layer: id range, occurrence histogram, bitvectors and K=1 counts of every job in
+// one launch per phase. Each phase's launch flattens the jobs' workgroups; pre (n_jobs + 1) is the exclusive
+// prefix of the workgroups over the jobs.
+struct InJob {
+    const int64_t* indptr;
+    const void* ids;
+    int64_t n_rows, nnz, lo, span;
+    int32_t id_bytes, n_cols;
+    int64_t pitch, n_words, slab_cols;
+    int32_t* hist;       // span bins (zeroed before the histogram launch)
+    const int32_t* lut;  // id - lo -> column, or -1
+    uint64_t* bv;        // (n_cols, pitch)
+    int64_t* mm;         // id range: [0] min, [1] max
+    int64_t* pc;         // K=1 counts (n_cols)
+};
+
 }  // namespace batch
 }  // namespace etm

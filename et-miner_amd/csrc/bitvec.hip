@@ -9,7 +9,9 @@
 
 #include <rocprim/device/device_radix_sort.hpp>
 
+#include "batch.h"
 #include "common.h"
+#include "internal.h"
 
 namespace {
 
@@ -40,15 +42,17 @@ struct IdVec<int32_t> {
     }
 };
 
-// Calls fn(id) for every id of ids[0..n), each thread keeping 4 vector loads in flight (grid-stride).
+// Calls fn(id) for every id of ids[0..n), each thread keeping 4 vector loads in flight, workgroup `blk` of
+// `nblk` striding over the ids.
 template <typename Id, class Fn>
-__device__ __forceinline__ void for_each_id(const Id* __restrict__ ids, int64_t n, Fn fn) {
+__device__ __forceinline__ void for_each_id_in(const Id* __restrict__ ids, int64_t n, int64_t blk, int64_t nblk,
+                                               Fn fn) {
     using IV = IdVec<Id>;
     constexpr int P = IV::kPer;
     const int64_t nv = ((uintptr_t)ids % 16 == 0) ? n / P : 0;
     const typename IV::V* vec = reinterpret_cast<const typename IV::V*>(ids);
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = nblk * blockDim.x;
+    int64_t i = blk * blockDim.x + threadIdx.x;
     for (; i + 3 * stride < nv; i += 4 * stride) {
         typename IV::V v[4];
 #pragma unroll
@@ -67,19 +71,32 @@ __device__ __forceinline__ void for_each_id(const Id* __restrict__ ids, int64_t 
 #pragma unroll
         for (int q = 0; q < P; ++q) fn(o[q]);
     }
-    for (int64_t j = nv * P + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) fn((int64_t)ids[j]);
+    for (int64_t j = nv * P + blk * blockDim.x + threadIdx.x; j < n; j += stride) fn((int64_t)ids[j]);
+}
+
+// The same over the whole grid.
+template <typename Id, class Fn>
+__device__ __forceinline__ void for_each_id(const Id* __restrict__ ids, int64_t n, Fn fn) {
+    for_each_id_in(ids, n, (int64_t)blockIdx.x, (int64_t)gridDim.x, fn);
 }
 
 // Range of the ids: each workgroup reduces its share and updates out[0] / out[1] only where it improves them
 // (the plain read first spares almost every atomic once an extreme has been seen).
+__device__ __forceinline__ void minmax_publish(int64_t lo, int64_t hi, int64_t* out);
+
 template <typename Id>
 __global__ __launch_bounds__(256) void k_ids_minmax(const Id* __restrict__ ids, int64_t n, int64_t* out) {
-    __shared__ int64_t lds_min[4], lds_max[4];
     int64_t lo = INT64_MAX, hi = INT64_MIN;
     for_each_id(ids, n, [&](int64_t v) {
         lo = v < lo ? v : lo;
         hi = v > hi ? v : hi;
     });
+    minmax_publish(lo, hi, out);
+}
+
+// The workgroup's extremes into out[0] / out[1] (atomics only where they improve them).
+__device__ __forceinline__ void minmax_publish(int64_t lo, int64_t hi, int64_t* out) {
+    __shared__ int64_t lds_min[4], lds_max[4];
     lo = etm::wave_min(lo);
     hi = etm::wave_max(hi);
     const int wid = threadIdx.x / 64;
@@ -144,17 +161,15 @@ __global__ __launch_bounds__(256) void k_hist_global(const Id* __restrict__ ids,
 constexpr int kTileWords = 8;
 constexpr int kTileCols = 1024;
 
+// Word tile `wt` x column slab `slab` of the builder, through the LDS tile `tile`.
 template <typename Id>
-__global__ __launch_bounds__(kBuildThreads) void k_bitvec_build_lds(const int64_t* __restrict__ indptr,
-                                                                    const Id* __restrict__ ids, int64_t n_rows,
-                                                                    int64_t lo, int64_t span,
-                                                                    const int32_t* __restrict__ lut, int32_t n_cols,
-                                                                    int64_t pitch, const int32_t* __restrict__ order,
-                                                                    int32_t slab_cols, uint64_t* __restrict__ out) {
-    // The slab's tile, sized at launch: slab_cols (at most kTileCols) columns x kTileWords words.
-    extern __shared__ unsigned long long tile[];
-    const int64_t w0 = (int64_t)blockIdx.x * kTileWords;
-    const int c0 = blockIdx.y * slab_cols;
+__device__ __forceinline__ void build_tile(const int64_t* __restrict__ indptr, const Id* __restrict__ ids,
+                                           int64_t n_rows, int64_t lo, int64_t span, const int32_t* __restrict__ lut,
+                                           int32_t n_cols, int64_t pitch, const int32_t* __restrict__ order,
+                                           int32_t slab_cols, uint64_t* __restrict__ out, int64_t wt, int slab,
+                                           unsigned long long* tile) {
+    const int64_t w0 = wt * kTileWords;
+    const int c0 = slab * slab_cols;
     const int nc = min(slab_cols, n_cols - c0);
     for (int t = threadIdx.x; t < nc * kTileWords; t += kBuildThreads) tile[t] = 0;
     __syncthreads();
@@ -194,6 +209,20 @@ __global__ __launch_bounds__(kBuildThreads) void k_bitvec_build_lds(const int64_
     }
 }
 
+template <typename Id>
+__global__ __launch_bounds__(kBuildThreads) void k_bitvec_build_lds(const int64_t* __restrict__ indptr,
+                                                                    const Id* __restrict__ ids, int64_t n_rows,
+                                                                    int64_t lo, int64_t span,
+                                                                    const int32_t* __restrict__ lut, int32_t n_cols,
+                                                                    int64_t pitch, const int32_t* __restrict__ order,
+                                                                    int32_t slab_cols, uint64_t* __restrict__ out) {
+    // The slab's tile, sized at launch: slab_cols (at most kTileCols) columns x kTileWords words.
+    extern __shared__ unsigned long long tile[];
+    build_tile(indptr, ids, n_rows, lo, span, lut, n_cols, pitch, order, slab_cols, out, (int64_t)blockIdx.x,
+               (int)blockIdx.y, tile);
+}
+
+
 // One workgroup per column: sum of popcounts over the column's words.
 __global__ __launch_bounds__(256) void k_colpop(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
                                                  int64_t* __restrict__ out) {
@@ -203,6 +232,107 @@ __global__ __launch_bounds__(256) void k_colpop(const uint64_t* __restrict__ bv,
     for (int64_t w = threadIdx.x; w < n_words; w += blockDim.x) s += __popcll(col[w]);
     s = etm::block_sum<256>(s, lds);
     if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// ── the input layer of a batch of jobs (batch.h InJob): one launch per phase over every job ──────────────
+
+using etm::batch::InJob;
+
+// The job of flattened workgroup b: pre[j] <= b < pre[j + 1].
+__device__ __forceinline__ int in_job_of(const int64_t* __restrict__ pre, int n_jobs, int64_t b) {
+    int lo = 0, hi = n_jobs;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_batch_minmax_init(const InJob* __restrict__ jobs, int n_jobs) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_jobs) return;
+    jobs[j].mm[0] = INT64_MAX;
+    jobs[j].mm[1] = INT64_MIN;
+}
+
+__global__ __launch_bounds__(256) void k_batch_minmax(const InJob* __restrict__ jobs, int n_jobs,
+                                                       const int64_t* __restrict__ pre) {
+    const int j = in_job_of(pre, n_jobs, blockIdx.x);
+    const InJob& J = jobs[j];
+    const int64_t blk = blockIdx.x - pre[j], nblk = pre[j + 1] - pre[j];
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    auto fn = [&](int64_t v) {
+        lo = v < lo ? v : lo;
+        hi = v > hi ? v : hi;
+    };
+    if (J.id_bytes == 8) for_each_id_in((const int64_t*)J.ids, J.nnz, blk, nblk, fn);
+    else for_each_id_in((const int32_t*)J.ids, J.nnz, blk, nblk, fn);
+    minmax_publish(lo, hi, J.mm);
+}
+
+// Histograms: spans of at most kLdsBins through an LDS copy per workgroup (as k_hist_lds), wider ones with global
+// atomics. The bins are zeroed beforehand.
+__global__ __launch_bounds__(256) void k_batch_hist(const InJob* __restrict__ jobs, int n_jobs,
+                                                     const int64_t* __restrict__ pre) {
+    __shared__ int32_t bins[kLdsBins];
+    const int j = in_job_of(pre, n_jobs, blockIdx.x);
+    const InJob& J = jobs[j];
+    const int64_t blk = blockIdx.x - pre[j], nblk = pre[j + 1] - pre[j];
+    const int64_t lo = J.lo, span = J.span;
+    int32_t* hist = J.hist;
+    if (span <= kLdsBins) {
+        for (int b = threadIdx.x; b < span; b += blockDim.x) bins[b] = 0;
+        __syncthreads();
+        auto fn = [&](int64_t id) {
+            const int64_t v = id - lo;
+            if (v >= 0 && v < span) atomicAdd(&bins[v], 1);
+        };
+        if (J.id_bytes == 8) for_each_id_in((const int64_t*)J.ids, J.nnz, blk, nblk, fn);
+        else for_each_id_in((const int32_t*)J.ids, J.nnz, blk, nblk, fn);
+        __syncthreads();
+        for (int b = threadIdx.x; b < span; b += blockDim.x)
+            if (bins[b]) atomicAdd(&hist[b], bins[b]);
+    } else {
+        auto fn = [&](int64_t id) {
+            const int64_t v = id - lo;
+            if (v >= 0 && v < span) atomicAdd(&hist[v], 1);
+        };
+        if (J.id_bytes == 8) for_each_id_in((const int64_t*)J.ids, J.nnz, blk, nblk, fn);
+        else for_each_id_in((const int32_t*)J.ids, J.nnz, blk, nblk, fn);
+    }
+}
+
+// Bitvectors: a job's workgroups are its (word tile, column slab) pairs, tile-major within a slab.
+__global__ __launch_bounds__(kBuildThreads) void k_batch_build(const InJob* __restrict__ jobs, int n_jobs,
+                                                                const int64_t* __restrict__ pre) {
+    extern __shared__ unsigned long long tile[];
+    const int j = in_job_of(pre, n_jobs, blockIdx.x);
+    const InJob& J = jobs[j];
+    const int64_t local = blockIdx.x - pre[j];
+    const int64_t tiles = (J.pitch + kTileWords - 1) / kTileWords;
+    const int64_t wt = local % tiles;
+    const int slab = (int)(local / tiles);
+    if (J.id_bytes == 8)
+        build_tile(J.indptr, (const int64_t*)J.ids, J.n_rows, J.lo, J.span, J.lut, J.n_cols, J.pitch, nullptr,
+                   (int32_t)J.slab_cols, J.bv, wt, slab, tile);
+    else
+        build_tile(J.indptr, (const int32_t*)J.ids, J.n_rows, J.lo, J.span, J.lut, J.n_cols, J.pitch, nullptr,
+                   (int32_t)J.slab_cols, J.bv, wt, slab, tile);
+}
+
+// K=1 counts: a workgroup per column of every job.
+__global__ __launch_bounds__(256) void k_batch_colpop(const InJob* __restrict__ jobs, int n_jobs,
+                                                       const int64_t* __restrict__ pre) {
+    __shared__ int64_t lds[8];
+    const int j = in_job_of(pre, n_jobs, blockIdx.x);
+    const InJob& J = jobs[j];
+    const int64_t c = blockIdx.x - pre[j];
+    const uint64_t* col = J.bv + c * J.pitch;
+    int64_t s = 0;
+    for (int64_t w = threadIdx.x; w < J.n_words; w += blockDim.x) s += __popcll(col[w]);
+    s = etm::block_sum<256>(s, lds);
+    if (threadIdx.x == 0) J.pc[c] = s;
 }
 
 // Row order key: bit 63 - r for each column of rank r < n_ranked the row holds (repeated ids set the same bit).
@@ -256,6 +386,58 @@ int grid_for(int64_t n, int per_block = 256, int cap = 8192) {
 }
 
 }  // namespace
+
+namespace etm {
+namespace detail {
+
+void batch_input_shape(etm::batch::InJob& j) {
+    const int64_t slabs = std::max<int64_t>(1, (j.n_cols + kTileCols - 1) / kTileCols);
+    j.slab_cols = (j.n_cols + slabs - 1) / slabs;
+}
+
+int64_t batch_input_blocks(int phase, const etm::batch::InJob& j) {
+    switch (phase) {
+        case 0:
+            return j.nnz > 0 ? grid_for(j.nnz, j.id_bytes == 4 ? 256 * 16 : 256 * 8, 1024) : 0;
+        case 1:
+            if (j.nnz == 0) return 0;
+            return j.span <= kLdsBins ? grid_for(j.nnz, (int)std::max<int64_t>(4 * j.span, 8192), j.span <= 2048 ? 1024 : 512)
+                                      : grid_for(j.nnz);
+        case 2:
+            return j.n_cols > 0 ? (j.pitch + kTileWords - 1) / kTileWords * ((j.n_cols + j.slab_cols - 1) / j.slab_cols)
+                                : 0;
+        default:
+            return j.n_cols;
+    }
+}
+
+hipError_t launch_batch_input(int phase, etm::batch::InJob* jobs, int n_jobs, const int64_t* pre, int64_t n_blocks,
+                              int64_t max_slab_cols, hipStream_t s) {
+    if (phase == 0) {
+        k_batch_minmax_init<<<(n_jobs + 255) / 256, 256, 0, s>>>(jobs, n_jobs);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (n_blocks <= 0) return hipSuccess;
+    if (n_blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
+    const unsigned g = (unsigned)n_blocks;
+    if (phase == 0) k_batch_minmax<<<g, 256, 0, s>>>(jobs, n_jobs, pre);
+    else if (phase == 1) k_batch_hist<<<g, 256, 0, s>>>(jobs, n_jobs, pre);
+    else if (phase == 2)
+        k_batch_build<<<g, kBuildThreads, (size_t)max_slab_cols * kTileWords * sizeof(unsigned long long), s>>>(
+            jobs, n_jobs, pre);
+    else k_batch_colpop<<<g, 256, 0, s>>>(jobs, n_jobs, pre);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_fill_i32<<<grid_for(n), 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace detail
+}  // namespace etm
 
 extern "C" int etm_ids_minmax(const void* d_ids, int id_bytes, int64_t n, int64_t* d_out, void* stream) {
     ETM_ARG(d_ids != nullptr && d_out != nullptr, "etm_ids_minmax: null pointer");

@@ -79,5 +79,14 @@ hipError_t launch_batch_count(const etm::batch::Job* jobs, int n_jobs, etm::batc
 hipError_t launch_batch_compact(etm::batch::Job* jobs, int n_jobs, hipStream_t s);
 hipError_t launch_batch_groups(etm::batch::Job* jobs, int n_jobs, etm::batch::Queue* q, bool init, hipStream_t s);
 
+// The batched input layer (bitvec.hip): per phase (0 id ranges, 1 histograms, 2 bitvectors, 3 K=1 counts) one
+// launch over n_jobs InJob records; pre holds the exclusive prefix of batch_input_blocks over the jobs and
+// n_blocks its total. batch_input_shape sets a job's column slab width (phase 2 needs max_slab_cols, the largest).
+void batch_input_shape(etm::batch::InJob& j);
+int64_t batch_input_blocks(int phase, const etm::batch::InJob& j);
+hipError_t launch_batch_input(int phase, etm::batch::InJob* jobs, int n_jobs, const int64_t* pre, int64_t n_blocks,
+                              int64_t max_slab_cols, hipStream_t s);
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+
 }  // namespace detail
 }  // namespace etm

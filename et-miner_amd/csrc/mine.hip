@@ -1091,20 +1091,50 @@ void mine_batch(const etm_batch_job* in, int n, int64_t span_limit, const etm_mi
     bt::Queue* d_q = a.get<bt::Queue>(1);
     std::vector<BatchHost> hj((size_t)n);
     auto has_data = [&](int i) { return in[i].nnz > 0 && in[i].n_rows > 0; };
-    // Id ranges.
+    // The input layer, one launch per phase over every job (etm::batch::InJob), one host sync per phase.
+    std::vector<bt::InJob> ij((size_t)n);
+    bt::InJob* d_in = a.get<bt::InJob>(n);
+    int64_t* d_pre = a.get<int64_t>(4 * ((int64_t)n + 1));
     int64_t* d_mm = a.get<int64_t>(2 * (int64_t)n);
-    for (int i = 0; i < n; ++i)
-        if (has_data(i)) check_rc(etm_ids_minmax(in[i].d_ids, in[i].id_bytes, in[i].nnz, d_mm + 2 * i, s), "etm_ids_minmax");
+    const size_t table_bytes = (size_t)n * sizeof(bt::InJob);
+    const size_t pre_bytes = ((size_t)n + 1) * 8;
+    // Exclusive prefix of phase p's workgroups over the jobs into pre; returns the total.
+    auto prefix = [&](int p, int64_t* pre) {
+        int64_t t = 0;
+        for (int i = 0; i < n; ++i) {
+            pre[i] = t;
+            t += has_data(i) ? etm::detail::batch_input_blocks(p, ij[(size_t)i]) : 0;
+        }
+        pre[n] = t;
+        return t;
+    };
+    for (int i = 0; i < n; ++i) {
+        bt::InJob& J = ij[(size_t)i];
+        J.indptr = in[i].d_indptr;
+        J.ids = in[i].d_ids;
+        J.n_rows = in[i].n_rows;
+        J.nnz = has_data(i) ? in[i].nnz : 0;
+        J.id_bytes = in[i].id_bytes;
+        J.mm = d_mm + 2 * i;
+    }
+    // Id ranges.
     int64_t total_span = 0;
     {
-        int64_t* h = (int64_t*)batch_pinned((size_t)n * 16);
-        check(hipMemcpyAsync(h, d_mm, (size_t)n * 16, hipMemcpyDeviceToHost, s), "readback");
+        char* hp = (char*)batch_pinned(table_bytes + pre_bytes + 16 * (size_t)n);
+        int64_t* h_pre = (int64_t*)(hp + table_bytes);
+        int64_t* h_mm = h_pre + n + 1;
+        std::memcpy(hp, ij.data(), table_bytes);
+        const int64_t blocks = prefix(0, h_pre);
+        check(hipMemcpyAsync(d_in, hp, table_bytes, hipMemcpyHostToDevice, s), "upload input records");
+        check(hipMemcpyAsync(d_pre, h_pre, pre_bytes, hipMemcpyHostToDevice, s), "upload input records");
+        check(etm::detail::launch_batch_input(0, d_in, n, d_pre, blocks, 0, s), "batch id ranges");
+        check(hipMemcpyAsync(h_mm, d_mm, (size_t)n * 16, hipMemcpyDeviceToHost, s), "readback");
         wait_stream(s);
         for (int i = 0; i < n; ++i) {
             if (!has_data(i)) continue;
             BatchHost& b = hj[(size_t)i];
-            b.lo = h[2 * i];
-            b.span = h[2 * i + 1] - h[2 * i] + 1;
+            b.lo = h_mm[2 * i];
+            b.span = h_mm[2 * i + 1] - h_mm[2 * i] + 1;
             if (b.span > span_limit)
                 throw Error(ETM_EINVAL, "job " + std::to_string(i) + ": item ids span " + std::to_string(b.span) +
                                             " values: code them to dense ids first");
@@ -1112,20 +1142,30 @@ void mine_batch(const etm_batch_job* in, int n, int64_t span_limit, const etm_mi
             total_span += b.span;
         }
     }
-    // Histograms down, LUTs up; the same pinned block later takes the K=1 counts (at most one per id).
+    // Histograms down, LUTs up, bitvectors and K=1 counts. The pinned block: records, three prefixes, histograms,
+    // LUTs, K=1 counts (at most one per id).
     int32_t* d_hist = a.get<int32_t>(total_span);
     int32_t* d_lut = a.get<int32_t>(total_span);
+    char* hp = (char*)batch_pinned(table_bytes + 3 * pre_bytes + 16 * (size_t)std::max<int64_t>(total_span, 1));
+    int64_t* h_pre1 = (int64_t*)(hp + table_bytes);
+    int64_t* h_pre2 = h_pre1 + n + 1;
+    int64_t* h_pre3 = h_pre2 + n + 1;
+    int32_t* h_hist = (int32_t*)(h_pre3 + n + 1);
+    int32_t* h_lut = h_hist + total_span;
+    int64_t* h_pc = (int64_t*)(h_lut + total_span);
     for (int i = 0; i < n; ++i) {
-        const BatchHost& b = hj[(size_t)i];
-        if (has_data(i))
-            check_rc(etm_ids_hist(in[i].d_ids, in[i].id_bytes, in[i].nnz, b.lo, b.span, d_hist + b.hist_off, s),
-                     "etm_ids_hist");
+        if (!has_data(i)) continue;
+        ij[(size_t)i].lo = hj[(size_t)i].lo;
+        ij[(size_t)i].span = hj[(size_t)i].span;
+        ij[(size_t)i].hist = d_hist + hj[(size_t)i].hist_off;
     }
-    int32_t* h = (int32_t*)batch_pinned((size_t)std::max<int64_t>(total_span, 1) * 16);
-    int32_t* h_hist = h;
-    int32_t* h_lut = h + total_span;
-    int64_t* h_pc = (int64_t*)(h + 2 * total_span);
     if (total_span) {
+        std::memcpy(hp, ij.data(), table_bytes);
+        const int64_t blocks = prefix(1, h_pre1);
+        check(etm::detail::launch_fill_i32(d_hist, total_span, 0, s), "zero histograms");
+        check(hipMemcpyAsync(d_in, hp, table_bytes, hipMemcpyHostToDevice, s), "upload input records");
+        check(hipMemcpyAsync(d_pre + (n + 1), h_pre1, pre_bytes, hipMemcpyHostToDevice, s), "upload input records");
+        check(etm::detail::launch_batch_input(1, d_in, n, d_pre + (n + 1), blocks, 0, s), "batch histograms");
         check(hipMemcpyAsync(h_hist, d_hist, (size_t)total_span * 4, hipMemcpyDeviceToHost, s), "readback");
         wait_stream(s);
     }
@@ -1146,21 +1186,38 @@ void mine_batch(const etm_batch_job* in, int n, int64_t span_limit, const etm_mi
         b.pc_off = total_cols;
         total_cols += b.n_cols;
     }
-    if (total_span) check(hipMemcpyAsync(d_lut, h_lut, (size_t)total_span * 4, hipMemcpyHostToDevice, s), "upload lut");
-    // Bitvectors and K=1 counts.
     int64_t* d_pc = a.get<int64_t>(total_cols);
+    int64_t max_slab = 1;
     for (int i = 0; i < n; ++i) {
         BatchHost& b = hj[(size_t)i];
-        if (!has_data(i) || b.n_cols == 0) continue;
+        bt::InJob& J = ij[(size_t)i];
+        if (!has_data(i) || b.n_cols == 0) {
+            J.n_cols = 0;
+            continue;
+        }
         b.n_words = (in[i].n_rows + 63) / 64;
         b.pitch = std::max<int64_t>(64, (b.n_words + 63) / 64 * 64);
         b.bv = a.get<uint64_t>((int64_t)b.n_cols * b.pitch);
-        check_rc(etm_bitvec_build(in[i].d_indptr, in[i].d_ids, in[i].id_bytes, in[i].n_rows, b.lo, b.span,
-                                  d_lut + b.hist_off, b.n_cols, b.pitch, nullptr, b.bv, s),
-                 "etm_bitvec_build");
-        check_rc(etm_column_popcounts(b.bv, b.n_cols, b.pitch, b.n_words, d_pc + b.pc_off, s), "etm_column_popcounts");
+        J.n_cols = b.n_cols;
+        J.n_words = b.n_words;
+        J.pitch = b.pitch;
+        J.lut = d_lut + b.hist_off;
+        J.bv = b.bv;
+        J.pc = d_pc + b.pc_off;
+        etm::detail::batch_input_shape(J);
+        max_slab = std::max(max_slab, J.slab_cols);
     }
     if (total_cols) {
+        std::memcpy(hp, ij.data(), table_bytes);
+        const int64_t blocks2 = prefix(2, h_pre2);
+        const int64_t blocks3 = prefix(3, h_pre3);
+        check(hipMemcpyAsync(d_lut, h_lut, (size_t)total_span * 4, hipMemcpyHostToDevice, s), "upload lut");
+        check(hipMemcpyAsync(d_in, hp, table_bytes, hipMemcpyHostToDevice, s), "upload input records");
+        check(hipMemcpyAsync(d_pre + 2 * (n + 1), h_pre2, 2 * pre_bytes, hipMemcpyHostToDevice, s),
+              "upload input records");
+        check(etm::detail::launch_batch_input(2, d_in, n, d_pre + 2 * (n + 1), blocks2, max_slab, s),
+              "batch bitvectors");
+        check(etm::detail::launch_batch_input(3, d_in, n, d_pre + 3 * (n + 1), blocks3, 0, s), "batch K=1 counts");
         check(hipMemcpyAsync(h_pc, d_pc, (size_t)total_cols * 8, hipMemcpyDeviceToHost, s), "readback");
         wait_stream(s);
     }
