@@ -31,7 +31,7 @@ MFMA Gram), each checked against the oracle's committed per-level digests.
 served: the served_jobs stream (512 independent jobs, seed 4: transactions log-uniform in 10K-2M, vocabulary
 500-20K, Zipf 0.9-1.3, a coin between 1 + Poisson(4|6|9) and lognormal(2.3, 0.6) baskets, min_support
 log-uniform in 0.001-0.05, max_length 3-10), every dataset resident in HBM before timing. The plain form mines
-the jobs one after another on one stream; the served form runs them through the request loop (8 per-job workers,
+the jobs one after another on one stream; the served form runs them through the request loop (10 per-job workers,
 one stream each, footprint admission, kernel settings tuned per shape bucket beforehand; 2 batch workers that mine
 the narrow jobs, up to 64 at a time, in batched calls whose levels advance in shared rounds) after one warm-up
 pass.
@@ -119,7 +119,7 @@ def main(argv=None) -> int:
     ap.add_argument("--row-order", default="auto", choices=["auto", "keep", "lex"],
                     help="bitvector row order of the native loop (counts never depend on it)")
     ap.add_argument("--served-jobs", type=int, default=512, help="jobs of the served stream (0: skip that leg)")
-    ap.add_argument("--served-workers", type=int, default=8, help="per-job workers of the served form")
+    ap.add_argument("--served-workers", type=int, default=10, help="per-job workers of the served form")
     ap.add_argument("--served-batch-workers", type=int, default=2,
                     help="batch workers of the served form (narrow jobs mined in batched calls; 0: none)")
     ap.add_argument("--served-max-batch", type=int, default=64, help="most jobs per batched call")

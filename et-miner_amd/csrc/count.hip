@@ -1483,10 +1483,17 @@ __global__ __launch_bounds__(256) void k_batch_count_small(const Job* __restrict
     const int lane = threadIdx.x & 63;
     const int64_t* pre = q->pre[C];
     const int64_t total = pre[n_jobs];
-    for (;;) {
-        unsigned long long got = 0;
-        if (lane == 0) got = atomicAdd(&q->head[C], 1ull);
-        const int64_t u = uniform64((int64_t)got);
+    // The first unit of every wave is its own index; the queue's head starts past them (one device-scope atomic
+    // per further unit, none at all when the round has no more units than waves).
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    int64_t u = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    for (;; u = -1) {
+        if (u < 0) {
+            if (total <= waves) return;
+            unsigned long long got = 0;
+            if (lane == 0) got = atomicAdd(&q->head[C], 1ull);
+            u = uniform64((int64_t)got + waves);
+        }
         if (u >= total) return;
         const int j = batch_job_of(pre, n_jobs, u);
         const Job& J = jobs[j];
@@ -1510,11 +1517,17 @@ __global__ __launch_bounds__(256) void k_batch_count_tiled(const Job* __restrict
     __shared__ unsigned long long s_u;
     const int64_t* pre = q->pre[etm::batch::kTiledQueue];
     const int64_t total = pre[n_jobs];
-    for (;;) {
-        if (threadIdx.x == 0) s_u = atomicAdd(&q->head[etm::batch::kTiledQueue], 1ull);
-        __syncthreads();
-        const int64_t u = uniform64((int64_t)s_u);
-        __syncthreads();
+    // As the small-group queues: the workgroup's own index first, the atomic head past the grid after that.
+    const int64_t blocks = gridDim.x;
+    for (int64_t first = 1;; first = 0) {
+        int64_t u = blockIdx.x;
+        if (!first) {
+            if (total <= blocks) return;
+            if (threadIdx.x == 0) s_u = atomicAdd(&q->head[etm::batch::kTiledQueue], 1ull);
+            __syncthreads();
+            u = uniform64((int64_t)s_u) + blocks;
+            __syncthreads();  // also: every thread is past the previous unit before this one's set-up
+        }
         if (u >= total) return;
         const int j = batch_job_of(pre, n_jobs, u);
         const Job& J = jobs[j];
