@@ -274,12 +274,15 @@ double ms_since(Clock::time_point t) {
 
 // Hands a finished level to the caller's callback (when there is one); a nonzero return stops the call.
 void report(const etm_mine_opts& o, const etm_levels& out, const etm_level_stat& st, const int32_t* h_rows,
-            const int32_t* h_counts, int64_t device_bytes, int64_t host_bytes, bool next_level) {
+            const int32_t* h_counts, int64_t device_bytes, int64_t host_bytes, bool next_level,
+            const int32_t* d_rows = nullptr, const int32_t* d_counts = nullptr) {
     if (o.level_cb == nullptr) return;
     etm_level_info info{};
     info.stat = st;
     info.h_rows = h_rows;
     info.h_counts = h_counts;
+    info.d_rows = d_rows;
+    info.d_counts = d_counts;
     info.h_col_to_item = out.col_to_item.empty() ? nullptr : out.col_to_item.data();
     info.n_cols = (int64_t)out.col_counts.size();
     info.device_bytes = device_bytes;
@@ -366,6 +369,8 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
     const bool prune = o.prune_non_free != 0;
     const bool track_free = prune || o.infer_counts != 0;
     const int64_t max_len = o.max_length > 0 ? std::min<int64_t>(o.max_length, n_cols) : n_cols;
+    if (o.emit_levels < ETM_EMIT_NONE || o.emit_levels > ETM_EMIT_DEVICE)
+        throw Error(ETM_EINVAL, "emit_levels must be one of ETM_EMIT_*");
     if (o.small_max != 0 && o.small_max != 4 && o.small_max != 8 && o.small_max != 16)
         throw Error(ETM_EINVAL, "small_max must be 0, 4, 8 or 16, got " + std::to_string(o.small_max));
     if (o.k3_schedule < ETM_SCHEDULE_AUTO || o.k3_schedule > ETM_SCHEDULE_LIVE)
@@ -709,7 +714,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
                             : (status == nullptr ? st.bytes_alg : 0);
         const int32_t* h_rows = nullptr;
         const int32_t* h_counts = nullptr;
-        if (emit && cur.n > 0 && o.level_cb) {
+        if (o.emit_levels == ETM_EMIT_HOST && cur.n > 0 && o.level_cb) {
             int32_t* buf = staging(cur.n * (k + 1));
             check(hipMemcpyAsync(buf, cur.rows, (size_t)(cur.n * k) * 4, hipMemcpyDeviceToHost, s), "level out");
             check(hipMemcpyAsync(buf + cur.n * k, cur.counts, (size_t)cur.n * 4, hipMemcpyDeviceToHost, s),
@@ -722,7 +727,8 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         out.stats.push_back(st);
         levels.push_back(emit ? Pending{nullptr, nullptr, 0, k} : Pending{cur.rows, cur.counts, cur.n, k});
         if (!emit) host_bytes += cur.n * (int64_t)(k + 1) * 4;
-        report(o, out, st, h_rows, h_counts, a.bytes(), host_bytes, k + 1 <= max_len && cur.n >= k + 1);
+        report(o, out, st, h_rows, h_counts, a.bytes(), host_bytes, k + 1 <= max_len && cur.n >= k + 1,
+               cur.n > 0 ? cur.rows : nullptr, cur.n > 0 ? cur.counts : nullptr);
         t_level = Clock::now();
         if (cur.n == 0) break;
         if (early) {

@@ -17,7 +17,7 @@ from ctypes import POINTER, c_char_p, c_int, c_int32, c_int64, c_size_t, c_void_
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libetm.so"
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 ETM_OK, ETM_EINVAL, ETM_ENOMEM, ETM_EHIP, ETM_EOVERFLOW, ETM_ENODEV, ETM_ECANCELED = 0, 1, 2, 3, 4, 5, 6
 CAND_COUNT, CAND_SKIP, CAND_INFER = 0, 1, 2
@@ -42,6 +42,8 @@ def deduction_setting() -> int:
         raise ValueError(f"ET_MINER_DEDUCTION must be 0 or 1, got {v!r}")
     return DEDUCTION_OFF if v == "0" else DEDUCTION_AUTO
 LAYOUT_LEX, LAYOUT_TRI = 0, 1
+#: etm_mine_opts.emit_levels (ETM_EMIT_*).
+EMIT_NONE, EMIT_HOST, EMIT_DEVICE = 0, 1, 2
 KERNEL_AUTO, KERNEL_VALU, KERNEL_MFMA, KERNEL_ROWS = 0, 1, 2, 3
 K2_KERNELS = {"auto": KERNEL_AUTO, "valu": KERNEL_VALU, "mfma": KERNEL_MFMA, "rows": KERNEL_ROWS}
 K3_KERNELS = {"auto": KERNEL_AUTO, "valu": KERNEL_VALU, "rows": KERNEL_ROWS}
@@ -54,6 +56,7 @@ SIGNATURES: dict = {
     "etm_abi_version": (c_int, []),
     "etm_last_error": (c_int, [c_char_p, c_size_t]),
     "etm_device_count": (c_int, [POINTER(c_int)]),
+    "etm_upload": (c_int, [P, c_int64, P, P]),
     "etm_ids_minmax": (c_int, [P, c_int, c_int64, P, P]),
     "etm_ids_hist": (c_int, [P, c_int, c_int64, c_int64, c_int64, P, P]),
     "etm_bitvec_build": (c_int, [P, P, c_int, c_int64, c_int64, c_int64, P, c_int32, c_int64, P, P, P]),
@@ -111,7 +114,7 @@ class LevelInfo(ctypes.Structure):
 
     _fields_ = [("stat", LevelStat), ("h_rows", POINTER(c_int32)), ("h_counts", POINTER(c_int32)),
                 ("h_col_to_item", POINTER(c_int64)), ("n_cols", c_int64), ("device_bytes", c_int64), ("host_bytes", c_int64), ("next_level", c_int32),
-                ("reserved", c_int32)]
+                ("reserved", c_int32), ("d_rows", c_void_p), ("d_counts", c_void_p)]
 
 
 #: etm_level_cb: int (*)(const etm_level_info*, void*)

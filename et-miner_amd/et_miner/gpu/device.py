@@ -63,18 +63,47 @@ def zeros(n, dtype: str):
     return t.zeros(shape, dtype=getattr(t, dtype), device=device())
 
 
+#: Arrays of at least this many bytes go through ``etm_upload`` (chunked pinned staging, parallel host copies).
+UPLOAD_CHUNKED_MIN = 4 << 20
+
+
 def to_device(a: np.ndarray, pinned: bool = True):
     """A host array copied to the device (through pinned memory, asynchronously on the current stream).
 
-    uint64 arrays (bitvector words) travel as int64 tensors: the kernels read the same bits."""
+    Large arrays are staged chunk by chunk through libetm's pinned ring (``etm_upload``): no pinned copy of the
+    whole array is made, and the host copy of each chunk overlaps the DMA of the previous one. uint64 arrays
+    (bitvector words) travel as int64 tensors: the kernels read the same bits."""
     t = torch()
     a = np.ascontiguousarray(a)
     if a.dtype == np.uint64:
         a = a.view(np.int64)
+    if pinned and a.nbytes >= UPLOAD_CHUNKED_MIN:
+        from .. import _lib
+
+        out = t.empty(a.shape, dtype=t.from_numpy(a[:0].reshape(-1)).dtype, device=device())
+        _lib.call("etm_upload", a.ctypes.data, a.nbytes, out.data_ptr(), stream_ptr())
+        return out
     host = t.from_numpy(a)
     if pinned and host.numel() > 0:
         host = host.pin_memory()
     return host.to(device(), non_blocking=pinned)
+
+
+class _RawDevice:
+    """A device range described by ``__cuda_array_interface__`` (for torch.as_tensor)."""
+
+    def __init__(self, p: int, n: int, typestr: str):
+        self.__cuda_array_interface__ = {"shape": (int(n),), "typestr": typestr, "data": (int(p), False),
+                                         "version": 2, "strides": None}
+
+
+def copy_from_pointer(p: int, n: int, dtype: str = "int32"):
+    """A new device tensor holding the n elements at device address p, copied on the current stream."""
+    t = torch()
+    typestr = {"int32": "<i4", "int64": "<i8", "uint8": "|u1"}[dtype]
+    if n == 0:
+        return t.empty(0, dtype=getattr(t, dtype), device=device())
+    return t.as_tensor(_RawDevice(p, n, typestr), device=device()).clone()
 
 
 def to_host(x) -> np.ndarray:

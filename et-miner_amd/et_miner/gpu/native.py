@@ -56,14 +56,17 @@ class LevelReport:
     host_bytes: int
     next_level: bool
     col_to_item: np.ndarray | None = None  # the item id of each column (etm_mine_csr)
+    d_rows: int | None = None  # with emit_levels="device", levels K >= 2: device pointers, valid during the call
+    d_counts: int | None = None
 
 
 class _Callback:
     """The ctypes level callback around a Python function of a LevelReport. An exception it raises stops the
     mining call (the callback returns nonzero) and is re-raised by ``run``."""
 
-    def __init__(self, fn, emit: bool):
-        self.fn, self.emit, self.error = fn, emit, None
+    def __init__(self, fn, emit):
+        self.mode = (_lib.EMIT_DEVICE if emit == "device" else _lib.EMIT_HOST if emit else _lib.EMIT_NONE)
+        self.fn, self.emit, self.error = fn, self.mode == _lib.EMIT_HOST, None
         self.c = _lib.LEVEL_CB(self._call)
 
     def _call(self, info_p, _user) -> int:
@@ -71,7 +74,11 @@ class _Callback:
             info = info_p.contents
             st = _stats_of(info.stat)
             rows = counts = None
-            if self.emit and st.n_frequent:
+            if self.mode == _lib.EMIT_DEVICE and st.k == 1 and st.n_frequent:
+                n = st.n_frequent
+                rows = np.ctypeslib.as_array(info.h_rows, shape=(n,)).reshape(n, 1).copy()
+                counts = np.ctypeslib.as_array(info.h_counts, shape=(n,)).astype(np.int64)
+            elif self.emit and st.n_frequent:
                 n, k = st.n_frequent, st.k
                 rows = np.ctypeslib.as_array(info.h_rows, shape=(n * k,)).reshape(n, k).copy()
                 counts = np.ctypeslib.as_array(info.h_counts, shape=(n,)).astype(np.int64)
@@ -81,7 +88,7 @@ class _Callback:
             if info.h_col_to_item and info.n_cols:
                 c2i = np.ctypeslib.as_array(info.h_col_to_item, shape=(info.n_cols,)).copy()
             self.fn(LevelReport(st, rows, counts, int(info.device_bytes), int(info.host_bytes),
-                                bool(info.next_level), c2i))
+                                bool(info.next_level), c2i, info.d_rows, info.d_counts))
             return 0
         except BaseException as e:  # noqa: BLE001 - handed back to the caller after the C call returns
             self.error = e
@@ -114,7 +121,7 @@ def _opts(min_count, max_length, prune, infer, time_kernels, k2_kernel="auto", k
                          _lib.deduction_setting(), _lib.ROW_ORDERS[row_order], int(bool(live_bytes)), 0,
                          _lib.K3_SCHEDULES[k3_schedule], int(small_max))
     if callback is not None:
-        opts.emit_levels = int(callback.emit)
+        opts.emit_levels = callback.mode
         opts.level_cb = callback.c
     return opts
 
@@ -168,7 +175,10 @@ def mine_csr_device(indptr_d, ids_d, n_rows: int, min_count: int, max_length=Non
     their most frequent items, with the tile skip), "keep", or "auto" (keeps the rows). ``on_level(report)`` is
     called with a LevelReport as each level completes (K=1 included); with ``emit_levels`` the report carries the
     level's rows and counts and the result keeps no itemsets (host memory holds one level at a time). An
-    exception raised by ``on_level`` stops mining and propagates. ``live_bytes`` fills ``LevelStats.bytes_live``.
+    exception raised by ``on_level`` stops mining and propagates. With ``emit_levels="device"`` the levels K >= 2
+    stay in device memory: the report carries ``d_rows`` / ``d_counts`` pointers, valid during the callback (work
+    it enqueues on the current stream is ordered before any reuse), and no level is copied to the host; K=1 comes
+    as host arrays. ``live_bytes`` fills ``LevelStats.bytes_live``.
     ``k3_schedule`` ("auto", "all", "live") and ``small_max`` (0, 4, 8, 16) choose how levels K >= 3 launch their
     counting (etm.h ETM_SCHEDULE_*, etm_mine_opts.small_max); neither changes a count."""
     torch = dev.torch()

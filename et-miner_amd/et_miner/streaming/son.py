@@ -6,9 +6,10 @@ local counts (``known``) and the slack (local min_count - 1) of the chunks that 
 the global min_count and is the exact count where no slack is left (``son.py:402-421``). Pass 2 counts the
 remaining candidates over every chunk with the batched itemset kernel, summing the chunks on the device.
 
-Transfers: each chunk's CSR is staged in pinned host buffers and copied with ``hipMemcpyAsync`` on a side
-stream; the copy of chunk i+1 is issued before chunk i is counted, so transfer overlaps counting. Only the
-CSR travels; bitvectors are built on the device.
+Transfers: each chunk's CSR is staged through pinned host buffers and copied with ``hipMemcpyAsync`` on a side
+stream (``etm_upload``); the copy of chunk i+1 is issued before chunk i is counted, so transfer overlaps
+counting. Only the CSR travels; bitvectors are built on the device, and pass 1's local levels stay on the device
+from the mining call to the union.
 """
 
 from __future__ import annotations
@@ -33,12 +34,13 @@ def _estimate_chunk_size_from_memory(memory_budget_gb: float, n_items_estimate: 
 
 
 class _Uploader:
-    """Double-buffered pinned staging of CSR chunks, copied on a side stream by a helper thread.
+    """CSR chunks copied to the device on a side stream by a helper thread.
 
-    ``issue(i)`` hands chunk i to the helper, which waits until the pinned buffer i % 2 is free (the copy of
-    chunk i-2 out of it has completed), stages the chunk's rows into it and enqueues the host-to-device copy
-    on the side stream. ``take(i)`` makes the current stream wait for that copy and issues chunk i+1, so the
-    staging and the copy of the next chunk run while the current one is counted.
+    ``issue(i)`` hands chunk i to the helper, which copies the chunk's rows with ``etm_upload`` (32 MiB pieces
+    staged through the helper's ring of pinned buffers by parallel host copies, each piece's DMA overlapping the
+    staging of the next) and rebases its offsets on the device. ``take(i)`` makes the current stream wait for
+    that copy and issues chunk i+1, so the staging and the copy of the next chunk run while the current one is
+    counted.
     """
 
     def __init__(self, indptr: np.ndarray, ids: np.ndarray, bounds: list):
@@ -50,34 +52,27 @@ class _Uploader:
         self.dev = dev
         self.device = dev.device()
         self.indptr, self.ids, self.bounds = indptr, ids, bounds
-        t = self.torch
-        max_rows = max(b - a for a, b in bounds)
-        max_nnz = max(int(indptr[b] - indptr[a]) for a, b in bounds)
-        self.stream = t.cuda.Stream(device=self.device)
-        self.host = [(t.empty(max_rows + 1, dtype=t.int64).pin_memory(),
-                      t.empty(max(1, max_nnz), dtype=t.int64).pin_memory()) for _ in range(2)]
-        self.buffer_done = [None, None]
+        self.stream = self.torch.cuda.Stream(device=self.device)
         self.pool = ThreadPoolExecutor(1)
         self.pending: dict = {}
 
     def _stage(self, i: int):
+        from .. import _lib
+
         t = self.torch
         a, b = self.bounds[i]
-        slot = i % 2
-        if self.buffer_done[slot] is not None:
-            self.buffer_done[slot].synchronize()
-        hp, hi = self.host[slot]
         lo, hi_e = int(self.indptr[a]), int(self.indptr[b])
         n, nnz = b - a, hi_e - lo
-        hp[: n + 1].numpy()[:] = self.indptr[a : b + 1] - lo
-        hi[:nnz].numpy()[:] = self.ids[lo:hi_e]
         with t.cuda.device(self.device), t.cuda.stream(self.stream):
-            d_ptr = hp[: n + 1].to(self.device, non_blocking=True)
-            d_ids = hi[:nnz].to(self.device, non_blocking=True)
+            d_ptr = t.empty(n + 1, dtype=t.int64, device=self.device)
+            d_ids = t.empty(max(1, nnz), dtype=t.int64, device=self.device)
+            h = self.stream.cuda_stream
+            _lib.call("etm_upload", self.indptr[a:].ctypes.data, (n + 1) * 8, d_ptr.data_ptr(), h)
+            _lib.call("etm_upload", self.ids[lo:].ctypes.data, nnz * 8, d_ids.data_ptr(), h)
+            d_ptr -= lo
             ev = t.cuda.Event()
             ev.record(self.stream)
-        self.buffer_done[slot] = ev
-        return d_ptr, d_ids, ev, n
+        return d_ptr, d_ids[:nnz], ev, n
 
     def issue(self, i: int) -> None:
         if 0 <= i < len(self.bounds) and i not in self.pending:
@@ -99,14 +94,29 @@ class _Uploader:
 
 
 def _local_levels(d_ptr, d_ids, n_rows: int, local_support: float, max_length, infer: bool):
-    """One chunk mined at the local threshold on the native level loop: (its column items, levels)."""
+    """One chunk mined at the local threshold on the native level loop: (its column items on the device, levels
+    [(k, n, device column-id rows (n * k int32), device counts (n int32))]). Levels K >= 2 never leave the
+    device: each is copied out of the call's arena as it completes."""
+    from ..gpu import device as dev
     from ..gpu.native import mine_csr_device
 
+    levels = []
+
+    def keep(rep):
+        n, k = rep.stats.n_frequent, rep.stats.k
+        if n == 0:
+            return
+        if k == 1:
+            levels.append((1, n, dev.to_device(np.ascontiguousarray(rep.rows.ravel(), dtype=np.int32)),
+                           dev.to_device(rep.counts.astype(np.int32))))
+        else:
+            levels.append((k, n, dev.copy_from_pointer(rep.d_rows, n * k), dev.copy_from_pointer(rep.d_counts, n)))
+
     res = mine_csr_device(d_ptr, d_ids, n_rows, _mining_min_count(local_support, n_rows), max_length,
-                          infer_counts=infer)
-    if not res.levels or len(res.col_to_item) == 0:
+                          infer_counts=infer, on_level=keep, emit_levels="device", levels=False)
+    if not levels or len(res.col_to_item) == 0:
         return None
-    return res.col_to_item, res.levels
+    return res.col_to_item, levels
 
 
 def apriori_streaming(
@@ -194,11 +204,10 @@ def son_csr(indptr, ids, min_support, max_length, chunk_size, local_support_fact
         if local is not None:
             col_items, levels = local
             chunk_items[ci] = dev.to_device(np.ascontiguousarray(col_items, dtype=np.int64))
-            for rows, counts in levels:
-                if len(rows):
-                    pending.setdefault(rows.shape[1], []).append((ci, rows, counts))
+            for k, n_k, d_rows, d_counts in levels:
+                pending.setdefault(k, []).append((ci, n_k, d_rows, d_counts))
         if progress_callback:
-            progress_callback("pass1", ci, len(bounds), {"candidates": sum(len(p[1]) for v in pending.values()
+            progress_callback("pass1", ci, len(bounds), {"candidates": sum(p[1] for v in pending.values()
                                                                            for p in v)})
     up.close()
     if not pending:
@@ -260,12 +269,13 @@ def son_csr(indptr, ids, min_support, max_length, chunk_size, local_support_fact
 
 def _device_union(parts: list, k: int, slacks: list, chunk_items: dict) -> tuple:
     """(item-id rows int64 (u, k), summed local counts, summed slack) of the distinct itemsets of length k
-    over the chunks' levels ``parts`` = [(chunk, column-id rows, counts)], through ``etm_union_insert``."""
+    over the chunks' device levels ``parts`` = [(chunk, rows, column-id rows int32, counts int32)], through
+    ``etm_union_insert``."""
     from .. import _lib
     from ..gpu import device as dev
 
     torch = dev.torch()
-    cap = sum(len(p[1]) for p in parts)
+    cap = sum(p[1] for p in parts)
     table_size = 1 << max(1, int(2 * cap - 1).bit_length())
     table = torch.full((table_size,), -1, dtype=torch.int32, device=dev.device())
     keys = dev.empty(cap * k, "int64")
@@ -273,10 +283,8 @@ def _device_union(parts: list, k: int, slacks: list, chunk_items: dict) -> tuple
     slack = dev.zeros(cap, "int64")
     n_unique = dev.zeros(1, "int64")
     overflow = dev.zeros(1, "int32")
-    for ci, rows, counts in parts:
-        d_rows = dev.to_device(np.ascontiguousarray(rows, dtype=np.int32))
-        d_counts = dev.to_device(np.ascontiguousarray(counts, dtype=np.int32))
-        _lib.call("etm_union_insert", d_rows.data_ptr(), d_counts.data_ptr(), len(rows), k,
+    for ci, n, d_rows, d_counts in parts:
+        _lib.call("etm_union_insert", d_rows.data_ptr(), d_counts.data_ptr(), n, k,
                   chunk_items[ci].data_ptr(), int(slacks[ci]), table.data_ptr(), table_size, keys.data_ptr(),
                   known.data_ptr(), slack.data_ptr(), n_unique.data_ptr(), cap, overflow.data_ptr(), dev.stream_ptr())
     u = int(dev.to_host(n_unique)[0])

@@ -70,8 +70,11 @@ enum { ETM_SCHEDULE_AUTO = 0, ETM_SCHEDULE_ALL = 1, ETM_SCHEDULE_LIVE = 2 };
 /* etm_mine_opts.row_order: the reference's row_order ("keep" / "lex", gpu/input_layer.py:469-529). */
 enum { ETM_ROW_ORDER_AUTO = 0, ETM_ROW_ORDER_KEEP = 1, ETM_ROW_ORDER_LEX = 2 };
 
+/* etm_mine_opts.emit_levels. */
+enum { ETM_EMIT_NONE = 0, ETM_EMIT_HOST = 1, ETM_EMIT_DEVICE = 2 };
+
 /* ABI version of this header (bumped on any signature or struct-meaning change). */
-#define ETM_ABI_VERSION 5
+#define ETM_ABI_VERSION 6
 int etm_abi_version(void);
 
 /* Copies the calling thread's last error message (NUL-terminated, truncated to n). */
@@ -79,6 +82,13 @@ int etm_last_error(char* h_buf, size_t n);
 
 /* Number of visible HIP devices (0 when none). */
 int etm_device_count(int* h_out);
+
+/* Host -> device copy of `bytes` bytes of pageable host memory, cut into 32 MiB chunks staged through the
+ * calling thread's ring of three pinned buffers: each chunk is copied into its buffer by up to 8 host threads
+ * while the previous chunk's DMA runs, and a buffer is refilled only after its DMA completed. Enqueued on
+ * `stream`; returns once the last chunk is enqueued, after which h_src may be reused. Replaces the reference's
+ * double-buffered pinned upload of the Arrow value buffers (gpu/input_layer.py:328-393). */
+int etm_upload(const void* h_src, int64_t bytes, void* d_dst, void* stream);
 
 /* Fills d_out[0] = min, d_out[1] = max of n item ids (int32 or int64 per id_bytes); n >= 1.
  * Replaces the id range scan of gpu/input_layer.py:396-405 (the bincount's span). */
@@ -351,6 +361,10 @@ typedef struct etm_level_info {
     int64_t host_bytes;      /* host memory of the levels kept for the result so far */
     int32_t next_level;      /* 1 when another level follows */
     int32_t reserved;
+    const int32_t* d_rows;   /* levels K >= 2: the level's n_frequent x k column ids in device memory, valid until
+                                the callback returns (copies enqueued on the call's stream are ordered before any
+                                reuse), else NULL */
+    const int32_t* d_counts; /* their counts, the same way */
 } etm_level_info;
 
 /* A nonzero return stops the mining call, which then fails with ETM_ECANCELED. Called on the calling thread. */
@@ -369,8 +383,10 @@ typedef struct etm_mine_opts {
     int32_t row_order;      /* etm_mine_csr: ETM_ROW_ORDER_LEX (rows sorted by their 64 most frequent columns,
                                and the tile skip), ETM_ROW_ORDER_KEEP, or ETM_ROW_ORDER_AUTO (keeps the rows) */
     int32_t live_bytes;     /* compute etm_level_stat.bytes_live (one extra pass over each level's candidates) */
-    int32_t emit_levels;    /* hand each level to level_cb as it completes and keep none for the result (host
-                               memory then holds one level at a time), as the reference's per-level flush */
+    int32_t emit_levels;    /* ETM_EMIT_HOST: hand each level to level_cb as it completes (host copies in h_rows,
+                               h_counts) and keep none for the result (host memory then holds one level at a time),
+                               as the reference's per-level flush; ETM_EMIT_DEVICE: the same with the level left in
+                               device memory only (d_rows, d_counts; no host copy of levels K >= 2) */
     int32_t k3_schedule;    /* ETM_SCHEDULE_*: how levels K >= 3 launch their counting units */
     int32_t small_max;      /* largest group (suffixes) counted by the wave-per-group kernels: 4, 8 or 16; 0 = 8 */
     etm_level_cb level_cb;  /* called after every level (K=1 included) when not NULL */
