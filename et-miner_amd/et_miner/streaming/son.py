@@ -24,6 +24,8 @@ from ..core.result import _min_count, _mining_min_count, empty_result, itemset_t
 
 #: The default local support factor (reference ``streaming/son.py:82``).
 LOCAL_SUPPORT_FACTOR = 0.95
+#: Chunks of pass 1 mined at a time (one stream and host thread each).
+PASS1_STREAMS = 2
 
 
 def _estimate_chunk_size_from_memory(memory_budget_gb: float, n_items_estimate: int = 1000) -> int:
@@ -196,40 +198,73 @@ def son_csr(indptr, ids, min_support, max_length, chunk_size, local_support_fact
     # Pass 1: local levels per chunk (column-id rows, counts, the chunk's column -> item map).
     if session:
         session.start_phase("pass1_local_mining")
+    # Two chunks are mined at a time, each on its own stream and host thread (a mining call waits on the host
+    # once per level; the other chunk's kernels fill those gaps). Chunk i goes to stream i % 2, after chunk i - 2.
+    from concurrent.futures import ThreadPoolExecutor
+
+    torch = dev.torch()
+    streams = [torch.cuda.Stream(device=dev.device()) for _ in range(PASS1_STREAMS)]
+
+    def mine_chunk(ci, d_ptr, d_ids, n):
+        with torch.cuda.device(dev.device()), torch.cuda.stream(streams[ci % PASS1_STREAMS]):
+            local = _local_levels(d_ptr, d_ids, n, local_s, max_length, use_generator_pruning)
+            if local is None:
+                return None
+            col_items, levels = local
+            return dev.to_device(np.ascontiguousarray(col_items, dtype=np.int64)), levels
+
     pending: dict = {}
     chunk_items: dict = {}
-    for ci in range(len(bounds)):
-        d_ptr, d_ids, n = up.take(ci)
-        local = _local_levels(d_ptr, d_ids, n, local_s, max_length, use_generator_pruning)
-        if local is not None:
-            col_items, levels = local
-            chunk_items[ci] = dev.to_device(np.ascontiguousarray(col_items, dtype=np.int64))
-            for k, n_k, d_rows, d_counts in levels:
-                pending.setdefault(k, []).append((ci, n_k, d_rows, d_counts))
-        if progress_callback:
-            progress_callback("pass1", ci, len(bounds), {"candidates": sum(p[1] for v in pending.values()
-                                                                           for p in v)})
+    futs: list = []
+    with ThreadPoolExecutor(PASS1_STREAMS) as pool:
+        for ci in range(len(bounds)):
+            if ci >= PASS1_STREAMS:
+                futs[ci - PASS1_STREAMS].result()
+            with torch.cuda.stream(streams[ci % PASS1_STREAMS]):
+                d_ptr, d_ids, n = up.take(ci)
+            futs.append(pool.submit(mine_chunk, ci, d_ptr, d_ids, n))
+        for ci, fut in enumerate(futs):
+            local = fut.result()
+            if local is not None:
+                chunk_items[ci], levels = local
+                for k, n_k, d_rows, d_counts in levels:
+                    pending.setdefault(k, []).append((ci, n_k, d_rows, d_counts))
+            if progress_callback:
+                progress_callback("pass1", ci, len(bounds), {"candidates": sum(p[1] for v in pending.values()
+                                                                               for p in v)})
     up.close()
+    # The union runs on the current stream: it follows the chunks' streams.
+    for st in streams:
+        torch.cuda.current_stream().wait_stream(st)
     if not pending:
         return itemset_table([], np.zeros(0, np.int64), n_total, with_counts)
     # The union of the chunks' levels, per length, in a device hash map: the local counts (known) and the slack
     # of the emitting chunks summed per itemset; the bound ``known + total_slack - slack`` drops what cannot
-    # reach min_count and is exact where no slack is left (reference son.py:402-421).
+    # reach min_count and is exact where no slack is left (reference son.py:402-421). The bound is applied on the
+    # device: only the exact survivors and the candidates pass 2 must count come back to the host.
+    torch = dev.torch()
     total_slack = sum(slacks)
-    bounded = {}
-    for k, parts in sorted(pending.items()):
-        keys, known, sl = _device_union(parts, k, slacks, chunk_items)
-        left = total_slack - sl
+    unions = {k: _device_union(parts, k, slacks, chunk_items) for k, parts in sorted(pending.items())}
+    sizes = dev.to_host(torch.stack([torch.stack([u[3][0], u[4][0].to(torch.int64)]) for u in unions.values()]))
+    exact, cands = {}, {}
+    for (k, (keys, known, sl, _, _)), (u, overflow) in zip(unions.items(), sizes.tolist()):
+        cap = len(known)
+        if overflow or u > cap:
+            raise RuntimeError(f"SON union of length {k}: {u} itemsets exceed the capacity {cap}")
+        keys, known, left = keys[: u * k].view(u, k), known[:u], total_slack - sl[:u]
         keep = known + left >= min_count
-        need = left[keep] > 0
-        bounded[k] = (keys[keep], np.where(need, 0, known[keep]).astype(np.int64), need)
+        need = keep & (left > 0)
+        done = keep & ~need & (known >= min_count)
+        exact[k] = (dev.to_host(keys[done]), dev.to_host(known[done]))
+        if bool(need.any()):
+            cands[k] = dev.to_host(keys[need])
     if session:
-        session.end_phase(n_candidates=sum(len(v[0]) for v in bounded.values()))
+        session.end_phase(n_candidates=sum(len(v[1]) for v in exact.values()) + sum(len(c) for c in cands.values()))
 
     # Pass 2: every candidate with slack left, counted over every chunk; totals summed on the device.
-    cands = {k: r[need] for k, (r, _, need) in bounded.items() if need.any()}
     if session:
         session.start_phase("pass2_global_counting")
+    counted = {}
     if cands:
         order = sorted(cands)
         all_items = np.unique(np.concatenate([cands[k].ravel() for k in order]))
@@ -254,23 +289,26 @@ def son_csr(indptr, ids, min_support, max_length, chunk_size, local_support_fact
         got = dev.to_host(totals)
         at = 0
         for k in order:
-            _, counts, need = bounded[k]
-            counts[need] = got[at : at + len(cands[k])]
+            counted[k] = got[at : at + len(cands[k])]
             at += len(cands[k])
     if session:
         session.end_phase(n_counted=sum(len(c) for c in cands.values()))
     levels = []
-    for k in sorted(bounded):
-        rows, counts, _ = bounded[k]
-        keep = counts >= min_count
-        levels.append((rows[keep], counts[keep]))
+    for k in sorted(exact):
+        rows, counts = exact[k]
+        if k in counted:
+            ok = counted[k] >= min_count
+            rows = np.concatenate([rows, cands[k][ok]])
+            counts = np.concatenate([counts, counted[k][ok]])
+        levels.append((rows, counts.astype(np.int64)))
     return itemset_table(levels, None, n_total, with_counts)
 
 
 def _device_union(parts: list, k: int, slacks: list, chunk_items: dict) -> tuple:
-    """(item-id rows int64 (u, k), summed local counts, summed slack) of the distinct itemsets of length k
-    over the chunks' device levels ``parts`` = [(chunk, rows, column-id rows int32, counts int32)], through
-    ``etm_union_insert``."""
+    """The distinct itemsets of length k over the chunks' device levels ``parts`` = [(chunk, rows, column-id rows
+    int32, counts int32)], inserted through ``etm_union_insert`` with nothing read back: device tensors of their
+    item-id rows (flat int64, cap x k), summed local counts and summed slack (the first u entries hold the u
+    itemsets), then u and the overflow flag (one element each)."""
     from .. import _lib
     from ..gpu import device as dev
 
@@ -287,10 +325,7 @@ def _device_union(parts: list, k: int, slacks: list, chunk_items: dict) -> tuple
         _lib.call("etm_union_insert", d_rows.data_ptr(), d_counts.data_ptr(), n, k,
                   chunk_items[ci].data_ptr(), int(slacks[ci]), table.data_ptr(), table_size, keys.data_ptr(),
                   known.data_ptr(), slack.data_ptr(), n_unique.data_ptr(), cap, overflow.data_ptr(), dev.stream_ptr())
-    u = int(dev.to_host(n_unique)[0])
-    if int(dev.to_host(overflow)[0]) or u > cap:
-        raise RuntimeError(f"SON union of length {k}: {u} itemsets exceed the capacity {cap}")
-    return (dev.to_host(keys[: u * k]).reshape(u, k), dev.to_host(known[:u]), dev.to_host(slack[:u]))
+    return keys, known, slack, n_unique, overflow
 
 
 def _drop_counts(table):
