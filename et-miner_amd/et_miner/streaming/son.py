@@ -24,8 +24,8 @@ from ..core.result import _min_count, _mining_min_count, empty_result, itemset_t
 
 #: The default local support factor (reference ``streaming/son.py:82``).
 LOCAL_SUPPORT_FACTOR = 0.95
-#: Chunks of pass 1 mined at a time (one stream and host thread each).
-PASS1_STREAMS = 2
+#: Chunks of pass 1 mined at a time (one stream and host thread each; 3 measured fastest).
+PASS1_STREAMS = 3
 
 
 def _estimate_chunk_size_from_memory(memory_budget_gb: float, n_items_estimate: int = 1000) -> int:
@@ -198,8 +198,10 @@ def son_csr(indptr, ids, min_support, max_length, chunk_size, local_support_fact
     # Pass 1: local levels per chunk (column-id rows, counts, the chunk's column -> item map).
     if session:
         session.start_phase("pass1_local_mining")
-    # Two chunks are mined at a time, each on its own stream and host thread (a mining call waits on the host
-    # once per level; the other chunk's kernels fill those gaps). Chunk i goes to stream i % 2, after chunk i - 2.
+    # PASS1_STREAMS chunks are mined at a time, each on its own stream and host thread (a mining call waits on the
+    # host once per level; the other chunks' kernels fill those gaps). Chunk i goes to stream i % PASS1_STREAMS,
+    # after chunk i - PASS1_STREAMS. Measured on deep_sparse_20m in 2M-row chunks: 1, 2, 3, 4 streams 137, 112,
+    # 102, 105 ms.
     from concurrent.futures import ThreadPoolExecutor
 
     torch = dev.torch()
