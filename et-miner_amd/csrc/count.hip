@@ -1190,136 +1190,206 @@ __global__ __launch_bounds__(256) void k_count_glist(const uint64_t* __restrict_
     }
 }
 
-// ── int8 MFMA Gram of a single-group level (the K=2 pair space) ───────────────────────────────────────────
+// ── the K=2 pair space (a single-group level) as a Gram on the matrix cores ─────────────────────────────
 //
-// One workgroup per (128 x 128 tile-pair of columns, word chunk). Per 4-word step (256 transactions = 8 K-steps
-// of 32 rows) the bits are expanded once into int8 0/1 operands in LDS and the 4 waves run v_mfma_i32_32x32x32_i8
-// on them: operand lane l carries column l % 32 and rows 16 (l / 32) .. +15 of the K-step for both A and B, so the
-// product is the exact Gram sum whatever the instruction's K order within a lane. Counts accumulate in int32
-// (exact for fewer than 2^31 transactions) and leave with one integer atomic per pair.
+// Each bit of a column becomes one fp4 (e2m1) operand element, 0 or 1.0 (nibble 0x2), and
+// v_mfma_scale_f32_32x32x64_f8f6f4 (unit scales) multiplies 32 columns by 32 columns over 64 rows: one bitvector
+// word per column per instruction. Operand lane l carries column l % 32 and the 32 rows of half l / 32 of the
+// word (32 nibbles, 16 bytes; the instruction reads four of the operand's eight registers for fp4) for both A and
+// B, so the product is the exact Gram sum whatever the instruction's order of the K elements within a lane. Each
+// product is 0 or 1 and fp32 adds integers below 2^24 exactly: a word chunk holds fewer than 2^24 rows, and the
+// per-chunk counts leave as int32 with one atomic per pair.
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef int v8i __attribute__((ext_vector_type(8)));
 typedef int v4i __attribute__((ext_vector_type(4)));
-typedef int v16i __attribute__((ext_vector_type(16)));
 
-constexpr int kMT = 128;  // columns per MFMA tile
-constexpr int kMW = 4;    // words per step: 256 rows, 8 K-steps of 32
-
-// Bits 4q..4q+3 of x as four 0/1 bytes (bit t -> byte t).
-__device__ __forceinline__ int spread4(uint32_t x, int q) {
-    return (int)((__umul24((x >> (4 * q)) & 0xFu, 0x00204081u)) & 0x01010101u);
+// Bits 0..7 of x as eight fp4 elements: nibble k = 0x2 (1.0) where bit k is set, else 0.
+__device__ __forceinline__ uint32_t fp4_spread8(uint32_t x) {
+    uint32_t t = (x & 0xFFu) << 1;
+    t = (t | (t << 12)) & 0x001E001Eu;
+    t = (t | (t << 6)) & 0x06060606u;
+    return (t | (t << 3)) & 0x22222222u;
 }
 
-// The 16 bits of x as sixteen 0/1 bytes (bit t -> byte t).
-__device__ __forceinline__ v4i spread16(uint32_t x) {
-    v4i v;
-    v[0] = spread4(x, 0);
-    v[1] = spread4(x, 1);
-    v[2] = spread4(x, 2);
-    v[3] = spread4(x, 3);
-    return v;
+// A workgroup of 8 waves covers a tile-pair of kGA x kGB columns (A rows i, B columns j) over a chunk of words,
+// in steps of kGW words. The waves split by role. Producers (waves 4-7, one per SIMD) expand the words of the
+// tile-pair's kGA + kGB columns for the next step into fp4 operands in LDS, [column][word][32 bytes] (an operand,
+// half a word of one column, is one 16-byte LDS read); each column is expanded once per workgroup and its words
+// are loaded one step ahead. Consumers (waves 0-3, one per SIMD) run a 2 x 4 block of MFMAs (64 A rows x 128 B
+// columns, fp32 accumulators) per word on the current step's operands. The two LDS buffers alternate, so on every
+// SIMD the expansion (VALU) of step t + 1 overlaps the MFMAs of step t; one barrier per step separates them.
+constexpr int kGA = 128;                     // A columns (rows of the Gram block) per tile-pair
+constexpr int kGB = 256;                     // B columns per tile-pair
+constexpr int kGW = 4;                       // words per step
+constexpr int kGRowB = 32 * kGW + 16;        // LDS bytes per column per buffer (16 of padding across banks)
+constexpr int kGBuf = (kGA + kGB) * kGRowB;  // 55,296 B per buffer
+constexpr int kGThreads = 512;
+constexpr int64_t kGMaxChunkWords = ((1ll << 24) - 1) / 64;  // rows of a chunk stay below 2^24
+
+// Tile-pairs of s columns: for B tile tb (kGB columns) the A tiles ta (kGA columns) with some i < j in the block,
+// ta < min(nA, 2 tb + 2) (kGB = 2 kGA); ordered by tb, then ta.
+__host__ __device__ inline int64_t gram_tile_pairs(int64_t s) {
+    const int64_t na = ceil_div(s, kGA), nb = ceil_div(s, kGB);
+    int64_t n = 0;
+    for (int64_t b = 0; b < nb; ++b) n += min(na, 2 * b + 2);
+    return n;
 }
 
-// K=2 pair counts of a wide column set as an int8 Gram on the matrix cores. A workgroup of 4 waves covers a
-// tile-pair of 128 x 128 columns over a chunk of words. Thread t owns column t of the tile-pair (A columns 0..127,
-// then B): per step of kMW words it expands its column's words (loaded into registers during the previous step)
-// into int8 0/1 bytes in LDS, [column][row] with rows contiguous, so an MFMA operand (16 rows of one column) is one
-// 16-byte LDS read; then each wave runs its 2 x 2 block of v_mfma_i32_32x32x32_i8 over the step's K-steps. Each
-// column is expanded once per workgroup (waves that expand the operands they use would expand each one twice).
-constexpr int kGRow = 64 * kMW + 16;  // LDS bytes per column: kMW words of rows + 16 of padding across banks
+__device__ __forceinline__ void gram_tile_pair(int64_t tp, int64_t s, int64_t* ta, int64_t* tb) {
+    const int64_t na = ceil_div(s, kGA), nb = ceil_div(s, kGB);
+    int64_t t = 0, b = 0;
+    for (; b < nb; ++b) {
+        const int64_t c = min(na, 2 * b + 2);
+        if (tp < t + c) break;
+        t += c;
+    }
+    *tb = b;
+    *ta = tp - t;
+}
 
-__global__ __launch_bounds__(256) void k_gram_mfma(const uint64_t* __restrict__ bv, int64_t pitch,
-                                                           int64_t n_words, int64_t chunk_words,
-                                                           const int32_t* __restrict__ cols, int64_t s, int64_t n_tp,
-                                                           int64_t n_chunks, int32_t* __restrict__ counts,
-                                                           int32_t layout) {
-    constexpr int T = kMT;  // columns per side
-    __shared__ __align__(16) uint8_t ops[2 * T * kGRow];
+__global__ __launch_bounds__(kGThreads) void k_gram_mfma(const uint64_t* __restrict__ bv, int64_t pitch,
+                                                         int64_t n_words, int64_t chunk_words,
+                                                         const int32_t* __restrict__ cols, int64_t s, int64_t n_tp,
+                                                         int64_t n_chunks, int32_t* __restrict__ counts,
+                                                         int32_t layout) {
+    __shared__ __align__(16) uint8_t ops[2 * kGBuf];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int64_t chunk_id, tp;
     if (!xcd_unit(blockIdx.x % kXcds, blockIdx.x / kXcds, n_tp, n_chunks, &chunk_id, &tp)) return;
-    const int64_t nt = (s + T - 1) / T;
-    int64_t ta, tb1;
-    etm::lex_pair_decode(uniform64(tp), nt + 1, &ta, &tb1);
+    int64_t ta, tb;
+    gram_tile_pair(uniform64(tp), s, &ta, &tb);
     ta = uniform64(ta);
-    const int64_t tb = uniform64(tb1) - 1;
-    // This thread's column: A column ta * T + tid, or B column tb * T + tid - T.
-    const int64_t mycol = tid < T ? ta * T + tid : tb * T + (tid - T);
-    const int64_t base = mycol < s ? (int64_t)cols[mycol] * pitch : -1;
-    const int wm = wave >> 1, wn = wave & 1;
-    v16i acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
+    tb = uniform64(tb);
     const int64_t w_begin = uniform64(chunk_id) * chunk_words;
     const int64_t w_end = min(n_words, w_begin + chunk_words);
-    // Two steps of this column's words in flight: the words of step s + 2 are requested while step s counts.
-    uint64_t next[kMW], after[kMW];
+    const int64_t n_steps = ceil_div(w_end - w_begin, kGW);
+    if (wave >= 4) {
+        // Producer thread p expands LDS columns p and (p < 128) p + 256: A columns 0..127, then B columns.
+        const int p = tid - 256;
+        int64_t base[2] = {-1, -1};
+        uint32_t at[2] = {0, 0};
 #pragma unroll
-    for (int w = 0; w < kMW; ++w) {
-        next[w] = (base >= 0 && w_begin + w < w_end) ? bv[base + w_begin + w] : 0ull;
-        after[w] = (base >= 0 && w_begin + kMW + w < w_end) ? bv[base + w_begin + kMW + w] : 0ull;
-    }
-    uint8_t* mine = ops + tid * kGRow;
-    const int sub = 16 * (lane >> 5);
-    for (int64_t wb = w_begin; wb < w_end; wb += kMW) {
-        // Expand this column's kMW words: byte r of the column's run = bit r % 64 of word r / 64.
-#pragma unroll
-        for (int w = 0; w < kMW; ++w)
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                *reinterpret_cast<v4i*>(mine + 64 * w + 16 * q) = spread16((uint32_t)(next[w] >> (16 * q)) & 0xFFFFu);
-        __syncthreads();
-        const int64_t nb = wb + 2 * kMW;
-#pragma unroll
-        for (int w = 0; w < kMW; ++w) {
-            next[w] = after[w];
-            after[w] = (base >= 0 && nb + w < w_end) ? bv[base + nb + w] : 0ull;
+        for (int q = 0; q < 2; ++q) {
+            const int c = p + 256 * q;
+            if (c >= kGA + kGB) continue;
+            const int64_t col = c < kGA ? ta * kGA + c : tb * kGB + (c - kGA);
+            at[q] = (uint32_t)c * kGRowB;
+            if (col < s) base[q] = (int64_t)cols[col] * pitch;
         }
+        const int nq = p < kGA + kGB - 256 ? 2 : 1;
+        uint64_t next[2][kGW];
 #pragma unroll
-        for (int ks = 0; ks < 2 * kMW; ++ks) {
-            v4i a[2], b[2];
+        for (int q = 0; q < 2; ++q)
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                a[t] = *reinterpret_cast<const v4i*>(ops + (wm * 64 + 32 * t + (lane & 31)) * kGRow + 32 * ks + sub);
-                b[t] = *reinterpret_cast<const v4i*>(ops + (T + wn * 64 + 32 * t + (lane & 31)) * kGRow + 32 * ks + sub);
+            for (int w = 0; w < kGW; ++w)
+                next[q][w] = (q < nq && base[q] >= 0 && w_begin + w < w_end) ? bv[base[q] + w_begin + w] : 0ull;
+        // Step t's operands into buffer t % 2; the words of step t + 1 are requested before the expansion.
+        auto expand = [&](int64_t t) {
+            uint8_t* buf = ops + (t & 1) * kGBuf;
+            uint64_t cur[2][kGW];
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int w = 0; w < kGW; ++w) cur[q][w] = next[q][w];
+            const int64_t wn = w_begin + (t + 1) * kGW;
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int w = 0; w < kGW; ++w)
+                    next[q][w] = (q < nq && base[q] >= 0 && wn + w < w_end) ? bv[base[q] + wn + w] : 0ull;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                if (q >= nq) continue;
+                uint8_t* mine = buf + at[q];
+#pragma unroll
+                for (int w = 0; w < kGW; ++w)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const uint32_t x = (uint32_t)(cur[q][w] >> (32 * h));
+                        v4i v;
+                        v[0] = (int)fp4_spread8(x);
+                        v[1] = (int)fp4_spread8(x >> 8);
+                        v[2] = (int)fp4_spread8(x >> 16);
+                        v[3] = (int)fp4_spread8(x >> 24);
+                        *reinterpret_cast<v4i*>(mine + 32 * w + 16 * h) = v;
+                    }
+            }
+        };
+        expand(0);
+        __syncthreads();
+        for (int64_t t = 0; t < n_steps; ++t) {
+            if (t + 1 < n_steps) expand(t + 1);
+            __syncthreads();
+        }
+        return;
+    }
+    // Consumers: wave (cm, cn) owns A rows cm * 64 .. + 63 and B columns cn * 128 .. + 127 of the block.
+    const int cm = wave >> 1, cn = wave & 1;
+    v16f acc[2][4];
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[mi][ni][r] = 0.0f;
+    const int half = 16 * (lane >> 5);
+    const uint32_t a_off = (uint32_t)(cm * 64 + (lane & 31)) * kGRowB + half;
+    const uint32_t b_off = (uint32_t)(kGA + cn * 128 + (lane & 31)) * kGRowB + half;
+    __syncthreads();
+    for (int64_t t = 0; t < n_steps; ++t) {
+        const uint8_t* buf = ops + (t & 1) * kGBuf;
+#pragma unroll
+        for (int w = 0; w < kGW; ++w) {
+            v8i a[2], b[4];
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi) {
+                const v4i x = *reinterpret_cast<const v4i*>(buf + a_off + mi * 32 * kGRowB + 32 * w);
+                a[mi] = v8i{x[0], x[1], x[2], x[3], 0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int ni = 0; ni < 4; ++ni) {
+                const v4i x = *reinterpret_cast<const v4i*>(buf + b_off + ni * 32 * kGRowB + 32 * w);
+                b[ni] = v8i{x[0], x[1], x[2], x[3], 0, 0, 0, 0};
             }
 #pragma unroll
             for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-                for (int ni = 0; ni < 2; ++ni)
-                    acc[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[mi], b[ni], acc[mi][ni], 0, 0, 0);
+                for (int ni = 0; ni < 4; ++ni)
+                    acc[mi][ni] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[mi], b[ni], acc[mi][ni], 4, 4, 0,
+                                                                                 127, 0, 127);
         }
-        __syncthreads();  // the next step's expansion overwrites the operands
+        __syncthreads();
     }
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi) {
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni) {
+        for (int ni = 0; ni < 4; ++ni) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int64_t i = ta * T + wm * 64 + mi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                const int64_t j = tb * T + wn * 64 + ni * 32 + (lane & 31);
-                if (i < j && j < s && acc[mi][ni][r]) atomicAdd(&counts[pair_slot(i, j, s, layout)], acc[mi][ni][r]);
+                const int64_t i = ta * kGA + cm * 64 + mi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int64_t j = tb * kGB + cn * 128 + ni * 32 + (lane & 31);
+                const int32_t c = (int32_t)acc[mi][ni][r];
+                if (i < j && j < s && c) atomicAdd(&counts[pair_slot(i, j, s, layout)], c);
             }
         }
     }
 }
 
-// Word chunk of the Gram: the workgroups (tile-pairs x chunks) fill whole rounds of the resident slots (two
-// 69.6 KiB-LDS workgroups per CU) as closely as possible over 2..8 rounds, the chunks per tile-pair rounded
-// down to a multiple of the 8 XCDs where at least 8 (xcd_unit then keeps each XCD on its own chunks). A last
-// round a third full costs the launch a third of a round.
+// Word chunk of the Gram: the workgroups (tile-pairs x chunks) fill whole rounds of the resident slots (one
+// 108 KiB-LDS workgroup per CU) as closely as possible over 2..8 rounds, the chunks per tile-pair rounded down to
+// a multiple of the 8 XCDs where at least 8 (xcd_unit then keeps each XCD on its own chunks), and no chunk of
+// 2^24 rows or more (the fp32 accumulators' exact range). A last round a third full costs the launch a third of
+// a round.
 int64_t gram_chunk(int64_t n_words, int64_t n_tp) {
-    constexpr int64_t kSlots = 2 * 256;
+    constexpr int64_t kSlots = 256;
     const int64_t max_chunks = std::max<int64_t>(1, n_words / 512);
+    const int64_t min_chunks = ceil_div(n_words, kGMaxChunkWords / kGW * kGW);
     int64_t best = 1;
     double best_fill = -1.0;
     for (int64_t rounds = 2; rounds <= 8; ++rounds) {
         int64_t ch = std::max<int64_t>(1, rounds * kSlots / std::max<int64_t>(n_tp, 1));
         if (ch >= kXcds) ch = ch / kXcds * kXcds;
-        ch = std::min(ch, max_chunks);
+        ch = std::max(std::min(ch, max_chunks), min_chunks);
         const int64_t units = ch * n_tp;
         const double fill = (double)units / (double)(ceil_div(units, kSlots) * kSlots);
         if (fill > best_fill + 0.02) {
@@ -1327,7 +1397,7 @@ int64_t gram_chunk(int64_t n_words, int64_t n_tp) {
             best = ch;
         }
     }
-    return ceil_div(ceil_div(n_words, best), kMW) * kMW;
+    return ceil_div(ceil_div(n_words, best), kGW) * kGW;
 }
 
 // One wave per (itemset, word chunk): AND of the itemset's columns, popcount, wave sum, one 64-bit add.
@@ -1759,17 +1829,15 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
         ETM_HIP(hipMemcpyAsync(&s_cols, d_group_start + 1, sizeof s_cols, hipMemcpyDeviceToHost, s));
         ETM_HIP(hipStreamSynchronize(s));
         if (kernel == ETM_KERNEL_MFMA || etm::detail::gram_preferred(s_cols, n_words)) {
-            // 2 x 2 blocks per wave (128-column tiles): measured faster than 4 x 4 blocks (256-column tiles, one
-            // wave per SIMD), which waste more edge pairs and hide load latency worse.
-            const int64_t nt = ceil_div(s_cols, kMT);
-            const int64_t n_tp = nt * (nt + 1) / 2;
-            const int64_t chunk = word_chunk > 0 ? ceil_div(word_chunk, kMW) * kMW : gram_chunk(n_words, n_tp);
+            const int64_t n_tp = gram_tile_pairs(s_cols);
+            const int64_t chunk = word_chunk > 0 ? std::min(ceil_div(word_chunk, kGW) * kGW, kGMaxChunkWords / kGW * kGW)
+                                                 : gram_chunk(n_words, n_tp);
             const int64_t n_chunks = ceil_div(n_words, chunk);
             const int64_t blocks = kXcds * xcd_units(n_tp, n_chunks);
             ETM_REQUIRE(blocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",
                         (long long)blocks);
-            k_gram_mfma<<<(unsigned)blocks, 256, 0, s>>>(d_bv, pitch, n_words, chunk, d_rows, s_cols, n_tp, n_chunks,
-                                                         d_counts, layout);
+            k_gram_mfma<<<(unsigned)blocks, kGThreads, 0, s>>>(d_bv, pitch, n_words, chunk, d_rows, s_cols, n_tp,
+                                                               n_chunks, d_counts, layout);
             ETM_LAUNCHED("k_gram_mfma");
             return ETM_OK;
         }
