@@ -1194,58 +1194,39 @@ __global__ __launch_bounds__(256) void k_count_glist(const uint64_t* __restrict_
 //
 // Each bit of a column becomes one fp4 (e2m1) operand element, 0 or 1.0 (nibble 0x2), and
 // v_mfma_scale_f32_32x32x64_f8f6f4 (unit scales) multiplies 32 columns by 32 columns over 64 rows: one bitvector
-// word per column per instruction. Operand lane l carries column l % 32 and the 32 rows of half l / 32 of the
-// word (32 nibbles, 16 bytes; the instruction reads four of the operand's eight registers for fp4) for both A and
-// B, so the product is the exact Gram sum whatever the instruction's order of the K elements within a lane. Each
-// product is 0 or 1 and fp32 adds integers below 2^24 exactly: a word chunk holds fewer than 2^24 rows, and the
-// per-chunk counts leave as int32 with one atomic per pair.
+// word per column per instruction. Operand lane l carries column l % 32 and half l / 32 of the word (32 rows as
+// 32 nibbles in four registers; the instruction reads four of the operand's eight for fp4). Which nibble holds
+// which row does not matter as long as A and B place rows alike, since the instruction sums products of
+// matching elements: register d of the half-word h takes bits d, d + 4, ..., d + 28 of h, each moved to bit 1 of
+// its nibble -- one shift and one AND per register. Each product is 0 or 1 and fp32 adds integers below 2^24
+// exactly: a word chunk holds fewer than 2^24 rows, and the per-chunk counts leave as int32 with one atomic per
+// pair.
 typedef float v16f __attribute__((ext_vector_type(16)));
 typedef int v8i __attribute__((ext_vector_type(8)));
-typedef int v4i __attribute__((ext_vector_type(4)));
 
-// Bits 0..7 of x as eight fp4 elements: nibble k = 0x2 (1.0) where bit k is set, else 0.
-__device__ __forceinline__ uint32_t fp4_spread8(uint32_t x) {
-    uint32_t t = (x & 0xFFu) << 1;
-    t = (t | (t << 12)) & 0x001E001Eu;
-    t = (t | (t << 6)) & 0x06060606u;
-    return (t | (t << 3)) & 0x22222222u;
+// The fp4 operand of a half-word h (32 rows): nibble n of register d is 0x2 where bit 4n + d of h is set.
+__device__ __forceinline__ v8i fp4_operand(uint32_t h) {
+    v8i v;
+    v[0] = (int)((h << 1) & 0x22222222u);
+    v[1] = (int)(h & 0x22222222u);
+    v[2] = (int)((h >> 1) & 0x22222222u);
+    v[3] = (int)((h >> 2) & 0x22222222u);
+    v[4] = v[5] = v[6] = v[7] = 0;
+    return v;
 }
 
-// A workgroup of 8 waves covers a tile-pair of kGA x kGB columns (A rows i, B columns j) over a chunk of words,
-// in steps of kGW words. The waves split by role. Producers (waves 4-7, one per SIMD) expand the words of the
-// tile-pair's kGA + kGB columns for the next step into fp4 operands in LDS, [column][word][32 bytes] (an operand,
-// half a word of one column, is one 16-byte LDS read); each column is expanded once per workgroup and its words
-// are loaded one step ahead. Consumers (waves 0-3, one per SIMD) run a 2 x 4 block of MFMAs (64 A rows x 128 B
-// columns, fp32 accumulators) per word on the current step's operands. The two LDS buffers alternate, so on every
-// SIMD the expansion (VALU) of step t + 1 overlaps the MFMAs of step t; one barrier per step separates them.
-constexpr int kGA = 128;                     // A columns (rows of the Gram block) per tile-pair
-constexpr int kGB = 256;                     // B columns per tile-pair
-constexpr int kGW = 4;                       // words per step
-constexpr int kGRowB = 32 * kGW + 16;        // LDS bytes per column per buffer (16 of padding across banks)
-constexpr int kGBuf = (kGA + kGB) * kGRowB;  // 55,296 B per buffer
+// A workgroup of 8 waves covers a tile-pair of kGT x kGT columns over a chunk of words, in steps of kGW words.
+// Per step all 512 threads stage the raw words of the tile-pair's 2 kGT columns for the next step in LDS (thread t
+// loads column t's kGW consecutive words; [column][word], a column's row padded to an odd multiple of 8 bytes so
+// that the 64 lanes of an operand read hit 64 distinct banks), while each wave runs its 2 x 4 block of MFMAs (64 A
+// columns x 128 B columns, fp32 accumulators) on the current step: per word it reads its six half-words from LDS
+// and expands them in registers. The two LDS buffers alternate; one barrier per step.
+constexpr int kGT = 256;                      // columns per tile (both sides)
+constexpr int kGW = 16;                       // words per step
+constexpr int kGRowB = 8 * kGW + 8;           // LDS bytes per column per buffer (136: 34 banks apart)
+constexpr int kGBuf = 2 * kGT * kGRowB;       // 69,632 B per buffer
 constexpr int kGThreads = 512;
-constexpr int64_t kGMaxChunkWords = ((1ll << 24) - 1) / 64;  // rows of a chunk stay below 2^24
-
-// Tile-pairs of s columns: for B tile tb (kGB columns) the A tiles ta (kGA columns) with some i < j in the block,
-// ta < min(nA, 2 tb + 2) (kGB = 2 kGA); ordered by tb, then ta.
-__host__ __device__ inline int64_t gram_tile_pairs(int64_t s) {
-    const int64_t na = ceil_div(s, kGA), nb = ceil_div(s, kGB);
-    int64_t n = 0;
-    for (int64_t b = 0; b < nb; ++b) n += min(na, 2 * b + 2);
-    return n;
-}
-
-__device__ __forceinline__ void gram_tile_pair(int64_t tp, int64_t s, int64_t* ta, int64_t* tb) {
-    const int64_t na = ceil_div(s, kGA), nb = ceil_div(s, kGB);
-    int64_t t = 0, b = 0;
-    for (; b < nb; ++b) {
-        const int64_t c = min(na, 2 * b + 2);
-        if (tp < t + c) break;
-        t += c;
-    }
-    *tb = b;
-    *ta = tp - t;
-}
+constexpr int64_t kGMaxChunkWords = ((1ll << 24) - 1) / 64 / kGW * kGW;  // rows of a chunk stay below 2^24
 
 __global__ __launch_bounds__(kGThreads) void k_gram_mfma(const uint64_t* __restrict__ bv, int64_t pitch,
                                                          int64_t n_words, int64_t chunk_words,
@@ -1256,74 +1237,31 @@ __global__ __launch_bounds__(kGThreads) void k_gram_mfma(const uint64_t* __restr
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int64_t chunk_id, tp;
     if (!xcd_unit(blockIdx.x % kXcds, blockIdx.x / kXcds, n_tp, n_chunks, &chunk_id, &tp)) return;
-    int64_t ta, tb;
-    gram_tile_pair(uniform64(tp), s, &ta, &tb);
+    const int64_t nt = ceil_div(s, kGT);
+    int64_t ta, tb1;
+    etm::lex_pair_decode(uniform64(tp), nt + 1, &ta, &tb1);
     ta = uniform64(ta);
-    tb = uniform64(tb);
+    const int64_t tb = uniform64(tb1) - 1;
     const int64_t w_begin = uniform64(chunk_id) * chunk_words;
     const int64_t w_end = min(n_words, w_begin + chunk_words);
     const int64_t n_steps = ceil_div(w_end - w_begin, kGW);
-    if (wave >= 4) {
-        // Producer thread p expands LDS columns p and (p < 128) p + 256: A columns 0..127, then B columns.
-        const int p = tid - 256;
-        int64_t base[2] = {-1, -1};
-        uint32_t at[2] = {0, 0};
+    // This thread's staging column: A columns 0..255, then B.
+    const int64_t mycol = tid < kGT ? ta * kGT + tid : tb * kGT + (tid - kGT);
+    const int64_t base = mycol < s ? (int64_t)cols[mycol] * pitch : -1;
+    uint64_t next[kGW];
+    auto load = [&](int64_t t) {
+        const int64_t w0 = w_begin + t * kGW;
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int c = p + 256 * q;
-            if (c >= kGA + kGB) continue;
-            const int64_t col = c < kGA ? ta * kGA + c : tb * kGB + (c - kGA);
-            at[q] = (uint32_t)c * kGRowB;
-            if (col < s) base[q] = (int64_t)cols[col] * pitch;
-        }
-        const int nq = p < kGA + kGB - 256 ? 2 : 1;
-        uint64_t next[2][kGW];
+        for (int w = 0; w < kGW; ++w) next[w] = (base >= 0 && w0 + w < w_end) ? bv[base + w0 + w] : 0ull;
+    };
+    auto stage = [&](int64_t t) {
+        uint64_t* row = reinterpret_cast<uint64_t*>(ops + (t & 1) * kGBuf + tid * kGRowB);
 #pragma unroll
-        for (int q = 0; q < 2; ++q)
-#pragma unroll
-            for (int w = 0; w < kGW; ++w)
-                next[q][w] = (q < nq && base[q] >= 0 && w_begin + w < w_end) ? bv[base[q] + w_begin + w] : 0ull;
-        // Step t's operands into buffer t % 2; the words of step t + 1 are requested before the expansion.
-        auto expand = [&](int64_t t) {
-            uint8_t* buf = ops + (t & 1) * kGBuf;
-            uint64_t cur[2][kGW];
-#pragma unroll
-            for (int q = 0; q < 2; ++q)
-#pragma unroll
-                for (int w = 0; w < kGW; ++w) cur[q][w] = next[q][w];
-            const int64_t wn = w_begin + (t + 1) * kGW;
-#pragma unroll
-            for (int q = 0; q < 2; ++q)
-#pragma unroll
-                for (int w = 0; w < kGW; ++w)
-                    next[q][w] = (q < nq && base[q] >= 0 && wn + w < w_end) ? bv[base[q] + wn + w] : 0ull;
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                if (q >= nq) continue;
-                uint8_t* mine = buf + at[q];
-#pragma unroll
-                for (int w = 0; w < kGW; ++w)
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        const uint32_t x = (uint32_t)(cur[q][w] >> (32 * h));
-                        v4i v;
-                        v[0] = (int)fp4_spread8(x);
-                        v[1] = (int)fp4_spread8(x >> 8);
-                        v[2] = (int)fp4_spread8(x >> 16);
-                        v[3] = (int)fp4_spread8(x >> 24);
-                        *reinterpret_cast<v4i*>(mine + 32 * w + 16 * h) = v;
-                    }
-            }
-        };
-        expand(0);
-        __syncthreads();
-        for (int64_t t = 0; t < n_steps; ++t) {
-            if (t + 1 < n_steps) expand(t + 1);
-            __syncthreads();
-        }
-        return;
-    }
-    // Consumers: wave (cm, cn) owns A rows cm * 64 .. + 63 and B columns cn * 128 .. + 127 of the block.
+        for (int w = 0; w < kGW; ++w) row[w] = next[w];
+    };
+    load(0);
+    stage(0);
+    // Wave (cm, cn) owns A columns cm * 64 .. + 63 and B columns cn * 128 .. + 127 of the tile-pair.
     const int cm = wave >> 1, cn = wave & 1;
     v16f acc[2][4];
 #pragma unroll
@@ -1332,25 +1270,22 @@ __global__ __launch_bounds__(kGThreads) void k_gram_mfma(const uint64_t* __restr
         for (int ni = 0; ni < 4; ++ni)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[mi][ni][r] = 0.0f;
-    const int half = 16 * (lane >> 5);
-    const uint32_t a_off = (uint32_t)(cm * 64 + (lane & 31)) * kGRowB + half;
-    const uint32_t b_off = (uint32_t)(kGA + cn * 128 + (lane & 31)) * kGRowB + half;
+    const uint32_t a_off = (uint32_t)(cm * 64 + (lane & 31)) * kGRowB + 4 * (lane >> 5);
+    const uint32_t b_off = (uint32_t)(kGT + cn * 128 + (lane & 31)) * kGRowB + 4 * (lane >> 5);
     __syncthreads();
     for (int64_t t = 0; t < n_steps; ++t) {
+        const bool more = t + 1 < n_steps;
+        if (more) load(t + 1);
         const uint8_t* buf = ops + (t & 1) * kGBuf;
-#pragma unroll
+#pragma unroll 4
         for (int w = 0; w < kGW; ++w) {
             v8i a[2], b[4];
 #pragma unroll
-            for (int mi = 0; mi < 2; ++mi) {
-                const v4i x = *reinterpret_cast<const v4i*>(buf + a_off + mi * 32 * kGRowB + 32 * w);
-                a[mi] = v8i{x[0], x[1], x[2], x[3], 0, 0, 0, 0};
-            }
+            for (int mi = 0; mi < 2; ++mi)
+                a[mi] = fp4_operand(*reinterpret_cast<const uint32_t*>(buf + a_off + mi * 32 * kGRowB + 8 * w));
 #pragma unroll
-            for (int ni = 0; ni < 4; ++ni) {
-                const v4i x = *reinterpret_cast<const v4i*>(buf + b_off + ni * 32 * kGRowB + 32 * w);
-                b[ni] = v8i{x[0], x[1], x[2], x[3], 0, 0, 0, 0};
-            }
+            for (int ni = 0; ni < 4; ++ni)
+                b[ni] = fp4_operand(*reinterpret_cast<const uint32_t*>(buf + b_off + ni * 32 * kGRowB + 8 * w));
 #pragma unroll
             for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
@@ -1358,6 +1293,7 @@ __global__ __launch_bounds__(kGThreads) void k_gram_mfma(const uint64_t* __restr
                     acc[mi][ni] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[mi], b[ni], acc[mi][ni], 4, 4, 0,
                                                                                  127, 0, 127);
         }
+        if (more) stage(t + 1);
         __syncthreads();
     }
 #pragma unroll
@@ -1366,8 +1302,8 @@ __global__ __launch_bounds__(kGThreads) void k_gram_mfma(const uint64_t* __restr
         for (int ni = 0; ni < 4; ++ni) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int64_t i = ta * kGA + cm * 64 + mi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                const int64_t j = tb * kGB + cn * 128 + ni * 32 + (lane & 31);
+                const int64_t i = ta * kGT + cm * 64 + mi * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int64_t j = tb * kGT + cn * 128 + ni * 32 + (lane & 31);
                 const int32_t c = (int32_t)acc[mi][ni][r];
                 if (i < j && j < s && c) atomicAdd(&counts[pair_slot(i, j, s, layout)], c);
             }
@@ -1375,15 +1311,21 @@ __global__ __launch_bounds__(kGThreads) void k_gram_mfma(const uint64_t* __restr
     }
 }
 
+// Tile-pairs of the Gram over s columns.
+int64_t gram_tile_pairs(int64_t s) {
+    const int64_t nt = ceil_div(s, kGT);
+    return nt * (nt + 1) / 2;
+}
+
 // Word chunk of the Gram: the workgroups (tile-pairs x chunks) fill whole rounds of the resident slots (one
-// 108 KiB-LDS workgroup per CU) as closely as possible over 2..8 rounds, the chunks per tile-pair rounded down to
+// 136 KiB-LDS workgroup per CU) as closely as possible over 2..8 rounds, the chunks per tile-pair rounded down to
 // a multiple of the 8 XCDs where at least 8 (xcd_unit then keeps each XCD on its own chunks), and no chunk of
 // 2^24 rows or more (the fp32 accumulators' exact range). A last round a third full costs the launch a third of
 // a round.
 int64_t gram_chunk(int64_t n_words, int64_t n_tp) {
     constexpr int64_t kSlots = 256;
     const int64_t max_chunks = std::max<int64_t>(1, n_words / 512);
-    const int64_t min_chunks = ceil_div(n_words, kGMaxChunkWords / kGW * kGW);
+    const int64_t min_chunks = ceil_div(n_words, kGMaxChunkWords);
     int64_t best = 1;
     double best_fill = -1.0;
     for (int64_t rounds = 2; rounds <= 8; ++rounds) {
@@ -1830,7 +1772,7 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
         ETM_HIP(hipStreamSynchronize(s));
         if (kernel == ETM_KERNEL_MFMA || etm::detail::gram_preferred(s_cols, n_words)) {
             const int64_t n_tp = gram_tile_pairs(s_cols);
-            const int64_t chunk = word_chunk > 0 ? std::min(ceil_div(word_chunk, kGW) * kGW, kGMaxChunkWords / kGW * kGW)
+            const int64_t chunk = word_chunk > 0 ? std::min(ceil_div(word_chunk, kGW) * kGW, kGMaxChunkWords)
                                                  : gram_chunk(n_words, n_tp);
             const int64_t n_chunks = ceil_div(n_words, chunk);
             const int64_t blocks = kXcds * xcd_units(n_tp, n_chunks);
