@@ -1217,14 +1217,15 @@ __device__ __forceinline__ v8i fp4_operand(uint32_t h) {
 
 // A workgroup of 8 waves covers a tile-pair of kGT x kGT columns over a chunk of words, in steps of kGW words.
 // Per step all 512 threads stage the raw words of the tile-pair's 2 kGT columns for the next step in LDS (thread t
-// loads column t's kGW consecutive words; [column][word], a column's row padded to an odd multiple of 8 bytes so
-// that the 64 lanes of an operand read hit 64 distinct banks), while each wave runs its 2 x 4 block of MFMAs (64 A
+// loads column t's kGW consecutive words; [column][word] as 32-bit halves, a column's row 33 dwords long so that
+// the 32 columns of an operand read fall in 32 distinct banks), while each wave runs its 2 x 4 block of MFMAs (64 A
 // columns x 128 B columns, fp32 accumulators) on the current step: per word it reads its six half-words from LDS
-// and expands them in registers. The two LDS buffers alternate; one barrier per step.
+// and expands them in registers. The two LDS buffers alternate; one barrier per step. (Measured against 4 waves
+// with 4 x 4 blocks, one wave per SIMD and accumulators in AGPRs: wide_vocab_2m K=2 2.92 against 3.54 ms.)
 constexpr int kGT = 256;                      // columns per tile (both sides)
 constexpr int kGW = 16;                       // words per step
-constexpr int kGRowB = 8 * kGW + 8;           // LDS bytes per column per buffer (136: 34 banks apart)
-constexpr int kGBuf = 2 * kGT * kGRowB;       // 69,632 B per buffer
+constexpr int kGRowB = 8 * kGW + 4;           // LDS bytes per column per buffer (132: 33 banks apart)
+constexpr int kGBuf = 2 * kGT * kGRowB;       // 67,584 B per buffer
 constexpr int kGThreads = 512;
 constexpr int64_t kGMaxChunkWords = ((1ll << 24) - 1) / 64 / kGW * kGW;  // rows of a chunk stay below 2^24
 
@@ -1255,9 +1256,12 @@ __global__ __launch_bounds__(kGThreads) void k_gram_mfma(const uint64_t* __restr
         for (int w = 0; w < kGW; ++w) next[w] = (base >= 0 && w0 + w < w_end) ? bv[base + w0 + w] : 0ull;
     };
     auto stage = [&](int64_t t) {
-        uint64_t* row = reinterpret_cast<uint64_t*>(ops + (t & 1) * kGBuf + tid * kGRowB);
+        uint32_t* row = reinterpret_cast<uint32_t*>(ops + (t & 1) * kGBuf + tid * kGRowB);
 #pragma unroll
-        for (int w = 0; w < kGW; ++w) row[w] = next[w];
+        for (int w = 0; w < kGW; ++w) {
+            row[2 * w] = (uint32_t)next[w];
+            row[2 * w + 1] = (uint32_t)(next[w] >> 32);
+        }
     };
     load(0);
     stage(0);
@@ -1318,7 +1322,7 @@ int64_t gram_tile_pairs(int64_t s) {
 }
 
 // Word chunk of the Gram: the workgroups (tile-pairs x chunks) fill whole rounds of the resident slots (one
-// 136 KiB-LDS workgroup per CU) as closely as possible over 2..8 rounds, the chunks per tile-pair rounded down to
+// 132 KiB-LDS workgroup per CU) as closely as possible over 2..8 rounds, the chunks per tile-pair rounded down to
 // a multiple of the 8 XCDs where at least 8 (xcd_unit then keeps each XCD on its own chunks), and no chunk of
 // 2^24 rows or more (the fp32 accumulators' exact range). A last round a third full costs the launch a third of
 // a round.
