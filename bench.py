@@ -25,7 +25,7 @@ workload (its first 1,000,000 transactions, same law and threshold), on this hos
 thread per host CPU, the median of three runs each; its lattice is checked equal to the GPU's on that sample.
 
 legs (secondary, not the headline): plain_60k, deep_k_1m (its 19.7 MB of bitvectors stay in the caches, so its
-rate is set by cache bandwidth, not HBM) and wide_vocab_2m with each K=2 kernel (VALU AND+popcount tiles, int8
+rate is set by cache bandwidth, not HBM) and wide_vocab_2m with each K=2 kernel (VALU AND+popcount tiles, fp4
 MFMA Gram), each checked against the oracle's committed per-level digests.
 
 served: the served_jobs stream (512 independent jobs, seed 4: transactions log-uniform in 10K-2M, vocabulary
@@ -61,9 +61,9 @@ sys.path.insert(0, str(ROOT / "tests"))
 
 HBM_PEAK_GBS = 8000.0
 VALU_CW_PEAK = 9.8e12  # candidate-words/s: 256 CUs x 64 lanes x 2.4 GHz / 4 VALU ops per word
-# candidate-words/s of the int8 MFMA Gram: 5.0 POP/s dense int8 / 128 ops per word (64 bit products, each a
-# multiply and an add)
-MFMA_I8_CW_PEAK = 5.0e15 / 128
+# candidate-words/s of the MFMA Gram: one v_mfma_scale_f32_32x32x64_f8f6f4 on fp4 operands counts 32 x 32 pairs over
+# one word (65,536 multiply-adds, 131,072 ops: 128 per candidate-word); MI355X's dense fp4 peak is 10.1 PFLOP/s.
+MFMA_FP4_CW_PEAK = 10.1e15 / 128
 CPU_SAMPLE_ROWS = 1_000_000
 DIGESTS = ROOT / "tests" / "golden" / "workload_digests.json"
 
@@ -213,7 +213,7 @@ def main(argv=None) -> int:
     k3_launch_levels = sum(1 for s in per_step if s.k >= 3)
     k2_ms, k2_cw = k2["ms"] / args.steps, k2["cw"] / args.steps
     k2_kernel = next((s.kernel for s in per_step if s.k == 2 and s.kernel), "valu")
-    k2_bound, k2_peak = {"valu": ("valu", VALU_CW_PEAK), "mfma": ("mfma", MFMA_I8_CW_PEAK)}.get(k2_kernel,
+    k2_bound, k2_peak = {"valu": ("valu", VALU_CW_PEAK), "mfma": ("mfma", MFMA_FP4_CW_PEAK)}.get(k2_kernel,
                                                                                                ("hbm", None))
     achieved = (k3_live / (k3_ms / 1000) / 1e9) if k3_ms > 0 else 0.0
     print(f"[bench] {args.steps} steps, {ms_per_step:.2f} ms/step, {n_eval:,} candidates/step, K>=3 counting "

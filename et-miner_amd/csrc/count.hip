@@ -28,6 +28,7 @@
 // A tile-pair or group none of whose candidates has status ETM_CAND_COUNT
 // returns before reading any bitvector.
 #include <algorithm>
+#include <cmath>
 
 #include "common.h"
 #include "internal.h"
@@ -1229,18 +1230,22 @@ constexpr int kGBuf = 2 * kGT * kGRowB;       // 67,584 B per buffer
 constexpr int kGThreads = 512;
 constexpr int64_t kGMaxChunkWords = ((1ll << 24) - 1) / 64 / kGW * kGW;  // rows of a chunk stay below 2^24
 
+// The grid covers the tile-pairs of an upper bound of the columns (n_tp of nt_ub tiles); the exact count s is the
+// level's one group's size, read on the device, and the tile-pairs past it return at once.
 __global__ __launch_bounds__(kGThreads) void k_gram_mfma(const uint64_t* __restrict__ bv, int64_t pitch,
                                                          int64_t n_words, int64_t chunk_words,
-                                                         const int32_t* __restrict__ cols, int64_t s, int64_t n_tp,
-                                                         int64_t n_chunks, int32_t* __restrict__ counts,
+                                                         const int32_t* __restrict__ cols,
+                                                         const int64_t* __restrict__ group_start, int64_t nt_ub,
+                                                         int64_t n_tp, int64_t n_chunks, int32_t* __restrict__ counts,
                                                          int32_t layout) {
     __shared__ __align__(16) uint8_t ops[2 * kGBuf];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int64_t chunk_id, tp;
     if (!xcd_unit(blockIdx.x % kXcds, blockIdx.x / kXcds, n_tp, n_chunks, &chunk_id, &tp)) return;
-    const int64_t nt = ceil_div(s, kGT);
+    const int64_t s = uniform64(group_start[1] - group_start[0]);
     int64_t ta, tb1;
-    etm::lex_pair_decode(uniform64(tp), nt + 1, &ta, &tb1);
+    etm::lex_pair_decode(uniform64(tp), nt_ub + 1, &ta, &tb1);
+    if (uniform64(tb1) - 1 >= ceil_div(s, kGT)) return;
     ta = uniform64(ta);
     const int64_t tb = uniform64(tb1) - 1;
     const int64_t w_begin = uniform64(chunk_id) * chunk_words;
@@ -1315,20 +1320,15 @@ __global__ __launch_bounds__(kGThreads) void k_gram_mfma(const uint64_t* __restr
     }
 }
 
-// Tile-pairs of the Gram over s columns.
-int64_t gram_tile_pairs(int64_t s) {
-    const int64_t nt = ceil_div(s, kGT);
-    return nt * (nt + 1) / 2;
-}
 
 // Word chunk of the Gram: the workgroups (tile-pairs x chunks) fill whole rounds of the resident slots (one
 // 132 KiB-LDS workgroup per CU) as closely as possible over 2..8 rounds, the chunks per tile-pair rounded down to
-// a multiple of the 8 XCDs where at least 8 (xcd_unit then keeps each XCD on its own chunks), and no chunk of
-// 2^24 rows or more (the fp32 accumulators' exact range). A last round a third full costs the launch a third of
-// a round.
+// a multiple of the 8 XCDs where at least 8 (xcd_unit then keeps each XCD on its own chunks), no chunk shorter
+// than four steps and none of 2^24 rows or more (the fp32 accumulators' exact range). A last round a third full
+// costs the launch a third of a round.
 int64_t gram_chunk(int64_t n_words, int64_t n_tp) {
     constexpr int64_t kSlots = 256;
-    const int64_t max_chunks = std::max<int64_t>(1, n_words / 512);
+    const int64_t max_chunks = std::max<int64_t>(1, n_words / (4 * kGW));
     const int64_t min_chunks = ceil_div(n_words, kGMaxChunkWords);
     int64_t best = 1;
     double best_fill = -1.0;
@@ -1767,23 +1767,26 @@ extern "C" int etm_count_level(const uint64_t* d_bv, int64_t pitch, int64_t n_wo
         ETM_REQUIRE(e == hipSuccess, ETM_EHIP, "launch of k_count_small failed: %s", hipGetErrorString(e));
         base += cls[c];
     }
-    // The pair space of a single-group level of one-item rows (K=2) without a status array: the int8 MFMA Gram
+    // The pair space of a single-group level of one-item rows (K=2) without a status array: the fp4 MFMA Gram
     // when asked for, or by default where it measured faster (gram_preferred), else the VALU tiled kernel.
     const bool gram_level = n_groups == 1 && m == 1 && d_status == nullptr;
     if (n_tile_pairs > 0 && gram_level && kernel != ETM_KERNEL_VALU) {
-        int64_t s_cols = 0;
-        ETM_HIP(hipMemcpyAsync(&s_cols, d_group_start + 1, sizeof s_cols, hipMemcpyDeviceToHost, s));
-        ETM_HIP(hipStreamSynchronize(s));
-        if (kernel == ETM_KERNEL_MFMA || etm::detail::gram_preferred(s_cols, n_words)) {
-            const int64_t n_tp = gram_tile_pairs(s_cols);
+        // n_tile_pairs counts 64-column tiles, nt64 (nt64 + 1) / 2 of them: the columns are at most 64 nt64.
+        int64_t nt64 = (int64_t)((std::sqrt(8.0 * (double)n_tile_pairs + 1.0) - 1.0) / 2.0);
+        while (nt64 * (nt64 + 1) / 2 < n_tile_pairs) ++nt64;
+        while (nt64 > 0 && (nt64 - 1) * nt64 / 2 >= n_tile_pairs) --nt64;
+        const int64_t s_ub = 64 * nt64;
+        if (kernel == ETM_KERNEL_MFMA || etm::detail::gram_preferred(s_ub, n_words)) {
+            const int64_t nt = ceil_div(s_ub, kGT);
+            const int64_t n_tp = nt * (nt + 1) / 2;
             const int64_t chunk = word_chunk > 0 ? std::min(ceil_div(word_chunk, kGW) * kGW, kGMaxChunkWords)
                                                  : gram_chunk(n_words, n_tp);
             const int64_t n_chunks = ceil_div(n_words, chunk);
             const int64_t blocks = kXcds * xcd_units(n_tp, n_chunks);
             ETM_REQUIRE(blocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",
                         (long long)blocks);
-            k_gram_mfma<<<(unsigned)blocks, kGThreads, 0, s>>>(d_bv, pitch, n_words, chunk, d_rows, s_cols, n_tp,
-                                                               n_chunks, d_counts, layout);
+            k_gram_mfma<<<(unsigned)blocks, kGThreads, 0, s>>>(d_bv, pitch, n_words, chunk, d_rows, d_group_start, nt,
+                                                               n_tp, n_chunks, d_counts, layout);
             ETM_LAUNCHED("k_gram_mfma");
             return ETM_OK;
         }

@@ -11,11 +11,11 @@ namespace detail {
 // Largest level (rows) / candidate space the single-workgroup fast paths take.
 constexpr int64_t kSingleMax = 32768;
 
-// The automatic K=2 kernel choice for a pair space of `cols` columns over `n_words` words: the int8 MFMA Gram
-// where it measured faster than the VALU AND+popcount tiles on MI355X (at least two 128-column tiles over
-// long columns: served jobs of 1.4-1.9M rows and wide_vocab_2m), the VALU tiles below (deep_k_1m's 157
-// columns, short columns).
-inline bool gram_preferred(int64_t cols, int64_t n_words) { return cols >= 256 && n_words >= 4096; }
+// The automatic K=2 kernel choice for a pair space of `cols` columns (an upper bound, rounded up to 64) over
+// `n_words` words: the fp4 MFMA Gram wherever it measured faster than the VALU AND+popcount tiles on MI355X (64 to
+// 512 columns over 256 to 31,250 words: everywhere except below 128 columns over 16,384 words or more, where the
+// two are within 10%).
+inline bool gram_preferred(int64_t cols, int64_t n_words) { return cols >= 128 || n_words < 16384; }
 
 hipError_t launch_groups_single(const int32_t* rows, const int64_t* d_n, int64_t n_cap, int32_t m, int32_t small_max,
                                 int64_t* group_start, int64_t* cum_pairs, int64_t* cum_tiles, int32_t* small_list,

@@ -569,7 +569,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
             }
             if (kern != ETM_KERNEL_ROWS)
                 kern = o.k2_kernel != ETM_KERNEL_AUTO ? o.k2_kernel
-                       : etm::detail::gram_preferred(prev.n, n_words) ? ETM_KERNEL_MFMA : ETM_KERNEL_VALU;
+                       : etm::detail::gram_preferred((prev.n + 63) / 64 * 64, n_words) ? ETM_KERNEL_MFMA : ETM_KERNEL_VALU;
         } else if (k == 3) {
             const int64_t words1 = (n1 + 63) / 64;
             const bool fits = n1 <= 65536 && n1 * words1 * 12 <= kRowsBitsMax;

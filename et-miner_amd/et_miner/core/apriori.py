@@ -259,7 +259,7 @@ def mine_csr(indptr, ids, min_support, max_length=None, prune_equal_support=Fals
     ``engine="native"`` runs the whole level loop inside libetm (one C call); ``engine="python"`` drives the
     same kernels level by level from Python through the operator wrappers (``timer`` hooks apply there).
     ``stats_out`` (a list) receives the per-level statistics. ``k2_kernel`` ("auto", "valu", "mfma",
-    "rows") picks the K=2 pair space's kernel: the int8 MFMA Gram, the VALU AND+popcount tiles or the row-wise
+    "rows") picks the K=2 pair space's kernel: the fp4 MFMA Gram, the VALU AND+popcount tiles or the row-wise
     kernel; ``k3_kernel`` ("auto", "valu", "rows") picks K=3's; ``row_order`` ("auto", "keep", "lex") the
     bitvectors' row order (native engine only; counts never depend on it).
 

@@ -7,7 +7,7 @@ exact, so the choice never changes a result, only its time. Three choices are tu
 with the earlier ones fixed at their best; the library's automatic choice stays unless another is more than 3%
 faster):
 
-* the K=2 kernel: the VALU AND+popcount tiles or the int8 MFMA Gram (from 128 columns), timed on the pair level
+* the K=2 kernel: the VALU AND+popcount tiles or the fp4 MFMA Gram, timed on the pair level
   alone; the row-wise kernel joins the candidates where the rows are sparse enough for it to have a chance
   (sum over rows of C(len, 2) below 1% of F(F-1)/2 x W, the reference's crossover times 2.5);
 * the launch schedule of levels K >= 3: every group and tile-pair, or only those holding a counted candidate

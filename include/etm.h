@@ -54,8 +54,8 @@ enum { ETM_CAND_COUNT = 0, ETM_CAND_SKIP = 1, ETM_CAND_INFER = 2 };
 enum { ETM_SUBSET_PRUNE = 1, ETM_SUBSET_INFER = 2 };
 /* Pair layouts of the K=2 count array: lexicographic (i-major), or the reference's b(b-1)/2 + a. */
 enum { ETM_LAYOUT_LEX = 0, ETM_LAYOUT_TRI = 1 };
-/* Kernel choice for the pair space of a single-group level (K=2): automatic (the int8 MFMA Gram for at least 256
- * columns over at least 4,096 words, else the VALU AND+popcount tiles), VALU tiles only, the MFMA Gram regardless
+/* Kernel choice for the pair space of a single-group level (K=2): automatic (the fp4 MFMA Gram, except below 128
+ * columns over at least 16,384 words: the VALU AND+popcount tiles), VALU tiles only, the MFMA Gram regardless
  * of size, or (etm_mine_* only) the row-wise kernel. */
 enum { ETM_KERNEL_AUTO = 0, ETM_KERNEL_VALU = 1, ETM_KERNEL_MFMA = 2, ETM_KERNEL_ROWS = 3 };
 

@@ -203,7 +203,7 @@ def test_workloads_match_oracle(name, n_rows, engine):
 
 @pytest.mark.parametrize("k2_kernel", ["valu", "mfma"])
 def test_k2_kernels_mine_identically(k2_kernel):
-    """The int8 MFMA Gram and the VALU tiles give the same lattice (wide vocabulary: hundreds of columns)."""
+    """The fp4 MFMA Gram and the VALU tiles give the same lattice (wide vocabulary: hundreds of columns)."""
     from workloads import workload
 
     w = workload("wide_vocab_2m", 150_000)

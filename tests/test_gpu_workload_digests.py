@@ -6,7 +6,7 @@ length, the number of frequent itemsets and a SHA-256 over every (itemset, count
 digest, so each itemset and each count is checked without running the oracle on the GPU host:
 
 * plain_60k (60,000 rows), deep_k_1m (1,000,000), wide_vocab_2m (2,000,000) with each K=2 kernel (the VALU
-  AND+popcount tiles and the int8 MFMA Gram), deep_sparse_20m (20,000,000 rows, max_length 16, generator
+  AND+popcount tiles and the fp4 MFMA Gram), deep_sparse_20m (20,000,000 rows, max_length 16, generator
   pruning) resident and by SON in 2,000,000-row chunks;
 * a fixed sample of 32 jobs of the served stream (jobs 0, 16, ..., 496 of the seed-4 stream: 10,000-2,000,000
   transactions each), every job alone on the plain path and in the mix of the served form.
