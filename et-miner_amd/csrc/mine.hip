@@ -1435,9 +1435,12 @@ extern "C" int etm_levels_stats(const etm_levels* lv, int32_t level, etm_level_s
     return ETM_OK;
 }
 
-extern "C" int etm_levels_flat(const etm_levels* lv, int64_t* h_totals, int64_t* h_values, int64_t* h_offsets,
-                               int64_t* h_counts) {
-    ETM_ARG(lv && h_totals, "etm_levels_flat: null pointer");
+namespace {
+
+// The flat export behind etm_levels_flat and etm_levels_frame: sizes into h_totals, then (when h_values and
+// h_offsets are given) the item ids, offsets and, where given, counts and supports (count / n_rows in double).
+int flat_export(const etm_levels* lv, int64_t* h_totals, int64_t* h_values, int64_t* h_offsets, int64_t* h_counts,
+                double* h_support, int64_t n_rows) {
     int64_t n_sets = 0, n_vals = 0;
     for (size_t l = 0; l < lv->k.size(); ++l) {
         n_sets += lv->n[l];
@@ -1445,7 +1448,8 @@ extern "C" int etm_levels_flat(const etm_levels* lv, int64_t* h_totals, int64_t*
     }
     h_totals[0] = n_sets;
     h_totals[1] = n_vals;
-    if (!h_values || !h_offsets || !h_counts) return ETM_OK;
+    if (!h_values || !h_offsets) return ETM_OK;
+    const double rows_d = (double)n_rows;
     const int64_t* c2i = lv->col_to_item.empty() ? nullptr : lv->col_to_item.data();
     // Each level's rows [lo, hi) land at fixed places (itemsets before the level, items before them), so ranges of
     // rows are written by separate threads when the result is large.
@@ -1472,10 +1476,11 @@ extern "C" int etm_levels_flat(const etm_levels* lv, int64_t* h_totals, int64_t*
         const int32_t* c = lv->counts[g.l];
         int64_t* v = h_values + g.at0;
         for (int64_t i = g.lo * k; i < g.hi * k; ++i) *v++ = c2i ? c2i[r[i]] : r[i];
-        for (int64_t i = g.lo; i < g.hi; ++i) {
-            h_counts[g.row0 + (i - g.lo)] = c[i];
-            h_offsets[g.row0 + (i - g.lo) + 1] = g.at0 + (i - g.lo + 1) * k;
-        }
+        for (int64_t i = g.lo; i < g.hi; ++i) h_offsets[g.row0 + (i - g.lo) + 1] = g.at0 + (i - g.lo + 1) * k;
+        if (h_counts)
+            for (int64_t i = g.lo; i < g.hi; ++i) h_counts[g.row0 + (i - g.lo)] = c[i];
+        if (h_support)  // IEEE division, as numpy's counts / n_rows
+            for (int64_t i = g.lo; i < g.hi; ++i) h_support[g.row0 + (i - g.lo)] = (double)c[i] / rows_d;
     };
     // Threads pay off from a few million item ids (starting one costs tens of microseconds; one thread writes
     // about a billion ids a second).
@@ -1493,6 +1498,23 @@ extern "C" int etm_levels_flat(const etm_levels* lv, int64_t* h_totals, int64_t*
         });
     for (auto& th : pool) th.join();
     return ETM_OK;
+}
+
+}  // namespace
+
+extern "C" int etm_levels_flat(const etm_levels* lv, int64_t* h_totals, int64_t* h_values, int64_t* h_offsets,
+                               int64_t* h_counts) {
+    ETM_ARG(lv && h_totals, "etm_levels_flat: null pointer");
+    if (!h_values || !h_offsets || !h_counts) h_values = h_offsets = h_counts = nullptr;
+    return flat_export(lv, h_totals, h_values, h_offsets, h_counts, nullptr, 1);
+}
+
+extern "C" int etm_levels_frame(const etm_levels* lv, int64_t* h_totals, int64_t* h_values, int64_t* h_offsets,
+                                int64_t* h_counts, double* h_support, int64_t n_rows) {
+    ETM_ARG(lv && h_totals, "etm_levels_frame: null pointer");
+    ETM_ARG((h_values == nullptr) == (h_offsets == nullptr), "etm_levels_frame: values and offsets go together");
+    ETM_ARG(h_support == nullptr || n_rows > 0, "etm_levels_frame: supports need n_rows > 0");
+    return flat_export(lv, h_totals, h_values, h_offsets, h_counts, h_support, n_rows);
 }
 
 etm_levels::~etm_levels() { pool_give(block, block_cap); }

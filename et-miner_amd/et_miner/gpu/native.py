@@ -26,14 +26,17 @@ class NativeResult:
     col_to_item: np.ndarray | None = None
     col_counts: np.ndarray | None = None
     flat: tuple | None = None  # (item ids int64, offsets int64, counts int64) over every level
+    support: np.ndarray | None = None  # count / support_rows per itemset (float64), when the export wrote it
+    support_rows: int = 0
 
     def table(self, n_rows: int, with_counts: bool = False):
         """The result as a pyarrow Table (itemset, support[, count]) built from the flat arrays."""
         import pyarrow as pa
 
         values, offsets, counts = self.flat
+        support = self.support if self.support is not None and self.support_rows == n_rows else counts / n_rows
         cols = {"itemset": pa.LargeListArray.from_arrays(pa.array(offsets), pa.array(values)),
-                "support": pa.array(counts / float(n_rows))}
+                "support": pa.array(support)}
         if with_counts:
             cols["count"] = pa.array(counts)
         return pa.table(cols)
@@ -126,19 +129,22 @@ def _opts(min_count, max_length, prune, infer, time_kernels, k2_kernel="auto", k
     return opts
 
 
-def _collect(handle, levels: bool = True) -> NativeResult:
-    """Reads a result handle (and frees it): the flat arrays always, the per-level arrays when ``levels``."""
+def _collect(handle, levels: bool = True, n_rows: int = 0) -> NativeResult:
+    """Reads a result handle (and frees it): the flat arrays always, the per-level arrays when ``levels``, and
+    with ``n_rows`` > 0 each itemset's support (count / n_rows), written by the library's export threads."""
     L = _lib.lib()
     res = NativeResult()
     try:
         tot = np.zeros(2, dtype=np.int64)
-        _lib.call("etm_levels_flat", handle, tot.ctypes.data, None, None, None)
+        _lib.call("etm_levels_frame", handle, tot.ctypes.data, None, None, None, None, 0)
         values = np.empty(tot[1], dtype=np.int64)
         offsets = np.empty(tot[0] + 1, dtype=np.int64)
         counts = np.empty(tot[0], dtype=np.int64)
-        _lib.call("etm_levels_flat", handle, tot.ctypes.data, values.ctypes.data, offsets.ctypes.data,
-                  counts.ctypes.data)
+        support = np.empty(tot[0], dtype=np.float64) if n_rows > 0 else None
+        _lib.call("etm_levels_frame", handle, tot.ctypes.data, values.ctypes.data, offsets.ctypes.data,
+                  counts.ctypes.data, None if support is None else support.ctypes.data, int(n_rows))
         res.flat = (values, offsets, counts)
+        res.support, res.support_rows = support, int(n_rows)
         n_levels = ctypes.c_int32()
         _lib.call("etm_levels_count", handle, ctypes.byref(n_levels))
         for lv in range(n_levels.value):
@@ -197,7 +203,7 @@ def mine_csr_device(indptr_d, ids_d, n_rows: int, min_count: int, max_length=Non
         _lib.call("etm_mine_csr", *args)
     else:
         cb.run("etm_mine_csr", *args)
-    return _collect(handle, levels)
+    return _collect(handle, levels, n_rows)
 
 
 def mine_batch_device(jobs, k2_kernel: str = "auto", k3_kernel: str = "auto", levels: bool = True) -> list:
@@ -252,4 +258,4 @@ def mine_bitvecs_device(bv_ptr: int, pitch: int, n_words: int, n_rows: int, col_
         _lib.call("etm_mine_bitvecs", *args)
     else:
         cb.run("etm_mine_bitvecs", *args)
-    return _collect(handle)
+    return _collect(handle, True, n_rows)

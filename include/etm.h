@@ -74,7 +74,7 @@ enum { ETM_ROW_ORDER_AUTO = 0, ETM_ROW_ORDER_KEEP = 1, ETM_ROW_ORDER_LEX = 2 };
 enum { ETM_EMIT_NONE = 0, ETM_EMIT_HOST = 1, ETM_EMIT_DEVICE = 2 };
 
 /* ABI version of this header (bumped on any signature or struct-meaning change). */
-#define ETM_ABI_VERSION 6
+#define ETM_ABI_VERSION 7
 int etm_abi_version(void);
 
 /* Copies the calling thread's last error message (NUL-terminated, truncated to n). */
@@ -451,6 +451,12 @@ int etm_levels_free(etm_levels* lv);
  * map, or column ids when the result has none), h_offsets (itemsets + 1) and h_counts. */
 int etm_levels_flat(const etm_levels* lv, int64_t* h_totals, int64_t* h_values, int64_t* h_offsets,
                     int64_t* h_counts);
+
+/* The same export for a result frame: item ids and offsets (both or neither), and, each optional, counts and
+ * supports (count / n_rows as an IEEE double division, the reference frame's `support = counts / n_rows`,
+ * core/cpu_miner.py:963), written by the exporting threads so that the caller divides nothing. */
+int etm_levels_frame(const etm_levels* lv, int64_t* h_totals, int64_t* h_values, int64_t* h_offsets,
+                     int64_t* h_counts, double* h_support, int64_t n_rows);
 
 #ifdef __cplusplus
 }

@@ -144,6 +144,28 @@ def test_readme_and_boundary():
 
 
 @pytest.mark.parametrize("engine", ENGINES)
+def test_support_column_is_count_over_rows(engine):
+    """The default frame's support (written by the library's export threads on the native route) is each
+    itemset's oracle count / n_rows as float64, bit for bit, in the reference frame's division."""
+    import pyarrow as pa
+
+    from et_miner.core.apriori import mine_csr
+
+    rng = np.random.default_rng(77)
+    indptr, ids = _csr(_rand_rows(rng, 7_001, 40, 4, dup=True))
+    n = len(indptr) - 1
+    want = miner.mine(indptr, ids, 0.01, None).as_dict()
+    frame = mine_csr(indptr, ids, 0.01, None, engine=engine)
+    table = frame if isinstance(frame, pa.Table) else frame.to_arrow()
+    sets = [tuple(sorted(x)) for x in table.column("itemset").to_pylist()]
+    got = dict(zip(sets, table.column("support").to_numpy()))
+    assert got.keys() == want.keys()
+    wk = list(want)
+    exact = np.array([want[k] for k in wk], dtype=np.int64) / n
+    assert np.array_equal(np.array([got[k] for k in wk]), exact)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("seed", range(8))
 def test_random_datasets_match_oracle(seed, engine):
     rng = np.random.default_rng(seed)
