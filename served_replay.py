@@ -3,6 +3,10 @@
     python served_replay.py --mode plain --jobs 512
     python served_replay.py --mode served --workers 8 --jobs 512
     python served_replay.py --mode served --workers 6 --batch-workers 2 --max-batch 64 --jobs 512
+    python served_replay.py --mode batch --max-batch 64
+
+``--mode batch`` keeps the stream's narrow jobs only (at most 256 frequent columns) and mines them through one
+batch worker, in batched calls of up to ``--max-batch`` jobs.
 
 Every job's dataset is generated on the host (before the GPU is touched) and made resident in HBM. The
 stream runs ``--passes`` times with every job queued at t=0 (the first pass warms the per-thread arenas); the
@@ -24,7 +28,7 @@ sys.path.insert(0, str(ROOT))
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--mode", choices=["plain", "served"], default="served")
+    ap.add_argument("--mode", choices=["plain", "served", "batch"], default="served")
     ap.add_argument("--jobs", type=int, default=512)
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--batch-workers", type=int, default=0, help="threads mining narrow jobs in batched calls")
@@ -33,9 +37,12 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
 
     from workloads import served_jobs
-    from et_miner.serve.server import Server, jobs_from_specs, run_plain, serve
+    from et_miner.serve.server import Server, batchable, jobs_from_specs, occurrence_profile, run_plain, serve
 
     jobs = jobs_from_specs(served_jobs(512)[: args.jobs], workers=min(16, os.cpu_count() or 1))
+    if args.mode == "batch":
+        jobs = [j for j in jobs if batchable(j, occurrence_profile(j.ids))]
+        args.workers, args.batch_workers = 1, 1
     import torch
 
     torch.cuda.set_device(0)
