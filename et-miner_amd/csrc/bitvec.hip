@@ -15,7 +15,7 @@
 
 namespace {
 
-constexpr int kBuildThreads = 256;
+constexpr int kBuildThreads = 512;
 
 // Ids as 16-byte vectors: 2 int64 or 4 int32 per load. The vector body covers the 16-byte aligned prefix
 // that holds whole vectors; the remaining ids go through scalar loads.
@@ -173,8 +173,9 @@ __device__ __forceinline__ void build_tile(const int64_t* __restrict__ indptr, c
     const int nc = min(slab_cols, n_cols - c0);
     for (int t = threadIdx.x; t < nc * kTileWords; t += kBuildThreads) tile[t] = 0;
     __syncthreads();
-    // Lane l of wave v takes the rows at positions w0 * 64 + 8 l + (v, v + 4): the 64 lanes of a wave spread
-    // over all kTileWords words, so a column held by many rows meets at most 8 lanes per address.
+    // Lane l of wave v takes the row at position w0 * 64 + 8 l + v (eight waves, one row a thread: twice the
+    // waves of four with two rows each, to hide the dependent indptr -> ids -> LUT loads): the 64 lanes of a
+    // wave spread over all kTileWords words, so a column held by many rows meets at most 8 lanes per address.
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int j = wv; j < 8; j += kBuildThreads / 64) {
         const int l = 8 * lane + j;  // 0..511: word l / 64, bit l % 64
@@ -223,15 +224,32 @@ __global__ __launch_bounds__(kBuildThreads) void k_bitvec_build_lds(const int64_
 }
 
 
-// One workgroup per column: sum of popcounts over the column's words.
+// One workgroup per (column, chunk of kColpopChunk words): the sum of popcounts over the chunk's words, four
+// loads in flight per thread. A column of one chunk stores its sum; with more chunks each adds its partial sum
+// to the zeroed output (one 64-bit atomic per workgroup).
+constexpr int64_t kColpopChunk = 16384;
+
 __global__ __launch_bounds__(256) void k_colpop(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
                                                  int64_t* __restrict__ out) {
     __shared__ int64_t lds[8];
     const uint64_t* col = bv + (int64_t)blockIdx.x * pitch;
+    const int64_t w0 = (int64_t)blockIdx.y * kColpopChunk;
+    const int64_t w1 = min(n_words, w0 + kColpopChunk);
     int64_t s = 0;
-    for (int64_t w = threadIdx.x; w < n_words; w += blockDim.x) s += __popcll(col[w]);
+    int64_t w = w0 + threadIdx.x;
+    for (; w + 768 < w1; w += 1024)
+        s += __popcll(col[w]) + __popcll(col[w + 256]) + __popcll(col[w + 512]) + __popcll(col[w + 768]);
+    for (; w < w1; w += 256) s += __popcll(col[w]);
     s = etm::block_sum<256>(s, lds);
-    if (threadIdx.x == 0) out[blockIdx.x] = s;
+    if (threadIdx.x == 0) {
+        if (gridDim.y == 1) out[blockIdx.x] = s;
+        else atomicAdd((unsigned long long*)&out[blockIdx.x], (unsigned long long)s);
+    }
+}
+
+__global__ void k_zero_counts(int64_t* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 // ── the input layer of a batch of jobs (batch.h InJob): one launch per phase over every job ──────────────
@@ -586,7 +604,15 @@ extern "C" int etm_column_popcounts(const uint64_t* d_bv, int32_t n_cols, int64_
     ETM_ARG(d_bv != nullptr && d_out != nullptr, "etm_column_popcounts: null pointer");
     ETM_ARG(n_cols >= 0 && n_words >= 0 && pitch >= n_words, "etm_column_popcounts: bad sizes");
     if (n_cols == 0) return ETM_OK;
-    k_colpop<<<n_cols, 256, 0, ETM_STREAM(stream)>>>(d_bv, pitch, n_words, d_out);
+    hipStream_t s = ETM_STREAM(stream);
+    const int64_t chunks = std::max<int64_t>(1, (n_words + kColpopChunk - 1) / kColpopChunk);
+    ETM_REQUIRE(chunks <= 65535, ETM_EOVERFLOW, "etm_column_popcounts: %lld word chunks exceed the grid",
+                (long long)chunks);
+    if (chunks > 1) {
+        k_zero_counts<<<(unsigned)((n_cols + 255) / 256), 256, 0, s>>>(d_out, n_cols);
+        ETM_LAUNCHED("k_zero_counts");
+    }
+    k_colpop<<<dim3((unsigned)n_cols, (unsigned)chunks), 256, 0, s>>>(d_bv, pitch, n_words, d_out);
     ETM_LAUNCHED("k_colpop");
     return ETM_OK;
 }

@@ -27,6 +27,14 @@
 //
 // A tile-pair or group none of whose candidates has status ETM_CAND_COUNT
 // returns before reading any bitvector.
+//
+// Further sections: the live-unit schedule (k_live_units lists the groups and
+// tile-pairs holding a counted candidate; k_count_small_live,
+// k_count_tiled_live and the register-only k_count_sparse_live count them),
+// the group-listed kernels for tiled groups of few counted pairs (k_gl_scan,
+// k_count_glist), the K=2 pair space on the fp4 matrix cores (k_gram_mfma),
+// arbitrary itemsets (k_count_itemsets), and the queue-fed counting kernels of
+// batched jobs (k_batch_plan, k_batch_count_small, k_batch_count_tiled).
 #include <algorithm>
 #include <cmath>
 

@@ -8,8 +8,13 @@
 namespace etm {
 namespace detail {
 
-// Largest level (rows) / candidate space the single-workgroup fast paths take.
+// Largest level (rows) / candidate space the single-workgroup fast paths take, and the most items (rows or
+// candidates x their items) they handle: one workgroup writes or compares every item, so wide levels go to the
+// multi-block paths sooner (deep_sparse_20m's K=13, 24,938 candidates of 13 items: 310 us compacted by one
+// workgroup).
 constexpr int64_t kSingleMax = 32768;
+constexpr int64_t kSingleItems = 65536;
+inline bool single_fits(int64_t n, int64_t items) { return n <= kSingleMax && n * items <= kSingleItems; }
 
 // The automatic K=2 kernel choice for a pair space of `cols` columns (an upper bound, rounded up to 64) over
 // `n_words` words: the fp4 MFMA Gram wherever it measured faster than the VALU AND+popcount tiles on MI355X (64 to

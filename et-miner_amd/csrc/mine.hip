@@ -324,7 +324,7 @@ GroupsDev alloc_groups(Arena& a, int64_t n) {
 // Group build of a level (single workgroup when small and d_n given; else the multi-block scans, which need n).
 void build_groups(Arena& a, const Level& lv, const int64_t* d_n, int64_t n_cap, GroupsDev& g, int64_t* d_totals,
                   int32_t small_max, hipStream_t s) {
-    if (n_cap <= etm::detail::kSingleMax) {
+    if (etm::detail::single_fits(n_cap, lv.m)) {
         check(etm::detail::launch_groups_single(lv.rows, d_n, n_cap, lv.m, small_max, g.gs, g.cp, g.ct, g.list,
                                                 d_totals, s),
               "group build");
@@ -634,8 +634,9 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         Level cur;
         cur.m = k;
         int64_t cap;
+        bool cap_exact = false;  // cap is the survivors' count itself, read back (not a bound)
         GroupsDev gn{};
-        if (n_cands <= etm::detail::kSingleMax) {
+        if (etm::detail::single_fits(n_cands, k)) {
             cap = n_cands;
             cur.rows = a.get<int32_t>(cap * k);
             cur.counts = a.get<int32_t>(cap);
@@ -651,6 +652,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
             check(hipMemcpyAsync(host.p, cur_scal, 8, hipMemcpyDeviceToHost, s), "readback");
             wait_stream(s);
             cap = host.p[0];
+            cap_exact = true;
             cur.rows = a.get<int32_t>(cap * k);
             cur.counts = a.get<int32_t>(cap);
             if (cap)
@@ -669,7 +671,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
                 sel.m = k;
                 sel.rows = a.get<int32_t>(cap * k);
                 sel.counts = a.get<int32_t>(cap);
-                if (cap <= etm::detail::kSingleMax) {
+                if (etm::detail::single_fits(cap, k)) {
                     check(etm::detail::launch_select_single(cur.rows, d_n, cap, k, cur.counts, cur.free, sel.rows,
                                                             sel.counts, cur_scal + 4, s),
                           "select");
@@ -694,13 +696,19 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
                 d_n = cur_scal + 4;
             }
         }
-        // The next level's groups, sized on the device when the level is small.
-        const bool early = cap <= etm::detail::kSingleMax && k + 1 <= max_len;
+        // The next level's groups, sized on the device when the level is small. A large level whose size the
+        // host already holds (no free-set selection after the count) builds them here as well, so that the
+        // readback below serves the level's scalars and the group totals at once.
+        const bool early = etm::detail::single_fits(cap, k) && k + 1 <= max_len;
+        const bool prebuilt = !early && cap_exact && !prune && cap > 0 && k + 1 <= max_len;
         if (early) {
             gn = alloc_groups(a, cap);
             check(etm::detail::launch_groups_single(cur.rows, d_n, cap, k, small_max, gn.gs, gn.cp, gn.ct, gn.list,
                                                     cur_scal + 8, s),
                   "group build");
+        } else if (prebuilt) {
+            gn = alloc_groups(a, cap);
+            build_groups(a, cur, nullptr, cap, gn, cur_scal + 8, small_max, s);
         }
         check(hipMemcpyAsync(host.p, cur_scal, kScal * 8, hipMemcpyDeviceToHost, s), "readback");
         wait_stream(s);
@@ -731,7 +739,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
                cur.n > 0 ? cur.rows : nullptr, cur.n > 0 ? cur.counts : nullptr);
         t_level = Clock::now();
         if (cur.n == 0) break;
-        if (early) {
+        if (early || prebuilt) {
             std::memcpy(tot, host.p + 8, sizeof tot);
             g = gn;
         } else if (k + 1 <= max_len) {
