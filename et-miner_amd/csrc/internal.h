@@ -2,6 +2,8 @@
 // and returns the launch status. A non-null d_n means "the count lives on the device at d_n, at most n_cap".
 #pragma once
 
+#include <functional>
+
 #include "batch.h"
 #include "common.h"
 
@@ -21,6 +23,11 @@ inline bool single_fits(int64_t n, int64_t items) { return n <= kSingleMax && n 
 // 512 columns over 256 to 31,250 words: everywhere except below 128 columns over 16,384 words or more, where the
 // two are within 10%).
 inline bool gram_preferred(int64_t cols, int64_t n_words) { return cols >= 128 || n_words < 16384; }
+
+// The process's pool of host threads (uploads, result exports): fn(0 .. parts - 1), part 0 on the caller, the
+// rest on the pool; returns when all are done. parallel_width() is the pool's threads plus the caller.
+int parallel_width();
+void parallel_for(int parts, const std::function<void(int)>& fn);
 
 hipError_t launch_groups_single(const int32_t* rows, const int64_t* d_n, int64_t n_cap, int32_t m, int32_t small_max,
                                 int64_t* group_start, int64_t* cum_pairs, int64_t* cum_tiles, int32_t* small_list,

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdlib>
 #include <cstring>
 #include <exception>
@@ -21,8 +22,53 @@
 #include <thread>
 #include <vector>
 
+#include <sys/mman.h>
+
 #include "common.h"
 #include "internal.h"
+
+// A host array in an anonymous mapping that grows by remapping (no copy), with huge pages advised: the exported
+// result of a large lattice, written by host threads while later levels are still being mined.
+class etm_host_array {
+   public:
+    etm_host_array() = default;
+    etm_host_array(const etm_host_array&) = delete;
+    etm_host_array& operator=(const etm_host_array&) = delete;
+    ~etm_host_array() {
+        if (p_) munmap(p_, cap_);
+    }
+    char* data() const { return (char*)p_; }
+    // Capacity of at least `bytes` (false when the mapping fails); the contents are kept.
+    bool reserve(size_t bytes) {
+        if (bytes <= cap_) return true;
+        constexpr size_t kHuge = 2u << 20;
+        size_t want = std::max(bytes, cap_ * 2);
+        want = (want + kHuge - 1) / kHuge * kHuge;
+        void* q = p_ ? mremap(p_, cap_, want, MREMAP_MAYMOVE)
+                     : mmap(nullptr, want, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
+        if (q == MAP_FAILED) return false;
+        (void)madvise(q, want, MADV_HUGEPAGE);
+        p_ = q;
+        cap_ = want;
+        return true;
+    }
+
+   private:
+    void* p_ = nullptr;
+    size_t cap_ = 0;
+};
+
+// The result as a frame (item ids, offsets, counts and supports over every level, in level order), built during
+// the call when the lattice has large levels.
+struct etm_frame {
+    etm_host_array values, offsets, counts, support;
+    int64_t n_sets = 0, n_vals = 0;
+    bool reserve(int64_t sets, int64_t vals) {
+        return values.reserve((size_t)std::max<int64_t>(vals, 1) * 8) && offsets.reserve((size_t)(sets + 1) * 8) &&
+               counts.reserve((size_t)std::max<int64_t>(sets, 1) * 8) &&
+               support.reserve((size_t)std::max<int64_t>(sets, 1) * 8);
+    }
+};
 
 // Host result of a mining call. Level 1 lives in host vectors; the levels from K=2 on stay in the pinned block
 // their device-to-host copy landed in (taken from a process-wide pool and given back when the result is
@@ -39,6 +85,10 @@ struct etm_levels {
     std::vector<int64_t> col_to_item;
     std::vector<int64_t> col_counts;
     std::vector<int32_t> owned;  // host copy of levels K >= 2 of a batched job (etm_mine_batch)
+    // Large levels copied to the host while later levels were mined: one pinned pool block each.
+    std::vector<std::pair<int32_t*, int64_t>> level_blocks;
+    etm_frame* frame = nullptr;  // the prebuilt frame, or null
+    bool rows_released = false;  // levels K >= 2 no longer readable (etm_levels_frame_view released them)
     ~etm_levels();
 };
 
@@ -266,6 +316,140 @@ void pool_give(int32_t* p, int64_t cap) {
     }
 }
 
+// Writes a level (rows: n x k column ids, cnts: n counts) at itemset set0 / item val0 of the frame: item ids
+// through c2i (or the column ids themselves), offsets, counts and supports (count / n_rows); ranges of rows go to
+// the host pool's threads when the level is large.
+void frame_write(etm_frame& f, const int32_t* rows, const int32_t* cnts, int64_t n, int32_t k, int64_t set0,
+                 int64_t val0, const int64_t* c2i, double n_rows_d) {
+    if (n <= 0) return;
+    int64_t* v = (int64_t*)f.values.data();
+    int64_t* off = (int64_t*)f.offsets.data();
+    int64_t* c = (int64_t*)f.counts.data();
+    double* sp = (double*)f.support.data();
+    const int parts = (int)std::min<int64_t>(etm::detail::parallel_width(), std::max<int64_t>(1, n * k >> 17));
+    const int64_t per = (n + parts - 1) / parts;
+    etm::detail::parallel_for(parts, [&](int p) {
+        const int64_t lo = p * per, hi = std::min(n, lo + per);
+        for (int64_t i = lo * k; i < hi * k; ++i) v[val0 + i] = c2i ? c2i[rows[i]] : rows[i];
+        for (int64_t i = lo; i < hi; ++i) {
+            off[set0 + i + 1] = val0 + (i + 1) * k;
+            c[set0 + i] = cnts[i];
+            sp[set0 + i] = (double)cnts[i] / n_rows_d;
+        }
+    });
+}
+
+// The calling thread's stream for result copies (non-blocking, default priority) and its events, created on
+// first use on a device and kept for the process.
+hipStream_t export_stream() {
+    static thread_local hipStream_t xs = nullptr;
+    static thread_local int xs_device = -1;
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    if (xs == nullptr || xs_device != dev) {
+        check(hipStreamCreateWithFlags(&xs, hipStreamNonBlocking), "export stream");
+        xs_device = dev;
+    }
+    return xs;
+}
+
+hipEvent_t export_event(size_t i) {
+    static thread_local std::vector<hipEvent_t> pool;
+    static thread_local int pool_device = -1;
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev != pool_device) {
+        pool.clear();  // events of another device stay with it
+        pool_device = dev;
+    }
+    while (pool.size() <= i) {
+        hipEvent_t e = nullptr;
+        check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+        pool.push_back(e);
+    }
+    return pool[i];
+}
+
+// A mining call's exporter: a host thread that waits for each large level's copy to land in pinned memory and
+// writes it into the frame while later levels are mined. finish() waits for every level pushed and rethrows the
+// first failure; the destructor waits as well (copies read device memory the caller frees afterwards).
+class Exporter {
+   public:
+    struct Task {
+        hipEvent_t done;
+        const int32_t* rows;
+        const int32_t* cnts;
+        int64_t n;
+        int32_t k;
+        int64_t set0, val0;
+    };
+    Exporter(etm_frame& f, const int64_t* c2i, double n_rows_d, int device)
+        : f_(f), c2i_(c2i), n_rows_d_(n_rows_d), device_(device) {
+        th_ = std::thread([this] { loop(); });
+    }
+    Exporter(const Exporter&) = delete;
+    Exporter& operator=(const Exporter&) = delete;
+    ~Exporter() { join(); }
+    void push(const Task& t) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            q_.push_back(t);
+        }
+        cv_.notify_one();
+    }
+    void finish() {
+        join();
+        if (err_) std::rethrow_exception(err_);
+    }
+
+   private:
+    void join() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            closed_ = true;
+        }
+        cv_.notify_one();
+        if (th_.joinable()) th_.join();
+    }
+    void loop() {
+        (void)hipSetDevice(device_);
+        for (;;) {
+            Task t;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return closed_ || !q_.empty(); });
+                if (q_.empty()) return;
+                t = q_.front();
+                q_.erase(q_.begin());
+            }
+            try {
+                const hipError_t e = hipEventSynchronize(t.done);  // the copy has landed (or failed)
+                if (err_) continue;
+                check(e, "result copy");
+                if (!f_.reserve(t.set0 + t.n, t.val0 + t.n * t.k))
+                    throw Error(ETM_ENOMEM, "the host result frame could not grow");
+                frame_write(f_, t.rows, t.cnts, t.n, t.k, t.set0, t.val0, c2i_, n_rows_d_);
+            } catch (...) {
+                if (!err_) err_ = std::current_exception();
+            }
+        }
+    }
+    etm_frame& f_;
+    const int64_t* c2i_;
+    double n_rows_d_;
+    int device_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::vector<Task> q_;
+    bool closed_ = false;
+    std::exception_ptr err_;
+    std::thread th_;
+};
+
+// Levels of at least this many ints (rows and counts) are copied to the host and exported as soon as they are
+// compacted; smaller ones come back in the final gather.
+constexpr int64_t kExportEarlyInts = 1 << 18;
+
 using Clock = std::chrono::steady_clock;
 
 double ms_since(Clock::time_point t) {
@@ -491,8 +675,15 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         int32_t* counts;
         int64_t n;
         int32_t k;
+        int32_t* host = nullptr;  // the pinned copy of a level exported early
+        int64_t set0 = 0, val0 = 0;  // its place in the frame
     };
     std::vector<Pending> levels;
+    // The frame: K=1 first, then each level in order; the exporter starts at the first large level.
+    int64_t frame_sets = emit ? 0 : n1, frame_vals = emit ? 0 : n1;
+    std::unique_ptr<Exporter> exporter;
+    hipStream_t xs = nullptr;
+    size_t n_export = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
 
     // Device scalars of every level (k <= 62 + 2), zeroed once.
@@ -733,7 +924,39 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         }
         st.wall_ms = (float)ms_since(t_level);
         out.stats.push_back(st);
-        levels.push_back(emit ? Pending{nullptr, nullptr, 0, k} : Pending{cur.rows, cur.counts, cur.n, k});
+        if (emit) {
+            levels.push_back(Pending{nullptr, nullptr, 0, k});
+        } else {
+            Pending pd{cur.rows, cur.counts, cur.n, k};
+            pd.set0 = frame_sets;
+            pd.val0 = frame_vals;
+            frame_sets += cur.n;
+            frame_vals += cur.n * k;
+            if (cur.n * (int64_t)(k + 1) >= kExportEarlyInts) {
+                // The level is final (the readback above waited for its compaction): copy it to the host on the
+                // export stream and let the exporter write it into the frame while the next levels run.
+                if (!exporter) {
+                    out.frame = new etm_frame();
+                    int dev = 0;
+                    check(hipGetDevice(&dev), "hipGetDevice");
+                    exporter = std::make_unique<Exporter>(*out.frame,
+                                                          out.col_to_item.empty() ? nullptr : out.col_to_item.data(),
+                                                          (double)n_rows, dev);
+                    xs = export_stream();
+                }
+                int64_t bcap = 0;
+                pd.host = pool_take(cur.n * (k + 1), &bcap);
+                out.level_blocks.emplace_back(pd.host, bcap);
+                check(hipMemcpyAsync(pd.host, cur.rows, (size_t)(cur.n * k) * 4, hipMemcpyDeviceToHost, xs),
+                      "level copy");
+                check(hipMemcpyAsync(pd.host + cur.n * k, cur.counts, (size_t)cur.n * 4, hipMemcpyDeviceToHost, xs),
+                      "level copy");
+                const hipEvent_t done = export_event(n_export++);
+                check(hipEventRecord(done, xs), "event record");
+                exporter->push(Exporter::Task{done, pd.host, pd.host + cur.n * k, cur.n, k, pd.set0, pd.val0});
+            }
+            levels.push_back(pd);
+        }
         if (!emit) host_bytes += cur.n * (int64_t)(k + 1) * 4;
         report(o, out, st, h_rows, h_counts, a.bytes(), host_bytes, k + 1 <= max_len && cur.n >= k + 1,
                cur.n > 0 ? cur.rows : nullptr, cur.n > 0 ? cur.counts : nullptr);
@@ -753,9 +976,10 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
         prev2 = prev;
         prev = cur;
     }
-    // Every level back to the host in one pass, through one pinned staging buffer.
+    // Every level not yet copied back to the host in one pass, through one pinned staging buffer.
     int64_t total_ints = 0;
-    for (const Pending& p : levels) total_ints += p.n * (p.k + 1);
+    for (const Pending& p : levels)
+        if (!p.host) total_ints += p.n * (p.k + 1);
     int64_t cap_ints = 0;
     int32_t* stage = pool_take(std::max<int64_t>(total_ints, 1), &cap_ints);
     out.block = stage;
@@ -776,6 +1000,7 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
             longest = 0;
         };
         for (const Pending& p : levels) {
+            if (p.host) continue;
             if (p.n) {
                 segs.s[ns++] = Seg{p.rows, p.n * p.k, at};
                 segs.s[ns++] = Seg{p.counts, p.n, at + p.n * p.k};
@@ -792,9 +1017,29 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
     for (const Pending& p : levels) {
         out.k.push_back(p.k);
         out.n.push_back(p.n);
+        if (p.host) {
+            out.rows.push_back(p.host);
+            out.counts.push_back(p.host + p.n * p.k);
+            continue;
+        }
         out.rows.push_back(stage + at);
         out.counts.push_back(stage + at + p.n * p.k);
         at += p.n * (p.k + 1);
+    }
+    if (exporter) {
+        // The frame: the exporter's levels, then K=1 and the levels of the final gather.
+        exporter->finish();
+        etm_frame& f = *out.frame;
+        if (!f.reserve(frame_sets, frame_vals)) throw Error(ETM_ENOMEM, "the host result frame could not grow");
+        ((int64_t*)f.offsets.data())[0] = 0;
+        const int64_t* c2i = out.col_to_item.empty() ? nullptr : out.col_to_item.data();
+        frame_write(f, out.k1_rows.data(), out.k1_counts.data(), n1, 1, 0, 0, c2i, (double)n_rows);
+        for (size_t l = 0; l < levels.size(); ++l) {
+            const Pending& p = levels[l];
+            if (!p.host) frame_write(f, out.rows[l + 1], out.counts[l + 1], p.n, p.k, p.set0, p.val0, c2i, (double)n_rows);
+        }
+        f.n_sets = frame_sets;
+        f.n_vals = frame_vals;
     }
     for (size_t i = 0; i < events.size(); ++i) {
         float ms = 0;
@@ -1419,6 +1664,7 @@ extern "C" int etm_levels_shape(const etm_levels* lv, int32_t level, int32_t* h_
 extern "C" int etm_levels_copy(const etm_levels* lv, int32_t level, int32_t* h_rows, int64_t* h_counts) {
     ETM_ARG(lv, "etm_levels_copy: null result");
     ETM_ARG(level >= 0 && level < (int32_t)lv->k.size(), "etm_levels_copy: level %d out of range", level);
+    ETM_ARG(level == 0 || !lv->rows_released, "etm_levels_copy: the levels were released by etm_levels_frame_view");
     const int64_t n = lv->n[level];
     if (h_rows && n) std::memcpy(h_rows, lv->rows[level], (size_t)(n * lv->k[level]) * 4);
     if (h_counts)
@@ -1513,6 +1759,7 @@ int flat_export(const etm_levels* lv, int64_t* h_totals, int64_t* h_values, int6
 extern "C" int etm_levels_flat(const etm_levels* lv, int64_t* h_totals, int64_t* h_values, int64_t* h_offsets,
                                int64_t* h_counts) {
     ETM_ARG(lv && h_totals, "etm_levels_flat: null pointer");
+    ETM_ARG(!lv->rows_released, "etm_levels_flat: the levels were released by etm_levels_frame_view");
     if (!h_values || !h_offsets || !h_counts) h_values = h_offsets = h_counts = nullptr;
     return flat_export(lv, h_totals, h_values, h_offsets, h_counts, nullptr, 1);
 }
@@ -1520,12 +1767,41 @@ extern "C" int etm_levels_flat(const etm_levels* lv, int64_t* h_totals, int64_t*
 extern "C" int etm_levels_frame(const etm_levels* lv, int64_t* h_totals, int64_t* h_values, int64_t* h_offsets,
                                 int64_t* h_counts, double* h_support, int64_t n_rows) {
     ETM_ARG(lv && h_totals, "etm_levels_frame: null pointer");
+    ETM_ARG(!lv->rows_released, "etm_levels_frame: the levels were released by etm_levels_frame_view");
     ETM_ARG((h_values == nullptr) == (h_offsets == nullptr), "etm_levels_frame: values and offsets go together");
     ETM_ARG(h_support == nullptr || n_rows > 0, "etm_levels_frame: supports need n_rows > 0");
     return flat_export(lv, h_totals, h_values, h_offsets, h_counts, h_support, n_rows);
 }
 
-etm_levels::~etm_levels() { pool_give(block, block_cap); }
+extern "C" int etm_levels_frame_view(etm_levels* lv, int64_t* h_totals, void** h_ptrs, int32_t release_rows) {
+    ETM_ARG(lv && h_totals && h_ptrs, "etm_levels_frame_view: null pointer");
+    if (lv->frame == nullptr) {
+        h_totals[0] = h_totals[1] = -1;
+        return ETM_OK;
+    }
+    const etm_frame& f = *lv->frame;
+    h_totals[0] = f.n_sets;
+    h_totals[1] = f.n_vals;
+    h_ptrs[0] = f.values.data();
+    h_ptrs[1] = f.offsets.data();
+    h_ptrs[2] = f.counts.data();
+    h_ptrs[3] = f.support.data();
+    if (release_rows && !lv->rows_released) {
+        pool_give(lv->block, lv->block_cap);
+        lv->block = nullptr;
+        lv->block_cap = 0;
+        for (auto& b : lv->level_blocks) pool_give(b.first, b.second);
+        lv->level_blocks.clear();
+        lv->rows_released = true;
+    }
+    return ETM_OK;
+}
+
+etm_levels::~etm_levels() {
+    pool_give(block, block_cap);
+    for (auto& b : level_blocks) pool_give(b.first, b.second);
+    delete frame;
+}
 
 extern "C" int etm_levels_free(etm_levels* lv) {
     delete lv;

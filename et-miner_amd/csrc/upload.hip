@@ -21,6 +21,7 @@ namespace {
 constexpr int64_t kChunk = 32ll << 20;  // bytes per staging buffer
 constexpr int kRing = 3;                // staging buffers per calling thread
 constexpr int kCopyThreads = 8;         // host threads per chunk copy (the caller included)
+constexpr int kPoolThreads = 15;        // threads of the shared pool (uploads and result exports)
 constexpr int64_t kParallelMin = 1 << 20;  // below this a chunk is copied by the caller alone
 
 // A fixed pool of host threads for the chunk copies, shared by every calling thread (tasks queue in order).
@@ -57,7 +58,7 @@ class CopyPool {
    private:
     CopyPool() {
         const unsigned hw = std::max(2u, std::thread::hardware_concurrency());
-        const int n = (int)std::min<unsigned>(kCopyThreads - 1, hw - 1);
+        const int n = (int)std::min<unsigned>(kPoolThreads, hw - 1);
         for (int t = 0; t < n; ++t) threads_.emplace_back([this] { loop(); });
         for (auto& t : threads_) t.detach();
     }
@@ -114,7 +115,8 @@ hipError_t ring_for_device(Ring& r) {
 
 void copy_parallel(char* dst, const char* src, int64_t n) {
     CopyPool& pool = CopyPool::get();
-    const int parts = n < kParallelMin ? 1 : std::min<int>(pool.size() + 1, (int)(n / (kParallelMin / 2)));
+    const int parts = n < kParallelMin ? 1 : std::min<int>(std::min(pool.size() + 1, kCopyThreads),
+                                                           (int)(n / (kParallelMin / 2)));
     if (parts <= 1) {
         std::memcpy(dst, src, (size_t)n);
         return;
@@ -127,6 +129,22 @@ void copy_parallel(char* dst, const char* src, int64_t n) {
 }
 
 }  // namespace
+
+namespace etm {
+namespace detail {
+
+int parallel_width() { return CopyPool::get().size() + 1; }
+
+void parallel_for(int parts, const std::function<void(int)>& fn) {
+    if (parts <= 1) {
+        if (parts == 1) fn(0);
+        return;
+    }
+    CopyPool::get().run(parts, fn);
+}
+
+}  // namespace detail
+}  // namespace etm
 
 extern "C" int etm_upload(const void* h_src, int64_t bytes, void* d_dst, void* stream) {
     ETM_ARG(bytes >= 0, "etm_upload: negative size");

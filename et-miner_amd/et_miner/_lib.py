@@ -17,7 +17,7 @@ from ctypes import POINTER, c_char_p, c_int, c_int32, c_int64, c_size_t, c_void_
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libetm.so"
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 ETM_OK, ETM_EINVAL, ETM_ENOMEM, ETM_EHIP, ETM_EOVERFLOW, ETM_ENODEV, ETM_ECANCELED = 0, 1, 2, 3, 4, 5, 6
 CAND_COUNT, CAND_SKIP, CAND_INFER = 0, 1, 2
@@ -99,6 +99,7 @@ SIGNATURES: dict = {
     "etm_levels_free": (c_int, [P]),
     "etm_levels_flat": (c_int, [P, P, P, P, P]),
     "etm_levels_frame": (c_int, [P, P, P, P, P, P, c_int64]),
+    "etm_levels_frame_view": (c_int, [P, P, P, c_int32]),
 }
 
 

@@ -129,22 +129,57 @@ def _opts(min_count, max_length, prune, infer, time_kernels, k2_kernel="auto", k
     return opts
 
 
+class _ResultOwner:
+    """Owns a result handle whose frame arrays are viewed in place: freed with the last array viewing them."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    def __del__(self):
+        if self.handle is None:
+            return
+        try:
+            _lib.lib().etm_levels_free(self.handle)
+        except Exception:  # interpreter shutdown: the library may be gone already
+            pass
+
+
+class _FrameView:
+    """One of a result's frame arrays, as numpy sees it (``np.asarray`` keeps this object, hence the owner)."""
+
+    def __init__(self, owner: _ResultOwner, ptr: int, n: int, typestr: str):
+        self.owner = owner
+        self.__array_interface__ = {"shape": (n,), "typestr": typestr, "data": (ptr, False), "version": 3}
+
+
+def _frame_view(handle):
+    """The frame the mining call built while it ran (large lattices), as numpy arrays over the library's memory,
+    or None; the result's per-level rows are released."""
+    tot = np.zeros(2, dtype=np.int64)
+    ptrs = (ctypes.c_void_p * 4)()
+    _lib.call("etm_levels_frame_view", handle, tot.ctypes.data, ptrs, 1)
+    if tot[0] < 0:
+        return None
+    owner = _ResultOwner(handle)
+    n_sets, n_vals = int(tot[0]), int(tot[1])
+
+    def view(i, n, typestr):
+        return np.asarray(_FrameView(owner, ptrs[i], n, typestr)) if n > 0 else np.empty(0, dtype=np.dtype(typestr))
+
+    try:
+        return view(0, n_vals, "<i8"), view(1, n_sets + 1, "<i8"), view(2, n_sets, "<i8"), view(3, n_sets, "<f8")
+    except BaseException:
+        owner.handle = None  # the caller still frees the handle
+        raise
+
+
 def _collect(handle, levels: bool = True, n_rows: int = 0) -> NativeResult:
-    """Reads a result handle (and frees it): the flat arrays always, the per-level arrays when ``levels``, and
-    with ``n_rows`` > 0 each itemset's support (count / n_rows), written by the library's export threads."""
+    """Reads a result handle (and frees it, or hands it to the arrays viewing its frame): the flat arrays
+    always, the per-level arrays when ``levels``, and with ``n_rows`` > 0 each itemset's support (count /
+    n_rows), written by the library's export threads."""
     L = _lib.lib()
     res = NativeResult()
     try:
-        tot = np.zeros(2, dtype=np.int64)
-        _lib.call("etm_levels_frame", handle, tot.ctypes.data, None, None, None, None, 0)
-        values = np.empty(tot[1], dtype=np.int64)
-        offsets = np.empty(tot[0] + 1, dtype=np.int64)
-        counts = np.empty(tot[0], dtype=np.int64)
-        support = np.empty(tot[0], dtype=np.float64) if n_rows > 0 else None
-        _lib.call("etm_levels_frame", handle, tot.ctypes.data, values.ctypes.data, offsets.ctypes.data,
-                  counts.ctypes.data, None if support is None else support.ctypes.data, int(n_rows))
-        res.flat = (values, offsets, counts)
-        res.support, res.support_rows = support, int(n_rows)
         n_levels = ctypes.c_int32()
         _lib.call("etm_levels_count", handle, ctypes.byref(n_levels))
         for lv in range(n_levels.value):
@@ -164,8 +199,26 @@ def _collect(handle, levels: bool = True, n_rows: int = 0) -> NativeResult:
         cnt = np.empty(n_cols.value, dtype=np.int64)
         _lib.call("etm_levels_columns", handle, ctypes.byref(n_cols), c2i.ctypes.data, cnt.ctypes.data)
         res.col_to_item, res.col_counts = c2i, cnt
+        view = _frame_view(handle)
+        if view is not None:
+            handle = None  # the arrays own it now
+            values, offsets, counts, support = view
+            res.flat = (values, offsets, counts)
+            res.support, res.support_rows = (support, int(n_rows)) if n_rows > 0 else (None, 0)
+            return res
+        tot = np.zeros(2, dtype=np.int64)
+        _lib.call("etm_levels_frame", handle, tot.ctypes.data, None, None, None, None, 0)
+        values = np.empty(tot[1], dtype=np.int64)
+        offsets = np.empty(tot[0] + 1, dtype=np.int64)
+        counts = np.empty(tot[0], dtype=np.int64)
+        support = np.empty(tot[0], dtype=np.float64) if n_rows > 0 else None
+        _lib.call("etm_levels_frame", handle, tot.ctypes.data, values.ctypes.data, offsets.ctypes.data,
+                  counts.ctypes.data, None if support is None else support.ctypes.data, int(n_rows))
+        res.flat = (values, offsets, counts)
+        res.support, res.support_rows = support, int(n_rows)
     finally:
-        L.etm_levels_free(handle)
+        if handle is not None:
+            L.etm_levels_free(handle)
     return res
 
 

@@ -74,7 +74,7 @@ enum { ETM_ROW_ORDER_AUTO = 0, ETM_ROW_ORDER_KEEP = 1, ETM_ROW_ORDER_LEX = 2 };
 enum { ETM_EMIT_NONE = 0, ETM_EMIT_HOST = 1, ETM_EMIT_DEVICE = 2 };
 
 /* ABI version of this header (bumped on any signature or struct-meaning change). */
-#define ETM_ABI_VERSION 7
+#define ETM_ABI_VERSION 8
 int etm_abi_version(void);
 
 /* Copies the calling thread's last error message (NUL-terminated, truncated to n). */
@@ -457,6 +457,16 @@ int etm_levels_flat(const etm_levels* lv, int64_t* h_totals, int64_t* h_values, 
  * core/cpu_miner.py:963), written by the exporting threads so that the caller divides nothing. */
 int etm_levels_frame(const etm_levels* lv, int64_t* h_totals, int64_t* h_values, int64_t* h_offsets,
                      int64_t* h_counts, double* h_support, int64_t n_rows);
+
+/* The frame a mining call built while it ran (etm_mine_csr / etm_mine_bitvecs with large levels: each level of at
+ * least 256K ints was copied to the host as soon as it was compacted and written out by host threads while the
+ * next levels were mined). h_totals[0] = itemsets, h_totals[1] = items, or both -1 when the result holds no such
+ * frame (then export with etm_levels_frame). h_ptrs[0..3] = item ids (int64), offsets (int64, itemsets + 1),
+ * counts (int64) and supports (double), the same values etm_levels_frame writes, owned by the result until
+ * etm_levels_free. With release_rows, the result's per-level rows go back to the pinned pool at once (later
+ * etm_levels_copy of levels K >= 2, etm_levels_flat and etm_levels_frame then fail). No counterpart in the
+ * reference: its frame is built on the host after the level loop (gpu/row_split.py:1172-1178). */
+int etm_levels_frame_view(etm_levels* lv, int64_t* h_totals, void** h_ptrs, int32_t release_rows);
 
 #ifdef __cplusplus
 }

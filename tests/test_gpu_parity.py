@@ -165,6 +165,51 @@ def test_support_column_is_count_over_rows(engine):
     assert np.array_equal(np.array([got[k] for k in wk]), exact)
 
 
+def test_early_exported_levels_match_oracle():
+    """Levels of at least 256K ints are copied to the host as soon as they are compacted and written into the
+    result frame by host threads while later levels run: a planted block of 40 items (91,390 itemsets at K=4,
+    457K ints) goes that way, K=1..3 through the final gather. The frame, the per-level arrays read before it, and
+    the default frame's supports all equal the oracle."""
+    import pyarrow as pa
+
+    from et_miner.core.apriori import mine_csr
+    from et_miner.core.result import _mining_min_count
+    from et_miner.gpu import device as dev
+    from et_miner.gpu.native import mine_csr_device
+
+    rng = np.random.default_rng(4040)
+    rows = []
+    for _ in range(3_000):
+        r = set(rng.choice(400, size=rng.integers(1, 6), replace=False).tolist())
+        if rng.random() < 0.3:
+            r |= set(range(1000, 1040))
+        rows.append(sorted(r))
+    indptr, ids = _csr(rows)
+    n = len(rows)
+    want = miner.mine(indptr, ids, 0.1, 4).as_dict()
+    assert sum(1 for s in want if len(s) == 4) == 91_390
+    # The frame (flat arrays viewed in place) and the per-level arrays.
+    d_ptr, d_ids = dev.to_device(indptr), dev.to_device(ids)
+    res = mine_csr_device(d_ptr, d_ids, n, _mining_min_count(0.1, n), 4, levels=True)
+    values, offsets, counts = res.flat
+    assert res.support is not None and res.support_rows == n
+    got = {tuple(values[offsets[i]:offsets[i + 1]].tolist()): int(counts[i]) for i in range(len(counts))}
+    assert got == want
+    assert np.array_equal(res.support, counts / n)
+    per_level = {}
+    for lv_rows, lv_counts in res.levels:
+        for r, c in zip(lv_rows, lv_counts):
+            per_level[tuple(res.col_to_item[r].tolist())] = int(c)
+    assert per_level == want
+    # The default frame through the API, twice in a row (the second call reuses the pooled pinned blocks).
+    for _ in range(2):
+        frame = mine_csr(indptr, ids, 0.1, 4)
+        table = frame if isinstance(frame, pa.Table) else frame.to_arrow()
+        sets = [tuple(x) for x in table.column("itemset").to_pylist()]
+        sup = table.column("support").to_numpy()
+        assert dict(zip(sets, sup)) == {s: c / n for s, c in want.items()}
+
+
 @pytest.mark.parametrize("engine", ENGINES)
 @pytest.mark.parametrize("seed", range(8))
 def test_random_datasets_match_oracle(seed, engine):
