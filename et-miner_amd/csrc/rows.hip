@@ -478,3 +478,23 @@ extern "C" int etm_union_insert(const int32_t* d_rows, const int32_t* d_counts, 
     ETM_LAUNCHED("k_union_insert");
     return ETM_OK;
 }
+
+namespace {
+
+// support[i] = counts[i] / n_rows as an IEEE double division (correctly rounded, as numpy's).
+__global__ __launch_bounds__(256) void k_support(const int64_t* __restrict__ counts, int64_t n, double rows,
+                                                  double* __restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (double)counts[i] / rows;
+}
+
+}  // namespace
+
+extern "C" int etm_support(const int64_t* d_counts, int64_t n, int64_t n_rows, double* d_out, void* stream) {
+    ETM_ARG(n == 0 || (d_counts && d_out), "etm_support: null pointer");
+    ETM_ARG(n_rows > 0, "etm_support: n_rows must be positive");
+    if (n == 0) return ETM_OK;
+    k_support<<<grid_u(n), 256, 0, ETM_STREAM(stream)>>>(d_counts, n, (double)n_rows, d_out);
+    ETM_LAUNCHED("k_support");
+    return ETM_OK;
+}

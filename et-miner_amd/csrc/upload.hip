@@ -1,11 +1,13 @@
-// Host -> device copies of pageable host arrays through a ring of pinned staging buffers.
+// Host <-> device copies of pageable host arrays through a ring of pinned staging buffers.
 //
 // A copy is cut into chunks of kChunk bytes. Each chunk is first copied into the calling thread's next pinned
 // staging buffer by up to kCopyThreads host threads, then handed to an asynchronous DMA on the caller's stream,
 // and the buffer's event is recorded behind it: while the DMA of chunk i runs, the host threads fill the buffer
 // of chunk i + 1. A buffer is refilled only once its previous DMA has completed. The call returns when the last
 // chunk is enqueued; stream order puts later work on the stream after the data. (Pinned memory can only be
-// allocated here, not registered in place, so every byte passes through a staging buffer once.)
+// allocated here, not registered in place, so every byte passes through a staging buffer once.) Downloads run
+// the other way: up to three chunks' DMAs in flight into the ring, each copied out by the host threads as it
+// lands, and the call returns with the data in place.
 #include <algorithm>
 #include <condition_variable>
 #include <cstring>
@@ -90,6 +92,11 @@ struct Ring {
     int next = 0;
 };
 
+Ring& thread_ring() {
+    static thread_local Ring ring;
+    return ring;
+}
+
 hipError_t ring_for_device(Ring& r) {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
@@ -146,11 +153,46 @@ void parallel_for(int parts, const std::function<void(int)>& fn) {
 }  // namespace detail
 }  // namespace etm
 
+extern "C" int etm_download(const void* d_src, int64_t bytes, void* h_dst, void* stream) {
+    ETM_ARG(bytes >= 0, "etm_download: negative size");
+    ETM_ARG(bytes == 0 || (d_src != nullptr && h_dst != nullptr), "etm_download: null pointer");
+    if (bytes == 0) return ETM_OK;
+    Ring& ring = thread_ring();  // the same thread's ring as etm_upload's
+    ETM_HIP(ring_for_device(ring));
+    hipStream_t s = ETM_STREAM(stream);
+    const char* src = (const char*)d_src;
+    char* dst = (char*)h_dst;
+    const int64_t n_chunks = (bytes + kChunk - 1) / kChunk;
+    // Chunk c lands in buffer c % kRing; up to kRing DMAs run ahead of the host copies out of the buffers.
+    auto issue = [&](int64_t c) -> hipError_t {
+        const int b = (int)(c % kRing);
+        if (ring.used[b]) {
+            const hipError_t e = hipEventSynchronize(ring.done[b]);  // an earlier upload still reading it
+            if (e != hipSuccess) return e;
+        }
+        const int64_t off = c * kChunk, n = std::min(kChunk, bytes - off);
+        hipError_t e = hipMemcpyAsync(ring.buf[b], src + off, (size_t)n, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipEventRecord(ring.done[b], s);
+        ring.used[b] = true;
+        return e;
+    };
+    for (int64_t c = 0; c < std::min<int64_t>(kRing, n_chunks); ++c) ETM_HIP(issue(c));
+    for (int64_t c = 0; c < n_chunks; ++c) {
+        const int b = (int)(c % kRing);
+        ETM_HIP(hipEventSynchronize(ring.done[b]));
+        const int64_t off = c * kChunk, n = std::min(kChunk, bytes - off);
+        copy_parallel(dst + off, ring.buf[b], n);
+        ring.used[b] = false;
+        if (c + kRing < n_chunks) ETM_HIP(issue(c + kRing));
+    }
+    return ETM_OK;
+}
+
 extern "C" int etm_upload(const void* h_src, int64_t bytes, void* d_dst, void* stream) {
     ETM_ARG(bytes >= 0, "etm_upload: negative size");
     ETM_ARG(bytes == 0 || (h_src != nullptr && d_dst != nullptr), "etm_upload: null pointer");
     if (bytes == 0) return ETM_OK;
-    static thread_local Ring ring;
+    Ring& ring = thread_ring();
     ETM_HIP(ring_for_device(ring));
     hipStream_t s = ETM_STREAM(stream);
     const char* src = (const char*)h_src;

@@ -17,7 +17,7 @@ from ctypes import POINTER, c_char_p, c_int, c_int32, c_int64, c_size_t, c_void_
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libetm.so"
-ABI_VERSION = 8
+ABI_VERSION = 10
 
 ETM_OK, ETM_EINVAL, ETM_ENOMEM, ETM_EHIP, ETM_EOVERFLOW, ETM_ENODEV, ETM_ECANCELED = 0, 1, 2, 3, 4, 5, 6
 CAND_COUNT, CAND_SKIP, CAND_INFER = 0, 1, 2
@@ -57,6 +57,8 @@ SIGNATURES: dict = {
     "etm_last_error": (c_int, [c_char_p, c_size_t]),
     "etm_device_count": (c_int, [POINTER(c_int)]),
     "etm_upload": (c_int, [P, c_int64, P, P]),
+    "etm_download": (c_int, [P, c_int64, P, P]),
+    "etm_support": (c_int, [P, c_int64, c_int64, P, P]),
     "etm_ids_minmax": (c_int, [P, c_int, c_int64, P, P]),
     "etm_ids_hist": (c_int, [P, c_int, c_int64, c_int64, c_int64, P, P]),
     "etm_bitvec_build": (c_int, [P, P, c_int, c_int64, c_int64, c_int64, P, c_int32, c_int64, P, P, P]),

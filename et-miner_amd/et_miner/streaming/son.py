@@ -248,20 +248,31 @@ def son_csr(indptr, ids, min_support, max_length, chunk_size, local_support_fact
     total_slack = sum(slacks)
     unions = {k: _device_union(parts, k, slacks, chunk_items) for k, parts in sorted(pending.items())}
     sizes = dev.to_host(torch.stack([torch.stack([u[3][0], u[4][0].to(torch.int64)]) for u in unions.values()]))
-    exact, cands = {}, {}
+    masks = {}
     for (k, (keys, known, sl, _, _)), (u, overflow) in zip(unions.items(), sizes.tolist()):
         cap = len(known)
         if overflow or u > cap:
             raise RuntimeError(f"SON union of length {k}: {u} itemsets exceed the capacity {cap}")
-        keys, known, left = keys[: u * k].view(u, k), known[:u], total_slack - sl[:u]
-        keep = known + left >= min_count
+        known_u, left = known[:u], total_slack - sl[:u]
+        keep = known_u + left >= min_count
         need = keep & (left > 0)
-        done = keep & ~need & (known >= min_count)
-        exact[k] = (dev.to_host(keys[done]), dev.to_host(known[done]))
-        if bool(need.any()):
+        masks[k] = (u, keep & ~need & (known_u >= min_count), need)
+    # How many itemsets each length keeps exactly and sends to pass 2, read back at once; the exact ones are then
+    # gathered on the device with no further sync (positions by a prefix sum, the rest into a spare row).
+    n_sel = dev.to_host(torch.stack([torch.stack([m[1].sum(), m[2].sum()]) for m in masks.values()]))
+    exact, cands = {}, {}
+    for (k, (u, done, need)), (n_done, n_need) in zip(masks.items(), n_sel.tolist()):
+        keys, known = unions[k][0][: u * k].view(u, k), unions[k][1][:u]
+        if n_done:
+            pos = torch.cumsum(done, 0) - 1
+            tgt = torch.where(done, pos, torch.full_like(pos, n_done))
+            rows = torch.empty((n_done + 1, k), dtype=keys.dtype, device=keys.device).index_copy_(0, tgt, keys)
+            cnts = torch.empty(n_done + 1, dtype=known.dtype, device=known.device).index_copy_(0, tgt, known)
+            exact[k] = (rows[:n_done], cnts[:n_done])
+        if n_need:
             cands[k] = dev.to_host(keys[need])
     if session:
-        session.end_phase(n_candidates=sum(len(v[1]) for v in exact.values()) + sum(len(c) for c in cands.values()))
+        session.end_phase(n_candidates=sum(int(m[0]) for m in n_sel.tolist()) + sum(len(c) for c in cands.values()))
 
     # Pass 2: every candidate with slack left, counted over every chunk; totals summed on the device.
     if session:
@@ -295,15 +306,52 @@ def son_csr(indptr, ids, min_support, max_length, chunk_size, local_support_fact
             at += len(cands[k])
     if session:
         session.end_phase(n_counted=sum(len(c) for c in cands.values()))
+    # The frame, assembled on the device (pass 2's survivors appended to their lengths) and copied back once.
     levels = []
-    for k in sorted(exact):
-        rows, counts = exact[k]
+    for k in sorted(set(exact) | set(counted)):
+        rows, cnts = exact.get(k, (None, None))
         if k in counted:
             ok = counted[k] >= min_count
-            rows = np.concatenate([rows, cands[k][ok]])
-            counts = np.concatenate([counts, counted[k][ok]])
-        levels.append((rows, counts.astype(np.int64)))
-    return itemset_table(levels, None, n_total, with_counts)
+            more_rows = dev.to_device(np.ascontiguousarray(cands[k][ok], dtype=np.int64))
+            more_cnts = dev.to_device(np.ascontiguousarray(counted[k][ok], dtype=np.int64))
+            rows = more_rows.view(-1, k) if rows is None else torch.cat([rows, more_rows.view(-1, k)])
+            cnts = more_cnts if cnts is None else torch.cat([cnts, more_cnts])
+        if len(cnts):
+            levels.append((k, rows, cnts))
+    return _frame_table(levels, n_total, with_counts)
+
+
+def _frame_table(levels: list, n_rows: int, with_counts: bool):
+    """The result Table (itemset, support[, count]) of device levels [(k, item-id rows (n, k) int64, counts (n)
+    int64)]: item ids, offsets, counts and supports (count / n_rows, IEEE double division) laid out in one device
+    buffer and brought back with one ``etm_download``; the host arrays are views of it."""
+    import pyarrow as pa
+
+    from .. import _lib
+    from ..gpu import device as dev
+
+    torch = dev.torch()
+    if not levels:
+        return itemset_table([], np.zeros(0, np.int64), n_rows, with_counts)
+    n_sets = sum(len(c) for _, _, c in levels)
+    n_vals = sum(len(c) * k for k, _, c in levels)
+    offs, base = [torch.zeros(1, dtype=torch.int64, device=dev.device())], 0
+    for k, _, c in levels:
+        offs.append(base + k * torch.arange(1, len(c) + 1, dtype=torch.int64, device=dev.device()))
+        base += k * len(c)
+    cnts = torch.cat([c.to(torch.int64) for _, _, c in levels])
+    support = torch.empty(n_sets, dtype=torch.float64, device=dev.device())
+    _lib.call("etm_support", cnts.data_ptr(), n_sets, int(n_rows), support.data_ptr(), dev.stream_ptr())
+    buf = torch.cat([torch.cat([r.reshape(-1) for _, r, _ in levels]), torch.cat(offs), cnts, support.view(torch.int64)])
+    host = dev.download(buf)
+    values, offsets = host[:n_vals], host[n_vals : n_vals + n_sets + 1]
+    counts = host[n_vals + n_sets + 1 : n_vals + 2 * n_sets + 1]
+    support = host[n_vals + 2 * n_sets + 1 :].view(np.float64)
+    cols = {"itemset": pa.LargeListArray.from_arrays(pa.array(offsets), pa.array(values)),
+            "support": pa.array(support)}
+    if with_counts:
+        cols["count"] = pa.array(counts)
+    return pa.table(cols)
 
 
 def _device_union(parts: list, k: int, slacks: list, chunk_items: dict) -> tuple:

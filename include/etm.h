@@ -74,7 +74,7 @@ enum { ETM_ROW_ORDER_AUTO = 0, ETM_ROW_ORDER_KEEP = 1, ETM_ROW_ORDER_LEX = 2 };
 enum { ETM_EMIT_NONE = 0, ETM_EMIT_HOST = 1, ETM_EMIT_DEVICE = 2 };
 
 /* ABI version of this header (bumped on any signature or struct-meaning change). */
-#define ETM_ABI_VERSION 8
+#define ETM_ABI_VERSION 10
 int etm_abi_version(void);
 
 /* Copies the calling thread's last error message (NUL-terminated, truncated to n). */
@@ -89,6 +89,12 @@ int etm_device_count(int* h_out);
  * `stream`; returns once the last chunk is enqueued, after which h_src may be reused. Replaces the reference's
  * double-buffered pinned upload of the Arrow value buffers (gpu/input_layer.py:328-393). */
 int etm_upload(const void* h_src, int64_t bytes, void* d_dst, void* stream);
+
+/* Device -> host copy into pageable host memory through the same ring: up to three 32 MiB chunks' DMAs in
+ * flight on `stream` (after the work already queued there), each copied out by up to 8 host threads as it lands.
+ * Returns when the data is in h_dst. No counterpart in the reference, whose results leave the device through
+ * CuPy's synchronous device-to-host copies (.get()); this is the streaming path's result copy. */
+int etm_download(const void* d_src, int64_t bytes, void* h_dst, void* stream);
 
 /* Fills d_out[0] = min, d_out[1] = max of n item ids (int32 or int64 per id_bytes); n >= 1.
  * Replaces the id range scan of gpu/input_layer.py:396-405 (the bincount's span). */
@@ -324,6 +330,11 @@ int etm_union_insert(const int32_t* d_rows, const int32_t* d_counts, int64_t n, 
                      const int64_t* d_col_to_item, int64_t slack, int32_t* d_table, int64_t table_size,
                      int64_t* d_keys, int64_t* d_known, int64_t* d_slack, int64_t* d_n_unique, int64_t cap,
                      int32_t* d_overflow, void* stream);
+
+/* Each itemset's support on the device: d_out[i] = d_counts[i] / n_rows as an IEEE double division, the result
+ * frame's `support = counts / n_rows` (core/cpu_miner.py:963) for a frame assembled on the device (the SON
+ * result, streaming/son.py's final table). */
+int etm_support(const int64_t* d_counts, int64_t n, int64_t n_rows, double* d_out, void* stream);
 
 /* ---- The native level loop --------------------------------------------------------------------------- */
 

@@ -306,8 +306,11 @@ def test_son_with_uneven_chunks(infer):
     second = [sorted(int(perm[x]) for x in r) for r in _rand_rows(rng, 6000, 40, 5)]
     indptr, ids = _csr(first + second)
     want = miner.mine(indptr, ids, 0.02, None, use_generator_pruning=infer).as_dict()
-    son = as_dict(son_csr(indptr, ids, 0.02, None, 2500, use_generator_pruning=infer, with_counts=True))
-    assert son == want
+    table = son_csr(indptr, ids, 0.02, None, 2500, use_generator_pruning=infer, with_counts=True)
+    assert as_dict(table) == want
+    # The frame is assembled on the device: its supports are the counts / n_rows of numpy's division, bit for bit.
+    counts = table.column("count").to_numpy()
+    assert np.array_equal(table.column("support").to_numpy(), counts / (len(indptr) - 1))
 
 
 def test_bitvecs_route_is_read_only_and_exact():
