@@ -920,12 +920,13 @@ __global__ __launch_bounds__(256) void k_live_units(const int32_t* __restrict__ 
 // all of the group's used columns staged in LDS: each column is read once per chunk. The other tiled groups keep
 // the tile-pair kernels (k_live_units skips the flagged ones).
 constexpr int kGlSparseMax = kSparseMax;   // fewer counted pairs: the tile-pair path's sparse wave kernel
-constexpr int kGlPairsMax = 4096;          // counted pairs of a group-listed group: 16 per thread
+constexpr int kGlPairsMax = 4096;          // counted pairs of a group-listed group: 8 per thread
+constexpr int kGlThreads = 512;            // threads of a k_count_glist workgroup
 constexpr int kGlColsMax = 2048;           // its used suffix columns (LDS slots)
 constexpr int64_t kGlScanMax = 1 << 22;    // candidates of a group that one workgroup scans
-constexpr int kGlPer = 16;                 // words a thread stages per step
-constexpr int kGlLdsWords = 256 * kGlPer;  // staging buffer of k_count_glist (32 KiB)
-constexpr int kGlMaxWords = 256;           // most words per staging step (a divisor of 256)
+constexpr int kGlPer = 8;                          // words a thread stages per step
+constexpr int kGlLdsWords = kGlThreads * kGlPer;   // staging buffer of k_count_glist (32 KiB)
+constexpr int kGlMaxWords = 256;                   // most words per staging step (a divisor of kGlThreads)
 constexpr int kGlPreMax = 4;               // prefix words loaded ahead with the columns (more: read in step)
 constexpr int kGlScanPer = 16;             // candidates per thread per step of k_gl_scan
 
@@ -1068,13 +1069,15 @@ __global__ __launch_bounds__(256) void k_gl_scan(const int32_t* __restrict__ til
     }
 }
 
-// One workgroup per (group-listed group, word chunk) unit. Per step of kw words the group's used columns are
-// staged [word][slot] (odd stride) in LDS, ANDed with the group's prefix; the next step's words are loaded into
-// registers while the current step counts, so a step costs its counting, not a global load round trip. Pairs:
-// with at most 256 of them, tpp = 256 / pow2ceil(pairs) threads share a pair's words (one accumulator each,
-// summed over the tpp lanes at the end); with more, thread t takes pairs t, t + 256, ... over every word. One
-// atomic per pair at the end. kw is the largest power of two (at most kGlMaxWords) whose block fits the buffer.
-__global__ __launch_bounds__(256) void k_count_glist(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
+// One workgroup of kGlThreads per (group-listed group, word chunk) unit. Per step of kw words the group's used
+// columns are staged [word][slot] (odd stride) in LDS, ANDed with the group's prefix; the next step's words are
+// loaded into registers while the current step counts, so a step costs its counting, not a global load round
+// trip. Pairs: with at most kGlThreads / 2 of them, tpp threads share a pair's words (one accumulator each, summed
+// over the tpp lanes at the end); with more, each thread takes a run of consecutive pairs over every word, reading
+// a shared A column's word once for the run. One atomic per pair at the end. kw is the largest power of two (at
+// most kGlMaxWords) whose block fits the buffer. (Eight waves of 107 registers: four waves a SIMD; four waves of
+// twice the pairs and staged words took 200 registers, two waves a SIMD.)
+__global__ __launch_bounds__(kGlThreads) void k_count_glist(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
                                                      const int32_t* __restrict__ rows, int32_t m,
                                                      const int64_t* __restrict__ group_start,
                                                      const int64_t* __restrict__ cum_pairs, GlistWs ws,
@@ -1107,27 +1110,35 @@ __global__ __launch_bounds__(256) void k_count_glist(const uint64_t* __restrict_
     const int np = m - 1;
     const int32_t* prow = rows + gs * m;
     const int n_used = e.n_used, n_pairs = e.n_pairs;
-    for (int c = tid; c < n_used; c += 256) col_id[c] = ws.cols[e.col_start + c];
+    for (int c = tid; c < n_used; c += kGlThreads) col_id[c] = ws.cols[e.col_start + c];
     const int stride = n_used | 1;
     const int lg = gl_step_log2(n_used);
     const int kw = 1 << lg;
-    // Pair work: tpp lanes per pair (tpp = 1: up to kGlPairsMax / 256 pairs a thread).
+    // Pair work: tpp lanes per pair, or (tpp = 1) thread t takes the npt consecutive pairs [t npt, (t + 1) npt):
+    // the list is in candidate order, so they mostly share their A column, whose word is then read from LDS
+    // once for all of them (bit q of new_a: pair q's A column differs from pair q - 1's). A thread's pairs past
+    // the list count slot 0 against itself and are dropped.
     int tpp = 1;
-    while (tpp < 4 && n_pairs * (2 * tpp) <= 256) tpp <<= 1;
+    while (tpp < 4 && n_pairs * (2 * tpp) <= kGlThreads) tpp <<= 1;
     const int sub = tid & (tpp - 1);
-    uint32_t slot[kGlPairsMax / 256];
-    int acc[kGlPairsMax / 256];
+    constexpr int kPer = kGlPairsMax / kGlThreads;
+    const int npt = (n_pairs + kGlThreads - 1) / kGlThreads;  // at most kPer
+    uint32_t slot[kPer];
+    int acc[kPer];
+    uint32_t new_a = 1u;
 #pragma unroll
-    for (int q = 0; q < kGlPairsMax / 256; ++q) {
-        const int pi = tpp > 1 ? (q == 0 ? tid / tpp : n_pairs) : tid + 256 * q;
+    for (int q = 0; q < kPer; ++q) {
+        const int pi = tpp > 1 ? (q == 0 ? tid / tpp : n_pairs) : (q < npt ? tid * npt + q : n_pairs);
         slot[q] = pi < n_pairs ? ws.pair_slots[e.pair_start + pi] : 0u;
         acc[q] = 0;
+        if (q > 0 && (slot[q] & 0xFFFFu) != (slot[q - 1] & 0xFFFFu)) new_a |= 1u << q;
     }
     __syncthreads();
     const int64_t w_begin = uniform64(ch) * chunk;
     const int64_t w_end = min(n_words, w_begin + chunk);
-    // kw divides 256: thread t stages word t % kw of every step, for slots t / kw + q * (256 / kw), q < kGlPer.
-    const int wq = tid & (kw - 1), c0 = tid >> lg, cstep = 256 >> lg;
+    // kw divides kGlThreads: thread t stages word t % kw of every step, for slots t / kw + q * (kGlThreads / kw),
+    // q < kGlPer.
+    const int wq = tid & (kw - 1), c0 = tid >> lg, cstep = kGlThreads >> lg;
     uint64_t nxt[kGlPer];
     uint64_t pw[kGlPreMax];
     auto load_step = [&](int64_t wb) {
@@ -1164,15 +1175,15 @@ __global__ __launch_bounds__(256) void k_count_glist(const uint64_t* __restrict_
                 acc[0] = a;
             }
         } else {
+            for (int w = 0; w < kw; ++w) {
+                const uint64_t* row = buf + w * stride;
+                uint64_t va = 0;
 #pragma unroll
-            for (int q = 0; q < kGlPairsMax / 256; ++q) {
-                if (tid + 256 * q < n_pairs) {
-                    const uint64_t* pa = buf + (slot[q] & 0xFFFFu);
-                    const uint64_t* pb = buf + (slot[q] >> 16);
-                    int a = acc[q];
-#pragma unroll 8
-                    for (int w = 0; w < kw; ++w) a = popc_acc(pa[w * stride] & pb[w * stride], a);
-                    acc[q] = a;
+                for (int q = 0; q < kPer; ++q) {
+                    if (q < npt) {  // uniform over the workgroup
+                        if ((new_a >> q) & 1u) va = row[slot[q] & 0xFFFFu];
+                        acc[q] = popc_acc(va & row[slot[q] >> 16], acc[q]);
+                    }
                 }
             }
         }
@@ -1190,9 +1201,9 @@ __global__ __launch_bounds__(256) void k_count_glist(const uint64_t* __restrict_
         return;
     }
 #pragma unroll
-    for (int q = 0; q < kGlPairsMax / 256; ++q) {
-        const int pi = tid + 256 * q;
-        if (pi < n_pairs && acc[q]) {
+    for (int q = 0; q < kPer; ++q) {
+        const int pi = tid * npt + q;
+        if (q < npt && pi < n_pairs && acc[q]) {
             const uint32_t ij = ws.pair_ij[e.pair_start + pi];
             atomicAdd(&counts[cbase + etm::lex_pair_index(ij & 0xFFFFu, ij >> 16, s)], acc[q]);
         }
@@ -1708,7 +1719,7 @@ int live_count_impl(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const 
         const int64_t blocks = kXcds * live_units_per_xcd(n_tiled, kLiveTiledTarget);
         ETM_REQUIRE(blocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",
                     (long long)blocks);
-        k_count_glist<<<(unsigned)blocks, 256, 0, s>>>(d_bv, pitch, n_words, d_rows, m, d_group_start, d_cum_pairs,
+        k_count_glist<<<(unsigned)blocks, kGlThreads, 0, s>>>(d_bv, pitch, n_words, d_rows, m, d_group_start, d_cum_pairs,
                                                        *gl, n_tiled, cnt + 8, d_counts, kLiveTiledTarget,
                                                        kLiveTiledMin);
         ETM_LAUNCHED("k_count_glist");
