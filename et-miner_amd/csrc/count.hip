@@ -439,7 +439,10 @@ __device__ __forceinline__ void tiled_unit(TiledShared& sh, const uint64_t* __re
     int& s_nlist = sh.nlist;
     unsigned long long* s_use = sh.use;
 
-    const int tid = threadIdx.x;
+    // The thread index passes through an empty asm statement, so that a persistent caller's loop cannot hoist
+    // the body's per-thread values out of it and keep them all live across units (255 registers instead of 114).
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
     const int64_t g = etm::wave_upper_index(cum_tiles, n_groups, tp);
     const int64_t gs = group_start[g];
     const int64_t s = group_start[g + 1] - gs;
@@ -1499,6 +1502,8 @@ __global__ __launch_bounds__(256) void k_batch_plan(Job* __restrict__ jobs, int 
 
 // The job holding unit u: pre[j] <= u < pre[j + 1] (pre[0] = 0 <= u < pre[n_jobs]; jobs without units are passed
 // over).
+// The job of unit u (wave-uniform for a wave-uniform u; made so explicitly, so that the job record's fields are
+// read by scalar loads into scalar registers).
 __device__ __forceinline__ int batch_job_of(const int64_t* __restrict__ pre, int n_jobs, int64_t u) {
     int lo = 0, hi = n_jobs;
     while (hi - lo > 1) {
@@ -1506,7 +1511,7 @@ __device__ __forceinline__ int batch_job_of(const int64_t* __restrict__ pre, int
         if (pre[mid] <= u) lo = mid;
         else hi = mid;
     }
-    return lo;
+    return __builtin_amdgcn_readfirstlane(lo);
 }
 
 // The small groups of size class C (S = 2, 4, 8 suffixes at most) of every job: each wave takes the next unit
