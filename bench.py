@@ -414,7 +414,7 @@ def _leg(w, data, mine_csr, table_digests, committed) -> dict:
 
 def _streaming_leg(w, data):
     """The workload mined from host arrays resident and by SON in chunk_size-row chunks (the CSR crosses PCIe in
-    both); wall seconds of one warm run each, results compared by digest."""
+    both); wall seconds, the median of three warm runs each, results compared by digest."""
     import numpy as np
 
     from et_miner.core.apriori import mine_csr
@@ -434,12 +434,16 @@ def _streaming_leg(w, data):
                       ("son", lambda: son_csr(data.indptr, ids, w.min_support, w.max_length, w.chunk_size,
                                               use_generator_pruning=True, with_counts=True))):
         run()  # warm: per-thread arenas sized, code objects loaded
-        t = time.perf_counter()
-        tb = run()
-        out[name + "_s"] = time.perf_counter() - t
+        times = []
+        for _ in range(3):
+            t = time.perf_counter()
+            tb = run()
+            times.append(time.perf_counter() - t)
+        out[name + "_s"] = sorted(times)[1]
         out[name + "_digest"] = digest(tb)
         out["itemsets"] = tb.num_rows
     out["identical_results"] = out.pop("resident_digest") == out.pop("son_digest")
+    out["son_over_resident"] = out["son_s"] / out["resident_s"]
     print(f"[bench] streaming: resident {out['resident_s']:.2f}s, son {out['son_s']:.2f}s, "
           f"identical={out['identical_results']}", file=sys.stderr, flush=True)
     return out
