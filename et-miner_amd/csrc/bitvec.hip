@@ -219,12 +219,8 @@ __global__ __launch_bounds__(kBuildThreads) void k_bitvec_build_lds(const int64_
                                                                     int32_t slab_cols, uint64_t* __restrict__ out) {
     // The slab's tile, sized at launch: slab_cols (at most kTileCols) columns x kTileWords words.
     extern __shared__ unsigned long long tile[];
-    // Word tiles in XCD order: workgroup b runs on XCD x = b % 8, and XCD x takes the consecutive tiles
-    // [x q + min(x, r), ...) of tiles = 8 q + r, one per workgroup in dispatch order, so that the two 64-byte halves
-    // of a column's 128-byte line are written through one L2.
-    const int64_t tiles = gridDim.x, q = tiles / 8, r = tiles % 8, x = blockIdx.x % 8;
-    const int64_t wt = x * q + min(x, r) + blockIdx.x / 8;
-    build_tile(indptr, ids, n_rows, lo, span, lut, n_cols, pitch, order, slab_cols, out, wt, (int)blockIdx.y, tile);
+    build_tile(indptr, ids, n_rows, lo, span, lut, n_cols, pitch, order, slab_cols, out, (int64_t)blockIdx.x,
+               (int)blockIdx.y, tile);
 }
 
 
