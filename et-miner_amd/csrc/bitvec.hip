@@ -416,7 +416,9 @@ void batch_input_shape(etm::batch::InJob& j) {
 int64_t batch_input_blocks(int phase, const etm::batch::InJob& j) {
     switch (phase) {
         case 0:
-            return j.nnz > 0 ? grid_for(j.nnz, j.id_bytes == 4 ? 256 * 16 : 256 * 8, 1024) : 0;
+            // 16K-32K ids a workgroup: the batch's jobs are small, and workgroups of 2K ids spent their time on
+            // their set-up and the job search (1 TB/s against the per-job scan's 5 TB/s).
+            return j.nnz > 0 ? grid_for(j.nnz, j.id_bytes == 4 ? 256 * 128 : 256 * 64, 1024) : 0;
         case 1:
             if (j.nnz == 0) return 0;
             return j.span <= kLdsBins ? grid_for(j.nnz, (int)std::max<int64_t>(4 * j.span, 8192), j.span <= 2048 ? 1024 : 512)

@@ -1455,7 +1455,7 @@ using etm::batch::Queue;
 // queues: every wave (workgroup) takes units until the queue's index passes its total, so these fixed shapes
 // serve any mix of jobs.
 constexpr int64_t kBatchSmallTarget = 2048, kBatchTiledTarget = 2048;
-constexpr int kBatchSmallBlocks = 1024, kBatchTiledBlocks = 512;
+constexpr int kBatchSmallBlocks = 1024, kBatchTiledBlocks = 768;  // the tiled kernel: three workgroups a CU
 
 // One workgroup, a thread per job: zeroes the job's round tallies, sizes its hash (a power of two >= 2 x the
 // previous level's rows) and its word chunks, and lays out the round's units per queue (a small class's groups
