@@ -554,11 +554,16 @@ __device__ __forceinline__ void tiled_unit(TiledShared& sh, const uint64_t* __re
             if ((uB >> c) & 1ull)
                 s_col[nA + __popcll(uB & ((1ull << c) - 1))] = (int64_t)rows[(gs + tb * kT + c) * m + np] * pitch;
         }
-        // This thread's list entries tid, tid + 256, ... as slots.
+        // This thread's list entries tid, tid + 256, ... as slots; or, with at most 128 live pairs, tpp threads
+        // per pair (pair tid / tpp), each counting every tpp-th word of a step (a short list otherwise leaves most
+        // of the workgroup idle while a few threads walk every word).
+        int tpp = 1;
+        while (tpp < 8 && n_live * (2 * tpp) <= 256) tpp <<= 1;
+        const int sub = tid & (tpp - 1);
         int la[kListMax / 256], lb[kListMax / 256], sum[kListMax / 256];
 #pragma unroll
         for (int e = 0; e < kListMax / 256; ++e) {
-            const int li = tid + 256 * e;
+            const int li = tpp > 1 ? (e == 0 ? tid / tpp : n_live) : tid + 256 * e;
             sum[e] = 0;
             la[e] = 0;
             lb[e] = 0;
@@ -582,25 +587,46 @@ __device__ __forceinline__ void tiled_unit(TiledShared& sh, const uint64_t* __re
 #pragma unroll 4
             for (int c = c0; c < nS; c += cstep) s_tiles[wq * stride + c] = in ? (bv[s_col[c] + wb + wq] & p) : 0ull;
             __syncthreads();
+            if (tpp > 1) {
+                if (tid / tpp < n_live) {
+                    int acc = sum[0];
+                    const uint64_t* pa = s_tiles + la[0];
+                    const uint64_t* pb = s_tiles + lb[0];
+                    for (int w = sub; w < kw; w += tpp) acc = popc_acc(pa[w * stride] & pb[w * stride], acc);
+                    sum[0] = acc;
+                }
+            } else {
 #pragma unroll
-            for (int e = 0; e < kListMax / 256; ++e) {
-                if (tid + 256 * e < n_live) {
-                    int acc = sum[e];
-                    const uint64_t* pa = s_tiles + la[e];
-                    const uint64_t* pb = s_tiles + lb[e];
+                for (int e = 0; e < kListMax / 256; ++e) {
+                    if (tid + 256 * e < n_live) {
+                        int acc = sum[e];
+                        const uint64_t* pa = s_tiles + la[e];
+                        const uint64_t* pb = s_tiles + lb[e];
 #pragma unroll 8
-                    for (int w = 0; w < kw; ++w) acc = popc_acc(pa[w * stride] & pb[w * stride], acc);
-                    sum[e] = acc;
+                        for (int w = 0; w < kw; ++w) acc = popc_acc(pa[w * stride] & pb[w * stride], acc);
+                        sum[e] = acc;
+                    }
                 }
             }
             __syncthreads();
         }
-#pragma unroll
-        for (int e = 0; e < kListMax / 256; ++e) {
-            const int li = tid + 256 * e;
-            if (li < n_live && sum[e]) {
+        if (tpp > 1) {
+            // The tpp lanes of a pair are adjacent in one wave: sum them, lane sub == 0 adds the pair's count.
+            int a = sum[0];
+            for (int off = tpp >> 1; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+            const int li = tid / tpp;
+            if (sub == 0 && li < n_live && a) {
                 const int64_t i = ta * kT + s_la[li], j = tb * kT + s_lb[li];
-                atomicAdd(&counts[cbase + pair_slot(i, j, s, layout)], sum[e]);
+                atomicAdd(&counts[cbase + pair_slot(i, j, s, layout)], a);
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < kListMax / 256; ++e) {
+                const int li = tid + 256 * e;
+                if (li < n_live && sum[e]) {
+                    const int64_t i = ta * kT + s_la[li], j = tb * kT + s_lb[li];
+                    atomicAdd(&counts[cbase + pair_slot(i, j, s, layout)], sum[e]);
+                }
             }
         }
         return;
