@@ -958,6 +958,7 @@ constexpr int kGlLdsWords = kGlThreads * kGlPer;   // staging buffer of k_count_
 constexpr int kGlMaxWords = 256;                   // most words per staging step (a divisor of kGlThreads)
 constexpr int kGlPreMax = 4;               // prefix words loaded ahead with the columns (more: read in step)
 constexpr int kGlScanPer = 16;             // candidates per thread per step of k_gl_scan
+constexpr int kGlScanThreads = 1024;       // threads of a k_gl_scan workgroup (one per tiled group)
 
 // Words per staging step of a group-listed group of n_used columns: the largest power of two (at most
 // kGlMaxWords) whose [word][slot] block (odd stride) fits kGlLdsWords.
@@ -986,7 +987,7 @@ struct GlistWs {
 
 // Counters of the mode: [0] heavy group-listed groups (listed from the front of ws.groups), [1] pair slots
 // taken, [2] column slots taken, [3] light ones (listed from the back).
-__global__ __launch_bounds__(256) void k_gl_scan(const int32_t* __restrict__ tiled_list, int64_t n_tiled,
+__global__ __launch_bounds__(kGlScanThreads) void k_gl_scan(const int32_t* __restrict__ tiled_list, int64_t n_tiled,
                                                  const int32_t* __restrict__ rows, int32_t m,
                                                  const int64_t* __restrict__ group_start,
                                                  const int64_t* __restrict__ cum_pairs,
@@ -994,7 +995,7 @@ __global__ __launch_bounds__(256) void k_gl_scan(const int32_t* __restrict__ til
                                                  int64_t* __restrict__ gl_cnt) {
     __shared__ uint64_t used[65536 / 64];
     __shared__ int32_t wpre[65536 / 64];
-    __shared__ int64_t lds[8];
+    __shared__ int64_t lds[kGlScanThreads / 64 + 1];
     __shared__ int64_t s_base[3];
     const int tid = threadIdx.x;
     const int32_t g = tiled_list[blockIdx.x];
@@ -1007,11 +1008,11 @@ __global__ __launch_bounds__(256) void k_gl_scan(const int32_t* __restrict__ til
         return;
     }
     const int nw = (int)((s + 63) / 64);
-    for (int w = tid; w < nw; w += 256) used[w] = 0;
+    for (int w = tid; w < nw; w += kGlScanThreads) used[w] = 0;
     __syncthreads();
     // Thread t takes candidates [t * 16, t * 16 + 16) of each step of 4096, decoded once and then walked in
     // candidate order ((i, j) -> (i, j + 1), or (i + 1, i + 2) at the end of row i).
-    constexpr int kStep = 256 * kGlScanPer;
+    constexpr int kStep = kGlScanThreads * kGlScanPer;
     // Pass 1: the counted pairs and the suffixes they use.
     int64_t n_pairs = 0;
     for (int64_t p0 = (int64_t)tid * kGlScanPer; p0 < P; p0 += kStep) {
@@ -1034,12 +1035,12 @@ __global__ __launch_bounds__(256) void k_gl_scan(const int32_t* __restrict__ til
             }
         }
     }
-    n_pairs = etm::block_sum<256>(n_pairs, lds);
+    n_pairs = etm::block_sum<kGlScanThreads>(n_pairs, lds);
     // Slots: exclusive popcount prefix of the used bitmap, four words a thread.
     int32_t cnt4 = 0;
     for (int w = 4 * tid; w < min(nw, 4 * tid + 4); ++w) cnt4 += __popcll(used[w]);
     int64_t n_used = 0;
-    const int64_t pre4 = etm::block_exclusive_scan<256>(cnt4, lds, &n_used);
+    const int64_t pre4 = etm::block_exclusive_scan<kGlScanThreads>(cnt4, lds, &n_used);
     {
         int32_t at = (int32_t)pre4;
         for (int w = 4 * tid; w < min(nw, 4 * tid + 4); ++w) {
@@ -1067,7 +1068,7 @@ __global__ __launch_bounds__(256) void k_gl_scan(const int32_t* __restrict__ til
         const uint64_t w = used[x >> 6];
         return (uint32_t)(wpre[x >> 6] + __popcll(w & ((1ull << (x & 63)) - 1)));
     };
-    for (int64_t i = tid; i < s; i += 256)
+    for (int64_t i = tid; i < s; i += kGlScanThreads)
         if ((used[i >> 6] >> (i & 63)) & 1ull) ws.cols[col_start + slot_of(i)] = rows[(gs + i) * m + np];
     // Pass 2: the pairs in candidate order (their A slots ascend, so a wave's A reads mostly broadcast).
     int64_t run = 0;
@@ -1078,7 +1079,7 @@ __global__ __launch_bounds__(256) void k_gl_scan(const int32_t* __restrict__ til
         for (int q = 0; q < kGlScanPer; ++q)
             if (p0 + q < P && status[cbase + p0 + q] == ETM_CAND_COUNT) live |= 1u << q;
         int64_t total = 0;
-        int64_t at = run + etm::block_exclusive_scan<256>(__popc(live), lds, &total);
+        int64_t at = run + etm::block_exclusive_scan<kGlScanThreads>(__popc(live), lds, &total);
         if (live) {
             int64_t i, j;
             etm::lex_pair_decode(p0, s, &i, &j);
@@ -1669,7 +1670,7 @@ int live_count_impl(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const 
     if (glist) {
         ETM_REQUIRE(n_tiled <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld tiled groups exceed the grid",
                     (long long)n_tiled);
-        k_gl_scan<<<(unsigned)n_tiled, 256, 0, s>>>(d_small_list + n_small, n_tiled, d_rows, m, d_group_start,
+        k_gl_scan<<<(unsigned)n_tiled, kGlScanThreads, 0, s>>>(d_small_list + n_small, n_tiled, d_rows, m, d_group_start,
                                                     d_cum_pairs, d_status, *gl, cnt + 8);
         ETM_LAUNCHED("k_gl_scan");
     }
