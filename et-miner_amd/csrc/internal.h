@@ -43,7 +43,14 @@ hipError_t launch_hash(const int32_t* rows, int64_t n_cap, const int64_t* d_n, i
                        int64_t table_size, hipStream_t s);
 hipError_t launch_free(const int32_t* rows, int64_t n_cap, const int64_t* d_n, int32_t k, const int32_t* counts,
                        const int32_t* prev_rows, const int32_t* prev_counts, const int32_t* prev_table,
-                       int64_t table_size, uint8_t* free_out, hipStream_t s);
+                       int64_t table_size, uint8_t* free_out, hipStream_t s, bool hinted = false);
+// The survivors of a level already counted by etm_level_compact's first call (workspace ws) written out, with
+// each survivor's free flag pre-set in free_hint (0: inferred count, hence not free; 1: to be computed by
+// launch_free(..., hinted = true)).
+hipError_t level_compact_hinted(const int32_t* rows, int32_t m, const int64_t* group_start, const int64_t* cum_pairs,
+                                int64_t n_groups, int64_t n_cands, const uint8_t* status, const int32_t* counts,
+                                const int32_t* gid, int64_t min_count, int32_t* out_rows, int32_t* out_counts,
+                                uint8_t* free_hint, const int64_t* ws, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
 
 // Level 2 (rows2: n2 lexsorted pairs of column ids < n_cols) as a pair index for the K=3 subset test: a bitmap

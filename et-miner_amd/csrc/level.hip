@@ -856,8 +856,14 @@ struct WriteSurvivor {
     int32_t* out_counts;
     int64_t* out_cand;
     const int32_t* gid;  // group of every candidate, or null (then found by binary search)
+    // With free_hint: each survivor's free flag before the free-flag pass, 0 where the survivor's count was
+    // inferred (its count is the smallest of its subsets', so one subset has it: not free), 1 where it is to be
+    // computed.
+    const uint8_t* status = nullptr;
+    uint8_t* free_hint = nullptr;
     __device__ void operator()(int64_t c, int64_t v, int64_t p) const {
         if (!v) return;
+        if (free_hint) free_hint[p] = (status != nullptr && status[c] == ETM_CAND_INFER) ? 0 : 1;
         const int64_t g = gid ? gid[c] : (n_groups == 1 ? 0 : etm::upper_index(cum_pairs, n_groups, c));
         const int64_t gs = group_start[g];
         const int64_t s = group_start[g + 1] - gs;
@@ -894,14 +900,16 @@ struct WriteSelected {
 
 // ── free flags ──────────────────────────────────────────────────────────
 
+// With hinted, free_out already holds 0 for the rows known not to be free (inferred counts), which are skipped.
 __global__ __launch_bounds__(256) void k_free(const int32_t* __restrict__ rows, int64_t n, int32_t k,
                                                const int32_t* __restrict__ counts,
                                                const int32_t* __restrict__ prev_rows,
                                                const int32_t* __restrict__ prev_counts,
                                                const int32_t* __restrict__ table, int64_t mask,
-                                               uint8_t* __restrict__ free_out, const int64_t* d_n = nullptr) {
+                                               uint8_t* __restrict__ free_out, const int64_t* d_n, bool hinted) {
     if (d_n) n = min(n, *d_n);
     for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+        if (hinted && free_out[x] == 0) continue;
         const int32_t* row = rows + x * k;
         uint64_t h = 0;
         for (int t = 0; t < k; ++t) h += item_hash(row[t]);
@@ -1532,11 +1540,21 @@ hipError_t launch_hash(const int32_t* rows, int64_t n_cap, const int64_t* d_n, i
 
 hipError_t launch_free(const int32_t* rows, int64_t n_cap, const int64_t* d_n, int32_t k, const int32_t* counts,
                        const int32_t* prev_rows, const int32_t* prev_counts, const int32_t* prev_table,
-                       int64_t table_size, uint8_t* free_out, hipStream_t s) {
+                       int64_t table_size, uint8_t* free_out, hipStream_t s, bool hinted) {
     if (n_cap == 0) return hipSuccess;
     k_free<<<grid_for(n_cap), 256, 0, s>>>(rows, n_cap, k, counts, prev_rows, prev_counts, prev_table,
-                                           table_size - 1, free_out, d_n);
+                                           table_size - 1, free_out, d_n, hinted);
     return hipGetLastError();
+}
+
+hipError_t level_compact_hinted(const int32_t* rows, int32_t m, const int64_t* group_start, const int64_t* cum_pairs,
+                                int64_t n_groups, int64_t n_cands, const uint8_t* status, const int32_t* counts,
+                                const int32_t* gid, int64_t min_count, int32_t* out_rows, int32_t* out_counts,
+                                uint8_t* free_hint, const int64_t* ws, hipStream_t s) {
+    Survives f{status, counts, min_count};
+    WriteSurvivor w{rows, m, group_start, cum_pairs, n_groups, counts, out_rows, out_counts, nullptr, gid, status,
+                    free_hint};
+    return etm::scan_finish(f, w, n_cands, ws, s);
 }
 
 hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s) {
@@ -1881,7 +1899,7 @@ extern "C" int etm_level_free(const int32_t* d_rows, int64_t n, int32_t k, const
     ETM_ARG(is_pow2(table_size), "etm_level_free: table_size must be a power of two");
     if (n == 0) return ETM_OK;
     k_free<<<grid_for(n), 256, 0, ETM_STREAM(stream)>>>(d_rows, n, k, d_counts, d_prev_rows, d_prev_counts,
-                                                        d_prev_table, table_size - 1, d_free);
+                                                        d_prev_table, table_size - 1, d_free, nullptr, false);
     ETM_LAUNCHED("k_free");
     return ETM_OK;
 }

@@ -847,16 +847,24 @@ void mine_levels(const uint64_t* d_bv, int64_t pitch, int64_t n_words, int64_t n
             cap_exact = true;
             cur.rows = a.get<int32_t>(cap * k);
             cur.counts = a.get<int32_t>(cap);
-            if (cap)
+            if (cap && track_free && status != nullptr) {
+                // The free flags start as hints: the survivors with inferred counts are not free (no lookup).
+                cur.free = a.get<uint8_t>(cap);
+                check(etm::detail::level_compact_hinted(prev.rows, m, g.gs, g.cp, n_groups, n_cands, status, counts,
+                                                        gid, mc, cur.rows, cur.counts, cur.free, ws, s),
+                      "etm_level_compact");
+            } else if (cap) {
                 check_rc(etm_level_compact(prev.rows, m, g.gs, g.cp, n_groups, n_cands, status, counts, gid, mc,
                                            cur.rows, cur.counts, nullptr, cur_scal + 0, ws, ws_n, s),
                          "etm_level_compact");
+            }
         }
         const int64_t* d_n = cur_scal + 0;
         if (track_free) {
-            cur.free = a.get<uint8_t>(cap);
+            const bool hinted = cur.free != nullptr;
+            if (!hinted) cur.free = a.get<uint8_t>(cap);
             check(etm::detail::launch_free(cur.rows, cap, d_n, k, cur.counts, prev.rows, prev.counts, prev.table,
-                                           prev.table_size, cur.free, s),
+                                           prev.table_size, cur.free, s, hinted),
                   "free flags");
             if (prune) {
                 Level sel;
