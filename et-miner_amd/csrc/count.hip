@@ -948,7 +948,10 @@ __global__ __launch_bounds__(256) void k_live_units(const int32_t* __restrict__ 
 // kGlColsMax), k_gl_scan lists them once per level, and k_count_glist counts each (group, word chunk) unit with
 // all of the group's used columns staged in LDS: each column is read once per chunk. The other tiled groups keep
 // the tile-pair kernels (k_live_units skips the flagged ones).
-constexpr int kGlSparseMax = kSparseMax;   // fewer counted pairs: the tile-pair path's sparse wave kernel
+//
+// Groups of more than kGlSparseMax counted pairs are group-listed; fewer go to the tile-pair path (its sparse
+// wave kernel). Measured on deep_sparse_20m's K >= 3 counting: 32: 8.70-8.93 ms, 16: 8.43-8.51, 8: 8.94, 4: 9.53.
+constexpr int kGlSparseMax = 16;
 constexpr int kGlPairsMax = 4096;          // counted pairs of a group-listed group: 8 per thread
 constexpr int kGlThreads = 512;            // threads of a k_count_glist workgroup
 constexpr int kGlColsMax = 2048;           // its used suffix columns (LDS slots)
