@@ -161,13 +161,14 @@ __global__ __launch_bounds__(256) void k_hist_global(const Id* __restrict__ ids,
 constexpr int kTileWords = 8;
 constexpr int kTileCols = 1024;
 
-// Word tile `wt` x column slab `slab` of the builder, through the LDS tile `tile`.
+// Word tile `wt` x column slab `slab` of the builder, through the LDS tile `tile`. With `part`, the tile's
+// popcount of each column (at most 512) goes to part[wt * n_cols + column] as well.
 template <typename Id>
 __device__ __forceinline__ void build_tile(const int64_t* __restrict__ indptr, const Id* __restrict__ ids,
                                            int64_t n_rows, int64_t lo, int64_t span, const int32_t* __restrict__ lut,
                                            int32_t n_cols, int64_t pitch, const int32_t* __restrict__ order,
                                            int32_t slab_cols, uint64_t* __restrict__ out, int64_t wt, int slab,
-                                           unsigned long long* tile) {
+                                           unsigned long long* tile, uint16_t* __restrict__ part = nullptr) {
     const int64_t w0 = wt * kTileWords;
     const int c0 = slab * slab_cols;
     const int nc = min(slab_cols, n_cols - c0);
@@ -204,9 +205,19 @@ __device__ __forceinline__ void build_tile(const int64_t* __restrict__ indptr, c
         for (; e < b; ++e) set((int64_t)ids[e]);
     }
     __syncthreads();
+    // A column's kTileWords words sit in kTileWords consecutive lanes (all active or all done together: both
+    // kBuildThreads and nc * kTileWords are multiples of kTileWords), so its popcount is a three-step butterfly.
     for (int t = threadIdx.x; t < nc * kTileWords; t += kBuildThreads) {
         const int c = t / kTileWords, w = t % kTileWords;
-        if (w0 + w < pitch) out[(int64_t)(c0 + c) * pitch + w0 + w] = tile[t];
+        const unsigned long long x = tile[t];
+        if (w0 + w < pitch) out[(int64_t)(c0 + c) * pitch + w0 + w] = x;
+        if (part != nullptr) {
+            int p = __popcll(x);
+            p += __shfl_xor(p, 1);
+            p += __shfl_xor(p, 2);
+            p += __shfl_xor(p, 4);
+            if (w == 0) part[wt * n_cols + c0 + c] = (uint16_t)p;
+        }
     }
 }
 
@@ -216,11 +227,27 @@ __global__ __launch_bounds__(kBuildThreads) void k_bitvec_build_lds(const int64_
                                                                     int64_t lo, int64_t span,
                                                                     const int32_t* __restrict__ lut, int32_t n_cols,
                                                                     int64_t pitch, const int32_t* __restrict__ order,
-                                                                    int32_t slab_cols, uint64_t* __restrict__ out) {
+                                                                    int32_t slab_cols, uint64_t* __restrict__ out,
+                                                                    uint16_t* __restrict__ part) {
     // The slab's tile, sized at launch: slab_cols (at most kTileCols) columns x kTileWords words.
     extern __shared__ unsigned long long tile[];
     build_tile(indptr, ids, n_rows, lo, span, lut, n_cols, pitch, order, slab_cols, out, (int64_t)blockIdx.x,
-               (int)blockIdx.y, tile);
+               (int)blockIdx.y, tile, part);
+}
+
+// Column popcounts from the builder's per-tile partials: a thread per column and chunk of kPartTiles tiles
+// (consecutive threads read consecutive columns of a tile row), one 64-bit atomic per (column, chunk) into the
+// zeroed output. Reads 2 bytes per (column, 8-word tile) where a pass over the bitvectors reads 64.
+constexpr int64_t kPartTiles = 128;
+
+__global__ __launch_bounds__(256) void k_colpop_parts(const uint16_t* __restrict__ part, int64_t n_tiles,
+                                                       int32_t n_cols, int64_t* __restrict__ out) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n_cols) return;
+    const int64_t t0 = (int64_t)blockIdx.y * kPartTiles, t1 = min(n_tiles, t0 + kPartTiles);
+    uint32_t s = 0;
+    for (int64_t t = t0; t < t1; ++t) s += part[t * n_cols + c];
+    atomicAdd((unsigned long long*)&out[c], (unsigned long long)s);
 }
 
 
@@ -571,7 +598,19 @@ extern "C" int etm_tile_occupancy(const uint64_t* d_bv, int64_t pitch, int64_t n
 extern "C" int etm_bitvec_build(const int64_t* d_indptr, const void* d_ids, int id_bytes, int64_t n_rows, int64_t lo,
                                 int64_t span, const int32_t* d_lut, int32_t n_cols, int64_t pitch,
                                 const int32_t* d_order, uint64_t* d_out, void* stream) {
+    return etm::detail::bitvec_build(d_indptr, d_ids, id_bytes, n_rows, lo, span, d_lut, n_cols, pitch, d_order, d_out,
+                                     nullptr, nullptr, stream);
+}
+
+int64_t etm::detail::bitvec_part_len(int64_t pitch, int32_t n_cols) {
+    return (pitch + kTileWords - 1) / kTileWords * n_cols;
+}
+
+int etm::detail::bitvec_build(const int64_t* d_indptr, const void* d_ids, int id_bytes, int64_t n_rows, int64_t lo,
+                              int64_t span, const int32_t* d_lut, int32_t n_cols, int64_t pitch, const int32_t* d_order,
+                              uint64_t* d_out, uint16_t* d_part, int64_t* d_counts, void* stream) {
     ETM_ARG(d_indptr != nullptr && d_lut != nullptr && d_out != nullptr, "etm_bitvec_build: null pointer");
+    ETM_ARG((d_part == nullptr) == (d_counts == nullptr), "etm_bitvec_build: partials need counts");
     ETM_ARG(id_bytes == 4 || id_bytes == 8, "etm_bitvec_build: id_bytes must be 4 or 8, got %d", id_bytes);
     ETM_ARG(n_rows >= 0 && n_cols >= 0 && span >= 1, "etm_bitvec_build: negative size");
     ETM_ARG(pitch >= (n_rows + 63) / 64, "etm_bitvec_build: pitch %lld < ceil(n_rows/64) = %lld",
@@ -591,12 +630,22 @@ extern "C" int etm_bitvec_build(const int64_t* d_indptr, const void* d_ids, int 
         if (id_bytes == 4)
             k_bitvec_build_lds<int32_t><<<grid, kBuildThreads, lds, s>>>(d_indptr, (const int32_t*)d_ids, n_rows, lo,
                                                                          span, d_lut, n_cols, pitch, d_order,
-                                                                         slab_cols, d_out);
+                                                                         slab_cols, d_out, d_part);
         else
             k_bitvec_build_lds<int64_t><<<grid, kBuildThreads, lds, s>>>(d_indptr, (const int64_t*)d_ids, n_rows, lo,
                                                                          span, d_lut, n_cols, pitch, d_order,
-                                                                         slab_cols, d_out);
+                                                                         slab_cols, d_out, d_part);
         ETM_LAUNCHED("k_bitvec_build_lds");
+    }
+    if (d_part != nullptr) {
+        const int64_t chunks = (tiles + kPartTiles - 1) / kPartTiles;
+        ETM_REQUIRE(chunks <= 65535, ETM_EOVERFLOW, "etm_bitvec_build: %lld tile chunks exceed the grid",
+                    (long long)chunks);
+        k_zero_counts<<<(unsigned)((n_cols + 255) / 256), 256, 0, s>>>(d_counts, n_cols);
+        ETM_LAUNCHED("k_zero_counts");
+        k_colpop_parts<<<dim3((unsigned)((n_cols + 255) / 256), (unsigned)chunks), 256, 0, s>>>(d_part, tiles, n_cols,
+                                                                                                d_counts);
+        ETM_LAUNCHED("k_colpop_parts");
     }
     return ETM_OK;
 }

@@ -107,5 +107,12 @@ hipError_t launch_batch_input(int phase, etm::batch::InJob* jobs, int n_jobs, co
                               int64_t max_slab_cols, hipStream_t s);
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 
+// etm_bitvec_build, and with d_part (bitvec_part_len(pitch, n_cols) uint16) and d_counts (n_cols) the column
+// popcounts as well, from per-tile partials the builder writes beside the bitvectors (no second pass over them).
+int64_t bitvec_part_len(int64_t pitch, int32_t n_cols);
+int bitvec_build(const int64_t* d_indptr, const void* d_ids, int id_bytes, int64_t n_rows, int64_t lo, int64_t span,
+                 const int32_t* d_lut, int32_t n_cols, int64_t pitch, const int32_t* d_order, uint64_t* d_out,
+                 uint16_t* d_part, int64_t* d_counts, void* stream);
+
 }  // namespace detail
 }  // namespace etm

@@ -1641,10 +1641,12 @@ extern "C" int etm_mine_csr(const int64_t* d_indptr, const void* d_ids, int id_b
                      "etm_row_order");
         }
         uint64_t* bv = a.get<uint64_t>((int64_t)n_cols * pitch);
-        check_rc(etm_bitvec_build(d_indptr, d_ids, id_bytes, n_rows, lo, span, d_lut, n_cols, pitch, order, bv, s),
-                 "etm_bitvec_build");
+        // The K=1 counts come with the build (per-tile partials), not from a second pass over the bitvectors.
         int64_t* pc = a.get<int64_t>(n_cols);
-        check_rc(etm_column_popcounts(bv, n_cols, pitch, n_words, pc, s), "etm_column_popcounts");
+        uint16_t* part = a.get<uint16_t>(etm::detail::bitvec_part_len(pitch, n_cols));
+        check_rc(etm::detail::bitvec_build(d_indptr, d_ids, id_bytes, n_rows, lo, span, d_lut, n_cols, pitch, order,
+                                           bv, part, pc, s),
+                 "etm_bitvec_build");
         res->col_counts.resize(n_cols);
         check(hipMemcpyAsync(res->col_counts.data(), pc, n_cols * 8, hipMemcpyDeviceToHost, s), "readback");
         wait_stream(s);

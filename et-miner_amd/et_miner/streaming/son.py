@@ -200,8 +200,8 @@ def son_csr(indptr, ids, min_support, max_length, chunk_size, local_support_fact
         session.start_phase("pass1_local_mining")
     # PASS1_STREAMS chunks are mined at a time, each on its own stream and host thread (a mining call waits on the
     # host once per level; the other chunks' kernels fill those gaps). Chunk i goes to stream i % PASS1_STREAMS,
-    # after chunk i - PASS1_STREAMS. Measured on deep_sparse_20m in 2M-row chunks: 1, 2, 3, 4 streams 137, 112,
-    # 102, 105 ms.
+    # after chunk i - PASS1_STREAMS. Measured on deep_sparse_20m in 2M-row chunks, the whole call: 1, 2, 3, 4, 6
+    # streams 108, 84, 79, 79, 75 ms (six streams hold twice the chunk workspaces for 5%).
     from concurrent.futures import ThreadPoolExecutor
 
     torch = dev.torch()
