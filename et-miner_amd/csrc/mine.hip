@@ -27,15 +27,64 @@
 #include "common.h"
 #include "internal.h"
 
+// Mappings of freed result arrays, kept for the next results (at most kMapPoolCount of them and kMapPoolBytes in
+// all, process-wide): their pages are already faulted in, so the export threads of the next call write at memory
+// speed instead of taking a page fault (and the kernel's zeroing of the page) per first touch.
+class MapPool {
+   public:
+    static constexpr size_t kMapPoolCount = 16;
+    static constexpr size_t kMapPoolBytes = size_t(2) << 30;
+    static MapPool& get() {
+        static MapPool* pool = new MapPool();  // kept for the process
+        return *pool;
+    }
+    // The largest kept mapping of at least `bytes`, or {nullptr, 0}.
+    std::pair<void*, size_t> take(size_t bytes) {
+        std::lock_guard<std::mutex> lk(mu_);
+        size_t best = maps_.size();
+        for (size_t i = 0; i < maps_.size(); ++i)
+            if (maps_[i].second >= bytes && (best == maps_.size() || maps_[i].second > maps_[best].second)) best = i;
+        if (best == maps_.size()) return {nullptr, 0};
+        const auto m = maps_[best];
+        maps_.erase(maps_.begin() + (std::ptrdiff_t)best);
+        held_ -= m.second;
+        return m;
+    }
+    // Keeps a mapping, dropping the smallest kept ones (or this one) past the limits.
+    void give(void* p, size_t cap) {
+        std::vector<std::pair<void*, size_t>> drop;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            maps_.emplace_back(p, cap);
+            held_ += cap;
+            while (!maps_.empty() && (maps_.size() > kMapPoolCount || held_ > kMapPoolBytes)) {
+                size_t small = 0;
+                for (size_t i = 1; i < maps_.size(); ++i)
+                    if (maps_[i].second < maps_[small].second) small = i;
+                drop.push_back(maps_[small]);
+                held_ -= maps_[small].second;
+                maps_.erase(maps_.begin() + (std::ptrdiff_t)small);
+            }
+        }
+        for (const auto& m : drop) munmap(m.first, m.second);
+    }
+
+   private:
+    std::mutex mu_;
+    std::vector<std::pair<void*, size_t>> maps_;
+    size_t held_ = 0;
+};
+
 // A host array in an anonymous mapping that grows by remapping (no copy), with huge pages advised: the exported
-// result of a large lattice, written by host threads while later levels are still being mined.
+// result of a large lattice, written by host threads while later levels are still being mined. A new array starts
+// in the largest kept mapping that fits, and goes back to the pool when freed.
 class etm_host_array {
    public:
     etm_host_array() = default;
     etm_host_array(const etm_host_array&) = delete;
     etm_host_array& operator=(const etm_host_array&) = delete;
     ~etm_host_array() {
-        if (p_) munmap(p_, cap_);
+        if (p_) MapPool::get().give(p_, cap_);
     }
     char* data() const { return (char*)p_; }
     // Capacity of at least `bytes` (false when the mapping fails); the contents are kept.
@@ -44,6 +93,14 @@ class etm_host_array {
         constexpr size_t kHuge = 2u << 20;
         size_t want = std::max(bytes, cap_ * 2);
         want = (want + kHuge - 1) / kHuge * kHuge;
+        if (p_ == nullptr) {
+            const auto m = MapPool::get().take(want);
+            if (m.first != nullptr) {
+                p_ = m.first;
+                cap_ = m.second;
+                return true;
+            }
+        }
         void* q = p_ ? mremap(p_, cap_, want, MREMAP_MAYMOVE)
                      : mmap(nullptr, want, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
         if (q == MAP_FAILED) return false;
