@@ -989,7 +989,7 @@ struct GlistWs {
 };
 
 // Counters of the mode: [0] heavy group-listed groups (listed from the front of ws.groups), [1] pair slots
-// taken, [2] column slots taken, [3] light ones (listed from the back).
+// taken, [2] column slots taken, [3] light ones (listed from the back), [4] k_count_glist's unit head.
 __global__ __launch_bounds__(kGlScanThreads) void k_gl_scan(const int32_t* __restrict__ tiled_list, int64_t n_tiled,
                                                  const int32_t* __restrict__ rows, int32_t m,
                                                  const int64_t* __restrict__ group_start,
@@ -1110,34 +1110,27 @@ __global__ __launch_bounds__(kGlScanThreads) void k_gl_scan(const int32_t* __res
 // a shared A column's word once for the run. One atomic per pair at the end. kw is the largest power of two (at
 // most kGlMaxWords) whose block fits the buffer. (Eight waves of 107 registers: four waves a SIMD; four waves of
 // twice the pairs and staged words took 200 registers, two waves a SIMD.)
-__global__ __launch_bounds__(kGlThreads) void k_count_glist(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
-                                                     const int32_t* __restrict__ rows, int32_t m,
-                                                     const int64_t* __restrict__ group_start,
-                                                     const int64_t* __restrict__ cum_pairs, GlistWs ws,
-                                                     int64_t n_tiled, const int64_t* __restrict__ gl_cnt,
-                                                     int32_t* __restrict__ counts, int64_t target,
-                                                     int64_t min_words) {
-    __shared__ uint64_t buf[kGlLdsWords];
-    __shared__ int32_t col_id[kGlColsMax];
-    const int64_t n_heavy = gl_cnt[0], n_light = gl_cnt[3];
-    const int64_t n_gl = n_heavy + n_light;
-    if (n_gl == 0) return;
-    int64_t chunk, n_chunks;
-    live_chunks(n_words, n_gl, target, kGlMaxWords, min_words, &chunk, &n_chunks);
-    // Units of the heavy groups first (chunk-major), then those of the light ones (listed from the back).
+// One (group, word chunk) unit of k_count_glist: unit u of the heavy groups' units (chunk-major), then the light
+// ones' (listed from the back). Every thread of the workgroup calls it with the same u.
+__device__ __forceinline__ void glist_unit(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
+                                           const int32_t* __restrict__ rows, int32_t m,
+                                           const int64_t* __restrict__ group_start,
+                                           const int64_t* __restrict__ cum_pairs, const GlistWs& ws, int64_t n_tiled,
+                                           int64_t n_heavy, int64_t n_light, int64_t chunk, int64_t n_chunks,
+                                           int32_t* __restrict__ counts, int64_t u, uint64_t* buf, int32_t* col_id) {
     int64_t ch, idx;
-    const int64_t u = blockIdx.x;
     if (u < n_heavy * n_chunks) {
         ch = u / n_heavy;
         idx = u % n_heavy;
     } else {
         const int64_t v = u - n_heavy * n_chunks;
-        if (v >= n_light * n_chunks) return;
         ch = v / n_light;
         idx = n_tiled - 1 - v % n_light;
     }
     const GlGroup e = ws.groups[uniform64(idx)];
-    const int tid = threadIdx.x;
+    // Laundered thread index: the persistent caller's loop keeps no hoisted per-thread values across units.
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
     const int64_t gs = group_start[e.g];
     const int64_t s = group_start[e.g + 1] - gs;
     const int np = m - 1;
@@ -1240,6 +1233,39 @@ __global__ __launch_bounds__(kGlThreads) void k_count_glist(const uint64_t* __re
             const uint32_t ij = ws.pair_ij[e.pair_start + pi];
             atomicAdd(&counts[cbase + etm::lex_pair_index(ij & 0xFFFFu, ij >> 16, s)], acc[q]);
         }
+    }
+}
+
+// A persistent grid over the (group-listed group, word chunk) units: each workgroup takes its own index as its
+// first unit, then the next from the level's head counter (gl_cnt[4], zero on entry), so a level with few or no
+// group-listed groups costs one pass of exits instead of a grid sized for the most units it could hold.
+constexpr int64_t kGlGrid = 512;  // workgroups of k_count_glist: two a CU (107 registers, 40 KiB of LDS)
+
+__global__ __launch_bounds__(kGlThreads) void k_count_glist(const uint64_t* __restrict__ bv, int64_t pitch, int64_t n_words,
+                                                     const int32_t* __restrict__ rows, int32_t m,
+                                                     const int64_t* __restrict__ group_start,
+                                                     const int64_t* __restrict__ cum_pairs, GlistWs ws,
+                                                     int64_t n_tiled, int64_t* __restrict__ gl_cnt,
+                                                     int32_t* __restrict__ counts, int64_t target,
+                                                     int64_t min_words) {
+    __shared__ uint64_t buf[kGlLdsWords];
+    __shared__ int32_t col_id[kGlColsMax];
+    __shared__ int64_t s_next;
+    const int64_t n_heavy = gl_cnt[0], n_light = gl_cnt[3];
+    const int64_t n_gl = n_heavy + n_light;
+    if (n_gl == 0) return;
+    int64_t chunk, n_chunks;
+    live_chunks(n_words, n_gl, target, kGlMaxWords, min_words, &chunk, &n_chunks);
+    const int64_t total = uniform64(n_gl * n_chunks);
+    int64_t u = blockIdx.x;
+    while (u < total) {
+        glist_unit(bv, pitch, n_words, rows, m, group_start, cum_pairs, ws, n_tiled, n_heavy, n_light, chunk, n_chunks,
+                   counts, u, buf, col_id);
+        // Every thread has passed the unit's barriers, so s_next is free; the barrier below also keeps the next
+        // unit's writes of col_id and buf behind this unit's last reads.
+        if (threadIdx.x == 0) s_next = (int64_t)gridDim.x + (int64_t)atomicAdd((unsigned long long*)&gl_cnt[4], 1ull);
+        __syncthreads();
+        u = uniform64(s_next);
     }
 }
 
@@ -1458,7 +1484,7 @@ hipError_t launch_small(const uint64_t* bv, int64_t pitch, int64_t n_words, int6
 }
 
 constexpr int64_t kLiveCounters = 16;  // int64: [0..3] live groups per small class, [4] live tile-pairs,
-                                       // [5] sparse tile-pairs, [8..11] the group-listed mode's counters
+                                       // [5] sparse tile-pairs, [8..12] the group-listed mode's counters
 
 // Units one XCD can be handed by xcd_unit when at most `ub` units are live and live_chunks cuts each into
 // n_chunks <= max(1, ceil(target / n) rounded up to a multiple of 8, capped) chunks: with n_chunks >= 8 an XCD
@@ -1751,9 +1777,7 @@ int live_count_impl(const uint64_t* d_bv, int64_t pitch, int64_t n_words, const 
         ETM_LAUNCHED("k_count_sparse_live");
     }
     if (glist) {
-        const int64_t blocks = kXcds * live_units_per_xcd(n_tiled, kLiveTiledTarget);
-        ETM_REQUIRE(blocks <= 0x7FFFFFFF, ETM_EOVERFLOW, "etm_count_level: %lld workgroups exceed the grid",
-                    (long long)blocks);
+        const int64_t blocks = std::min<int64_t>(kXcds * live_units_per_xcd(n_tiled, kLiveTiledTarget), kGlGrid);
         k_count_glist<<<(unsigned)blocks, kGlThreads, 0, s>>>(d_bv, pitch, n_words, d_rows, m, d_group_start, d_cum_pairs,
                                                        *gl, n_tiled, cnt + 8, d_counts, kLiveTiledTarget,
                                                        kLiveTiledMin);
