@@ -27,4 +27,6 @@ def test_traffic_profile_is_readable():
     assert isinstance(traffic, int) and traffic > 0
     headline = bench.measured_traffic("deep_sparse_20m")
     assert isinstance(headline, int) and 10e9 < headline < 40e9
-    assert bench.measured_traffic("wide_vocab_2m") is None  # no PMC traffic profile for this workload
+    wide = bench.measured_traffic("wide_vocab_2m")
+    assert isinstance(wide, int) and 1e9 < wide < 20e9
+    assert bench.measured_traffic("plain_60k") is None  # no PMC traffic profile for this workload
